@@ -1,0 +1,935 @@
+// Triangle-counting kernels for gfx950 (wave64) and the tc_count dispatcher.
+//
+// Every kernel computes the forward count over the oriented CSR,
+//     T = sum_u sum_{v in N+(u)} |N+(u) & N+(v)|,
+// with uint64 per-lane partials, a wave reduction and one atomic per wave.
+// Integer sums are order-free, so any schedule gives the same exact count.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "internal.h"
+
+namespace tcx {
+namespace {
+
+constexpr int TB = 256;
+
+__device__ inline uint64_t gtid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ inline uint64_t gstride() { return (uint64_t)gridDim.x * blockDim.x; }
+
+__device__ inline unsigned long long wave_sum(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__device__ inline void flush_count(unsigned long long cnt, unsigned long long* total) {
+  cnt = wave_sum(cnt);
+  if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(total, cnt);
+}
+
+// x in a[lo, hi) for ascending a
+__device__ inline bool contains(const uint32_t* a, uint32_t lo, uint32_t hi, uint32_t x) {
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    uint32_t v = a[mid];
+    if (v == x) return true;
+    if (v < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return false;
+}
+
+// largest e in [0, E) with pre[e] <= t  (pre ascending, pre[0] = 0 <= t)
+__device__ inline uint32_t owner_of(const uint32_t* pre, uint32_t E, uint32_t t) {
+  uint32_t lo = 0, hi = E - 1;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// ---- work models -----------------------------------------------------------
+
+// Hu/tc.cu:22-35: W(u) = (sum_{v in N+(u)} d+(v)) * (unsigned)log2(d+(u)).
+// floor(log2(d)) for d >= 1 is 31 - clz(d) exactly.
+__global__ void k_hu_work(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+                          uint32_t n, int64_t* __restrict__ w) {
+  for (uint64_t u = gtid(); u < n; u += gstride()) {
+    uint32_t b = rp[u], e = rp[u + 1];
+    int64_t tot = 0;
+    for (uint32_t k = b; k < e; ++k) {
+      uint32_t v = col[k];
+      tot += rp[v + 1] - rp[v];
+    }
+    uint32_t d = e - b;
+    w[u] = d == 0 ? 0 : tot * (int64_t)(31 - __clz(d));
+  }
+}
+
+// Hu/tc.cu:36-67 binarySearchValue (direction -1) over an inclusive prefix.
+__device__ inline uint32_t hu_bsv(const int64_t* a, int64_t value, uint32_t len) {
+  int64_t s = 0, e = (int64_t)len - 1, right = 0;
+  bool found = false;
+  int64_t mid = (s + e) / 2;
+  while (s <= e) {
+    int64_t x = a[mid];
+    if (x == value) { right = mid; found = true; break; }
+    if (x < value) s = mid + 1;
+    else { if (e == 0) break; e = mid - 1; }
+    mid = (s + e) / 2;
+  }
+  if (!found) return (uint32_t)s;
+  while (right - 1 >= 0 && a[right - 1] == value) --right;
+  return (uint32_t)right;
+}
+
+__global__ void k_block_starts(const int64_t* __restrict__ pre, uint32_t n, int64_t step,
+                               uint32_t nblocks, int32_t* __restrict__ start) {
+  for (uint64_t i = gtid(); i <= nblocks; i += gstride()) {
+    if (i == 0) start[0] = 0;
+    else if (i == nblocks) start[nblocks] = (int32_t)n;
+    else start[i] = (int32_t)hu_bsv(pre, (int64_t)i * step + 1, n);
+  }
+}
+
+// Balanced work per vertex for the AUTO engine: wedges + staged list; hubs
+// (d+ > slice) are scheduled separately and weigh 0 here.
+__global__ void k_auto_work(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+                            uint32_t n, uint32_t slice, int64_t* __restrict__ w) {
+  for (uint64_t u = gtid(); u < n; u += gstride()) {
+    uint32_t b = rp[u], e = rp[u + 1];
+    uint32_t d = e - b;
+    int64_t tot = 0;
+    if (d >= 2 && d <= slice) {
+      for (uint32_t k = b; k < e; ++k) {
+        uint32_t v = col[k];
+        tot += rp[v + 1] - rp[v];
+      }
+      tot += d;
+    }
+    w[u] = tot;
+  }
+}
+
+// Fox/tc.cu:18-33: W(u,v) = int(min(d+u, d+v) * log2(max + 2)).
+__global__ void k_fox_work(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+                           const uint32_t* __restrict__ esrc, uint64_t m, int32_t* __restrict__ w) {
+  for (uint64_t i = gtid(); i < m; i += gstride()) {
+    uint32_t u = esrc[i], v = col[i];
+    int a = (int)(rp[u + 1] - rp[u]), b = (int)(rp[v + 1] - rp[v]);
+    int large = a > b ? a : b;
+    w[i] = (int)((a + b - large) * log2((double)large + 2));
+  }
+}
+
+__global__ void k_fox_bin(const int32_t* __restrict__ w, uint64_t m, const int32_t* thr, int nthr,
+                          uint8_t* __restrict__ bin, unsigned long long* counts) {
+  for (uint64_t i = gtid(); i < m; i += gstride()) {
+    int32_t x = w[i];
+    int j = 0;
+    while (j < nthr && x >= thr[j]) ++j;
+    bin[i] = (uint8_t)j;
+    atomicAdd(counts + j, 1ull);
+  }
+}
+
+__global__ void k_fox_scatter(const uint8_t* __restrict__ bin, uint64_t m,
+                              unsigned long long* cursor, uint32_t* __restrict__ eids) {
+  for (uint64_t i = gtid(); i < m; i += gstride()) {
+    unsigned long long p = atomicAdd(cursor + bin[i], 1ull);
+    eids[p] = (uint32_t)i;
+  }
+}
+
+__global__ void k_row_flags(const uint32_t* __restrict__ rp, uint32_t n, uint32_t min_d,
+                            uint32_t max_d, uint8_t* __restrict__ f) {
+  for (uint64_t u = gtid(); u < n; u += gstride()) {
+    uint32_t d = rp[u + 1] - rp[u];
+    f[u] = (d >= min_d && d <= max_d) ? 1 : 0;
+  }
+}
+
+// ---- Hu: workgroup per vertex range, N+(u) slices staged in LDS ----------
+//
+// Hu/tc.cu:69-201 re-designed for wave64: the range's adjacency slice is
+// copied into LDS by all lanes (coalesced), the wedges u->v->w of the slice
+// are flattened with an LDS prefix sum so consecutive lanes read consecutive
+// w from HBM, and w is binary-searched in the LDS copy of N+(u).  A vertex
+// whose list exceeds the slice searches N+(u) in global memory instead
+// (the reference's cacheWorked == false path).
+template <int SLICE, int ROWCAP>
+struct HuLds {
+  uint32_t col[SLICE];
+  uint32_t vst[SLICE];
+  uint32_t pre[SLICE + 1];
+  uint16_t row[SLICE];
+  uint32_t rof[ROWCAP + 1];
+  uint32_t wsum[TB / 64];
+  uint32_t u2;
+};
+
+// Stage edges [ebase, ebase+E) into LDS: v, start of N+(v), wedge prefix.
+template <int SLICE, int ROWCAP>
+__device__ inline uint32_t stage_edges(HuLds<SLICE, ROWCAP>& L, const uint32_t* __restrict__ rp,
+                                       const uint32_t* __restrict__ col, uint32_t ebase, uint32_t E) {
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t e = tid; e < E; e += TB) {
+    uint32_t v = col[ebase + e];
+    uint32_t vs = rp[v];
+    L.col[e] = v;
+    L.vst[e] = vs;
+    L.pre[e] = rp[v + 1] - vs;
+  }
+  __syncthreads();
+  // block exclusive scan, blocked layout: thread t owns [t*K, t*K+K)
+  constexpr int K = SLICE / TB;
+  uint32_t loc[K];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    uint32_t e = tid * K + k;
+    uint32_t x = e < E ? L.pre[e] : 0;
+    loc[k] = sum;
+    sum += x;
+  }
+  // inclusive wave scan of per-thread sums
+  const uint32_t lane = tid & 63, wv = tid >> 6;
+  uint32_t inc = sum;
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t o = __shfl_up(inc, off, 64);
+    if (lane >= (uint32_t)off) inc += o;
+  }
+  if (lane == 63) L.wsum[wv] = inc;
+  __syncthreads();
+  uint32_t wbase = 0;
+  for (uint32_t i = 0; i < wv; ++i) wbase += L.wsum[i];
+  uint32_t tbase = wbase + inc - sum;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    uint32_t e = tid * K + k;
+    if (e < E) L.pre[e] = tbase + loc[k];
+  }
+  uint32_t total = 0;
+  for (uint32_t i = 0; i < TB / 64; ++i) total += L.wsum[i];
+  if (tid == 0) L.pre[E] = total;
+  __syncthreads();
+  return total;
+}
+
+template <int SLICE, int ROWCAP>
+__global__ __launch_bounds__(TB) void k_hu(const uint32_t* __restrict__ rp,
+                                           const uint32_t* __restrict__ col,
+                                           const int32_t* __restrict__ start, int skip_hubs,
+                                           unsigned long long* total) {
+  __shared__ HuLds<SLICE, ROWCAP> L;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t ub = (uint32_t)start[blockIdx.x], ue = (uint32_t)start[blockIdx.x + 1];
+  unsigned long long cnt = 0;
+  uint32_t u = ub;
+  while (u < ue) {
+    if (tid == 0) {
+      uint32_t base = rp[u];
+      if (rp[u + 1] - base > SLICE) {
+        L.u2 = u;  // hub: its list does not fit the slice
+      } else {
+        uint32_t lo = u + 1, hi = min(ue, u + ROWCAP);
+        while (lo < hi) {
+          uint32_t mid = (lo + hi + 1) >> 1;
+          if (rp[mid] - base <= SLICE) lo = mid;
+          else hi = mid - 1;
+        }
+        L.u2 = lo;
+      }
+    }
+    __syncthreads();
+    const uint32_t u2 = L.u2;
+    if (u2 == u && skip_hubs) {
+      // scheduled by the hub kernel
+      __syncthreads();
+      u = u + 1;
+      continue;
+    }
+    if (u2 == u) {
+      // hub path: N+(u) stays in HBM/L2
+      const uint32_t ab = rp[u], du = rp[u + 1] - ab;
+      for (uint32_t k0 = 0; k0 < du; k0 += SLICE) {
+        uint32_t E = min((uint32_t)SLICE, du - k0);
+        uint32_t T = stage_edges(L, rp, col, ab + k0, E);
+        for (uint32_t t = tid; t < T; t += TB) {
+          uint32_t e = owner_of(L.pre, E, t);
+          uint32_t w = col[L.vst[e] + (t - L.pre[e])];
+          cnt += contains(col, ab, ab + du, w);
+        }
+        __syncthreads();
+      }
+      u = u + 1;
+      continue;
+    }
+    const uint32_t base = rp[u];
+    const uint32_t E = rp[u2] - base;
+    const uint32_t R = u2 - u;
+    for (uint32_t r = tid; r <= R; r += TB) L.rof[r] = rp[u + r] - base;
+    uint32_t T = stage_edges(L, rp, col, base, E);  // has a barrier after rof too
+    for (uint32_t e = tid; e < E; e += TB) {
+      uint32_t lo = 0, hi = R - 1;  // row r with rof[r] <= e < rof[r+1]
+      while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (L.rof[mid] <= e) lo = mid;
+        else hi = mid - 1;
+      }
+      L.row[e] = (uint16_t)lo;
+    }
+    __syncthreads();
+    for (uint32_t t = tid; t < T; t += TB) {
+      uint32_t e = owner_of(L.pre, E, t);
+      uint32_t w = col[L.vst[e] + (t - L.pre[e])];
+      uint32_t r = L.row[e];
+      cnt += contains(L.col, L.rof[r], L.rof[r + 1], w);
+    }
+    __syncthreads();
+    u = u2;
+  }
+  flush_count(cnt, total);
+}
+
+// AUTO hub kernel: task = (hub u, first edge k0); the workgroup takes edges
+// [k0, k0 + HUB_CHUNK) of u, stages N+(u) in LDS when it fits, and checks the
+// flattened wedges against it.
+constexpr int kHubChunk = 1024;
+constexpr int kHubLds = 8192;
+
+__global__ __launch_bounds__(TB) void k_hub(const uint32_t* __restrict__ rp,
+                                            const uint32_t* __restrict__ col,
+                                            const uint2* __restrict__ tasks, uint32_t ntasks,
+                                            unsigned long long* total) {
+  __shared__ HuLds<kHubChunk, 4> L;
+  __shared__ uint32_t s_nu[kHubLds];
+  const uint32_t tid = threadIdx.x;
+  unsigned long long cnt = 0;
+  for (uint32_t ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
+    const uint2 task = tasks[ti];
+    const uint32_t u = task.x, k0 = task.y;
+    const uint32_t ab = rp[u], du = rp[u + 1] - ab;
+    const bool in_lds = du <= (uint32_t)kHubLds;
+    if (in_lds)
+      for (uint32_t i = tid; i < du; i += TB) s_nu[i] = col[ab + i];
+    const uint32_t E = min((uint32_t)kHubChunk, du - k0);
+    uint32_t T = stage_edges(L, rp, col, ab + k0, E);
+    for (uint32_t t = tid; t < T; t += TB) {
+      uint32_t e = owner_of(L.pre, E, t);
+      uint32_t w = col[L.vst[e] + (t - L.pre[e])];
+      cnt += in_lds ? contains(s_nu, 0, du, w) : contains(col, ab, ab + du, w);
+    }
+    __syncthreads();
+  }
+  flush_count(cnt, total);
+}
+
+// ---- Bisson: wave per row, bitmap of N+(u) for long rows -----------------
+//
+// Bisson/tc.cu:18-104 re-designed: rows with d+ > min_deg mark N+(u) in a
+// per-workgroup global bitmap (L2-resident) plus a coarse LDS filter, probe
+// the flattened wedges, then clear only the words they set (the reference
+// memsets the whole n-bit map per row).  Short rows keep N+(u) in LDS and
+// binary-search it.
+constexpr int kCoarseWords = 2048;  // 64K-bit LDS filter
+
+__global__ __launch_bounds__(64) void k_bisson(const uint32_t* __restrict__ rp,
+                                               const uint32_t* __restrict__ col,
+                                               const uint32_t* __restrict__ rows, uint64_t nrows,
+                                               uint32_t* __restrict__ bitmaps, uint64_t words,
+                                               uint32_t min_deg, uint32_t cshift,
+                                               unsigned long long* total) {
+  __shared__ uint32_t s_coarse[kCoarseWords];
+  __shared__ uint32_t s_list[64];
+  __shared__ uint32_t s_vs[64];
+  __shared__ uint32_t s_pre[65];
+  const uint32_t lane = threadIdx.x;
+  uint32_t* bm = bitmaps + (uint64_t)blockIdx.x * words;
+  for (uint32_t i = lane; i < kCoarseWords; i += 64) s_coarse[i] = 0;
+  __syncthreads();
+  unsigned long long cnt = 0;
+  for (uint64_t ri = blockIdx.x; ri < nrows; ri += gridDim.x) {
+    const uint32_t u = rows[ri];
+    const uint32_t base = rp[u], du = rp[u + 1] - base;
+    const bool big = du > min_deg;
+    if (big) {
+      for (uint32_t e = lane; e < du; e += 64) {
+        uint32_t w = col[base + e];
+        atomicOr(bm + (w >> 5), 1u << (w & 31));
+        uint32_t cw = w >> cshift;
+        atomicOr(s_coarse + (cw >> 5), 1u << (cw & 31));
+      }
+    } else {
+      s_list[lane] = lane < du ? col[base + lane] : 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    for (uint32_t k0 = 0; k0 < du; k0 += 64) {
+      const uint32_t E = min(64u, du - k0);
+      uint32_t c = 0;
+      if (lane < E) {
+        uint32_t v = col[base + k0 + lane];
+        uint32_t vs = rp[v];
+        s_vs[lane] = vs;
+        c = rp[v + 1] - vs;
+      }
+      uint32_t inc = c;
+      for (int off = 1; off < 64; off <<= 1) {
+        uint32_t o = __shfl_up(inc, off, 64);
+        if (lane >= (uint32_t)off) inc += o;
+      }
+      s_pre[lane] = inc - c;
+      if (lane == 63) s_pre[64] = inc;
+      __syncthreads();
+      const uint32_t T = s_pre[64];
+      for (uint32_t t = lane; t < T; t += 64) {
+        uint32_t e = owner_of(s_pre, E, t);
+        uint32_t w = col[s_vs[e] + (t - s_pre[e])];
+        if (big) {
+          uint32_t cw = w >> cshift;
+          if (s_coarse[cw >> 5] & (1u << (cw & 31))) {
+            uint32_t word = __hip_atomic_load(bm + (w >> 5), __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+            cnt += (word >> (w & 31)) & 1u;
+          }
+        } else {
+          cnt += contains(s_list, 0, du, w);
+        }
+      }
+      __syncthreads();
+    }
+    if (big) {
+      for (uint32_t e = lane; e < du; e += 64) {
+        uint32_t w = col[base + e];
+        bm[w >> 5] = 0u;
+        s_coarse[(w >> cshift) >> 5] = 0u;
+      }
+      __syncthreads();
+    }
+  }
+  flush_count(cnt, total);
+}
+
+// ---- Fox: lanes-per-edge bins --------------------------------------------
+// Fox/tc.cu:64-125: each group of L lanes takes one edge and binary-searches
+// its strided share of the shorter list in the longer one.
+template <int L>
+__global__ __launch_bounds__(TB) void k_fox(const uint32_t* __restrict__ rp,
+                                            const uint32_t* __restrict__ col,
+                                            const uint32_t* __restrict__ esrc,
+                                            const uint32_t* __restrict__ eids, uint64_t count,
+                                            unsigned long long* total) {
+  const uint64_t g = gtid() / L, ng = gstride() / L;
+  const uint32_t lane = threadIdx.x % L;
+  unsigned long long cnt = 0;
+  for (uint64_t i = g; i < count; i += ng) {
+    uint32_t e = eids[i];
+    uint32_t u = esrc[e], v = col[e];
+    uint32_t a = rp[u], da = rp[u + 1] - a, b = rp[v], db = rp[v + 1] - b;
+    if (da < db) {
+      uint32_t t = a; a = b; b = t;
+      t = da; da = db; db = t;
+    }
+    for (uint32_t j = lane; j < db; j += L) cnt += contains(col, a, a + da, col[b + j]);
+  }
+  flush_count(cnt, total);
+}
+
+// ---- TriCore: wave per edge, 64 sampled pivots of the longer list ---------
+// tricore/src/tricount_gpu.cu:215-290 with 64 samples (one per lane); the
+// pivots live in LDS and each lane does the two-phase search for its
+// strided elements of the shorter list.  Optional node-range filter on the
+// second endpoint implements the 2-D partition passes (:337-406).
+__global__ __launch_bounds__(64) void k_tricore(const uint32_t* __restrict__ rp,
+                                                const uint32_t* __restrict__ col,
+                                                const uint32_t* __restrict__ esrc, uint64_t e_lo,
+                                                uint64_t e_hi, uint32_t v_lo, uint32_t v_hi,
+                                                unsigned long long* total) {
+  __shared__ uint32_t s_samp[64];
+  const uint32_t lane = threadIdx.x;
+  unsigned long long cnt = 0;
+  for (uint64_t e = e_lo + blockIdx.x; e < e_hi; e += gridDim.x) {
+    uint32_t u = esrc[e], v = col[e];
+    if (v < v_lo || v >= v_hi) continue;
+    uint32_t a = rp[u], da = rp[u + 1] - a, b = rp[v], db = rp[v + 1] - b;
+    if (da < db) {
+      uint32_t t = a; a = b; b = t;
+      t = da; da = db; db = t;
+    }
+    if (db == 0) continue;
+    s_samp[lane] = col[a + (uint32_t)(((uint64_t)lane * da) >> 6)];
+    __syncthreads();
+    for (uint32_t j = lane; j < db; j += 64) {
+      uint32_t x = col[b + j];
+      if (s_samp[0] > x) continue;
+      uint32_t lo = 0, hi = 63;
+      while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (s_samp[mid] <= x) lo = mid;
+        else hi = mid - 1;
+      }
+      uint32_t seg_lo = (uint32_t)(((uint64_t)lo * da) >> 6);
+      uint32_t seg_hi = lo == 63 ? da : (uint32_t)(((uint64_t)(lo + 1) * da) >> 6);
+      cnt += contains(col + a, seg_lo, seg_hi, x);
+    }
+    __syncthreads();
+  }
+  flush_count(cnt, total);
+}
+
+// ---- per-vertex counts (wave per vertex, global binary search) ----------
+__global__ __launch_bounds__(TB) void k_per_vertex(const uint32_t* __restrict__ rp,
+                                                   const uint32_t* __restrict__ col, uint32_t n,
+                                                   unsigned long long* __restrict__ tri) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t wave = gtid() >> 6, nwaves = gstride() >> 6;
+  for (uint64_t u = wave; u < n; u += nwaves) {
+    const uint32_t ab = rp[u], du = rp[u + 1] - ab;
+    unsigned long long cnt = 0;
+    for (uint32_t k = 0; k < du; ++k) {
+      uint32_t v = col[ab + k];
+      uint32_t vb = rp[v], ve = rp[v + 1];
+      for (uint32_t j = vb + lane; j < ve; j += 64) cnt += contains(col, ab, ab + du, col[j]);
+    }
+    cnt = wave_sum(cnt);
+    if (lane == 0) tri[u] = cnt;
+  }
+}
+
+}  // namespace
+
+// ---- schedule builders ------------------------------------------------------
+
+static int ensure_version(tc_graph* g) {
+  SchedCache& sc = g->sc;
+  if (sc.version != g->sched_version) {
+    sc.hu_nblocks = 0;
+    sc.auto_nblocks = 0;
+    sc.auto_key = 0;
+    sc.n_hub_tasks = 0;
+    sc.rows_ok = false;
+    sc.fox_lanes = 0;
+    sc.version = g->sched_version;
+  }
+  return TC_OK;
+}
+
+static unsigned grid_for(uint64_t n, int tb = TB) {
+  uint64_t b = (n + tb - 1) / tb;
+  if (b < 1) b = 1;
+  if (b > (1u << 20)) b = 1u << 20;
+  return (unsigned)b;
+}
+
+// start[] over a per-vertex work array with the Hu partition rule.
+static int block_starts(tc_ctx* c, tc_graph* g, DevBuf& w, uint32_t nblocks, DevBuf& start) {
+  const uint32_t n = g->n;
+  DevBuf pre;
+  TCX_TRY(pre.reserve(8 * ((size_t)n + 1)));
+  TCX_TRY(inclusive_scan_i64(c, w.as<int64_t>(), pre.as<int64_t>(), n));
+  int64_t last = 0;
+  if (n) {
+    TCX_HIP(hipMemcpyAsync(&last, pre.as<int64_t>() + n - 1, 8, hipMemcpyDeviceToHost, c->stream));
+    TCX_HIP(hipStreamSynchronize(c->stream));
+  }
+  int64_t step = (last + nblocks - 1) / nblocks;  // Hu/tc.cu:294
+  TCX_TRY(start.reserve(4 * ((size_t)nblocks + 1)));
+  hipLaunchKernelGGL(k_block_starts, dim3(grid_for(nblocks + 1ull)), dim3(TB), 0, c->stream,
+                     pre.as<int64_t>(), n, step, nblocks, start.as<int32_t>());
+  TCX_HIP(hipGetLastError());
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  return TC_OK;
+}
+
+static int hu_schedule(tc_ctx* c, tc_graph* g, uint32_t nblocks) {
+  SchedCache& sc = g->sc;
+  if (sc.hu_nblocks == nblocks) return TC_OK;
+  DevBuf w;
+  TCX_TRY(w.reserve(8 * ((size_t)g->n + 1)));
+  hipLaunchKernelGGL(k_hu_work, dim3(grid_for(g->n)), dim3(TB), 0, c->stream,
+                     g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(), g->n, w.as<int64_t>());
+  TCX_HIP(hipGetLastError());
+  TCX_TRY(block_starts(c, g, w, nblocks, sc.hu_start));
+  sc.hu_nblocks = nblocks;
+  return TC_OK;
+}
+
+static int rows_schedule(tc_ctx* c, tc_graph* g) {
+  SchedCache& sc = g->sc;
+  if (sc.rows_ok) return TC_OK;
+  DevBuf flags;
+  TCX_TRY(flags.reserve((size_t)g->n + 1));
+  TCX_TRY(sc.rows.reserve(4 * ((size_t)g->n + 1)));
+  hipLaunchKernelGGL(k_row_flags, dim3(grid_for(g->n)), dim3(TB), 0, c->stream,
+                     g->rowptr.as<uint32_t>(), g->n, 2u, 0xFFFFFFFFu, flags.as<uint8_t>());
+  TCX_HIP(hipGetLastError());
+  TCX_TRY(select_indices(c, flags.as<uint8_t>(), g->n, sc.rows.as<uint32_t>(), &sc.nrows));
+  sc.rows_ok = true;
+  return TC_OK;
+}
+
+// Fox thresholds, Fox/tc.cu:316-331, generalised to max_lanes = 64.
+static void fox_thresholds(int max_work, uint32_t max_lanes, std::vector<int32_t>& thr) {
+  thr.clear();
+  int workPerThreadB = 8;
+  double maxPowerOf8 = std::log(max_work) / std::log(workPerThreadB);
+  int prev = INT32_MIN;
+  for (uint32_t i = 2; i <= max_lanes; i *= 2) {
+    double index = (double)(maxPowerOf8 - 1) * (i - 1) / (double)max_lanes + 1;
+    int p = powl(workPerThreadB, index);
+    if (p < prev) p = prev;  // keep bins ordered where the reference's would overlap
+    thr.push_back(p);
+    prev = p;
+  }
+}
+
+static int fox_schedule(tc_ctx* c, tc_graph* g, uint32_t max_lanes) {
+  SchedCache& sc = g->sc;
+  if (sc.fox_lanes == max_lanes) return TC_OK;
+  const uint64_t m = g->m_or;
+  DevBuf w, bin;
+  TCX_TRY(w.reserve(4 * std::max<uint64_t>(m, 1)));
+  TCX_TRY(bin.reserve(std::max<uint64_t>(m, 1)));
+  TCX_TRY(sc.fox_eids.reserve(4 * std::max<uint64_t>(m, 1)));
+  hipLaunchKernelGGL(k_fox_work, dim3(grid_for(m)), dim3(TB), 0, c->stream,
+                     g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(), g->esrc.as<uint32_t>(), m,
+                     w.as<int32_t>());
+  TCX_HIP(hipGetLastError());
+  uint32_t mx = 0;
+  TCX_TRY(max_u32(c, w.as<uint32_t>(), m, &mx));
+  std::vector<int32_t> thr;
+  fox_thresholds((int)mx, max_lanes, thr);
+  int nthr = (int)thr.size();
+  const int nb = nthr + 1;
+  TCX_TRY(c->scratch.reserve(4096));
+  int32_t* d_thr = c->scratch.as<int32_t>();
+  unsigned long long* d_cnt = reinterpret_cast<unsigned long long*>(c->scratch.as<char>() + 256);
+  unsigned long long* d_cur = reinterpret_cast<unsigned long long*>(c->scratch.as<char>() + 512);
+  TCX_HIP(hipMemcpyAsync(d_thr, thr.data(), 4 * nthr, hipMemcpyHostToDevice, c->stream));
+  TCX_HIP(hipMemsetAsync(d_cnt, 0, 8 * nb, c->stream));
+  hipLaunchKernelGGL(k_fox_bin, dim3(grid_for(m)), dim3(TB), 0, c->stream, w.as<int32_t>(), m,
+                     d_thr, nthr, bin.as<uint8_t>(), d_cnt);
+  TCX_HIP(hipGetLastError());
+  std::vector<unsigned long long> cnt(nb);
+  TCX_HIP(hipMemcpyAsync(cnt.data(), d_cnt, 8 * nb, hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  sc.fox_off.assign(nb + 1, 0);
+  for (int j = 0; j < nb; ++j) sc.fox_off[j + 1] = sc.fox_off[j] + cnt[j];
+  std::vector<unsigned long long> cur(sc.fox_off.begin(), sc.fox_off.end() - 1);
+  TCX_HIP(hipMemcpyAsync(d_cur, cur.data(), 8 * nb, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_fox_scatter, dim3(grid_for(m)), dim3(TB), 0, c->stream, bin.as<uint8_t>(),
+                     m, d_cur, sc.fox_eids.as<uint32_t>());
+  TCX_HIP(hipGetLastError());
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  sc.fox_lanes = max_lanes;
+  return TC_OK;
+}
+
+static int auto_schedule(tc_ctx* c, tc_graph* g, uint32_t nblocks, uint32_t slice) {
+  SchedCache& sc = g->sc;
+  uint32_t key = nblocks * 131u + slice;
+  if (sc.auto_key == key && sc.auto_nblocks == nblocks) return TC_OK;
+  DevBuf w;
+  TCX_TRY(w.reserve(8 * ((size_t)g->n + 1)));
+  hipLaunchKernelGGL(k_auto_work, dim3(grid_for(g->n)), dim3(TB), 0, c->stream,
+                     g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(), g->n, slice, w.as<int64_t>());
+  TCX_HIP(hipGetLastError());
+  TCX_TRY(block_starts(c, g, w, nblocks, sc.auto_start));
+  // hub tasks (d+ > slice)
+  DevBuf flags, hubs;
+  TCX_TRY(flags.reserve((size_t)g->n + 1));
+  TCX_TRY(hubs.reserve(4 * ((size_t)g->n + 1)));
+  hipLaunchKernelGGL(k_row_flags, dim3(grid_for(g->n)), dim3(TB), 0, c->stream,
+                     g->rowptr.as<uint32_t>(), g->n, slice + 1, 0xFFFFFFFFu, flags.as<uint8_t>());
+  TCX_HIP(hipGetLastError());
+  uint64_t nh = 0;
+  TCX_TRY(select_indices(c, flags.as<uint8_t>(), g->n, hubs.as<uint32_t>(), &nh));
+  std::vector<uint2> tasks;
+  if (nh) {
+    std::vector<uint32_t> h(nh);
+    TCX_HIP(hipMemcpyAsync(h.data(), hubs.p, 4 * nh, hipMemcpyDeviceToHost, c->stream));
+    std::vector<uint32_t> rb(nh), re(nh);
+    for (uint64_t i = 0; i < nh; ++i) {
+      TCX_HIP(hipMemcpyAsync(&rb[i], g->rowptr.as<uint32_t>() + h[i], 4, hipMemcpyDeviceToHost,
+                             c->stream));
+      TCX_HIP(hipMemcpyAsync(&re[i], g->rowptr.as<uint32_t>() + h[i] + 1, 4,
+                             hipMemcpyDeviceToHost, c->stream));
+    }
+    TCX_HIP(hipStreamSynchronize(c->stream));
+    for (uint64_t i = 0; i < nh; ++i)
+      for (uint32_t k = 0; k < re[i] - rb[i]; k += kHubChunk) tasks.push_back(make_uint2(h[i], k));
+  }
+  sc.n_hub_tasks = (uint32_t)tasks.size();
+  if (!tasks.empty()) {
+    TCX_TRY(sc.hub_tasks.reserve(sizeof(uint2) * tasks.size()));
+    TCX_HIP(hipMemcpyAsync(sc.hub_tasks.p, tasks.data(), sizeof(uint2) * tasks.size(),
+                           hipMemcpyHostToDevice, c->stream));
+    TCX_HIP(hipStreamSynchronize(c->stream));
+  }
+  sc.auto_key = key;
+  sc.auto_nblocks = nblocks;
+  return TC_OK;
+}
+
+}  // namespace tcx
+
+using namespace tcx;
+
+namespace {
+
+struct Timer {
+  tc_ctx* c;
+  float ms = 0;
+  explicit Timer(tc_ctx* cc) : c(cc) {}
+  int begin() {
+    TCX_HIP(hipEventRecord(c->ev0, c->stream));
+    return TC_OK;
+  }
+  int end() {
+    TCX_HIP(hipEventRecord(c->ev1, c->stream));
+    TCX_HIP(hipEventSynchronize(c->ev1));
+    TCX_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    return TC_OK;
+  }
+};
+
+int check_ready(tc_ctx* c, tc_graph* g) {
+  if (!c || !g) return fail(TC_ERR_ARG, "NULL argument");
+  if (g->ctx != c) return fail(TC_ERR_ARG, "graph belongs to another context");
+  if (g->orient_mode < 0) return fail(TC_ERR_STATE, "graph must be oriented before counting");
+  TCX_HIP(hipSetDevice(c->device));
+  return TC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tc_count(tc_ctx* c, tc_graph* g, const tc_sched* sched_in, uint64_t* count,
+             tc_timing* timing) {
+  TCX_TRY(check_ready(c, g));
+  if (!count) return fail(TC_ERR_ARG, "NULL count");
+  tc_sched s;
+  memset(&s, 0, sizeof(s));
+  if (sched_in) s = *sched_in;
+  else s.kernel = TC_KERNEL_AUTO;
+  if (s.kernel < TC_KERNEL_HU || s.kernel > TC_KERNEL_AUTO)
+    return fail(TC_ERR_ARG, "unknown kernel %d", s.kernel);
+  ensure_version(g);
+  TCX_TRY(c->accum.reserve(64));
+  unsigned long long* acc = c->accum.as<unsigned long long>();
+  tc_timing tm;
+  memset(&tm, 0, sizeof(tm));
+  *count = 0;
+  if (g->m_or == 0 || g->n == 0) {
+    if (timing) *timing = tm;
+    return TC_OK;
+  }
+  const uint32_t* rp = g->rowptr.as<uint32_t>();
+  const uint32_t* col = g->col.as<uint32_t>();
+  Timer prep(c), run(c);
+  TCX_TRY(prep.begin());
+  // schedule (cached per graph)
+  switch (s.kernel) {
+    case TC_KERNEL_HU:
+      TCX_TRY(hu_schedule(c, g, s.grid_size ? s.grid_size : 30000u));  // Hu/tc.cu:7
+      break;
+    case TC_KERNEL_BISSON:
+      TCX_TRY(rows_schedule(c, g));
+      break;
+    case TC_KERNEL_FOX: {
+      uint32_t L = s.fox_max_lanes ? s.fox_max_lanes : 64;
+      if (L != 1 && L != 2 && L != 4 && L != 8 && L != 16 && L != 32 && L != 64)
+        return fail(TC_ERR_ARG, "fox_max_lanes must be a power of two <= 64");
+      TCX_TRY(fox_schedule(c, g, L));
+      break;
+    }
+    case TC_KERNEL_AUTO:
+      TCX_TRY(auto_schedule(c, g, s.grid_size ? s.grid_size : (uint32_t)c->num_cus * 16u, 2048));
+      break;
+    default:
+      break;
+  }
+  TCX_TRY(prep.end());
+  TCX_HIP(hipMemsetAsync(acc, 0, sizeof(unsigned long long), c->stream));
+  TCX_TRY(run.begin());
+  switch (s.kernel) {
+    case TC_KERNEL_HU: {
+      hipLaunchKernelGGL((k_hu<2048, 1024>), dim3(g->sc.hu_nblocks), dim3(TB), 0, c->stream, rp,
+                         col, g->sc.hu_start.as<int32_t>(), 0, acc);
+      tm.launches = 1;
+      break;
+    }
+    case TC_KERNEL_BISSON: {
+      uint32_t G = s.grid_size ? s.grid_size : (uint32_t)c->num_cus * 8u;
+      uint32_t min_deg = s.bitmap_min_deg ? s.bitmap_min_deg : 64;
+      if (min_deg > 64) min_deg = 64;  // the short-row path holds at most one list per wave
+      uint64_t words = ((uint64_t)g->n + 31) / 32;
+      uint64_t need = words * G;
+      if (c->bitmap_words < need) {
+        TCX_TRY(c->bitmaps.reserve(4 * need));
+        TCX_HIP(hipMemsetAsync(c->bitmaps.p, 0, 4 * need, c->stream));
+        c->bitmap_words = need;
+      }
+      uint32_t cshift = 0;
+      while (((uint64_t)(g->n - 1) >> cshift) >= (uint64_t)kCoarseWords * 32) ++cshift;
+      hipLaunchKernelGGL(k_bisson, dim3(G), dim3(64), 0, c->stream, rp, col,
+                         g->sc.rows.as<uint32_t>(), g->sc.nrows, c->bitmaps.as<uint32_t>(), words,
+                         min_deg, cshift, acc);
+      tm.launches = 1;
+      break;
+    }
+    case TC_KERNEL_FOX: {
+      const auto& off = g->sc.fox_off;
+      const uint32_t* eids = g->sc.fox_eids.as<uint32_t>();
+      const uint32_t* es = g->esrc.as<uint32_t>();
+      for (size_t j = 0; j + 1 < off.size(); ++j) {
+        uint64_t cnt = off[j + 1] - off[j];
+        if (!cnt) continue;
+        const uint32_t* ids = eids + off[j];
+        uint32_t lanes = 1u << j;
+        unsigned grid = (unsigned)std::min<uint64_t>(20000, (cnt * lanes + TB - 1) / TB);
+        switch (lanes) {
+          case 1: hipLaunchKernelGGL(k_fox<1>, dim3(grid), dim3(TB), 0, c->stream, rp, col, es, ids, cnt, acc); break;
+          case 2: hipLaunchKernelGGL(k_fox<2>, dim3(grid), dim3(TB), 0, c->stream, rp, col, es, ids, cnt, acc); break;
+          case 4: hipLaunchKernelGGL(k_fox<4>, dim3(grid), dim3(TB), 0, c->stream, rp, col, es, ids, cnt, acc); break;
+          case 8: hipLaunchKernelGGL(k_fox<8>, dim3(grid), dim3(TB), 0, c->stream, rp, col, es, ids, cnt, acc); break;
+          case 16: hipLaunchKernelGGL(k_fox<16>, dim3(grid), dim3(TB), 0, c->stream, rp, col, es, ids, cnt, acc); break;
+          case 32: hipLaunchKernelGGL(k_fox<32>, dim3(grid), dim3(TB), 0, c->stream, rp, col, es, ids, cnt, acc); break;
+          default: hipLaunchKernelGGL(k_fox<64>, dim3(grid), dim3(TB), 0, c->stream, rp, col, es, ids, cnt, acc); break;
+        }
+        tm.launches++;
+      }
+      break;
+    }
+    case TC_KERNEL_TRICORE: {
+      uint32_t parts = s.tricore_parts ? s.tricore_parts : 1;
+      uint32_t G = s.grid_size ? s.grid_size : (uint32_t)c->num_cus * 32u;
+      const uint32_t* es = g->esrc.as<uint32_t>();
+      if (parts > g->n) parts = g->n;
+      // node_split as in tricore/src/main.cpp:78-86
+      std::vector<uint32_t> split(parts + 1);
+      for (uint32_t i = 0; i <= parts; ++i)
+        split[i] = i == parts ? g->n : (uint32_t)((uint64_t)(g->n / parts) * i);
+      std::vector<uint32_t> ebeg(parts + 1);
+      for (uint32_t i = 0; i <= parts; ++i)
+        TCX_HIP(hipMemcpyAsync(&ebeg[i], rp + split[i], 4, hipMemcpyDeviceToHost, c->stream));
+      TCX_HIP(hipStreamSynchronize(c->stream));
+      TCX_TRY(run.begin());
+      for (uint32_t pm = 0; pm < parts; ++pm) {
+        for (uint32_t pn = pm; pn < parts; ++pn) {
+          hipLaunchKernelGGL(k_tricore, dim3(G), dim3(64), 0, c->stream, rp, col, es,
+                             (uint64_t)ebeg[pm], (uint64_t)ebeg[pm + 1], split[pn], split[pn + 1], acc);
+          tm.launches++;
+          if (pm != pn) {
+            hipLaunchKernelGGL(k_tricore, dim3(G), dim3(64), 0, c->stream, rp, col, es,
+                               (uint64_t)ebeg[pn], (uint64_t)ebeg[pn + 1], split[pm], split[pm + 1],
+                               acc);
+            tm.launches++;
+          }
+        }
+      }
+      break;
+    }
+    case TC_KERNEL_AUTO: {
+      hipLaunchKernelGGL((k_hu<2048, 1024>), dim3(g->sc.auto_nblocks), dim3(TB), 0, c->stream, rp,
+                         col, g->sc.auto_start.as<int32_t>(), 1, acc);
+      tm.launches = 1;
+      if (g->sc.n_hub_tasks) {
+        unsigned grid = std::min<uint32_t>(g->sc.n_hub_tasks, (uint32_t)c->num_cus * 8u);
+        hipLaunchKernelGGL(k_hub, dim3(grid), dim3(TB), 0, c->stream, rp, col,
+                           g->sc.hub_tasks.as<uint2>(), g->sc.n_hub_tasks, acc);
+        tm.launches++;
+      }
+      break;
+    }
+  }
+  TCX_HIP(hipGetLastError());
+  TCX_TRY(run.end());
+  unsigned long long h = 0;
+  TCX_HIP(hipMemcpyAsync(&h, acc, 8, hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  *count = h;
+  tm.ms_count = run.ms;
+  tm.ms_prep = prep.ms;
+  tm.ms_dominant = run.ms;
+  if (timing) *timing = tm;
+  return TC_OK;
+}
+
+int tc_count_per_vertex(tc_ctx* c, tc_graph* g, uint64_t* tri) {
+  TCX_TRY(check_ready(c, g));
+  if (!tri) return fail(TC_ERR_ARG, "NULL output");
+  if (g->n == 0) return TC_OK;
+  DevBuf d;
+  TCX_TRY(d.reserve(8 * (size_t)g->n));
+  hipLaunchKernelGGL(k_per_vertex, dim3(std::min<uint64_t>(grid_for(g->n * 64ull), 65536)),
+                     dim3(TB), 0, c->stream, g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(), g->n,
+                     d.as<unsigned long long>());
+  TCX_HIP(hipGetLastError());
+  TCX_HIP(hipMemcpyAsync(tri, d.p, 8 * (size_t)g->n, hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  return TC_OK;
+}
+
+int tc_hu_workload(tc_ctx* c, tc_graph* g, int64_t* w) {
+  TCX_TRY(check_ready(c, g));
+  if (!w) return fail(TC_ERR_ARG, "NULL output");
+  if (g->n == 0) return TC_OK;
+  DevBuf d;
+  TCX_TRY(d.reserve(8 * (size_t)g->n));
+  hipLaunchKernelGGL(k_hu_work, dim3(grid_for(g->n)), dim3(TB), 0, c->stream,
+                     g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(), g->n, d.as<int64_t>());
+  TCX_HIP(hipGetLastError());
+  TCX_HIP(hipMemcpyAsync(w, d.p, 8 * (size_t)g->n, hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  return TC_OK;
+}
+
+int tc_hu_block_starts(tc_ctx* c, tc_graph* g, uint32_t nblocks, int32_t* start) {
+  TCX_TRY(check_ready(c, g));
+  if (!start || nblocks == 0) return fail(TC_ERR_ARG, "bad arguments");
+  ensure_version(g);
+  TCX_TRY(hu_schedule(c, g, nblocks));
+  TCX_HIP(hipMemcpyAsync(start, g->sc.hu_start.p, 4 * ((size_t)nblocks + 1), hipMemcpyDeviceToHost,
+                         c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  return TC_OK;
+}
+
+int tc_fox_workload(tc_ctx* c, tc_graph* g, int32_t* w) {
+  TCX_TRY(check_ready(c, g));
+  if (!w) return fail(TC_ERR_ARG, "NULL output");
+  if (g->m_or == 0) return TC_OK;
+  DevBuf d;
+  TCX_TRY(d.reserve(4 * g->m_or));
+  hipLaunchKernelGGL(k_fox_work, dim3(grid_for(g->m_or)), dim3(TB), 0, c->stream,
+                     g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(), g->esrc.as<uint32_t>(), g->m_or,
+                     d.as<int32_t>());
+  TCX_HIP(hipGetLastError());
+  TCX_HIP(hipMemcpyAsync(w, d.p, 4 * g->m_or, hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  return TC_OK;
+}
+
+int tc_fox_bin_sizes(tc_ctx* c, tc_graph* g, uint32_t max_lanes, uint64_t* count) {
+  TCX_TRY(check_ready(c, g));
+  if (!count) return fail(TC_ERR_ARG, "NULL output");
+  if (max_lanes == 0 || max_lanes > 64 || (max_lanes & (max_lanes - 1)))
+    return fail(TC_ERR_ARG, "max_lanes must be a power of two <= 64");
+  ensure_version(g);
+  uint32_t nb = 1;
+  for (uint32_t l = max_lanes; l > 1; l >>= 1) ++nb;
+  if (g->m_or == 0) {
+    for (uint32_t j = 0; j < nb; ++j) count[j] = 0;
+    return TC_OK;
+  }
+  TCX_TRY(fox_schedule(c, g, max_lanes));
+  for (uint32_t j = 0; j < nb; ++j) count[j] = g->sc.fox_off[j + 1] - g->sc.fox_off[j];
+  return TC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,775 @@
+// Device graph: upload, simplification, orientation (ID / degree / A-direction
+// / k-core), A-order relabelling and oriented-CSR construction, all in HBM.
+//
+// Layout in HBM (one graph):
+//   src, dst   : uint32[m_raw]   raw records as read from the edge file
+//   rowptr     : uint32[n+1]     oriented CSR offsets
+//   col        : uint32[m+]      oriented CSR targets, ascending per row
+//   esrc       : uint32[m+]      source of each CSR edge (edge-centric kernels)
+//   vkey       : int32[n]        A-direction priority or core number
+#include <algorithm>
+#include <cmath>
+
+#include "internal.h"
+
+namespace tcx {
+namespace {
+
+constexpr int TB = 256;
+
+inline unsigned grid_for(uint64_t n, int tb = TB) {
+  uint64_t b = (n + tb - 1) / tb;
+  if (b < 1) b = 1;
+  if (b > (1u << 20)) b = 1u << 20;
+  return (unsigned)b;
+}
+
+__device__ inline uint64_t gtid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ inline uint64_t gstride() { return (uint64_t)gridDim.x * blockDim.x; }
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+  for (uint64_t i = gtid(); i < n; i += gstride()) p[i] = v;
+}
+__global__ void k_fill_i32(int32_t* p, int32_t v, uint64_t n) {
+  for (uint64_t i = gtid(); i < n; i += gstride()) p[i] = v;
+}
+
+// Degree over the raw records (duplicates and self-loops counted),
+// Hu/preTC.cu:119-123, Bisson/preTC.cu:65-76, tricount_gpu.cu:37-47.
+__global__ void k_degree_raw(const uint32_t* __restrict__ s, const uint32_t* __restrict__ d,
+                             uint64_t m, uint32_t* deg) {
+  for (uint64_t i = gtid(); i < m; i += gstride()) {
+    atomicAdd(deg + s[i], 1u);
+    atomicAdd(deg + d[i], 1u);
+  }
+}
+
+// (prune) flag[v] = deg[v] <= 1, Hu/preTC.cu:139-144.
+__global__ void k_low_degree_flag(const uint32_t* deg, uint32_t n, uint32_t* flag) {
+  for (uint64_t v = gtid(); v < n; v += gstride()) flag[v] = deg[v] <= 1 ? 1u : 0u;
+}
+
+// Orient one raw record into a packed (src<<32 | dst) key; self-loops and
+// pruned endpoints become the sentinel (n_out << 32), which sorts last.
+__global__ void k_orient_keys(const uint32_t* __restrict__ s, const uint32_t* __restrict__ d,
+                              uint64_t m, const uint32_t* __restrict__ deg, int mode,
+                              const uint32_t* __restrict__ pruned_before, uint32_t n_out,
+                              uint64_t* __restrict__ keys) {
+  const uint64_t sentinel = (uint64_t)n_out << 32;
+  for (uint64_t i = gtid(); i < m; i += gstride()) {
+    uint32_t a = s[i], b = d[i];
+    bool swap;
+    if (mode == TC_ORIENT_ID) {
+      swap = a < b;  // src = larger id (Fox/preTC.cu:94-98)
+    } else if (mode == TC_ORIENT_DEGREE) {
+      uint32_t da = deg[a], db = deg[b];
+      swap = da > db || (da == db && a < b);  // Hu/preTC.cu:129
+    } else {
+      uint32_t da = deg[a], db = deg[b];
+      swap = da > db || (da == db && a > b);  // tricount_gpu.cu:76
+    }
+    uint32_t os = swap ? b : a, od = swap ? a : b;
+    uint64_t key;
+    if (os == od) {
+      key = sentinel;
+    } else if (pruned_before) {
+      if (deg[os] <= 1 || deg[od] <= 1) key = sentinel;  // Hu/preTC.cu:151-160
+      else key = ((uint64_t)(os - pruned_before[os]) << 32) | (od - pruned_before[od]);
+    } else {
+      key = ((uint64_t)os << 32) | od;
+    }
+    keys[i] = key;
+  }
+}
+
+// Canonical simple-edge key (min<<32 | max); self-loops -> sentinel.
+__global__ void k_simple_keys(const uint32_t* __restrict__ s, const uint32_t* __restrict__ d,
+                              uint64_t m, uint32_t n, uint64_t* __restrict__ keys) {
+  for (uint64_t i = gtid(); i < m; i += gstride()) {
+    uint32_t a = s[i], b = d[i];
+    uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    keys[i] = a == b ? ((uint64_t)n << 32) : (((uint64_t)lo << 32) | hi);
+  }
+}
+
+// Both directions of every non-loop record (A-direction-seri.cpp:215-221).
+__global__ void k_sym_keys(const uint32_t* __restrict__ s, const uint32_t* __restrict__ d,
+                           uint64_t m, uint32_t n, uint64_t* __restrict__ keys) {
+  for (uint64_t i = gtid(); i < m; i += gstride()) {
+    uint32_t a = s[i], b = d[i];
+    if (a == b) {
+      keys[2 * i] = (uint64_t)n << 32;
+      keys[2 * i + 1] = (uint64_t)n << 32;
+    } else {
+      keys[2 * i] = ((uint64_t)a << 32) | b;
+      keys[2 * i + 1] = ((uint64_t)b << 32) | a;
+    }
+  }
+}
+
+__global__ void k_unzip(const uint64_t* __restrict__ keys, uint64_t m, uint32_t* __restrict__ s,
+                        uint32_t* __restrict__ d) {
+  for (uint64_t i = gtid(); i < m; i += gstride()) {
+    uint64_t k = keys[i];
+    s[i] = (uint32_t)(k >> 32);
+    d[i] = (uint32_t)k;
+  }
+}
+
+// Row offsets + targets + sources from keys sorted by (src,dst); one thread
+// per key plus one for the tail (tricount_gpu.cu:127-137 builds the same
+// offsets from a sorted first-endpoint array).
+__global__ void k_csr_from_sorted(const uint64_t* __restrict__ keys, uint64_t m, uint32_t n,
+                                  uint32_t* __restrict__ rowptr, uint32_t* __restrict__ col,
+                                  uint32_t* __restrict__ esrc) {
+  for (uint64_t i = gtid(); i <= m; i += gstride()) {
+    int64_t prev = i > 0 ? (int64_t)(keys[i - 1] >> 32) : -1;
+    int64_t cur = i < m ? (int64_t)(keys[i] >> 32) : (int64_t)n;
+    for (int64_t r = prev + 1; r <= cur; ++r) rowptr[r] = (uint32_t)i;
+    if (i < m) {
+      col[i] = (uint32_t)keys[i];
+      if (esrc) esrc[i] = (uint32_t)(keys[i] >> 32);
+    }
+  }
+}
+
+__global__ void k_indegree(const uint32_t* __restrict__ col, uint64_t m, uint32_t* indeg) {
+  for (uint64_t i = gtid(); i < m; i += gstride()) atomicAdd(indeg + col[i], 1u);
+}
+
+__device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// acc[0] += sum d+^2, acc[1] += sum d- d+, acc[2] = max d+
+__global__ void k_csr_stats(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ indeg,
+                            uint32_t n, unsigned long long* acc) {
+  unsigned long long s1 = 0, s2 = 0;
+  uint32_t mx = 0;
+  for (uint64_t v = gtid(); v < n; v += gstride()) {
+    uint64_t d = rowptr[v + 1] - rowptr[v];
+    s1 += d * d;
+    s2 += d * (uint64_t)indeg[v];
+    mx = d > mx ? (uint32_t)d : mx;
+  }
+  s1 = wave_sum_u64(s1);
+  s2 = wave_sum_u64(s2);
+  for (int off = 32; off > 0; off >>= 1) {
+    uint32_t o = __shfl_down(mx, off, 64);
+    mx = o > mx ? o : mx;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(acc + 0, s1);
+    atomicAdd(acc + 1, s2);
+    atomicMax(acc + 2, (unsigned long long)mx);
+  }
+}
+
+// ---- A-direction peel (edge_directing/A-direction-seri.cpp:35-153) --------
+//
+// The serial peel assigns a vertex the label of the frontier level during
+// which its runtime degree first drops to the threshold, so processing one
+// whole level in parallel reproduces its labels exactly.  Runtime degrees
+// start at twice the simple degree (each neighbour is listed twice, :44-73)
+// and each peeled neighbour removes 2.
+__global__ void k_adir_init(const uint32_t* __restrict__ rp, uint32_t n, int32_t* rtd,
+                            int32_t* prio) {
+  for (uint64_t v = gtid(); v < n; v += gstride()) {
+    rtd[v] = 2 * (int32_t)(rp[v + 1] - rp[v]);
+    prio[v] = -1;
+  }
+}
+
+__global__ void k_adir_scan(uint32_t n, const int32_t* __restrict__ rtd, int32_t* prio, float t,
+                            int32_t label, uint32_t* frontier, uint32_t* fcount) {
+  for (uint64_t v = gtid(); v < n; v += gstride()) {
+    if (prio[v] == -1 && (float)rtd[v] <= t) {
+      prio[v] = label;
+      frontier[atomicAdd(fcount, 1u)] = (uint32_t)v;
+    }
+  }
+}
+
+__global__ void k_adir_expand(const uint32_t* __restrict__ frontier, uint32_t fsize,
+                              const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+                              int32_t* rtd, int32_t* prio, float t, int32_t label,
+                              uint32_t* next, uint32_t* ncount) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t wave = gtid() >> 6, nwaves = gstride() >> 6;
+  for (uint64_t f = wave; f < fsize; f += nwaves) {
+    uint32_t v = frontier[f];
+    for (uint32_t e = rp[v] + lane; e < rp[v + 1]; e += 64) {
+      uint32_t y = col[e];
+      if (__hip_atomic_load(prio + y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != -1) continue;
+      int32_t old = atomicSub(rtd + y, 2);
+      if ((float)old > t && (float)(old - 2) <= t) {
+        prio[y] = label;
+        next[atomicAdd(ncount, 1u)] = y;
+      }
+    }
+  }
+}
+
+__global__ void k_min_unpeeled(uint32_t n, const int32_t* __restrict__ val,
+                               const int32_t* __restrict__ state, int32_t* out) {
+  int32_t mn = INT32_MAX;
+  for (uint64_t v = gtid(); v < n; v += gstride())
+    if (state[v] == -1 && val[v] < mn) mn = val[v];
+  for (int off = 32; off > 0; off >>= 1) {
+    int32_t o = __shfl_down(mn, off, 64);
+    mn = o < mn ? o : mn;
+  }
+  if ((threadIdx.x & 63) == 0 && mn != INT32_MAX) atomicMin(out, mn);
+}
+
+// ---- k-core (edge_directing/kcore_direction.cpp:91-126) --------------------
+// Level-synchronous peel: at level k every vertex whose remaining degree is
+// <= k is removed with core number k; core numbers are order-independent.
+__global__ void k_core_init(const uint32_t* __restrict__ rp, uint32_t n, int32_t* deg,
+                            int32_t* core) {
+  for (uint64_t v = gtid(); v < n; v += gstride()) {
+    deg[v] = (int32_t)(rp[v + 1] - rp[v]);
+    core[v] = -1;
+  }
+}
+
+__global__ void k_core_scan(uint32_t n, const int32_t* __restrict__ deg, int32_t* core, int32_t k,
+                            uint32_t* frontier, uint32_t* fcount) {
+  for (uint64_t v = gtid(); v < n; v += gstride()) {
+    if (core[v] == -1 && deg[v] <= k) {
+      core[v] = k;
+      frontier[atomicAdd(fcount, 1u)] = (uint32_t)v;
+    }
+  }
+}
+
+__global__ void k_core_expand(const uint32_t* __restrict__ frontier, uint32_t fsize,
+                              const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+                              int32_t* deg, int32_t* core, int32_t k, uint32_t* next,
+                              uint32_t* ncount) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t wave = gtid() >> 6, nwaves = gstride() >> 6;
+  for (uint64_t f = wave; f < fsize; f += nwaves) {
+    uint32_t v = frontier[f];
+    for (uint32_t e = rp[v] + lane; e < rp[v + 1]; e += 64) {
+      uint32_t y = col[e];
+      if (__hip_atomic_load(core + y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != -1) continue;
+      int32_t old = atomicSub(deg + y, 1);
+      if (old == k + 1) {
+        core[y] = k;
+        next[atomicAdd(ncount, 1u)] = y;
+      }
+    }
+  }
+}
+
+// keep[e] for a sorted symmetric key list: ADIR (mode 2) or KCORE (mode 3).
+__global__ void k_sym_keep(const uint64_t* __restrict__ keys, uint64_t m2,
+                           const int32_t* __restrict__ key, int mode, uint8_t* keep) {
+  for (uint64_t e = gtid(); e < m2; e += gstride()) {
+    uint64_t kk = keys[e];
+    uint32_t i = (uint32_t)(kk >> 32), j = (uint32_t)kk;
+    int32_t pi = key[i], pj = key[j];
+    bool k;
+    if (mode == TC_ORIENT_ADIR) k = pj > pi || (pj == pi && i < j);  // A-direction-seri.cpp:259-261
+    else k = pi < pj || (pi == pj && i < j);                         // kcore_direction.cpp:137-138
+    keep[e] = k ? 1 : 0;
+  }
+}
+
+// A-order helpers: oriented out-degree over raw records (A-order.cpp:88-99)
+// and the relabelled "reordered" records (:219-228).
+__global__ void k_aorder_outdeg(const uint32_t* __restrict__ s, const uint32_t* __restrict__ d,
+                                uint64_t m, const uint32_t* __restrict__ deg, uint32_t* rtd) {
+  for (uint64_t i = gtid(); i < m; i += gstride()) {
+    uint32_t a = s[i], b = d[i];
+    bool swap = deg[a] > deg[b] || (deg[a] == deg[b] && a < b);
+    atomicAdd(rtd + (swap ? b : a), 1u);
+  }
+}
+
+__global__ void k_aorder_apply(uint32_t* s, uint32_t* d, uint64_t m, const uint32_t* __restrict__ deg,
+                               const uint32_t* __restrict__ perm) {
+  for (uint64_t i = gtid(); i < m; i += gstride()) {
+    uint32_t a = s[i], b = d[i];
+    bool swap = deg[a] > deg[b] || (deg[a] == deg[b] && a < b);
+    uint32_t os = swap ? b : a, od = swap ? a : b;
+    s[i] = perm[os];
+    d[i] = perm[od];
+  }
+}
+
+int read_u32(tc_ctx* c, const uint32_t* dptr, uint32_t* out) {
+  TCX_HIP(hipMemcpyAsync(out, dptr, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  return TC_OK;
+}
+
+}  // namespace
+
+// Sort keys (in `keys`, scratch of the same size in `tmp`), drop duplicates
+// and the sentinel, build the graph's oriented CSR.
+static int csr_from_keys(tc_ctx* c, tc_graph* g, DevBuf& keys, DevBuf& tmp, uint64_t nkeys,
+                         uint32_t n_out, bool already_sorted_unique) {
+  uint64_t* sorted = keys.as<uint64_t>();
+  uint64_t nuniq = nkeys;
+  uint64_t* uniq = sorted;
+  if (!already_sorted_unique) {
+    int end_bit = 32 + bit_width_u32(n_out);
+    TCX_TRY(sort_u64(c, keys.as<uint64_t>(), tmp.as<uint64_t>(), nkeys, end_bit, &sorted));
+    uint64_t* other = sorted == keys.as<uint64_t>() ? tmp.as<uint64_t>() : keys.as<uint64_t>();
+    TCX_TRY(unique_u64(c, sorted, other, nkeys, &nuniq));
+    uniq = other;
+    if (nuniq > 0) {
+      uint64_t last = 0;
+      TCX_HIP(hipMemcpyAsync(&last, uniq + nuniq - 1, 8, hipMemcpyDeviceToHost, c->stream));
+      TCX_HIP(hipStreamSynchronize(c->stream));
+      if ((uint32_t)(last >> 32) == n_out) --nuniq;  // sentinel
+    }
+  }
+  if (nuniq >= 0xFFFFFFFFull)
+    return fail(TC_ERR_RANGE, "%llu oriented edges exceed the 32-bit CSR offsets",
+                (unsigned long long)nuniq);
+  g->n = n_out;
+  g->m_or = nuniq;
+  TCX_TRY(g->rowptr.reserve(sizeof(uint32_t) * ((size_t)n_out + 1)));
+  TCX_TRY(g->col.reserve(sizeof(uint32_t) * std::max<uint64_t>(nuniq, 1)));
+  TCX_TRY(g->esrc.reserve(sizeof(uint32_t) * std::max<uint64_t>(nuniq, 1)));
+  hipLaunchKernelGGL(k_csr_from_sorted, dim3(grid_for(nuniq + 1)), dim3(TB), 0, c->stream, uniq,
+                     nuniq, n_out, g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(),
+                     g->esrc.as<uint32_t>());
+  TCX_HIP(hipGetLastError());
+  g->sched_version++;
+  return compute_csr_stats(c, g);
+}
+
+int compute_csr_stats(tc_ctx* c, tc_graph* g) {
+  DevBuf indeg;
+  TCX_TRY(indeg.reserve(sizeof(uint32_t) * ((size_t)g->n + 1)));
+  TCX_TRY(c->accum.reserve(64));
+  unsigned long long* acc = c->accum.as<unsigned long long>();
+  TCX_HIP(hipMemsetAsync(acc, 0, 3 * sizeof(unsigned long long), c->stream));
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(g->n)), dim3(TB), 0, c->stream,
+                     indeg.as<uint32_t>(), 0u, (uint64_t)g->n);
+  hipLaunchKernelGGL(k_indegree, dim3(grid_for(g->m_or)), dim3(TB), 0, c->stream,
+                     g->col.as<uint32_t>(), g->m_or, indeg.as<uint32_t>());
+  hipLaunchKernelGGL(k_csr_stats, dim3(grid_for(g->n)), dim3(TB), 0, c->stream,
+                     g->rowptr.as<uint32_t>(), indeg.as<uint32_t>(), g->n, acc);
+  TCX_HIP(hipGetLastError());
+  unsigned long long h[3];
+  TCX_HIP(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  g->sum_out_sq = h[0];
+  g->sum_in_out = h[1];
+  g->max_out = (uint32_t)h[2];
+  return TC_OK;
+}
+
+// Simple symmetric adjacency as sorted unique keys (both directions).
+static int sym_keys(tc_ctx* c, tc_graph* g, DevBuf& keys, DevBuf& tmp, uint64_t** out,
+                    uint64_t* m2) {
+  uint64_t nk = 2 * g->m_raw;
+  TCX_TRY(keys.reserve(8 * std::max<uint64_t>(nk, 1)));
+  TCX_TRY(tmp.reserve(8 * std::max<uint64_t>(nk, 1)));
+  hipLaunchKernelGGL(k_sym_keys, dim3(grid_for(g->m_raw)), dim3(TB), 0, c->stream,
+                     g->src.as<uint32_t>(), g->dst.as<uint32_t>(), g->m_raw, g->n,
+                     keys.as<uint64_t>());
+  TCX_HIP(hipGetLastError());
+  uint64_t* sorted;
+  TCX_TRY(sort_u64(c, keys.as<uint64_t>(), tmp.as<uint64_t>(), nk,
+                   32 + bit_width_u32(g->n), &sorted));
+  uint64_t* other = sorted == keys.as<uint64_t>() ? tmp.as<uint64_t>() : keys.as<uint64_t>();
+  uint64_t nu = 0;
+  TCX_TRY(unique_u64(c, sorted, other, nk, &nu));
+  if (nu > 0) {
+    uint64_t last = 0;
+    TCX_HIP(hipMemcpyAsync(&last, other + nu - 1, 8, hipMemcpyDeviceToHost, c->stream));
+    TCX_HIP(hipStreamSynchronize(c->stream));
+    if ((uint32_t)(last >> 32) == g->n) --nu;
+  }
+  *out = other;
+  *m2 = nu;
+  return TC_OK;
+}
+
+static int sym_csr(tc_ctx* c, const uint64_t* keys, uint64_t m2, uint32_t n, DevBuf& rp,
+                   DevBuf& col) {
+  TCX_TRY(rp.reserve(sizeof(uint32_t) * ((size_t)n + 1)));
+  TCX_TRY(col.reserve(sizeof(uint32_t) * std::max<uint64_t>(m2, 1)));
+  hipLaunchKernelGGL(k_csr_from_sorted, dim3(grid_for(m2 + 1)), dim3(TB), 0, c->stream, keys, m2,
+                     n, rp.as<uint32_t>(), col.as<uint32_t>(), (uint32_t*)nullptr);
+  TCX_HIP(hipGetLastError());
+  return TC_OK;
+}
+
+static int adir_peel(tc_ctx* c, uint32_t n, uint64_t m2, const uint32_t* rp, const uint32_t* col,
+                     int32_t* prio) {
+  DevBuf rtd, fa, fb;
+  TCX_TRY(rtd.reserve(sizeof(int32_t) * ((size_t)n + 1)));
+  TCX_TRY(fa.reserve(sizeof(uint32_t) * ((size_t)n + 1)));
+  TCX_TRY(fb.reserve(sizeof(uint32_t) * ((size_t)n + 1)));
+  TCX_TRY(c->scratch.reserve(64));
+  uint32_t* cnt = c->scratch.as<uint32_t>();
+  hipLaunchKernelGGL(k_adir_init, dim3(grid_for(n)), dim3(TB), 0, c->stream, rp, n,
+                     rtd.as<int32_t>(), prio);
+  TCX_HIP(hipGetLastError());
+  // A-direction-seri.cpp:37 and :77, with edgeNum = 2m (totalEdge, :228,:252)
+  unsigned int edgeNum = (unsigned int)m2;
+  int nodeNum = (int)n;
+  float threshold = ((float)edgeNum) / nodeNum + 1;
+  int threshold_old = threshold;
+  int32_t label = 0;
+  uint64_t peeled = 0;
+  uint32_t* F = fa.as<uint32_t>();
+  uint32_t* N = fb.as<uint32_t>();
+  const unsigned expand_grid = 4096;
+  for (;;) {
+    TCX_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), c->stream));
+    hipLaunchKernelGGL(k_adir_scan, dim3(grid_for(n)), dim3(TB), 0, c->stream, n,
+                       rtd.as<int32_t>(), prio, threshold, label, F, cnt);
+    TCX_HIP(hipGetLastError());
+    uint32_t fs = 0;
+    TCX_TRY(read_u32(c, cnt, &fs));
+    peeled += fs;
+    label++;
+    while (fs > 0) {
+      TCX_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), c->stream));
+      hipLaunchKernelGGL(k_adir_expand, dim3(std::min<unsigned>(expand_grid, grid_for(fs * 64ull))),
+                         dim3(TB), 0, c->stream, F, fs, rp, col, rtd.as<int32_t>(), prio,
+                         threshold, label, N, cnt);
+      TCX_HIP(hipGetLastError());
+      uint32_t ns = 0;
+      TCX_TRY(read_u32(c, cnt, &ns));
+      peeled += ns;
+      std::swap(F, N);
+      fs = ns;
+      label++;
+    }
+    if (peeled >= n) break;
+    threshold += threshold_old;  // :127
+    // Rounds whose scan finds nothing only advance the label and threshold.
+    TCX_HIP(hipMemsetAsync(cnt, 0x7f, sizeof(int32_t), c->stream));
+    hipLaunchKernelGGL(k_min_unpeeled, dim3(grid_for(n)), dim3(TB), 0, c->stream, n,
+                       rtd.as<int32_t>(), prio, (int32_t*)cnt);
+    TCX_HIP(hipGetLastError());
+    int32_t minr = 0;
+    TCX_TRY(read_u32(c, cnt, (uint32_t*)&minr));
+    while (!((float)minr <= threshold)) {
+      label++;
+      threshold += threshold_old;
+    }
+  }
+  return TC_OK;
+}
+
+static int kcore_peel(tc_ctx* c, uint32_t n, const uint32_t* rp, const uint32_t* col,
+                      int32_t* core) {
+  DevBuf deg, fa, fb;
+  TCX_TRY(deg.reserve(sizeof(int32_t) * ((size_t)n + 1)));
+  TCX_TRY(fa.reserve(sizeof(uint32_t) * ((size_t)n + 1)));
+  TCX_TRY(fb.reserve(sizeof(uint32_t) * ((size_t)n + 1)));
+  TCX_TRY(c->scratch.reserve(64));
+  uint32_t* cnt = c->scratch.as<uint32_t>();
+  hipLaunchKernelGGL(k_core_init, dim3(grid_for(n)), dim3(TB), 0, c->stream, rp, n,
+                     deg.as<int32_t>(), core);
+  TCX_HIP(hipGetLastError());
+  uint64_t removed = 0;
+  int32_t k = 0;
+  uint32_t* F = fa.as<uint32_t>();
+  uint32_t* N = fb.as<uint32_t>();
+  while (removed < n) {
+    TCX_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), c->stream));
+    hipLaunchKernelGGL(k_core_scan, dim3(grid_for(n)), dim3(TB), 0, c->stream, n,
+                       deg.as<int32_t>(), core, k, F, cnt);
+    TCX_HIP(hipGetLastError());
+    uint32_t fs = 0;
+    TCX_TRY(read_u32(c, cnt, &fs));
+    removed += fs;
+    while (fs > 0) {
+      TCX_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), c->stream));
+      hipLaunchKernelGGL(k_core_expand, dim3(std::min<unsigned>(4096, grid_for(fs * 64ull))),
+                         dim3(TB), 0, c->stream, F, fs, rp, col, deg.as<int32_t>(), core, k, N,
+                         cnt);
+      TCX_HIP(hipGetLastError());
+      uint32_t ns = 0;
+      TCX_TRY(read_u32(c, cnt, &ns));
+      removed += ns;
+      std::swap(F, N);
+      fs = ns;
+    }
+    if (removed >= n) break;
+    TCX_HIP(hipMemsetAsync(cnt, 0x7f, sizeof(int32_t), c->stream));
+    hipLaunchKernelGGL(k_min_unpeeled, dim3(grid_for(n)), dim3(TB), 0, c->stream, n,
+                       deg.as<int32_t>(), core, (int32_t*)cnt);
+    TCX_HIP(hipGetLastError());
+    int32_t mind = 0;
+    TCX_TRY(read_u32(c, cnt, (uint32_t*)&mind));
+    k = std::max(k + 1, mind);
+  }
+  return TC_OK;
+}
+
+}  // namespace tcx
+
+using namespace tcx;
+
+static int check_graph(tc_graph* g) {
+  if (!g || !g->ctx) return fail(TC_ERR_ARG, "NULL graph");
+  TCX_HIP(hipSetDevice(g->ctx->device));
+  return TC_OK;
+}
+
+extern "C" {
+
+int tc_graph_create(tc_ctx* c, const uint32_t* src, const uint32_t* dst, uint64_t m,
+                    uint32_t n_hint, tc_graph** out) {
+  if (!c || !out) return fail(TC_ERR_ARG, "NULL argument");
+  if (m && (!src || !dst)) return fail(TC_ERR_ARG, "NULL edge arrays");
+  *out = nullptr;
+  TCX_HIP(hipSetDevice(c->device));
+  tc_graph* g = new tc_graph();
+  g->ctx = c;
+  g->m_raw = m;
+  int st = g->src.reserve(4 * std::max<uint64_t>(m, 1));
+  if (st == TC_OK) st = g->dst.reserve(4 * std::max<uint64_t>(m, 1));
+  if (st != TC_OK) { delete g; return st; }
+  if (m) {
+    hipError_t e1 = hipMemcpyAsync(g->src.p, src, 4 * m, hipMemcpyHostToDevice, c->stream);
+    hipError_t e2 = hipMemcpyAsync(g->dst.p, dst, 4 * m, hipMemcpyHostToDevice, c->stream);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+      delete g;
+      return fail(TC_ERR_HIP, "edge upload failed");
+    }
+    uint32_t ms = 0, md = 0;
+    st = max_u32(c, g->src.as<uint32_t>(), m, &ms);
+    if (st == TC_OK) st = max_u32(c, g->dst.as<uint32_t>(), m, &md);
+    if (st != TC_OK) { delete g; return st; }
+    uint32_t mx = std::max(ms, md);
+    if (mx >= 0x7FFFFFFFu) {
+      delete g;
+      return fail(TC_ERR_RANGE, "vertex id %u exceeds 2^31-2", mx);
+    }
+    if (n_hint && mx >= n_hint) {
+      delete g;
+      return fail(TC_ERR_RANGE, "vertex id %u >= n_hint %u", mx, n_hint);
+    }
+    g->n = n_hint ? n_hint : mx + 1;
+  } else {
+    g->n = n_hint;
+  }
+  *out = g;
+  return TC_OK;
+}
+
+int tc_graph_destroy(tc_graph* g) {
+  if (!g) return TC_OK;
+  if (g->ctx) {
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipStreamSynchronize(g->ctx->stream);
+  }
+  delete g;
+  return TC_OK;
+}
+
+int tc_graph_simplify(tc_ctx* c, tc_graph* g) {
+  TCX_TRY(check_graph(g));
+  if (c != g->ctx) return fail(TC_ERR_ARG, "graph belongs to another context");
+  if (g->m_raw == 0) return TC_OK;
+  DevBuf keys, tmp;
+  TCX_TRY(keys.reserve(8 * g->m_raw));
+  TCX_TRY(tmp.reserve(8 * g->m_raw));
+  hipLaunchKernelGGL(k_simple_keys, dim3(grid_for(g->m_raw)), dim3(TB), 0, c->stream,
+                     g->src.as<uint32_t>(), g->dst.as<uint32_t>(), g->m_raw, g->n,
+                     keys.as<uint64_t>());
+  TCX_HIP(hipGetLastError());
+  uint64_t* sorted;
+  TCX_TRY(sort_u64(c, keys.as<uint64_t>(), tmp.as<uint64_t>(), g->m_raw,
+                   32 + bit_width_u32(g->n), &sorted));
+  uint64_t* other = sorted == keys.as<uint64_t>() ? tmp.as<uint64_t>() : keys.as<uint64_t>();
+  uint64_t nu = 0;
+  TCX_TRY(unique_u64(c, sorted, other, g->m_raw, &nu));
+  if (nu > 0) {
+    uint64_t last = 0;
+    TCX_HIP(hipMemcpyAsync(&last, other + nu - 1, 8, hipMemcpyDeviceToHost, c->stream));
+    TCX_HIP(hipStreamSynchronize(c->stream));
+    if ((uint32_t)(last >> 32) == g->n) --nu;
+  }
+  hipLaunchKernelGGL(k_unzip, dim3(grid_for(nu)), dim3(TB), 0, c->stream, other, nu,
+                     g->src.as<uint32_t>(), g->dst.as<uint32_t>());
+  TCX_HIP(hipGetLastError());
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  g->m_raw = nu;
+  return TC_OK;
+}
+
+int tc_graph_relabel_aorder(tc_ctx* c, tc_graph* g, uint32_t* perm_out) {
+  TCX_TRY(check_graph(g));
+  if (c != g->ctx) return fail(TC_ERR_ARG, "graph belongs to another context");
+  uint32_t n = g->n;
+  if (n == 0) return fail(TC_ERR_UNSUPPORTED, "A-order on an empty graph");
+  DevBuf deg, rtd, perm;
+  TCX_TRY(deg.reserve(4 * (size_t)n));
+  TCX_TRY(rtd.reserve(4 * (size_t)n));
+  TCX_TRY(perm.reserve(4 * (size_t)n));
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(TB), 0, c->stream, deg.as<uint32_t>(),
+                     0u, (uint64_t)n);
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(TB), 0, c->stream, rtd.as<uint32_t>(),
+                     0u, (uint64_t)n);
+  hipLaunchKernelGGL(k_degree_raw, dim3(grid_for(g->m_raw)), dim3(TB), 0, c->stream,
+                     g->src.as<uint32_t>(), g->dst.as<uint32_t>(), g->m_raw, deg.as<uint32_t>());
+  hipLaunchKernelGGL(k_aorder_outdeg, dim3(grid_for(g->m_raw)), dim3(TB), 0, c->stream,
+                     g->src.as<uint32_t>(), g->dst.as<uint32_t>(), g->m_raw, deg.as<uint32_t>(),
+                     rtd.as<uint32_t>());
+  TCX_HIP(hipGetLastError());
+  std::vector<uint32_t> h_rtd(n), h_perm(n);
+  TCX_HIP(hipMemcpyAsync(h_rtd.data(), rtd.p, 4 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  TCX_TRY(aorder_perm_from_outdeg(h_rtd.data(), n, h_perm.data()));
+  TCX_HIP(hipMemcpyAsync(perm.p, h_perm.data(), 4 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_aorder_apply, dim3(grid_for(g->m_raw)), dim3(TB), 0, c->stream,
+                     g->src.as<uint32_t>(), g->dst.as<uint32_t>(), g->m_raw, deg.as<uint32_t>(),
+                     perm.as<uint32_t>());
+  TCX_HIP(hipGetLastError());
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  if (perm_out) std::copy(h_perm.begin(), h_perm.end(), perm_out);
+  g->orient_mode = -1;  // any CSR built before is stale
+  return TC_OK;
+}
+
+int tc_graph_orient(tc_ctx* c, tc_graph* g, int mode, uint32_t flags) {
+  TCX_TRY(check_graph(g));
+  if (c != g->ctx) return fail(TC_ERR_ARG, "graph belongs to another context");
+  if (mode < TC_ORIENT_ID || mode > TC_ORIENT_DEGREE_TRICORE)
+    return fail(TC_ERR_ARG, "unknown orientation mode %d", mode);
+  if ((flags & TC_FLAG_PRUNE_DEG1) && mode != TC_ORIENT_DEGREE)
+    return fail(TC_ERR_ARG, "degree<=1 pruning applies to TC_ORIENT_DEGREE only");
+  const uint32_t n = g->n;
+  const uint64_t m = g->m_raw;
+  if (mode == TC_ORIENT_ID || mode == TC_ORIENT_DEGREE || mode == TC_ORIENT_DEGREE_TRICORE) {
+    DevBuf deg, pruned, keys, tmp;
+    TCX_TRY(deg.reserve(4 * ((size_t)n + 1)));
+    TCX_TRY(keys.reserve(8 * std::max<uint64_t>(m, 1)));
+    TCX_TRY(tmp.reserve(8 * std::max<uint64_t>(m, 1)));
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n + 1)), dim3(TB), 0, c->stream,
+                       deg.as<uint32_t>(), 0u, (uint64_t)n + 1);
+    if (mode != TC_ORIENT_ID)
+      hipLaunchKernelGGL(k_degree_raw, dim3(grid_for(m)), dim3(TB), 0, c->stream,
+                         g->src.as<uint32_t>(), g->dst.as<uint32_t>(), m, deg.as<uint32_t>());
+    TCX_HIP(hipGetLastError());
+    uint32_t n_out = n;
+    const uint32_t* pruned_before = nullptr;
+    if (flags & TC_FLAG_PRUNE_DEG1) {
+      DevBuf flag;
+      TCX_TRY(flag.reserve(4 * ((size_t)n + 1)));
+      TCX_TRY(pruned.reserve(4 * ((size_t)n + 1)));
+      hipLaunchKernelGGL(k_low_degree_flag, dim3(grid_for(n)), dim3(TB), 0, c->stream,
+                         deg.as<uint32_t>(), n, flag.as<uint32_t>());
+      TCX_HIP(hipGetLastError());
+      TCX_TRY(exclusive_scan_u32(c, flag.as<uint32_t>(), pruned.as<uint32_t>(), n));
+      uint32_t lastp = 0, lastf = 0;
+      if (n) {
+        TCX_TRY(read_u32(c, pruned.as<uint32_t>() + n - 1, &lastp));
+        TCX_TRY(read_u32(c, flag.as<uint32_t>() + n - 1, &lastf));
+      }
+      n_out = n - (lastp + lastf);
+      pruned_before = pruned.as<uint32_t>();
+      TCX_HIP(hipStreamSynchronize(c->stream));
+    }
+    hipLaunchKernelGGL(k_orient_keys, dim3(grid_for(m)), dim3(TB), 0, c->stream,
+                       g->src.as<uint32_t>(), g->dst.as<uint32_t>(), m, deg.as<uint32_t>(), mode,
+                       pruned_before, n_out, keys.as<uint64_t>());
+    TCX_HIP(hipGetLastError());
+    TCX_TRY(csr_from_keys(c, g, keys, tmp, m, n_out, false));
+    g->orient_mode = mode;
+    return TC_OK;
+  }
+  // A-direction / k-core: peel the simple symmetric graph, then keep the
+  // symmetric entries that point up the peel order.
+  DevBuf keys, tmp, srp, scol, keep;
+  uint64_t* skeys = nullptr;
+  uint64_t m2 = 0;
+  TCX_TRY(sym_keys(c, g, keys, tmp, &skeys, &m2));
+  TCX_TRY(sym_csr(c, skeys, m2, n, srp, scol));
+  TCX_TRY(g->vkey.reserve(4 * ((size_t)n + 1)));
+  if (n > 0) {
+    if (mode == TC_ORIENT_ADIR)
+      TCX_TRY(adir_peel(c, n, m2, srp.as<uint32_t>(), scol.as<uint32_t>(), g->vkey.as<int32_t>()));
+    else
+      TCX_TRY(kcore_peel(c, n, srp.as<uint32_t>(), scol.as<uint32_t>(), g->vkey.as<int32_t>()));
+  }
+  TCX_TRY(keep.reserve(std::max<uint64_t>(m2, 1)));
+  hipLaunchKernelGGL(k_sym_keep, dim3(grid_for(m2)), dim3(TB), 0, c->stream, skeys, m2,
+                     g->vkey.as<int32_t>(), mode, keep.as<uint8_t>());
+  TCX_HIP(hipGetLastError());
+  uint64_t* dst_keys = skeys == keys.as<uint64_t>() ? tmp.as<uint64_t>() : keys.as<uint64_t>();
+  uint64_t mo = 0;
+  TCX_TRY(select_flagged_u64(c, skeys, keep.as<uint8_t>(), dst_keys, m2, &mo));
+  DevBuf* holder = dst_keys == keys.as<uint64_t>() ? &keys : &tmp;
+  DevBuf* spare = holder == &keys ? &tmp : &keys;
+  TCX_TRY(csr_from_keys(c, g, *holder, *spare, mo, n, true));
+  g->orient_mode = mode;
+  return TC_OK;
+}
+
+int tc_graph_info_get(const tc_graph* g, tc_graph_info* info) {
+  if (!g || !info) return fail(TC_ERR_ARG, "NULL argument");
+  info->n = g->n;
+  info->max_out_degree = g->max_out;
+  info->m_raw = g->m_raw;
+  info->m_oriented = g->orient_mode >= 0 ? g->m_or : 0;
+  info->orient_mode = g->orient_mode;
+  info->pad0 = 0;
+  info->sum_out_sq = g->orient_mode >= 0 ? g->sum_out_sq : 0;
+  info->sum_in_out = g->orient_mode >= 0 ? g->sum_in_out : 0;
+  double n1 = (double)g->n + 1, mo = (double)info->m_oriented;
+  info->bytes_alg = 4 * n1 + 8 * mo + 4.0 * (double)info->sum_out_sq + 4.0 * (double)info->sum_in_out;
+  info->bytes_min = 4 * n1 + 4 * mo;
+  return TC_OK;
+}
+
+int tc_graph_get_csr(const tc_graph* g, uint32_t* rowptr, uint32_t* col) {
+  if (!g || !g->ctx) return fail(TC_ERR_ARG, "NULL graph");
+  if (g->orient_mode < 0) return fail(TC_ERR_STATE, "graph not oriented");
+  TCX_HIP(hipSetDevice(g->ctx->device));
+  hipStream_t s = g->ctx->stream;
+  if (rowptr)
+    TCX_HIP(hipMemcpyAsync(rowptr, g->rowptr.p, 4 * ((size_t)g->n + 1), hipMemcpyDeviceToHost, s));
+  if (col && g->m_or)
+    TCX_HIP(hipMemcpyAsync(col, g->col.p, 4 * g->m_or, hipMemcpyDeviceToHost, s));
+  TCX_HIP(hipStreamSynchronize(s));
+  return TC_OK;
+}
+
+int tc_graph_get_edges(const tc_graph* g, uint32_t* src, uint32_t* dst) {
+  if (!g || !g->ctx) return fail(TC_ERR_ARG, "NULL graph");
+  TCX_HIP(hipSetDevice(g->ctx->device));
+  hipStream_t s = g->ctx->stream;
+  if (g->m_raw) {
+    if (src) TCX_HIP(hipMemcpyAsync(src, g->src.p, 4 * g->m_raw, hipMemcpyDeviceToHost, s));
+    if (dst) TCX_HIP(hipMemcpyAsync(dst, g->dst.p, 4 * g->m_raw, hipMemcpyDeviceToHost, s));
+  }
+  TCX_HIP(hipStreamSynchronize(s));
+  return TC_OK;
+}
+
+int tc_graph_get_vertex_key(const tc_graph* g, int32_t* key) {
+  if (!g || !g->ctx || !key) return fail(TC_ERR_ARG, "NULL argument");
+  if (g->orient_mode != TC_ORIENT_ADIR && g->orient_mode != TC_ORIENT_KCORE)
+    return fail(TC_ERR_STATE, "vertex keys exist after ADIR or KCORE orientation only");
+  TCX_HIP(hipSetDevice(g->ctx->device));
+  if (g->n)
+    TCX_HIP(hipMemcpyAsync(key, g->vkey.p, 4 * (size_t)g->n, hipMemcpyDeviceToHost, g->ctx->stream));
+  TCX_HIP(hipStreamSynchronize(g->ctx->stream));
+  return TC_OK;
+}
+
+int tc_graph_device_csr(const tc_graph* g, const uint32_t** rowptr, const uint32_t** col) {
+  if (!g || !rowptr || !col) return fail(TC_ERR_ARG, "NULL argument");
+  if (g->orient_mode < 0) return fail(TC_ERR_STATE, "graph not oriented");
+  *rowptr = g->rowptr.as<uint32_t>();
+  *col = g->col.as<uint32_t>();
+  return TC_OK;
+}
+
+}  // extern "C"

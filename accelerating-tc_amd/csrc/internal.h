@@ -1,0 +1,145 @@
+// Internal declarations shared by the engine's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/tc_amd.h"
+
+namespace tcx {
+
+// ---- error plumbing ------------------------------------------------------
+void set_error(const char* fmt, ...);
+int fail(int status, const char* fmt, ...);
+
+#define TCX_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t _e = (call);                                                  \
+    if (_e != hipSuccess)                                                    \
+      return ::tcx::fail(_e == hipErrorOutOfMemory ? TC_ERR_NOMEM : TC_ERR_HIP, \
+                         "%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), \
+                         __FILE__, __LINE__);                                \
+  } while (0)
+
+#define TCX_TRY(expr)          \
+  do {                         \
+    int _s = (expr);           \
+    if (_s != TC_OK) return _s; \
+  } while (0)
+
+// ---- device buffer ------------------------------------------------------
+// Owns one hipMalloc'd region; grows on demand (contents are not kept).
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  int reserve(size_t want);
+  void release();
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace tcx
+
+struct tc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  uint64_t total_mem = 0;
+  int num_cus = 256;
+  tcx::DevBuf temp;     // rocprim temp storage
+  tcx::DevBuf scratch;  // small per-call scratch (counters, bitmaps)
+  tcx::DevBuf accum;    // uint64 accumulators
+  tcx::DevBuf prim_out; // 64-byte result slot of the device-wide primitives
+  tcx::DevBuf bitmaps;  // Bisson per-workgroup bitmaps, all-zero between calls
+  uint64_t bitmap_words = 0;
+};
+
+namespace tcx {
+// Per-graph schedules, rebuilt when the oriented CSR changes.
+struct SchedCache {
+  uint64_t version = ~0ull;
+  uint32_t hu_nblocks = 0;
+  DevBuf hu_start;                 // int32[nblocks+1]
+  uint32_t auto_key = 0;
+  uint32_t auto_nblocks = 0;
+  DevBuf auto_start;               // int32[nblocks+1]
+  DevBuf hub_tasks;                // uint2 (hub vertex, first edge)
+  uint32_t n_hub_tasks = 0;
+  bool rows_ok = false;
+  DevBuf rows;                     // rows with d+ >= 2
+  uint64_t nrows = 0;
+  uint32_t fox_lanes = 0;
+  DevBuf fox_eids;                 // edge ids grouped by lanes-per-edge bin
+  std::vector<uint64_t> fox_off;   // bin offsets (bins+1)
+  float ms_prep = 0;
+};
+}  // namespace tcx
+
+struct tc_graph {
+  tc_ctx* ctx = nullptr;
+  uint32_t n = 0;
+  uint64_t m_raw = 0;
+  tcx::DevBuf src, dst;     // raw records (uint32 each)
+  // oriented CSR
+  int orient_mode = -1;
+  uint64_t m_or = 0;
+  uint32_t max_out = 0;
+  tcx::DevBuf rowptr;       // uint32[n+1]
+  tcx::DevBuf col;          // uint32[m_or]
+  tcx::DevBuf esrc;         // uint32[m_or]  source of each CSR edge
+  tcx::DevBuf vkey;         // int32[n] ADIR priority / KCORE core number
+  uint64_t sum_out_sq = 0, sum_in_out = 0;
+  // cached schedules (valid for the current CSR)
+  uint64_t sched_version = 0;
+  tcx::SchedCache sc;
+};
+
+namespace tcx {
+
+// ---- primitives (prims.hip) ---------------------------------------------
+int sort_u64(tc_ctx* c, uint64_t* keys, uint64_t* tmp, uint64_t n, int end_bit,
+             uint64_t** sorted);
+int unique_u64(tc_ctx* c, const uint64_t* in, uint64_t* out, uint64_t n,
+               uint64_t* n_out);
+int exclusive_scan_u32(tc_ctx* c, const uint32_t* in, uint32_t* out, uint64_t n);
+int exclusive_scan_u64(tc_ctx* c, const uint64_t* in, uint64_t* out, uint64_t n);
+int inclusive_scan_i64(tc_ctx* c, const int64_t* in, int64_t* out, uint64_t n);
+int max_u32(tc_ctx* c, const uint32_t* in, uint64_t n, uint32_t* result);
+int sum_u64(tc_ctx* c, const uint64_t* in, uint64_t n, uint64_t* result);
+int min_i32(tc_ctx* c, const int32_t* in, uint64_t n, int32_t* result);
+int select_flagged_u32(tc_ctx* c, const uint32_t* in, const uint8_t* flags,
+                       uint32_t* out, uint64_t n, uint64_t* n_out);
+int select_indices(tc_ctx* c, const uint8_t* flags, uint64_t n, uint32_t* out,
+                   uint64_t* n_out);
+int select_flagged_u64(tc_ctx* c, const uint64_t* in, const uint8_t* flags,
+                       uint64_t* out, uint64_t n, uint64_t* n_out);
+
+// ---- graph helpers (graph.hip) -------------------------------------------
+int build_csr_from_keys(tc_ctx* c, tc_graph* g, uint64_t* keys, uint64_t nkeys,
+                        int end_bit);
+int compute_csr_stats(tc_ctx* c, tc_graph* g);
+int build_sym(tc_ctx* c, tc_graph* g, DevBuf& rowptr, DevBuf& col, uint64_t* m2);
+
+// ---- A-order (aorder.cpp, host heap) -------------------------------------
+int aorder_perm_from_outdeg(const uint32_t* rtd, uint32_t n, uint32_t* perm);
+
+// ---- host parallel-for ----------------------------------------------------
+void parallel_for(uint64_t n, void (*fn)(uint64_t begin, uint64_t end, void* arg),
+                  void* arg, unsigned max_threads = 0);
+
+inline int bit_width_u32(uint32_t x) {
+  int b = 0;
+  while (x) { ++b; x >>= 1; }
+  return b;
+}
+
+constexpr int kWave = 64;
+
+}  // namespace tcx

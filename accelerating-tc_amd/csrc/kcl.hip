@@ -1,0 +1,333 @@
+// k-clique counting over an oriented (acyclic) CSR, k-clique/kcl.cu:16-215.
+//
+// The reference grows embeddings level by level and materialises every
+// (k-1)-level in HBM.  Here each root u owns the sub-graph induced by N+(u):
+//  * BITMAP path (d+(u) <= 512): the sub-graph's adjacency is built as a
+//    d x d bit matrix in LDS; a lane takes one second vertex and walks the
+//    remaining levels with AND + popcount on W-word bitsets.
+//  * MERGE path (any d+(u)): a wave walks the clique tree depth-first;
+//    candidate lists live in per-wave HBM scratch and each intersection is
+//    a wave-wide binary-search + ballot compaction.
+// Every k-clique has a unique source in the DAG and a unique chain order,
+// so it is counted exactly once on either path.
+#include <algorithm>
+#include <cstring>
+
+#include "internal.h"
+
+namespace tcx {
+namespace {
+
+constexpr int kMaxK = 12;
+
+__device__ inline uint64_t gtid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ inline uint64_t gstride() { return (uint64_t)gridDim.x * blockDim.x; }
+
+__device__ inline unsigned long long wave_sum(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__device__ inline int find_idx(const uint32_t* a, uint32_t n, uint32_t x) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    uint32_t v = a[mid];
+    if (v == x) return (int)mid;
+    if (v < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return -1;
+}
+
+__device__ inline bool contains(const uint32_t* a, uint32_t lo, uint32_t hi, uint32_t x) {
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    uint32_t v = a[mid];
+    if (v == x) return true;
+    if (v < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return false;
+}
+
+__global__ void k_deg_flags(const uint32_t* __restrict__ rp, uint32_t n, uint32_t lo, uint32_t hi,
+                            uint8_t* __restrict__ f) {
+  for (uint64_t u = gtid(); u < n; u += gstride()) {
+    uint32_t d = rp[u + 1] - rp[u];
+    f[u] = (d >= lo && d <= hi) ? 1 : 0;
+  }
+}
+
+// ---- BITMAP path ------------------------------------------------------------
+constexpr int kBitmapMaxK = 9;  // LDS DFS stack depth bound (k - 3 levels)
+
+template <int W>
+__global__ __launch_bounds__(64) void k_kcl_bitmap(const uint32_t* __restrict__ rp,
+                                                   const uint32_t* __restrict__ col,
+                                                   const uint32_t* __restrict__ roots,
+                                                   uint64_t nroots, uint32_t k,
+                                                   unsigned long long* total) {
+  constexpr int D = 64 * W;
+  constexpr int LV = kBitmapMaxK - 3;
+  __shared__ uint32_t s_list[D];
+  __shared__ uint64_t s_rows[D * W];
+  __shared__ uint64_t s_stack[64 * LV * 2 * W];  // per lane: (set, remaining) per level
+  const uint32_t lane = threadIdx.x;
+  unsigned long long cnt = 0;
+  uint64_t* my = s_stack + (uint64_t)lane * LV * 2 * W;
+  const int depth = (int)k - 2;  // vertices still to choose after u and the lane's first vertex
+  for (uint64_t ri = blockIdx.x; ri < nroots; ri += gridDim.x) {
+    const uint32_t u = roots[ri];
+    const uint32_t base = rp[u], d = rp[u + 1] - base;
+    for (uint32_t i = lane; i < d; i += 64) s_list[i] = col[base + i];
+    __syncthreads();
+    // row i = local indices of N+(c_i) & N+(u)
+    for (uint32_t i = lane; i < d; i += 64) {
+      const uint32_t ci = s_list[i];
+      uint64_t row[W];
+#pragma unroll
+      for (int w = 0; w < W; ++w) row[w] = 0;
+      for (uint32_t e = rp[ci]; e < rp[ci + 1]; ++e) {
+        int j = find_idx(s_list, d, col[e]);
+        if (j >= 0) row[j >> 6] |= 1ull << (j & 63);
+      }
+#pragma unroll
+      for (int w = 0; w < W; ++w) s_rows[i * W + w] = row[w];
+    }
+    __syncthreads();
+    for (uint32_t i = lane; i < d; i += 64) {
+      const uint64_t* r0 = s_rows + i * W;
+      if (depth == 1) {
+        for (int w = 0; w < W; ++w) cnt += __popcll(r0[w]);
+        continue;
+      }
+      // DFS over levels 0..depth-2; level L holds the candidates for the
+      // (L+3)-th clique vertex and the part of them not yet tried.
+      for (int w = 0; w < W; ++w) {
+        my[w] = r0[w];
+        my[W + w] = r0[w];
+      }
+      int L = 0;
+      while (L >= 0) {
+        uint64_t* set = my + (2 * L) * W;
+        uint64_t* rem = set + W;
+        int j = -1;
+        for (int w = 0; w < W; ++w) {
+          if (rem[w]) {
+            int b = __ffsll((long long)rem[w]) - 1;
+            rem[w] &= rem[w] - 1;
+            j = w * 64 + b;
+            break;
+          }
+        }
+        if (j < 0) {
+          --L;
+          continue;
+        }
+        const uint64_t* rj = s_rows + j * W;
+        if (L == depth - 2) {
+          for (int w = 0; w < W; ++w) cnt += __popcll(set[w] & rj[w]);
+        } else {
+          uint64_t* nset = set + 2 * W;
+          uint64_t any = 0;
+          for (int w = 0; w < W; ++w) {
+            uint64_t x = set[w] & rj[w];
+            nset[w] = x;
+            nset[W + w] = x;
+            any |= x;
+          }
+          if (any) ++L;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  cnt = wave_sum(cnt);
+  if (lane == 0 && cnt) atomicAdd(total, cnt);
+}
+
+// ---- MERGE path -------------------------------------------------------------
+// Level L holds the candidates after choosing u and L-1 more vertices
+// (level 1 = N+(u)).  Levels 2..k-2 are materialised in the wave's scratch
+// (slot L-2); at level k-2 every candidate x adds |S & N+(x)| directly.
+// The DFS control is uniform across the wave: every lane writes the same
+// values to s_size / s_idx, so no barrier is needed for them.
+__global__ __launch_bounds__(64) void k_kcl_merge(const uint32_t* __restrict__ rp,
+                                                  const uint32_t* __restrict__ col,
+                                                  const uint32_t* __restrict__ roots,
+                                                  uint64_t nroots, uint32_t k, uint32_t max_d,
+                                                  uint32_t slots, uint32_t* __restrict__ scratch,
+                                                  unsigned long long* total) {
+  __shared__ uint32_t s_size[kMaxK];
+  __shared__ uint32_t s_idx[kMaxK];
+  const uint32_t lane = threadIdx.x;
+  unsigned long long cnt = 0;
+  uint32_t* mine = scratch + (uint64_t)blockIdx.x * slots * max_d;
+  const int last = (int)k - 2;
+  for (uint64_t ri = blockIdx.x; ri < nroots; ri += gridDim.x) {
+    const uint32_t u = roots[ri];
+    const uint32_t ub = rp[u], d = rp[u + 1] - ub;
+    s_size[1] = d;
+    s_idx[1] = 0;
+    int L = 1;
+    while (L >= 1) {
+      const uint32_t* S = L == 1 ? col + ub : mine + (uint64_t)(L - 2) * max_d;
+      const uint32_t sz = s_size[L];
+      if (L == last) {
+        for (uint32_t ix = 0; ix < sz; ++ix) {
+          const uint32_t x = S[ix];
+          const uint32_t xb = rp[x], xe = rp[x + 1];
+          for (uint32_t t = lane; t < sz; t += 64) cnt += contains(col, xb, xe, S[t]);
+        }
+        --L;
+        continue;
+      }
+      const uint32_t ix = s_idx[L];
+      if (ix >= sz) {
+        --L;
+        continue;
+      }
+      s_idx[L] = ix + 1;
+      const uint32_t x = S[ix];
+      const uint32_t xb = rp[x], xe = rp[x + 1];
+      uint32_t* out = mine + (uint64_t)(L - 1) * max_d;
+      uint32_t nsz = 0;
+      for (uint32_t t0 = 0; t0 < sz; t0 += 64) {
+        const uint32_t t = t0 + lane;
+        const bool keep = t < sz && contains(col, xb, xe, S[t]);
+        const uint64_t bal = __ballot(keep);
+        const uint32_t pos = __popcll(bal & ((1ull << lane) - 1));
+        if (keep) out[nsz + pos] = S[t];
+        nsz += __popcll(bal);
+      }
+      __syncthreads();  // compacted list visible to every lane
+      if (nsz > 0) {
+        s_size[L + 1] = nsz;
+        s_idx[L + 1] = 0;
+        ++L;
+      }
+    }
+    __syncthreads();
+  }
+  cnt = wave_sum(cnt);
+  if (lane == 0 && cnt) atomicAdd(total, cnt);
+}
+
+}  // namespace
+}  // namespace tcx
+
+using namespace tcx;
+
+static unsigned grid_for(uint64_t n, int tb = 256) {
+  uint64_t b = (n + tb - 1) / tb;
+  if (b < 1) b = 1;
+  if (b > (1u << 20)) b = 1u << 20;
+  return (unsigned)b;
+}
+
+extern "C" {
+
+int kcl_count(tc_ctx* c, tc_graph* g, uint32_t k, int variant, uint64_t* count,
+              tc_timing* timing) {
+  if (!c || !g || !count) return fail(TC_ERR_ARG, "NULL argument");
+  if (g->ctx != c) return fail(TC_ERR_ARG, "graph belongs to another context");
+  if (g->orient_mode < 0) return fail(TC_ERR_STATE, "graph must be oriented before counting");
+  if (k == 0 || k > (uint32_t)kMaxK) return fail(TC_ERR_RANGE, "k must be in 1..%d", kMaxK);
+  if (variant < KCL_VARIANT_AUTO || variant > KCL_VARIANT_MERGE)
+    return fail(TC_ERR_ARG, "unknown k-clique variant %d", variant);
+  TCX_HIP(hipSetDevice(c->device));
+  tc_timing tm;
+  memset(&tm, 0, sizeof(tm));
+  *count = 0;
+  if (k == 1) { *count = g->n; if (timing) *timing = tm; return TC_OK; }
+  if (k == 2) { *count = g->m_or; if (timing) *timing = tm; return TC_OK; }
+  if (g->m_or == 0) { if (timing) *timing = tm; return TC_OK; }
+  const uint32_t* rp = g->rowptr.as<uint32_t>();
+  const uint32_t* col = g->col.as<uint32_t>();
+  // root buckets: [lo, hi] of d+ and the path that takes them
+  struct Bucket { uint32_t lo, hi; int w; };  // w = bitmap words, 0 = merge
+  std::vector<Bucket> buckets;
+  const uint32_t need = k - 1;  // a root needs k-1 out-neighbours
+  if (variant == KCL_VARIANT_MERGE || k > (uint32_t)kBitmapMaxK) {
+    buckets.push_back({need, 0xFFFFFFFFu, 0});
+  } else {
+    buckets.push_back({need, 64, 1});
+    buckets.push_back({65, 128, 2});
+    buckets.push_back({129, 256, 4});
+    buckets.push_back({257, 512, 8});
+    buckets.push_back({513, 0xFFFFFFFFu, 0});
+  }
+  DevBuf flags, roots, scratch;
+  TCX_TRY(flags.reserve((size_t)g->n + 1));
+  TCX_TRY(roots.reserve(4 * ((size_t)g->n + 1)));
+  TCX_TRY(c->accum.reserve(64));
+  unsigned long long* acc = c->accum.as<unsigned long long>();
+  TCX_HIP(hipMemsetAsync(acc, 0, 8, c->stream));
+  float total_ms = 0;
+  for (const Bucket& b : buckets) {
+    uint32_t lo = std::max(b.lo, need);
+    if (lo > b.hi) continue;
+    hipLaunchKernelGGL(k_deg_flags, dim3(grid_for(g->n)), dim3(256), 0, c->stream, rp, g->n, lo,
+                       b.hi, flags.as<uint8_t>());
+    TCX_HIP(hipGetLastError());
+    uint64_t nr = 0;
+    TCX_TRY(select_indices(c, flags.as<uint8_t>(), g->n, roots.as<uint32_t>(), &nr));
+    if (nr == 0) continue;
+    TCX_HIP(hipEventRecord(c->ev0, c->stream));
+    if (b.w == 0) {
+      uint32_t G = (uint32_t)std::min<uint64_t>(nr, (uint64_t)c->num_cus * 8);
+      uint32_t maxd = std::max<uint32_t>(g->max_out, 1);
+      uint32_t slots = k > 3 ? k - 3 : 1;
+      TCX_TRY(scratch.reserve(4ull * G * slots * maxd));
+      hipLaunchKernelGGL(k_kcl_merge, dim3(G), dim3(64), 0, c->stream, rp, col,
+                         roots.as<uint32_t>(), nr, k, maxd, slots, scratch.as<uint32_t>(), acc);
+    } else {
+      uint32_t G = (uint32_t)std::min<uint64_t>(nr, (uint64_t)c->num_cus * 16);
+      switch (b.w) {
+        case 1: hipLaunchKernelGGL(k_kcl_bitmap<1>, dim3(G), dim3(64), 0, c->stream, rp, col, roots.as<uint32_t>(), nr, k, acc); break;
+        case 2: hipLaunchKernelGGL(k_kcl_bitmap<2>, dim3(G), dim3(64), 0, c->stream, rp, col, roots.as<uint32_t>(), nr, k, acc); break;
+        case 4: hipLaunchKernelGGL(k_kcl_bitmap<4>, dim3(G), dim3(64), 0, c->stream, rp, col, roots.as<uint32_t>(), nr, k, acc); break;
+        default: hipLaunchKernelGGL(k_kcl_bitmap<8>, dim3(G), dim3(64), 0, c->stream, rp, col, roots.as<uint32_t>(), nr, k, acc); break;
+      }
+    }
+    TCX_HIP(hipGetLastError());
+    TCX_HIP(hipEventRecord(c->ev1, c->stream));
+    TCX_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    TCX_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    total_ms += ms;
+    if (ms > tm.ms_dominant) {
+      tm.ms_dominant = ms;
+      tm.dominant_kernel = (uint32_t)b.w;
+    }
+    tm.launches++;
+  }
+  unsigned long long h = 0;
+  TCX_HIP(hipMemcpyAsync(&h, acc, 8, hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  *count = h;
+  tm.ms_count = total_ms;
+  if (timing) *timing = tm;
+  return TC_OK;
+}
+
+int kcl_gpu_solver(const char* path, uint32_t k, uint64_t* total, size_t n_chunk) {
+  (void)n_chunk;
+  if (!path || !total) return fail(TC_ERR_ARG, "NULL argument");
+  uint64_t m = 0;
+  TCX_TRY(tc_file_edge_count(path, &m));
+  std::vector<uint32_t> s(m), d(m);
+  TCX_TRY(tc_read_edges(path, s.data(), d.data(), m, &m));
+  tc_ctx* c = nullptr;
+  TCX_TRY(tc_ctx_create(0, &c));
+  tc_graph* g = nullptr;
+  int st = tc_graph_create(c, s.data(), d.data(), m, 0, &g);
+  if (st == TC_OK) st = tc_graph_orient(c, g, TC_ORIENT_KCORE, 0);
+  if (st == TC_OK) st = kcl_count(c, g, k, KCL_VARIANT_AUTO, total, nullptr);
+  tc_graph_destroy(g);
+  tc_ctx_destroy(c);
+  return st;
+}
+
+}  // extern "C"

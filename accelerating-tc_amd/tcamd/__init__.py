@@ -1,0 +1,462 @@
+"""Python binding of libtcamd.so, the MI355X triangle-counting / k-clique engine.
+
+The engine's boundary is the C ABI in include/tc_amd.h; this module binds it
+with ctypes and mirrors the reference's pipeline one step per call:
+
+    edges = read_edges("g.bin")                    # tools/list2bin format
+    with Context() as ctx:
+        g = ctx.graph(*edges)                      # upload raw records to HBM
+        g.orient(ORIENT_DEGREE)                    # Hu/preTC.cu orientation + CSR
+        count, timing = g.count(KERNEL_HU)         # Hu/tc.cu triangleCountKernel
+
+Every call goes to the HIP library; there is no CPU fallback.  A missing or
+unbuilt library raises at import time.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libtcamd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"libtcamd.so not found at {LIB_PATH}; build it with "
+        "`make -C accelerating-tc_amd lib` (or __graft_entry__.build())")
+
+_lib = C.CDLL(LIB_PATH)
+
+OK = 0
+ERR_ARG, ERR_IO, ERR_FORMAT, ERR_RANGE, ERR_NOMEM, ERR_HIP, ERR_UNSUPPORTED, ERR_STATE = (
+    -1, -2, -3, -4, -5, -6, -7, -8)
+
+ORIENT_ID, ORIENT_DEGREE, ORIENT_ADIR, ORIENT_KCORE, ORIENT_DEGREE_TRICORE = 0, 1, 2, 3, 4
+FLAG_PRUNE_DEG1 = 1
+KERNEL_HU, KERNEL_BISSON, KERNEL_FOX, KERNEL_TRICORE, KERNEL_AUTO = 0, 1, 2, 3, 4
+KERNEL_NAMES = {"hu": KERNEL_HU, "bisson": KERNEL_BISSON, "fox": KERNEL_FOX,
+                "tricore": KERNEL_TRICORE, "auto": KERNEL_AUTO}
+KCL_AUTO, KCL_BITMAP, KCL_MERGE = 0, 1, 2
+
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+
+class GraphInfo(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("max_out_degree", C.c_uint32), ("m_raw", C.c_uint64),
+                ("m_oriented", C.c_uint64), ("orient_mode", C.c_int32), ("pad0", C.c_int32),
+                ("sum_out_sq", C.c_uint64), ("sum_in_out", C.c_uint64),
+                ("bytes_alg", C.c_double), ("bytes_min", C.c_double)]
+
+
+class Sched(C.Structure):
+    _fields_ = [("kernel", C.c_int32), ("block_size", C.c_uint32), ("grid_size", C.c_uint32),
+                ("lds_slice", C.c_uint32), ("bitmap_min_deg", C.c_uint32),
+                ("fox_max_lanes", C.c_uint32), ("tricore_parts", C.c_uint32),
+                ("auto_bins", C.c_uint32), ("auto_thresholds", C.c_uint32 * 4),
+                ("reserved", C.c_uint32 * 4)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("ms_count", C.c_float), ("ms_prep", C.c_float), ("launches", C.c_uint32),
+                ("dominant_kernel", C.c_uint32), ("ms_dominant", C.c_float),
+                ("reserved", C.c_float)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+def _sig(name, res, args):
+    f = getattr(_lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_sig("tc_status_str", C.c_char_p, [C.c_int])
+_sig("tc_last_error", C.c_char_p, [])
+_sig("tc_ctx_create", C.c_int, [C.c_int, C.POINTER(_vp)])
+_sig("tc_ctx_destroy", C.c_int, [_vp])
+_sig("tc_ctx_device_memory", C.c_int, [_vp, C.POINTER(C.c_uint64)])
+_sig("tc_file_edge_count", C.c_int, [C.c_char_p, C.POINTER(C.c_uint64)])
+_sig("tc_read_edges", C.c_int, [C.c_char_p, _u32p, _u32p, C.c_uint64, C.POINTER(C.c_uint64)])
+_sig("tc_write_edges", C.c_int, [C.c_char_p, _u32p, _u32p, C.c_uint64])
+_sig("tc_list2bin", C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_uint64)])
+_sig("tc_bin2list", C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_uint64)])
+_sig("tc_gen_er", C.c_int, [C.c_uint32, C.c_uint64, C.c_uint64, _u32p, _u32p])
+_sig("tc_gen_rmat", C.c_int, [C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_double,
+                              C.c_uint64, _u32p, _u32p])
+_sig("tc_gen_ba", C.c_int, [C.c_uint32, C.c_uint32, C.c_uint64, _u32p, _u32p,
+                            C.POINTER(C.c_uint64)])
+_sig("tc_gen_rgg", C.c_int, [C.c_uint32, C.c_double, C.c_uint64, _vp, _vp,
+                             C.POINTER(C.c_uint64)])
+_sig("tc_graph_create", C.c_int, [_vp, _vp, _vp, C.c_uint64, C.c_uint32, C.POINTER(_vp)])
+_sig("tc_graph_destroy", C.c_int, [_vp])
+_sig("tc_graph_simplify", C.c_int, [_vp, _vp])
+_sig("tc_graph_relabel_aorder", C.c_int, [_vp, _vp, _vp])
+_sig("tc_graph_orient", C.c_int, [_vp, _vp, C.c_int, C.c_uint32])
+_sig("tc_graph_info_get", C.c_int, [_vp, C.POINTER(GraphInfo)])
+_sig("tc_graph_get_csr", C.c_int, [_vp, _vp, _vp])
+_sig("tc_graph_get_edges", C.c_int, [_vp, _vp, _vp])
+_sig("tc_graph_get_vertex_key", C.c_int, [_vp, _i32p])
+_sig("tc_hu_workload", C.c_int, [_vp, _vp, _i64p])
+_sig("tc_hu_block_starts", C.c_int, [_vp, _vp, C.c_uint32, _i32p])
+_sig("tc_fox_workload", C.c_int, [_vp, _vp, _i32p])
+_sig("tc_fox_bin_sizes", C.c_int, [_vp, _vp, C.c_uint32, _u64p])
+_sig("tc_count", C.c_int, [_vp, _vp, C.POINTER(Sched), C.POINTER(C.c_uint64),
+                           C.POINTER(Timing)])
+_sig("tc_count_per_vertex", C.c_int, [_vp, _vp, _u64p])
+_sig("kcl_count", C.c_int, [_vp, _vp, C.c_uint32, C.c_int, C.POINTER(C.c_uint64),
+                            C.POINTER(Timing)])
+_sig("kcl_gpu_solver", C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_uint64), C.c_size_t])
+_sig("tc_tricore_init_gpu", C.c_uint64, [])
+_sig("tc_tricore_cal_degree", None, [_u64p, C.c_uint64, _u32p, C.c_uint32])
+_sig("tc_tricore_redirect_edges", None, [_u64p, C.c_uint64, _u32p, C.c_uint32])
+_sig("tc_tricore_cal_part_num", C.c_uint32, [_u64p, C.c_uint64, C.c_uint32])
+_sig("tc_tricore_split", None, [_u64p, C.c_uint64, C.c_uint64, C.POINTER(C.POINTER(C.c_uint32)),
+                                _u32p, _u64p])
+_sig("tc_tricore_tricount_gpu", C.c_uint64, [C.c_uint64, _vp, _u64p, _u32p,
+                                             C.POINTER(C.POINTER(C.c_uint32))])
+
+EXPORTED = [
+    "tc_status_str", "tc_last_error", "tc_ctx_create", "tc_ctx_destroy", "tc_ctx_device_memory",
+    "tc_file_edge_count", "tc_read_edges", "tc_write_edges", "tc_list2bin", "tc_bin2list",
+    "tc_gen_er", "tc_gen_rmat", "tc_gen_ba", "tc_gen_rgg", "tc_graph_create", "tc_graph_destroy",
+    "tc_graph_simplify", "tc_graph_relabel_aorder", "tc_graph_orient", "tc_graph_info_get",
+    "tc_graph_get_csr", "tc_graph_get_edges", "tc_graph_get_vertex_key", "tc_graph_device_csr",
+    "tc_hu_workload", "tc_hu_block_starts", "tc_fox_workload", "tc_fox_bin_sizes", "tc_count",
+    "tc_count_per_vertex", "kcl_count", "kcl_gpu_solver", "tc_tricore_init_gpu",
+    "tc_tricore_cal_degree", "tc_tricore_redirect_edges", "tc_tricore_cal_part_num",
+    "tc_tricore_split", "tc_tricore_tricount_gpu",
+]
+
+
+class TCError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        msg = _lib.tc_last_error().decode(errors="replace")
+        self.status = status
+        super().__init__(f"{where}: {_lib.tc_status_str(status).decode()} ({status}): {msg}")
+
+
+def _check(status: int, where: str):
+    if status != OK:
+        raise TCError(status, where)
+
+
+def last_error() -> str:
+    return _lib.tc_last_error().decode(errors="replace")
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+# ---- files and generators ----------------------------------------------------
+
+def file_edge_count(path: str) -> int:
+    m = C.c_uint64()
+    _check(_lib.tc_file_edge_count(path.encode(), C.byref(m)), "tc_file_edge_count")
+    return m.value
+
+
+def read_edges(path: str):
+    m = file_edge_count(path)
+    src = np.empty(max(m, 1), np.uint32)
+    dst = np.empty(max(m, 1), np.uint32)
+    got = C.c_uint64()
+    _check(_lib.tc_read_edges(path.encode(), src, dst, m, C.byref(got)), "tc_read_edges")
+    return src[:m], dst[:m]
+
+
+def write_edges(path: str, src, dst):
+    src, dst = _u32(src), _u32(dst)
+    if len(src) == 0:
+        src = np.zeros(1, np.uint32)
+        dst = np.zeros(1, np.uint32)
+        m = 0
+    else:
+        m = len(src)
+    _check(_lib.tc_write_edges(path.encode(), src, dst, m), "tc_write_edges")
+
+
+def list2bin(text_path: str, bin_path: str) -> int:
+    m = C.c_uint64()
+    _check(_lib.tc_list2bin(text_path.encode(), bin_path.encode(), C.byref(m)), "tc_list2bin")
+    return m.value
+
+
+def bin2list(bin_path: str, text_path: str) -> int:
+    m = C.c_uint64()
+    _check(_lib.tc_bin2list(bin_path.encode(), text_path.encode(), C.byref(m)), "tc_bin2list")
+    return m.value
+
+
+def gen_er(n: int, m: int, seed: int):
+    s = np.empty(max(m, 1), np.uint32)
+    d = np.empty(max(m, 1), np.uint32)
+    _check(_lib.tc_gen_er(n, m, seed, s, d), "tc_gen_er")
+    return s[:m], d[:m]
+
+
+def gen_rmat(scale: int, nedges: int, a=0.57, b=0.19, c=0.19, seed: int = 0):
+    s = np.empty(max(nedges, 1), np.uint32)
+    d = np.empty(max(nedges, 1), np.uint32)
+    _check(_lib.tc_gen_rmat(scale, nedges, a, b, c, seed, s, d), "tc_gen_rmat")
+    return s[:nedges], d[:nedges]
+
+
+def gen_ba(n: int, k: int, seed: int):
+    m = (n - k) * k
+    s = np.empty(max(m, 1), np.uint32)
+    d = np.empty(max(m, 1), np.uint32)
+    got = C.c_uint64()
+    _check(_lib.tc_gen_ba(n, k, seed, s, d, C.byref(got)), "tc_gen_ba")
+    return s[:got.value], d[:got.value]
+
+
+def gen_rgg(n: int, r: float, seed: int):
+    m = C.c_uint64()
+    _check(_lib.tc_gen_rgg(n, r, seed, None, None, C.byref(m)), "tc_gen_rgg")
+    s = np.empty(max(m.value, 1), np.uint32)
+    d = np.empty(max(m.value, 1), np.uint32)
+    _check(_lib.tc_gen_rgg(n, r, seed, s.ctypes.data, d.ctypes.data, C.byref(m)), "tc_gen_rgg")
+    return s[:m.value], d[:m.value]
+
+
+def rgg_radius(n: int, avg_degree: float) -> float:
+    """Radius giving the target average degree in the unit square."""
+    return float(np.sqrt(avg_degree / (np.pi * (n - 1))))
+
+
+# ---- context / graph ---------------------------------------------------------
+
+@dataclass
+class CountResult:
+    count: int
+    timing: dict
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = _vp()
+        _check(_lib.tc_ctx_create(device, C.byref(h)), "tc_ctx_create")
+        self._h = h
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_memory(self) -> int:
+        b = C.c_uint64()
+        _check(_lib.tc_ctx_device_memory(self._h, C.byref(b)), "tc_ctx_device_memory")
+        return b.value
+
+    def close(self):
+        if self._h:
+            _lib.tc_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def graph(self, src, dst, n: int = 0) -> "Graph":
+        return Graph(self, src, dst, n)
+
+
+class Graph:
+    def __init__(self, ctx: Context, src, dst, n: int = 0):
+        src, dst = _u32(src), _u32(dst)
+        if len(src) != len(dst):
+            raise ValueError("src and dst differ in length")
+        self.ctx = ctx
+        h = _vp()
+        m = len(src)
+        _check(_lib.tc_graph_create(ctx.handle, src.ctypes.data if m else None,
+                                    dst.ctypes.data if m else None, m, n, C.byref(h)),
+               "tc_graph_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.tc_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- preprocessing --
+    def simplify(self):
+        _check(_lib.tc_graph_simplify(self.ctx.handle, self._h), "tc_graph_simplify")
+        return self
+
+    def relabel_aorder(self):
+        n = self.info().n
+        perm = np.empty(max(n, 1), np.uint32)
+        _check(_lib.tc_graph_relabel_aorder(self.ctx.handle, self._h, perm.ctypes.data),
+               "tc_graph_relabel_aorder")
+        return perm[:n]
+
+    def orient(self, mode: int, prune_deg1: bool = False):
+        _check(_lib.tc_graph_orient(self.ctx.handle, self._h, mode,
+                                    FLAG_PRUNE_DEG1 if prune_deg1 else 0), "tc_graph_orient")
+        return self
+
+    def info(self) -> GraphInfo:
+        i = GraphInfo()
+        _check(_lib.tc_graph_info_get(self._h, C.byref(i)), "tc_graph_info_get")
+        return i
+
+    def csr(self):
+        i = self.info()
+        rp = np.empty(i.n + 1, np.uint32)
+        col = np.empty(max(i.m_oriented, 1), np.uint32)
+        _check(_lib.tc_graph_get_csr(self._h, rp.ctypes.data, col.ctypes.data), "tc_graph_get_csr")
+        return rp, col[:i.m_oriented]
+
+    def edges(self):
+        i = self.info()
+        s = np.empty(max(i.m_raw, 1), np.uint32)
+        d = np.empty(max(i.m_raw, 1), np.uint32)
+        _check(_lib.tc_graph_get_edges(self._h, s.ctypes.data, d.ctypes.data), "tc_graph_get_edges")
+        return s[:i.m_raw], d[:i.m_raw]
+
+    def vertex_key(self):
+        n = self.info().n
+        k = np.empty(max(n, 1), np.int32)
+        _check(_lib.tc_graph_get_vertex_key(self._h, k), "tc_graph_get_vertex_key")
+        return k[:n]
+
+    # -- work models --
+    def hu_workload(self):
+        n = self.info().n
+        w = np.empty(max(n, 1), np.int64)
+        _check(_lib.tc_hu_workload(self.ctx.handle, self._h, w), "tc_hu_workload")
+        return w[:n]
+
+    def hu_block_starts(self, nblocks: int):
+        s = np.empty(nblocks + 1, np.int32)
+        _check(_lib.tc_hu_block_starts(self.ctx.handle, self._h, nblocks, s), "tc_hu_block_starts")
+        return s
+
+    def fox_workload(self):
+        m = self.info().m_oriented
+        w = np.empty(max(m, 1), np.int32)
+        _check(_lib.tc_fox_workload(self.ctx.handle, self._h, w), "tc_fox_workload")
+        return w[:m]
+
+    def fox_bin_sizes(self, max_lanes: int = 32):
+        nb = int(np.log2(max_lanes)) + 1
+        out = np.zeros(nb, np.uint64)
+        _check(_lib.tc_fox_bin_sizes(self.ctx.handle, self._h, max_lanes, out), "tc_fox_bin_sizes")
+        return out
+
+    # -- counting --
+    def count(self, kernel=KERNEL_AUTO, **sched) -> CountResult:
+        if isinstance(kernel, str):
+            kernel = KERNEL_NAMES[kernel]
+        s = Sched()
+        s.kernel = kernel
+        for k, v in sched.items():
+            if k == "auto_thresholds":
+                for i, x in enumerate(v):
+                    s.auto_thresholds[i] = x
+            else:
+                setattr(s, k, v)
+        out = C.c_uint64()
+        t = Timing()
+        _check(_lib.tc_count(self.ctx.handle, self._h, C.byref(s), C.byref(out), C.byref(t)),
+               "tc_count")
+        return CountResult(out.value, t.as_dict())
+
+    def count_per_vertex(self):
+        n = self.info().n
+        tri = np.zeros(max(n, 1), np.uint64)
+        _check(_lib.tc_count_per_vertex(self.ctx.handle, self._h, tri), "tc_count_per_vertex")
+        return tri[:n]
+
+    def kclique(self, k: int, variant: int = KCL_AUTO) -> CountResult:
+        out = C.c_uint64()
+        t = Timing()
+        _check(_lib.kcl_count(self.ctx.handle, self._h, k, variant, C.byref(out), C.byref(t)),
+               "kcl_count")
+        return CountResult(out.value, t.as_dict())
+
+
+def kcl_gpu_solver(path: str, k: int, n_chunk: int = 1) -> int:
+    out = C.c_uint64()
+    _check(_lib.kcl_gpu_solver(path.encode(), k, C.byref(out), n_chunk), "kcl_gpu_solver")
+    return out.value
+
+
+# ---- TriCore function API (tricore/include/tricount.h:7-17) ------------------
+
+class tricore:
+    @staticmethod
+    def init_gpu() -> int:
+        return int(_lib.tc_tricore_init_gpu())
+
+    @staticmethod
+    def pack(src, dst):
+        """uint64 view of the record: FIRST = high word (tricore/include/util.h:14-16)."""
+        return (_u32(src).astype(np.uint64) << np.uint64(32)) | _u32(dst).astype(np.uint64)
+
+    @staticmethod
+    def cal_degree(edges, node_count: int):
+        edges = np.ascontiguousarray(edges, np.uint64)
+        deg = np.zeros(max(node_count, 1), np.uint32)
+        _lib.tc_tricore_cal_degree(edges, len(edges), deg, node_count)
+        return deg[:node_count]
+
+    @staticmethod
+    def redirect_edges(edges, deg):
+        deg = _u32(deg)
+        _lib.tc_tricore_redirect_edges(edges, len(edges), deg, len(deg))
+        return edges
+
+    @staticmethod
+    def cal_part_num(edges, node_count: int) -> int:
+        return int(_lib.tc_tricore_cal_part_num(edges, len(edges), node_count))
+
+    @staticmethod
+    def count(edges, node_count: int, part_num: int) -> int:
+        """split + tricount_gpu as tricore/src/main.cpp:75-95 chains them."""
+        edges = np.ascontiguousarray(edges, np.uint64).copy()
+        split = np.array([node_count if i == part_num else (node_count // part_num) * i
+                          for i in range(part_num + 1)], np.uint32)
+        node_index = (C.POINTER(C.c_uint32) * part_num)()
+        adj_count = np.zeros(part_num + 1, np.uint64)
+        _lib.tc_tricore_split(edges, len(edges), part_num, node_index, split, adj_count)
+        if any(not bool(node_index[i]) for i in range(part_num)):
+            raise RuntimeError("tricore split failed: " + last_error())
+        total = _lib.tc_tricore_tricount_gpu(part_num, edges.ctypes.data, adj_count, split,
+                                             node_index)
+        libc = C.CDLL(None)
+        libc.free.argtypes = [C.c_void_p]
+        for i in range(part_num):
+            libc.free(C.cast(node_index[i], C.c_void_p))
+        return int(total)

@@ -1,0 +1,221 @@
+/*
+ * tc_amd.h -- C ABI of the MI355X triangle-counting / k-clique engine.
+ *
+ * The reference (pkumod/accelerating-TC) has no library API: each method is a
+ * stand-alone program whose boundary is argv + an edge file + one stdout
+ * line.  These entry points are the function-level form of that boundary.
+ * Each declaration cites the reference code whose role it takes over.
+ *
+ * Conventions
+ *  - Every int-returning function returns TC_OK (0) or a negative TC_ERR_*;
+ *    nothing calls exit().  tc_last_error() gives the message of the last
+ *    failure on the calling thread.
+ *  - Host pointers passed in are borrowed read-only unless documented as
+ *    outputs; outputs are caller-allocated.
+ *  - A tc_ctx owns one HIP stream and is not thread-safe; separate contexts
+ *    are independent.  A tc_graph / tc_batch is bound to the context that
+ *    created it and owns its device memory until destroyed.
+ *  - Vertex ids are uint32; the edge-file record is {int32 src, int32 dst},
+ *    8 bytes, no header (tools/list2bin.cpp:3-6,16-20).
+ */
+#ifndef TC_AMD_H
+#define TC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status ------------------------------------------------------------ */
+#define TC_OK 0
+#define TC_ERR_ARG (-1)         /* bad argument / NULL pointer              */
+#define TC_ERR_IO (-2)          /* open/read/write failure                  */
+#define TC_ERR_FORMAT (-3)      /* file size not a multiple of the record   */
+#define TC_ERR_RANGE (-4)       /* id / size beyond what the engine handles */
+#define TC_ERR_NOMEM (-5)       /* device or host allocation failed         */
+#define TC_ERR_HIP (-6)         /* HIP runtime error                        */
+#define TC_ERR_UNSUPPORTED (-7) /* input outside an algorithm's domain      */
+#define TC_ERR_STATE (-8)       /* call order (e.g. count before orient)    */
+
+/* ---- enums ------------------------------------------------------------- */
+/* Edge orientation (DAG construction). */
+#define TC_ORIENT_ID 0          /* src = larger id: Fox/preTC.cu:92-101, Bisson/preTC.cu:136-144 */
+#define TC_ORIENT_DEGREE 1      /* Hu/preTC.cu:125-133 (ties from the larger id) */
+#define TC_ORIENT_ADIR 2        /* A-direction peel: edge_directing/A-direction-seri.cpp:35-153,252-273 */
+#define TC_ORIENT_KCORE 3       /* edge_directing/kcore_direction.cpp:91-146 */
+#define TC_ORIENT_DEGREE_TRICORE 4 /* tricore/src/tricount_gpu.cu:69-80 (ties from the smaller id) */
+
+/* Orientation flags. */
+#define TC_FLAG_PRUNE_DEG1 1u   /* Hu/preTC.cu:135-169: drop degree<=1 vertices, compact ids */
+
+/* Counting method. */
+#define TC_KERNEL_HU 0          /* Hu/tc.cu:69-201      */
+#define TC_KERNEL_BISSON 1      /* Bisson/tc.cu:18-104  */
+#define TC_KERNEL_FOX 2         /* Fox/tc.cu:64-125     */
+#define TC_KERNEL_TRICORE 3     /* tricore/src/tricount_gpu.cu:215-290 */
+#define TC_KERNEL_AUTO 4        /* degree-binned engine (fixed or learned bins) */
+
+/* k-clique variant. */
+#define KCL_VARIANT_AUTO 0      /* per-root choice (learned predictor when loaded) */
+#define KCL_VARIANT_BITMAP 1    /* LDS bitmap sub-graph path                       */
+#define KCL_VARIANT_MERGE 2     /* sorted-list intersection path                   */
+
+typedef struct tc_ctx tc_ctx;
+typedef struct tc_graph tc_graph;
+typedef struct tc_batch tc_batch;
+
+typedef struct tc_graph_info {
+  uint32_t n;               /* vertex count (max id + 1, or after pruning)      */
+  uint32_t max_out_degree;  /* max d+ of the oriented CSR                       */
+  uint64_t m_raw;           /* records currently held as raw edges              */
+  uint64_t m_oriented;      /* oriented unique edges (= undirected simple edges) */
+  int32_t orient_mode;      /* TC_ORIENT_*, -1 before tc_graph_orient           */
+  int32_t pad0;
+  uint64_t sum_out_sq;      /* sum_u d+(u)^2                                    */
+  uint64_t sum_in_out;      /* sum_v d-(v) d+(v)  (= wedge count)               */
+  double bytes_alg;         /* B_alg = 4(n+1) + 8m+ + 4 sum d+^2 + 4 sum d- d+  */
+  double bytes_min;         /* 4(n+1) + 4m+                                     */
+} tc_graph_info;
+
+/* Per-call schedule.  Zero fields take the defaults listed. */
+typedef struct tc_sched {
+  int32_t kernel;           /* TC_KERNEL_*                                       */
+  uint32_t block_size;      /* threads per workgroup (default 256)               */
+  uint32_t grid_size;       /* Hu: work-balanced blocks (default 8192);
+                               Bisson/TriCore: resident workgroups               */
+  uint32_t lds_slice;       /* Hu/AUTO: adjacency entries staged in LDS (2048)  */
+  uint32_t bitmap_min_deg;  /* Bisson: d+ above which the bitmap path runs (64)  */
+  uint32_t fox_max_lanes;   /* Fox: widest lanes-per-edge bin, 32 or 64 (64)     */
+  uint32_t tricore_parts;   /* TriCore: node-range partitions (1)                */
+  uint32_t auto_bins;       /* AUTO: 0 = fixed bins, 1 = learned (model loaded)  */
+  uint32_t auto_thresholds[4]; /* AUTO: d+ bin edges (default 8, 64, 512, 4096) */
+  uint32_t reserved[4];
+} tc_sched;
+
+typedef struct tc_timing {
+  float ms_count;           /* hipEvent time of the count kernels only           */
+  float ms_prep;            /* schedule building (work model, bins) on device    */
+  uint32_t launches;        /* count-kernel launches                             */
+  uint32_t dominant_kernel; /* index of the kernel with the most time            */
+  float ms_dominant;        /* its hipEvent time                                 */
+  float reserved;
+} tc_timing;
+
+/* ---- errors / context -------------------------------------------------- */
+const char* tc_status_str(int status);
+const char* tc_last_error(void);
+/* Picks the device and creates the stream; replaces the per-program
+ * cudaSetDevice + warm-up (Hu/tc.cu:209-219, tricore/src/tricount_gpu.cu:29-35). */
+int tc_ctx_create(int device, tc_ctx** out);
+int tc_ctx_destroy(tc_ctx* ctx);
+/* Total device memory in bytes (tricore init_gpu(), tricount_gpu.cu:29-35). */
+int tc_ctx_device_memory(const tc_ctx* ctx, uint64_t* bytes);
+
+/* ---- L0: edge files (tools/list2bin.cpp, tools/bin2list.cpp) ---------- */
+/* Record count = size/8; a size that is not a multiple of 8 is TC_ERR_FORMAT
+ * (tricore/src/main.cpp:38-41). */
+int tc_file_edge_count(const char* path, uint64_t* m);
+/* Whole-file read into caller arrays of capacity cap (Hu/preTC.cu:68-86). */
+int tc_read_edges(const char* path, uint32_t* src, uint32_t* dst, uint64_t cap,
+                  uint64_t* m);
+int tc_write_edges(const char* path, const uint32_t* src, const uint32_t* dst,
+                   uint64_t m);
+/* Text "src dst" lines <-> binary records (tools/list2bin.cpp:8-24,
+ * tools/bin2list.cpp:8-22). */
+int tc_list2bin(const char* text_path, const char* bin_path, uint64_t* m);
+int tc_bin2list(const char* bin_path, const char* text_path, uint64_t* m);
+
+/* ---- seeded synthetic generators (deterministic for any thread count) -- */
+/* Erdos-Renyi: exactly m unique undirected pairs, no self-loops. */
+int tc_gen_er(uint32_t n, uint64_t m, uint64_t seed, uint32_t* src, uint32_t* dst);
+/* R-MAT: nedges raw draws over 2^scale ids, quadrant probs (a,b,c,1-a-b-c). */
+int tc_gen_rmat(uint32_t scale, uint64_t nedges, double a, double b, double c,
+                uint64_t seed, uint32_t* src, uint32_t* dst);
+/* Barabasi-Albert: n vertices, k attachments each; m_out = edges written
+ * (capacity needed: n*k). */
+int tc_gen_ba(uint32_t n, uint32_t k, uint64_t seed, uint32_t* src, uint32_t* dst,
+              uint64_t* m_out);
+/* Random geometric graph in the unit square, radius r.  Call with src=NULL
+ * to get the edge count in *m_out, then again with arrays of that size. */
+int tc_gen_rgg(uint32_t n, double r, uint64_t seed, uint32_t* src, uint32_t* dst,
+               uint64_t* m_out);
+
+/* ---- L1/L2: device graph ---------------------------------------------- */
+/* Uploads raw records (host arrays) to HBM.  n = max id + 1 when n_hint is
+ * 0 (Hu/preTC.cu:90-113); ids must be < n_hint otherwise. */
+int tc_graph_create(tc_ctx* ctx, const uint32_t* src, const uint32_t* dst,
+                    uint64_t m, uint32_t n_hint, tc_graph** out);
+int tc_graph_destroy(tc_graph* g);
+/* Replaces the raw records by the simple undirected edge set: self-loops
+ * dropped, duplicates merged, each edge once as (min,max), sorted. */
+int tc_graph_simplify(tc_ctx* ctx, tc_graph* g);
+/* A-order relabelling of the raw records (vertex_ordering/A-order/A-order.cpp:62-230);
+ * the records are rewritten with new ids.  perm (optional, n entries) gets
+ * old->new.  TC_ERR_UNSUPPORTED outside the reference's defined range. */
+int tc_graph_relabel_aorder(tc_ctx* ctx, tc_graph* g, uint32_t* perm);
+/* Orients the raw records and builds the oriented CSR in HBM
+ * (Hu/preTC.cu:116-291, Fox/preTC.cu:88-203, tricore/src/tricount_gpu.cu:37-203,
+ * edge_directing/A-direction-seri.cpp:35-273, edge_directing/kcore_direction.cpp:25-146). */
+int tc_graph_orient(tc_ctx* ctx, tc_graph* g, int mode, uint32_t flags);
+int tc_graph_info_get(const tc_graph* g, tc_graph_info* info);
+/* Copies of device state (host arrays of the documented sizes). */
+int tc_graph_get_csr(const tc_graph* g, uint32_t* rowptr /*n+1*/, uint32_t* col /*m+*/);
+int tc_graph_get_edges(const tc_graph* g, uint32_t* src, uint32_t* dst /*m_raw*/);
+/* ADIR: peel priorities (A-direction-seri.cpp numbering); KCORE: core numbers. */
+int tc_graph_get_vertex_key(const tc_graph* g, int32_t* key /*n*/);
+/* Device addresses of the CSR (for callers sharing HBM buffers). */
+int tc_graph_device_csr(const tc_graph* g, const uint32_t** rowptr,
+                        const uint32_t** col);
+
+/* ---- L3: work models / schedules -------------------------------------- */
+/* Hu per-vertex work W(u), Hu/tc.cu:22-35 (n entries). */
+int tc_hu_workload(tc_ctx* ctx, tc_graph* g, int64_t* w);
+/* Hu block start vertices, Hu/tc.cu:286-301 (nblocks+1 entries). */
+int tc_hu_block_starts(tc_ctx* ctx, tc_graph* g, uint32_t nblocks, int32_t* start);
+/* Fox per-edge work in CSR edge order, Fox/tc.cu:18-33 (m+ entries). */
+int tc_fox_workload(tc_ctx* ctx, tc_graph* g, int32_t* w);
+/* Fox bin sizes: count[j] = edges in the bin of 2^j lanes, j = 0..log2(max_lanes),
+ * bins per Fox/tc.cu:316-331. */
+int tc_fox_bin_sizes(tc_ctx* ctx, tc_graph* g, uint32_t max_lanes, uint64_t* count);
+
+/* ---- L4/L5: counting --------------------------------------------------- */
+/* Exact triangle count of the oriented CSR (uint64; the reference prints it as
+ * "There are N triangles in the input graph.", Hu/tc.cu:337).  sched may be
+ * NULL (AUTO with defaults).  timing may be NULL. */
+int tc_count(tc_ctx* ctx, tc_graph* g, const tc_sched* sched, uint64_t* count,
+             tc_timing* timing);
+/* Per-vertex forward counts tri[u] (n entries), same kernels. */
+int tc_count_per_vertex(tc_ctx* ctx, tc_graph* g, uint64_t* tri);
+
+/* ---- k-clique (k-clique/kcl.cu:16-215) -------------------------------- */
+int kcl_count(tc_ctx* ctx, tc_graph* g, uint32_t k, int variant, uint64_t* count,
+              tc_timing* timing);
+/* File-level form of kcl_gpu_solver (k-clique/kcl.h:6-7): reads a list2bin
+ * file, k-core orientation, counts k-cliques.  n_chunk is accepted for
+ * signature parity (HBM holds every level; chunks are not needed). */
+int kcl_gpu_solver(const char* path, uint32_t k, uint64_t* total, size_t n_chunk);
+
+/* ---- TriCore function API (tricore/include/tricount.h:7-17) ----------- */
+/* Same signatures and meaning as the reference; edges are the uint64 view of
+ * the record, FIRST = high word (tricore/include/util.h:14-16).  Failures are
+ * reported through tc_last_error() and a 0 / no-op result. */
+uint64_t tc_tricore_init_gpu(void);
+void tc_tricore_cal_degree(const uint64_t* edges, uint64_t edge_count, uint32_t* deg,
+                           uint32_t node_count);
+void tc_tricore_redirect_edges(uint64_t* edges, uint64_t edge_count, const uint32_t* deg,
+                               uint32_t node_count);
+uint32_t tc_tricore_cal_part_num(const uint64_t* edges, uint64_t edge_count,
+                                 uint32_t node_count);
+void tc_tricore_split(uint64_t* edges, uint64_t edge_count, uint64_t part_num,
+                      uint32_t** node_index, const uint32_t* node_split,
+                      uint64_t* adj_count);
+uint64_t tc_tricore_tricount_gpu(uint64_t part_num, uint32_t* adj,
+                                 const uint64_t* adj_count, const uint32_t* node_split,
+                                 uint32_t** node_index);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TC_AMD_H */

@@ -1,0 +1,262 @@
+"""GPU parity: every kernel, orientation and preprocessing step of libtcamd.so
+against the CPU oracle on the same seeded inputs (bit-exact)."""
+import numpy as np
+import pytest
+
+import oracle as orc
+import tcamd
+from graphs import CASES, clique, er, hub, rmat
+
+pytestmark = pytest.mark.gpu
+
+ORIENTS = {
+    "id": tcamd.ORIENT_ID,
+    "degree": tcamd.ORIENT_DEGREE,
+    "tricore": tcamd.ORIENT_DEGREE_TRICORE,
+    "adir": tcamd.ORIENT_ADIR,
+    "kcore": tcamd.ORIENT_KCORE,
+}
+KERNELS = ["hu", "bisson", "fox", "tricore", "auto"]
+
+_truth = {}
+
+
+def truth(name):
+    if name not in _truth:
+        s, d = CASES[name]()
+        _truth[name] = orc.pipeline_count(s, d, "degree")[0]
+    return _truth[name]
+
+
+def oracle_csr(s, d, mode):
+    n = orc.num_nodes(s, d)
+    if mode in ("id", "degree", "tricore"):
+        deg = orc.degree_raw(s, d, n)
+        m = {"id": orc.ORIENT_ID, "degree": orc.ORIENT_DEGREE, "tricore": orc.ORIENT_DEG_TRICORE}[mode]
+        os_, od = orc.orient_simple(s, d, deg, m)
+        return orc.build_csr(os_, od, n)
+    srp, scol = orc.build_sym(s, d, n)
+    if mode == "adir":
+        key = orc.adir_priority(n, srp, scol)
+        os_, od = orc.adir_orient(n, srp, scol, key)
+    else:
+        key = orc.kcore(n, srp, scol)
+        os_, od = orc.kcore_orient(n, srp, scol, key)
+    return orc.build_csr(os_, od, n)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("orient", list(ORIENTS))
+def test_count_every_kernel(ctx, case, orient):
+    s, d = CASES[case]()
+    expect = truth(case)
+    print(f"case={case} orient={orient} m={len(s)} expect={expect}", flush=True)
+    g = ctx.graph(s, d).orient(ORIENTS[orient])
+    for k in KERNELS:
+        got = g.count(k).count
+        assert got == expect, f"{k} on {case}/{orient}: {got} != {expect}"
+    g.close()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("orient", list(ORIENTS))
+def test_oriented_csr_matches_oracle(ctx, case, orient):
+    s, d = CASES[case]()
+    rp_o, col_o = oracle_csr(s, d, orient)
+    g = ctx.graph(s, d).orient(ORIENTS[orient])
+    rp, col = g.csr()
+    np.testing.assert_array_equal(rp, rp_o)
+    np.testing.assert_array_equal(col, col_o)
+    info = g.info()
+    dplus = np.diff(rp_o.astype(np.int64))
+    indeg = np.bincount(col_o, minlength=len(dplus)).astype(np.int64)
+    assert info.sum_out_sq == int((dplus * dplus).sum())
+    assert info.sum_in_out == int((dplus * indeg).sum())
+    assert info.max_out_degree == (int(dplus.max()) if len(dplus) else 0)
+    g.close()
+
+
+@pytest.mark.parametrize("case", ["er", "rmat12", "ba", "messy", "hub"])
+def test_adir_priorities_exact(ctx, case):
+    s, d = CASES[case]()
+    n = orc.num_nodes(s, d)
+    srp, scol = orc.build_sym(s, d, n)
+    expect = orc.adir_priority(n, srp, scol)
+    g = ctx.graph(s, d).orient(tcamd.ORIENT_ADIR)
+    np.testing.assert_array_equal(g.vertex_key(), expect)
+    g.close()
+
+
+@pytest.mark.parametrize("case", ["er", "rmat12", "ba", "rgg", "messy"])
+def test_kcore_numbers_exact(ctx, case):
+    s, d = CASES[case]()
+    n = orc.num_nodes(s, d)
+    srp, scol = orc.build_sym(s, d, n)
+    expect = orc.kcore(n, srp, scol)
+    g = ctx.graph(s, d).orient(tcamd.ORIENT_KCORE)
+    np.testing.assert_array_equal(g.vertex_key(), expect.astype(np.int32))
+    g.close()
+
+
+@pytest.mark.parametrize("case", ["er", "rmat12", "ba", "messy"])
+def test_prune_deg1_matches_oracle(ctx, case):
+    s, d = CASES[case]()
+    n = orc.num_nodes(s, d)
+    deg = orc.degree_raw(s, d, n)
+    os_, od = orc.orient_simple(s, d, deg, orc.ORIENT_DEGREE)
+    ps, pd, n2 = orc.prune_deg1(os_, od, n, deg)
+    rp_o, col_o = orc.build_csr(ps, pd, n2)
+    g = ctx.graph(s, d).orient(tcamd.ORIENT_DEGREE, prune_deg1=True)
+    rp, col = g.csr()
+    assert g.info().n == n2
+    np.testing.assert_array_equal(rp, rp_o)
+    np.testing.assert_array_equal(col, col_o)
+    assert g.count("hu").count == truth(case)
+    g.close()
+
+
+@pytest.mark.parametrize("case", ["er", "rmat12", "ba", "rgg"])
+def test_aorder_matches_oracle(ctx, case):
+    s, d = CASES[case]()
+    n = orc.num_nodes(s, d)
+    res = orc.aorder(s, d, n)
+    assert res is not None
+    perm_o, os_, od = res
+    g = ctx.graph(s, d)
+    perm = g.relabel_aorder()
+    np.testing.assert_array_equal(perm, perm_o)
+    rs, rd = g.edges()
+    np.testing.assert_array_equal(rs, perm_o[os_])
+    np.testing.assert_array_equal(rd, perm_o[od])
+    g.orient(tcamd.ORIENT_ADIR)
+    for k in KERNELS:
+        assert g.count(k).count == truth(case)
+    g.close()
+
+
+@pytest.mark.parametrize("case", ["er", "rmat12", "hub"])
+def test_work_models_match_oracle(ctx, case):
+    s, d = CASES[case]()
+    rp_o, col_o = oracle_csr(s, d, "degree")
+    g = ctx.graph(s, d).orient(tcamd.ORIENT_DEGREE)
+    w = g.hu_workload()
+    w_o = orc.hu_workload(rp_o, col_o)
+    np.testing.assert_array_equal(w, w_o)
+    for nb in (1, 7, 300, 30000):
+        np.testing.assert_array_equal(g.hu_block_starts(nb), orc.hu_block_starts(w_o, nb))
+    fw = g.fox_workload()
+    esrc = np.repeat(np.arange(len(rp_o) - 1, dtype=np.uint32), np.diff(rp_o.astype(np.int64)))
+    fw_o = orc.fox_workload(esrc, col_o, np.diff(rp_o.astype(np.int64)).astype(np.uint32))
+    np.testing.assert_array_equal(fw, fw_o)
+    for lanes in (32, 64):
+        bins = g.fox_bin_sizes(lanes)
+        ref = orc.fox_bins(np.sort(fw_o), lanes)
+        starts = [int(ref[1 << j]) for j in range(len(bins))] + [len(fw_o)]
+        assert [int(x) for x in bins] == [starts[j + 1] - starts[j] for j in range(len(bins))]
+    g.close()
+
+
+@pytest.mark.parametrize("case", ["er", "rmat12", "hub", "messy"])
+def test_per_vertex_counts(ctx, case):
+    s, d = CASES[case]()
+    rp_o, col_o = oracle_csr(s, d, "degree")
+    g = ctx.graph(s, d).orient(tcamd.ORIENT_DEGREE)
+    np.testing.assert_array_equal(g.count_per_vertex(), orc.count_per_vertex(rp_o, col_o))
+    g.close()
+
+
+@pytest.mark.parametrize("sched", [
+    dict(kernel="bisson", bitmap_min_deg=2),
+    dict(kernel="bisson", bitmap_min_deg=16, grid_size=7),
+    dict(kernel="fox", fox_max_lanes=32),
+    dict(kernel="fox", fox_max_lanes=1),
+    dict(kernel="tricore", tricore_parts=3),
+    dict(kernel="tricore", tricore_parts=8, grid_size=5),
+    dict(kernel="hu", grid_size=1),
+    dict(kernel="hu", grid_size=977),
+    dict(kernel="auto", grid_size=3),
+])
+@pytest.mark.parametrize("case", ["rmat12", "hub", "k6", "messy"])
+def test_schedule_variants(ctx, case, sched):
+    s, d = CASES[case]()
+    sched = dict(sched)
+    k = sched.pop("kernel")
+    for orient in (tcamd.ORIENT_ID, tcamd.ORIENT_DEGREE):
+        g = ctx.graph(s, d).orient(orient)
+        assert g.count(k, **sched).count == truth(case)
+        g.close()
+
+
+def test_empty_and_tiny(ctx):
+    e = np.zeros(0, np.uint32)
+    g = ctx.graph(e, e)
+    g.orient(tcamd.ORIENT_DEGREE)
+    for k in KERNELS:
+        assert g.count(k).count == 0
+    assert g.kclique(4).count == 0
+    g.close()
+    for (s, d, t) in [([0], [1], 0), ([0, 1, 2], [1, 2, 0], 1), ([3, 3], [3, 3], 0)]:
+        g = ctx.graph(np.array(s, np.uint32), np.array(d, np.uint32))
+        for o in ORIENTS.values():
+            g.orient(o)
+            for k in KERNELS:
+                assert g.count(k).count == t
+
+
+@pytest.mark.parametrize("case", ["er", "rmat12", "rgg", "k6", "hub"])
+@pytest.mark.parametrize("variant", [tcamd.KCL_AUTO, tcamd.KCL_MERGE])
+def test_kclique_matches_oracle(ctx, case, variant):
+    s, d = CASES[case]()
+    rp_o, col_o = oracle_csr(s, d, "kcore")
+    g = ctx.graph(s, d).orient(tcamd.ORIENT_KCORE)
+    for k in range(3, 8):
+        expect = orc.kclique(rp_o, col_o, k)
+        got = g.kclique(k, variant).count
+        assert got == expect, f"k={k} variant={variant}: {got} != {expect}"
+    g.close()
+
+
+def test_kclique_dense_bitmap_widths(ctx):
+    # K_n with n around each LDS bitmap width boundary: C(n, k) exactly
+    from math import comb
+    for n in (40, 70, 130, 200, 300):
+        s, d = clique(n)
+        g = ctx.graph(s, d).orient(tcamd.ORIENT_KCORE)
+        for k in (3, 4, 5):
+            assert g.kclique(k).count == comb(n, k), (n, k)
+        g.close()
+
+
+def test_tricore_function_api(ctx):
+    s, d = rmat(11, 8, 5)
+    n = orc.num_nodes(s, d)
+    edges = tcamd.tricore.pack(s, d)
+    deg = tcamd.tricore.cal_degree(edges, n)
+    np.testing.assert_array_equal(deg, orc.degree_raw(s, d, n))
+    red = tcamd.tricore.redirect_edges(edges.copy(), deg)
+    os_, od = orc.orient_simple(s, d, deg, orc.ORIENT_DEG_TRICORE)
+    np.testing.assert_array_equal(red, tcamd.tricore.pack(os_, od))
+    assert tcamd.tricore.cal_part_num(red, n) == 1
+    expect = orc.pipeline_count(s, d, "degree")[0]
+    for parts in (1, 2, 3, 7):
+        assert tcamd.tricore.count(red, n, parts) == expect
+
+
+def test_kcl_gpu_solver_file(ctx, tmp_path):
+    s, d = er(500, 6000, 9)
+    p = str(tmp_path / "g.bin")
+    tcamd.write_edges(p, s, d)
+    rp_o, col_o = oracle_csr(s, d, "kcore")
+    assert tcamd.kcl_gpu_solver(p, 4) == orc.kclique(rp_o, col_o, 4)
+
+
+def test_larger_rmat_all_kernels(ctx):
+    s, d = rmat(16, 16, 22)
+    expect, rp_o, col_o = orc.pipeline_count(s, d, "degree")
+    g = ctx.graph(s, d)
+    g.simplify()
+    g.relabel_aorder()
+    g.orient(tcamd.ORIENT_ADIR)
+    for k in KERNELS:
+        assert g.count(k).count == expect, k
+    g.close()
