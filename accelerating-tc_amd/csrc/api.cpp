@@ -126,6 +126,8 @@ int tc_ctx_destroy(tc_ctx* c) {
   c->scratch.release();
   c->accum.release();
   c->prim_out.release();
+  for (hipEvent_t e : c->lev)
+    if (e) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);

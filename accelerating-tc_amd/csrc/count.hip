@@ -95,23 +95,27 @@ __global__ void k_block_starts(const int64_t* __restrict__ pre, uint32_t n, int6
   }
 }
 
-// Balanced work per vertex for the AUTO engine: wedges + staged list; hubs
-// (d+ > slice) are scheduled separately and weigh 0 here.
+// AUTO engine per-vertex work: wedges sum_{v in N+(u)} d+(v) plus d+(u).
 __global__ void k_auto_work(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
-                            uint32_t n, uint32_t slice, int64_t* __restrict__ w) {
+                            uint32_t n, uint64_t* __restrict__ key) {
   for (uint64_t u = gtid(); u < n; u += gstride()) {
     uint32_t b = rp[u], e = rp[u + 1];
-    uint32_t d = e - b;
-    int64_t tot = 0;
-    if (d >= 2 && d <= slice) {
-      for (uint32_t k = b; k < e; ++k) {
-        uint32_t v = col[k];
-        tot += rp[v + 1] - rp[v];
-      }
-      tot += d;
+    uint64_t tot = e - b;
+    for (uint32_t k = b; k < e; ++k) {
+      uint32_t v = col[k];
+      tot += rp[v + 1] - rp[v];
     }
-    w[u] = tot;
+    key[u] = ~tot;  // ascending sort of ~work = descending work
   }
+}
+
+__global__ void k_gather_u8(const uint8_t* __restrict__ in, const uint32_t* __restrict__ idx,
+                            uint32_t n, uint8_t* __restrict__ out) {
+  for (uint64_t i = gtid(); i < n; i += gstride()) out[i] = in[idx[i]];
+}
+
+__global__ void k_iota(uint32_t* __restrict__ p, uint32_t n) {
+  for (uint64_t i = gtid(); i < n; i += gstride()) p[i] = (uint32_t)i;
 }
 
 // Fox/tc.cu:18-33: W(u,v) = int(min(d+u, d+v) * log2(max + 2)).
@@ -125,23 +129,22 @@ __global__ void k_fox_work(const uint32_t* __restrict__ rp, const uint32_t* __re
   }
 }
 
+// Bin of each edge (number of thresholds <= w) plus a block-level histogram,
+// so the global counters see one atomic per block and bin.
 __global__ void k_fox_bin(const int32_t* __restrict__ w, uint64_t m, const int32_t* thr, int nthr,
-                          uint8_t* __restrict__ bin, unsigned long long* counts) {
+                          uint32_t* __restrict__ bin, unsigned long long* counts) {
+  __shared__ unsigned long long s_cnt[8];
+  if (threadIdx.x < 8) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
   for (uint64_t i = gtid(); i < m; i += gstride()) {
     int32_t x = w[i];
     int j = 0;
     while (j < nthr && x >= thr[j]) ++j;
-    bin[i] = (uint8_t)j;
-    atomicAdd(counts + j, 1ull);
+    bin[i] = (uint32_t)j;
+    atomicAdd(s_cnt + j, 1ull);
   }
-}
-
-__global__ void k_fox_scatter(const uint8_t* __restrict__ bin, uint64_t m,
-                              unsigned long long* cursor, uint32_t* __restrict__ eids) {
-  for (uint64_t i = gtid(); i < m; i += gstride()) {
-    unsigned long long p = atomicAdd(cursor + bin[i], 1ull);
-    eids[p] = (uint32_t)i;
-  }
+  __syncthreads();
+  if (threadIdx.x < 8 && s_cnt[threadIdx.x]) atomicAdd(counts + threadIdx.x, s_cnt[threadIdx.x]);
 }
 
 __global__ void k_row_flags(const uint32_t* __restrict__ rp, uint32_t n, uint32_t min_d,
@@ -174,14 +177,20 @@ struct HuLds {
 // Stage edges [ebase, ebase+E) into LDS: v, start of N+(v), wedge prefix.
 template <int SLICE, int ROWCAP>
 __device__ inline uint32_t stage_edges(HuLds<SLICE, ROWCAP>& L, const uint32_t* __restrict__ rp,
-                                       const uint32_t* __restrict__ col, uint32_t ebase, uint32_t E) {
+                                       const uint32_t* __restrict__ col, uint32_t ebase, uint32_t E,
+                                       bool by_row = false) {
   const uint32_t tid = threadIdx.x;
   for (uint32_t e = tid; e < E; e += TB) {
     uint32_t v = col[ebase + e];
     uint32_t vs = rp[v];
     L.col[e] = v;
     L.vst[e] = vs;
-    L.pre[e] = rp[v + 1] - vs;
+    uint32_t c = rp[v + 1] - vs;
+    if (by_row) {
+      const uint32_t r = L.row[e];
+      if (L.rof[r + 1] - L.rof[r] < 2) c = 0;
+    }
+    L.pre[e] = c;
   }
   __syncthreads();
   // block exclusive scan, blocked layout: thread t owns [t*K, t*K+K)
@@ -272,7 +281,7 @@ __global__ __launch_bounds__(TB) void k_hu(const uint32_t* __restrict__ rp,
     const uint32_t E = rp[u2] - base;
     const uint32_t R = u2 - u;
     for (uint32_t r = tid; r <= R; r += TB) L.rof[r] = rp[u + r] - base;
-    uint32_t T = stage_edges(L, rp, col, base, E);  // has a barrier after rof too
+    __syncthreads();
     for (uint32_t e = tid; e < E; e += TB) {
       uint32_t lo = 0, hi = R - 1;  // row r with rof[r] <= e < rof[r+1]
       while (lo < hi) {
@@ -282,7 +291,9 @@ __global__ __launch_bounds__(TB) void k_hu(const uint32_t* __restrict__ rp,
       }
       L.row[e] = (uint16_t)lo;
     }
-    __syncthreads();
+    // stage_edges begins with a barrier-free pass that writes col/vst/pre,
+    // then synchronises; rows with d+(u) < 2 get no wedges.
+    uint32_t T = stage_edges(L, rp, col, base, E, true);
     for (uint32_t t = tid; t < T; t += TB) {
       uint32_t e = owner_of(L.pre, E, t);
       uint32_t w = col[L.vst[e] + (t - L.pre[e])];
@@ -295,35 +306,126 @@ __global__ __launch_bounds__(TB) void k_hu(const uint32_t* __restrict__ rp,
   flush_count(cnt, total);
 }
 
-// AUTO hub kernel: task = (hub u, first edge k0); the workgroup takes edges
-// [k0, k0 + HUB_CHUNK) of u, stages N+(u) in LDS when it fits, and checks the
-// flattened wedges against it.
-constexpr int kHubChunk = 1024;
-constexpr int kHubLds = 8192;
+// ---- AUTO: vertex-centric, N+(u) as an LDS hash set ----------------------
+//
+// Multiplicative hashing with linear probing; the table holds N+(u) at load
+// <= 1/2, so a probe costs ~1.5 LDS reads on average.  Every wedge
+// u->v->w reads w from N+(v) with consecutive lanes on consecutive entries.
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
-__global__ __launch_bounds__(TB) void k_hub(const uint32_t* __restrict__ rp,
-                                            const uint32_t* __restrict__ col,
-                                            const uint2* __restrict__ tasks, uint32_t ntasks,
-                                            unsigned long long* total) {
-  __shared__ HuLds<kHubChunk, 4> L;
-  __shared__ uint32_t s_nu[kHubLds];
-  const uint32_t tid = threadIdx.x;
+// Branchless membership test in an ascending LDS list a[0, n): the loop
+// count depends on n only, so lanes of a wave never diverge.
+__device__ inline uint32_t lds_has(const uint32_t* a, uint32_t n, uint32_t x) {
+  const uint32_t* p = a;
+  uint32_t len = n;
+  while (len > 1) {
+    const uint32_t half = len >> 1;
+    p = p[half] <= x ? p + half : p;
+    len -= half;
+  }
+  return (n > 0 && *p == x) ? 1u : 0u;
+}
+
+// Streams N+(v) = col[vb, ve) with the wave's lanes, four 64-entry chunks in
+// flight per lane, and counts the entries present in the LDS list a[0, n).
+__device__ inline uint32_t count_list(const uint32_t* __restrict__ col, uint32_t vb, uint32_t ve,
+                                      uint32_t lane, const uint32_t* a, uint32_t n) {
+  uint32_t c = 0;
+  for (uint32_t k = vb + lane; k < ve; k += 256) {
+    const uint32_t w0 = col[k];
+    const uint32_t w1 = k + 64 < ve ? col[k + 64] : kEmpty;
+    const uint32_t w2 = k + 128 < ve ? col[k + 128] : kEmpty;
+    const uint32_t w3 = k + 192 < ve ? col[k + 192] : kEmpty;
+    c += lds_has(a, n, w0);
+    c += lds_has(a, n, w1);
+    c += lds_has(a, n, w2);
+    c += lds_has(a, n, w3);
+  }
+  return c;
+}
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Small vertices (2 <= d+ <= 64): one wave per vertex, N+(u) in the wave's
+// LDS slot; lane j preloads the row bounds of the j-th out-neighbour.
+__global__ __launch_bounds__(TB) void k_vsmall(const uint32_t* __restrict__ rp,
+                                               const uint32_t* __restrict__ col,
+                                               const uint32_t* __restrict__ verts,
+                                               uint64_t nverts, unsigned long long* total) {
+  __shared__ uint32_t s_list[TB / 64][64];
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t* L = s_list[wv];
+  const uint64_t wave = gtid() >> 6, nwaves = gstride() >> 6;
   unsigned long long cnt = 0;
-  for (uint32_t ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
-    const uint2 task = tasks[ti];
-    const uint32_t u = task.x, k0 = task.y;
-    const uint32_t ab = rp[u], du = rp[u + 1] - ab;
-    const bool in_lds = du <= (uint32_t)kHubLds;
-    if (in_lds)
-      for (uint32_t i = tid; i < du; i += TB) s_nu[i] = col[ab + i];
-    const uint32_t E = min((uint32_t)kHubChunk, du - k0);
-    uint32_t T = stage_edges(L, rp, col, ab + k0, E);
-    for (uint32_t t = tid; t < T; t += TB) {
-      uint32_t e = owner_of(L.pre, E, t);
-      uint32_t w = col[L.vst[e] + (t - L.pre[e])];
-      cnt += in_lds ? contains(s_nu, 0, du, w) : contains(col, ab, ab + du, w);
+  for (uint64_t i = wave; i < nverts; i += nwaves) {
+    const uint32_t u = verts[i];
+    const uint32_t ub = rp[u], du = rp[u + 1] - ub;
+    uint32_t mine = kEmpty, mvb = 0, mve = 0;
+    if (lane < du) {
+      mine = col[ub + lane];
+      mvb = rp[mine];
+      mve = rp[mine + 1];
     }
+    L[lane] = mine;
+    wave_sync();
+    for (uint32_t j = 0; j < du; ++j) {
+      const uint32_t vb = __shfl(mvb, j, 64), ve = __shfl(mve, j, 64);
+      cnt += count_list(col, vb, ve, lane, L, du);
+    }
+    wave_sync();
+  }
+  flush_count(cnt, total);
+}
+
+// Large vertices: a workgroup per vertex, taken from a work-sorted list
+// through an atomic cursor (persistent grid).  The block copies N+(u) into
+// LDS and stages the row bounds of its out-neighbours, so each wave's walk
+// over N+(v) lists waits only on the list loads themselves.  Lists longer
+// than the LDS copy are searched in global memory.
+constexpr int kStage = 1024;
+
+template <int CAP>
+__global__ __launch_bounds__(TB) void k_vlist(const uint32_t* __restrict__ rp,
+                                              const uint32_t* __restrict__ col,
+                                              const uint32_t* __restrict__ tasks, uint32_t ntasks,
+                                              uint32_t* cursor, unsigned long long* total) {
+  __shared__ uint32_t s_u[CAP];
+  __shared__ uint32_t s_vb[kStage], s_ve[kStage];
+  __shared__ uint32_t s_t;
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  unsigned long long cnt = 0;
+  for (;;) {
+    if (tid == 0) s_t = atomicAdd(cursor, 1u);
     __syncthreads();
+    const uint32_t t = s_t;
+    if (t >= ntasks) break;
+    const uint32_t u = tasks[t];
+    const uint32_t ub = rp[u], du = rp[u + 1] - ub;
+    const bool in_lds = du <= (uint32_t)CAP;
+    if (in_lds)
+      for (uint32_t i = tid; i < du; i += TB) s_u[i] = col[ub + i];
+    for (uint32_t e0 = 0; e0 < du; e0 += kStage) {
+      const uint32_t E = min((uint32_t)kStage, du - e0);
+      for (uint32_t i = tid; i < E; i += TB) {
+        const uint32_t v = col[ub + e0 + i];
+        s_vb[i] = rp[v];
+        s_ve[i] = rp[v + 1];
+      }
+      __syncthreads();
+      for (uint32_t e = wv; e < E; e += TB / 64) {
+        const uint32_t vb = s_vb[e], ve = s_ve[e];
+        if (in_lds) {
+          cnt += count_list(col, vb, ve, lane, s_u, du);
+        } else {
+          for (uint32_t k = vb + lane; k < ve; k += 64) cnt += contains(col, ub, ub + du, col[k]);
+        }
+      }
+      __syncthreads();
+    }
   }
   flush_count(cnt, total);
 }
@@ -507,9 +609,7 @@ static int ensure_version(tc_graph* g) {
   SchedCache& sc = g->sc;
   if (sc.version != g->sched_version) {
     sc.hu_nblocks = 0;
-    sc.auto_nblocks = 0;
-    sc.auto_key = 0;
-    sc.n_hub_tasks = 0;
+    sc.auto_ok = false;
     sc.rows_ok = false;
     sc.fox_lanes = 0;
     sc.version = g->sched_version;
@@ -590,10 +690,12 @@ static int fox_schedule(tc_ctx* c, tc_graph* g, uint32_t max_lanes) {
   SchedCache& sc = g->sc;
   if (sc.fox_lanes == max_lanes) return TC_OK;
   const uint64_t m = g->m_or;
-  DevBuf w, bin;
+  DevBuf w, bin, btmp, ids, itmp;
   TCX_TRY(w.reserve(4 * std::max<uint64_t>(m, 1)));
-  TCX_TRY(bin.reserve(std::max<uint64_t>(m, 1)));
-  TCX_TRY(sc.fox_eids.reserve(4 * std::max<uint64_t>(m, 1)));
+  TCX_TRY(bin.reserve(4 * std::max<uint64_t>(m, 1)));
+  TCX_TRY(btmp.reserve(4 * std::max<uint64_t>(m, 1)));
+  TCX_TRY(ids.reserve(4 * std::max<uint64_t>(m, 1)));
+  TCX_TRY(itmp.reserve(4 * std::max<uint64_t>(m, 1)));
   hipLaunchKernelGGL(k_fox_work, dim3(grid_for(m)), dim3(TB), 0, c->stream,
                      g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(), g->esrc.as<uint32_t>(), m,
                      w.as<int32_t>());
@@ -607,70 +709,80 @@ static int fox_schedule(tc_ctx* c, tc_graph* g, uint32_t max_lanes) {
   TCX_TRY(c->scratch.reserve(4096));
   int32_t* d_thr = c->scratch.as<int32_t>();
   unsigned long long* d_cnt = reinterpret_cast<unsigned long long*>(c->scratch.as<char>() + 256);
-  unsigned long long* d_cur = reinterpret_cast<unsigned long long*>(c->scratch.as<char>() + 512);
   TCX_HIP(hipMemcpyAsync(d_thr, thr.data(), 4 * nthr, hipMemcpyHostToDevice, c->stream));
-  TCX_HIP(hipMemsetAsync(d_cnt, 0, 8 * nb, c->stream));
-  hipLaunchKernelGGL(k_fox_bin, dim3(grid_for(m)), dim3(TB), 0, c->stream, w.as<int32_t>(), m,
-                     d_thr, nthr, bin.as<uint8_t>(), d_cnt);
+  TCX_HIP(hipMemsetAsync(d_cnt, 0, 8 * 8, c->stream));
+  hipLaunchKernelGGL(k_fox_bin, dim3(std::min<unsigned>(grid_for(m), 4096)), dim3(TB), 0, c->stream,
+                     w.as<int32_t>(), m, d_thr, nthr, bin.as<uint32_t>(), d_cnt);
+  hipLaunchKernelGGL(k_iota, dim3(grid_for(m)), dim3(TB), 0, c->stream, ids.as<uint32_t>(),
+                     (uint32_t)m);
   TCX_HIP(hipGetLastError());
   std::vector<unsigned long long> cnt(nb);
   TCX_HIP(hipMemcpyAsync(cnt.data(), d_cnt, 8 * nb, hipMemcpyDeviceToHost, c->stream));
   TCX_HIP(hipStreamSynchronize(c->stream));
   sc.fox_off.assign(nb + 1, 0);
   for (int j = 0; j < nb; ++j) sc.fox_off[j + 1] = sc.fox_off[j] + cnt[j];
-  std::vector<unsigned long long> cur(sc.fox_off.begin(), sc.fox_off.end() - 1);
-  TCX_HIP(hipMemcpyAsync(d_cur, cur.data(), 8 * nb, hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_fox_scatter, dim3(grid_for(m)), dim3(TB), 0, c->stream, bin.as<uint8_t>(),
-                     m, d_cur, sc.fox_eids.as<uint32_t>());
-  TCX_HIP(hipGetLastError());
+  // stable grouping of edge ids by bin (keys have at most 3 bits)
+  uint32_t *sk, *sv;
+  TCX_TRY(sort_pairs_u32_u32(c, bin.as<uint32_t>(), btmp.as<uint32_t>(), ids.as<uint32_t>(),
+                             itmp.as<uint32_t>(), m, 3, &sk, &sv));
+  TCX_TRY(sc.fox_eids.reserve(4 * std::max<uint64_t>(m, 1)));
+  TCX_HIP(hipMemcpyAsync(sc.fox_eids.p, sv, 4 * m, hipMemcpyDeviceToDevice, c->stream));
   TCX_HIP(hipStreamSynchronize(c->stream));
   sc.fox_lanes = max_lanes;
   return TC_OK;
 }
 
-static int auto_schedule(tc_ctx* c, tc_graph* g, uint32_t nblocks, uint32_t slice) {
+static int gather_flags(tc_ctx* c, const uint8_t* in, const uint32_t* idx, uint32_t n,
+                        uint8_t* out) {
+  hipLaunchKernelGGL(k_gather_u8, dim3(grid_for(n)), dim3(TB), 0, c->stream, in, idx, n, out);
+  TCX_HIP(hipGetLastError());
+  return TC_OK;
+}
+
+static int auto_schedule(tc_ctx* c, tc_graph* g) {
   SchedCache& sc = g->sc;
-  uint32_t key = nblocks * 131u + slice;
-  if (sc.auto_key == key && sc.auto_nblocks == nblocks) return TC_OK;
-  DevBuf w;
-  TCX_TRY(w.reserve(8 * ((size_t)g->n + 1)));
-  hipLaunchKernelGGL(k_auto_work, dim3(grid_for(g->n)), dim3(TB), 0, c->stream,
-                     g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(), g->n, slice, w.as<int64_t>());
+  if (sc.auto_ok) return TC_OK;
+  const uint32_t n = g->n;
+  const uint32_t* rp = g->rowptr.as<uint32_t>();
+  DevBuf flags, key, ktmp, ids, itmp;
+  TCX_TRY(flags.reserve((size_t)n + 1));
+  TCX_TRY(sc.auto_small.reserve(4 * ((size_t)n + 1)));
+  hipLaunchKernelGGL(k_row_flags, dim3(grid_for(n)), dim3(TB), 0, c->stream, rp, n, 2u, 64u,
+                     flags.as<uint8_t>());
   TCX_HIP(hipGetLastError());
-  TCX_TRY(block_starts(c, g, w, nblocks, sc.auto_start));
-  // hub tasks (d+ > slice)
-  DevBuf flags, hubs;
-  TCX_TRY(flags.reserve((size_t)g->n + 1));
-  TCX_TRY(hubs.reserve(4 * ((size_t)g->n + 1)));
-  hipLaunchKernelGGL(k_row_flags, dim3(grid_for(g->n)), dim3(TB), 0, c->stream,
-                     g->rowptr.as<uint32_t>(), g->n, slice + 1, 0xFFFFFFFFu, flags.as<uint8_t>());
+  TCX_TRY(select_indices(c, flags.as<uint8_t>(), n, sc.auto_small.as<uint32_t>(), &sc.n_small));
+  // large and huge vertices sorted by descending work
+  TCX_TRY(key.reserve(8 * ((size_t)n + 1)));
+  TCX_TRY(ktmp.reserve(8 * ((size_t)n + 1)));
+  TCX_TRY(ids.reserve(4 * ((size_t)n + 1)));
+  TCX_TRY(itmp.reserve(4 * ((size_t)n + 1)));
+  hipLaunchKernelGGL(k_auto_work, dim3(grid_for(n)), dim3(TB), 0, c->stream, rp,
+                     g->col.as<uint32_t>(), n, key.as<uint64_t>());
+  hipLaunchKernelGGL(k_iota, dim3(grid_for(n)), dim3(TB), 0, c->stream, ids.as<uint32_t>(), n);
   TCX_HIP(hipGetLastError());
-  uint64_t nh = 0;
-  TCX_TRY(select_indices(c, flags.as<uint8_t>(), g->n, hubs.as<uint32_t>(), &nh));
-  std::vector<uint2> tasks;
-  if (nh) {
-    std::vector<uint32_t> h(nh);
-    TCX_HIP(hipMemcpyAsync(h.data(), hubs.p, 4 * nh, hipMemcpyDeviceToHost, c->stream));
-    std::vector<uint32_t> rb(nh), re(nh);
-    for (uint64_t i = 0; i < nh; ++i) {
-      TCX_HIP(hipMemcpyAsync(&rb[i], g->rowptr.as<uint32_t>() + h[i], 4, hipMemcpyDeviceToHost,
-                             c->stream));
-      TCX_HIP(hipMemcpyAsync(&re[i], g->rowptr.as<uint32_t>() + h[i] + 1, 4,
-                             hipMemcpyDeviceToHost, c->stream));
-    }
-    TCX_HIP(hipStreamSynchronize(c->stream));
-    for (uint64_t i = 0; i < nh; ++i)
-      for (uint32_t k = 0; k < re[i] - rb[i]; k += kHubChunk) tasks.push_back(make_uint2(h[i], k));
-  }
-  sc.n_hub_tasks = (uint32_t)tasks.size();
-  if (!tasks.empty()) {
-    TCX_TRY(sc.hub_tasks.reserve(sizeof(uint2) * tasks.size()));
-    TCX_HIP(hipMemcpyAsync(sc.hub_tasks.p, tasks.data(), sizeof(uint2) * tasks.size(),
-                           hipMemcpyHostToDevice, c->stream));
-    TCX_HIP(hipStreamSynchronize(c->stream));
-  }
-  sc.auto_key = key;
-  sc.auto_nblocks = nblocks;
+  uint64_t* sk;
+  uint32_t* sv;
+  TCX_TRY(sort_pairs_u64_u32(c, key.as<uint64_t>(), ktmp.as<uint64_t>(), ids.as<uint32_t>(),
+                             itmp.as<uint32_t>(), n, 64, &sk, &sv));
+  // sv = vertices by descending work; keep the two degree classes in order
+  DevBuf dflag;
+  TCX_TRY(dflag.reserve((size_t)n + 1));
+  TCX_TRY(sc.auto_large.reserve(4 * ((size_t)n + 1)));
+  TCX_TRY(sc.auto_huge.reserve(4 * ((size_t)n + 1)));
+  hipLaunchKernelGGL(k_row_flags, dim3(grid_for(n)), dim3(TB), 0, c->stream, rp, n, 65u, 2048u,
+                     flags.as<uint8_t>());
+  TCX_HIP(hipGetLastError());
+  // flags are indexed by vertex; gather them into sorted order
+  TCX_TRY(gather_flags(c, flags.as<uint8_t>(), sv, n, dflag.as<uint8_t>()));
+  TCX_TRY(select_flagged_u32(c, sv, dflag.as<uint8_t>(), sc.auto_large.as<uint32_t>(), n,
+                             &sc.n_large));
+  hipLaunchKernelGGL(k_row_flags, dim3(grid_for(n)), dim3(TB), 0, c->stream, rp, n, 2049u,
+                     0xFFFFFFFFu, flags.as<uint8_t>());
+  TCX_HIP(hipGetLastError());
+  TCX_TRY(gather_flags(c, flags.as<uint8_t>(), sv, n, dflag.as<uint8_t>()));
+  TCX_TRY(select_flagged_u32(c, sv, dflag.as<uint8_t>(), sc.auto_huge.as<uint32_t>(), n,
+                             &sc.n_huge));
+  sc.auto_ok = true;
   return TC_OK;
 }
 
@@ -692,6 +804,38 @@ struct Timer {
     TCX_HIP(hipEventRecord(c->ev1, c->stream));
     TCX_HIP(hipEventSynchronize(c->ev1));
     TCX_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    return TC_OK;
+  }
+};
+
+// Event marks between consecutive count launches: mark() before each launch,
+// finish() after the last; gives the total and the longest launch.
+struct LaunchMarks {
+  tc_ctx* c;
+  int n = 0;
+  explicit LaunchMarks(tc_ctx* cc) : c(cc) {}
+  int mark() {
+    if (n >= 16) return TC_OK;  // later launches fold into the last interval
+    if (!c->lev[n]) TCX_HIP(hipEventCreate(&c->lev[n]));
+    TCX_HIP(hipEventRecord(c->lev[n], c->stream));
+    ++n;
+    return TC_OK;
+  }
+  int finish(tc_timing* tm) {
+    if (!c->lev[n]) TCX_HIP(hipEventCreate(&c->lev[n]));
+    TCX_HIP(hipEventRecord(c->lev[n], c->stream));
+    TCX_HIP(hipEventSynchronize(c->lev[n]));
+    tm->ms_count = 0;
+    tm->ms_dominant = 0;
+    for (int i = 0; i < n; ++i) {
+      float ms = 0;
+      TCX_HIP(hipEventElapsedTime(&ms, c->lev[i], c->lev[i + 1]));
+      tm->ms_count += ms;
+      if (ms > tm->ms_dominant) {
+        tm->ms_dominant = ms;
+        tm->dominant_kernel = (uint32_t)i;
+      }
+    }
     return TC_OK;
   }
 };
@@ -730,7 +874,8 @@ int tc_count(tc_ctx* c, tc_graph* g, const tc_sched* sched_in, uint64_t* count,
   }
   const uint32_t* rp = g->rowptr.as<uint32_t>();
   const uint32_t* col = g->col.as<uint32_t>();
-  Timer prep(c), run(c);
+  Timer prep(c);
+  LaunchMarks lm(c);
   TCX_TRY(prep.begin());
   // schedule (cached per graph)
   switch (s.kernel) {
@@ -748,16 +893,16 @@ int tc_count(tc_ctx* c, tc_graph* g, const tc_sched* sched_in, uint64_t* count,
       break;
     }
     case TC_KERNEL_AUTO:
-      TCX_TRY(auto_schedule(c, g, s.grid_size ? s.grid_size : (uint32_t)c->num_cus * 16u, 2048));
+      TCX_TRY(auto_schedule(c, g));
       break;
     default:
       break;
   }
   TCX_TRY(prep.end());
   TCX_HIP(hipMemsetAsync(acc, 0, sizeof(unsigned long long), c->stream));
-  TCX_TRY(run.begin());
   switch (s.kernel) {
     case TC_KERNEL_HU: {
+      TCX_TRY(lm.mark());
       hipLaunchKernelGGL((k_hu<2048, 1024>), dim3(g->sc.hu_nblocks), dim3(TB), 0, c->stream, rp,
                          col, g->sc.hu_start.as<int32_t>(), 0, acc);
       tm.launches = 1;
@@ -776,6 +921,7 @@ int tc_count(tc_ctx* c, tc_graph* g, const tc_sched* sched_in, uint64_t* count,
       }
       uint32_t cshift = 0;
       while (((uint64_t)(g->n - 1) >> cshift) >= (uint64_t)kCoarseWords * 32) ++cshift;
+      TCX_TRY(lm.mark());
       hipLaunchKernelGGL(k_bisson, dim3(G), dim3(64), 0, c->stream, rp, col,
                          g->sc.rows.as<uint32_t>(), g->sc.nrows, c->bitmaps.as<uint32_t>(), words,
                          min_deg, cshift, acc);
@@ -792,6 +938,7 @@ int tc_count(tc_ctx* c, tc_graph* g, const tc_sched* sched_in, uint64_t* count,
         const uint32_t* ids = eids + off[j];
         uint32_t lanes = 1u << j;
         unsigned grid = (unsigned)std::min<uint64_t>(20000, (cnt * lanes + TB - 1) / TB);
+        TCX_TRY(lm.mark());
         switch (lanes) {
           case 1: hipLaunchKernelGGL(k_fox<1>, dim3(grid), dim3(TB), 0, c->stream, rp, col, es, ids, cnt, acc); break;
           case 2: hipLaunchKernelGGL(k_fox<2>, dim3(grid), dim3(TB), 0, c->stream, rp, col, es, ids, cnt, acc); break;
@@ -818,13 +965,14 @@ int tc_count(tc_ctx* c, tc_graph* g, const tc_sched* sched_in, uint64_t* count,
       for (uint32_t i = 0; i <= parts; ++i)
         TCX_HIP(hipMemcpyAsync(&ebeg[i], rp + split[i], 4, hipMemcpyDeviceToHost, c->stream));
       TCX_HIP(hipStreamSynchronize(c->stream));
-      TCX_TRY(run.begin());
       for (uint32_t pm = 0; pm < parts; ++pm) {
         for (uint32_t pn = pm; pn < parts; ++pn) {
+          TCX_TRY(lm.mark());
           hipLaunchKernelGGL(k_tricore, dim3(G), dim3(64), 0, c->stream, rp, col, es,
                              (uint64_t)ebeg[pm], (uint64_t)ebeg[pm + 1], split[pn], split[pn + 1], acc);
           tm.launches++;
           if (pm != pn) {
+            TCX_TRY(lm.mark());
             hipLaunchKernelGGL(k_tricore, dim3(G), dim3(64), 0, c->stream, rp, col, es,
                                (uint64_t)ebeg[pn], (uint64_t)ebeg[pn + 1], split[pm], split[pm + 1],
                                acc);
@@ -835,27 +983,42 @@ int tc_count(tc_ctx* c, tc_graph* g, const tc_sched* sched_in, uint64_t* count,
       break;
     }
     case TC_KERNEL_AUTO: {
-      hipLaunchKernelGGL((k_hu<2048, 1024>), dim3(g->sc.auto_nblocks), dim3(TB), 0, c->stream, rp,
-                         col, g->sc.auto_start.as<int32_t>(), 1, acc);
-      tm.launches = 1;
-      if (g->sc.n_hub_tasks) {
-        unsigned grid = std::min<uint32_t>(g->sc.n_hub_tasks, (uint32_t)c->num_cus * 8u);
-        hipLaunchKernelGGL(k_hub, dim3(grid), dim3(TB), 0, c->stream, rp, col,
-                           g->sc.hub_tasks.as<uint2>(), g->sc.n_hub_tasks, acc);
+      const SchedCache& sc = g->sc;
+      TCX_TRY(c->scratch.reserve(64));
+      uint32_t* cursor = c->scratch.as<uint32_t>();
+      TCX_HIP(hipMemsetAsync(cursor, 0, 2 * sizeof(uint32_t), c->stream));
+      const uint32_t G = s.grid_size ? s.grid_size : (uint32_t)c->num_cus * 8u;
+      if (sc.n_large) {
+        TCX_TRY(lm.mark());
+        hipLaunchKernelGGL(k_vlist<2048>, dim3(std::min<uint64_t>(G, sc.n_large)), dim3(TB), 0,
+                           c->stream, rp, col, sc.auto_large.as<uint32_t>(), (uint32_t)sc.n_large,
+                           cursor, acc);
+        tm.launches++;
+      }
+      if (sc.n_huge) {
+        TCX_TRY(lm.mark());
+        hipLaunchKernelGGL(k_vlist<8192>, dim3(std::min<uint64_t>((uint64_t)c->num_cus * 4, sc.n_huge)),
+                           dim3(TB), 0, c->stream, rp, col, sc.auto_huge.as<uint32_t>(),
+                           (uint32_t)sc.n_huge, cursor + 1, acc);
+        tm.launches++;
+      }
+      if (sc.n_small) {
+        TCX_TRY(lm.mark());
+        unsigned grid = (unsigned)std::min<uint64_t>((uint64_t)c->num_cus * 16, (sc.n_small + 3) / 4);
+        hipLaunchKernelGGL(k_vsmall, dim3(grid), dim3(TB), 0, c->stream, rp, col,
+                           sc.auto_small.as<uint32_t>(), sc.n_small, acc);
         tm.launches++;
       }
       break;
     }
   }
   TCX_HIP(hipGetLastError());
-  TCX_TRY(run.end());
+  TCX_TRY(lm.finish(&tm));
   unsigned long long h = 0;
   TCX_HIP(hipMemcpyAsync(&h, acc, 8, hipMemcpyDeviceToHost, c->stream));
   TCX_HIP(hipStreamSynchronize(c->stream));
   *count = h;
-  tm.ms_count = run.ms;
   tm.ms_prep = prep.ms;
-  tm.ms_dominant = run.ms;
   if (timing) *timing = tm;
   return TC_OK;
 }

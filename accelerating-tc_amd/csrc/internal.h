@@ -51,6 +51,7 @@ struct tc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t lev[17] = {};  // per-launch timing marks (created on first use)
   uint64_t total_mem = 0;
   int num_cus = 256;
   tcx::DevBuf temp;     // rocprim temp storage
@@ -67,11 +68,11 @@ struct SchedCache {
   uint64_t version = ~0ull;
   uint32_t hu_nblocks = 0;
   DevBuf hu_start;                 // int32[nblocks+1]
-  uint32_t auto_key = 0;
-  uint32_t auto_nblocks = 0;
-  DevBuf auto_start;               // int32[nblocks+1]
-  DevBuf hub_tasks;                // uint2 (hub vertex, first edge)
-  uint32_t n_hub_tasks = 0;
+  bool auto_ok = false;
+  DevBuf auto_small;               // vertices with 2 <= d+ <= 64, id order
+  DevBuf auto_large;               // d+ in (64, 2048], by descending work
+  DevBuf auto_huge;                // d+ > 2048, by descending work
+  uint64_t n_small = 0, n_large = 0, n_huge = 0;
   bool rows_ok = false;
   DevBuf rows;                     // rows with d+ >= 2
   uint64_t nrows = 0;
@@ -106,6 +107,12 @@ namespace tcx {
 // ---- primitives (prims.hip) ---------------------------------------------
 int sort_u64(tc_ctx* c, uint64_t* keys, uint64_t* tmp, uint64_t n, int end_bit,
              uint64_t** sorted);
+int sort_pairs_u64_u32(tc_ctx* c, uint64_t* keys, uint64_t* ktmp, uint32_t* vals,
+                       uint32_t* vtmp, uint64_t n, int end_bit, uint64_t** skeys,
+                       uint32_t** svals);
+int sort_pairs_u32_u32(tc_ctx* c, uint32_t* keys, uint32_t* ktmp, uint32_t* vals,
+                       uint32_t* vtmp, uint64_t n, int end_bit, uint32_t** skeys,
+                       uint32_t** svals);
 int unique_u64(tc_ctx* c, const uint64_t* in, uint64_t* out, uint64_t n,
                uint64_t* n_out);
 int exclusive_scan_u32(tc_ctx* c, const uint32_t* in, uint32_t* out, uint64_t n);

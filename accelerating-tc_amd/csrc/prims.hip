@@ -31,6 +31,38 @@ int sort_u64(tc_ctx* c, uint64_t* keys, uint64_t* tmp, uint64_t n, int end_bit,
   return TC_OK;
 }
 
+int sort_pairs_u64_u32(tc_ctx* c, uint64_t* keys, uint64_t* ktmp, uint32_t* vals,
+                       uint32_t* vtmp, uint64_t n, int end_bit, uint64_t** skeys,
+                       uint32_t** svals) {
+  *skeys = keys;
+  *svals = vals;
+  if (n <= 1) return TC_OK;
+  rocprim::double_buffer<uint64_t> kb(keys, ktmp);
+  rocprim::double_buffer<uint32_t> vb(vals, vtmp);
+  TCX_TRY(run_prim(c, [&](void* t, size_t& b) {
+    return rocprim::radix_sort_pairs(t, b, kb, vb, (size_t)n, 0u, (unsigned)end_bit, c->stream);
+  }));
+  *skeys = kb.current();
+  *svals = vb.current();
+  return TC_OK;
+}
+
+int sort_pairs_u32_u32(tc_ctx* c, uint32_t* keys, uint32_t* ktmp, uint32_t* vals,
+                       uint32_t* vtmp, uint64_t n, int end_bit, uint32_t** skeys,
+                       uint32_t** svals) {
+  *skeys = keys;
+  *svals = vals;
+  if (n <= 1) return TC_OK;
+  rocprim::double_buffer<uint32_t> kb(keys, ktmp);
+  rocprim::double_buffer<uint32_t> vb(vals, vtmp);
+  TCX_TRY(run_prim(c, [&](void* t, size_t& b) {
+    return rocprim::radix_sort_pairs(t, b, kb, vb, (size_t)n, 0u, (unsigned)end_bit, c->stream);
+  }));
+  *skeys = kb.current();
+  *svals = vb.current();
+  return TC_OK;
+}
+
 int unique_u64(tc_ctx* c, const uint64_t* in, uint64_t* out, uint64_t n,
                uint64_t* n_out) {
   if (n == 0) { *n_out = 0; return TC_OK; }

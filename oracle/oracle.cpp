@@ -505,6 +505,14 @@ uint64_t orc_count_rows(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
   return total;
 }
 
+uint64_t orc_count_rows_strided(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
+                                uint32_t start, uint32_t stride) {
+  uint64_t total = 0;
+  if (stride == 0) stride = 1;
+  for (uint64_t u = start; u < n; u += stride) total += vertex_tri(rowptr, col, (uint32_t)u);
+  return total;
+}
+
 void orc_count_per_vertex(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
                           uint64_t* tri) {
   for (uint32_t u = 0; u < n; ++u) tri[u] = vertex_tri(rowptr, col, u);

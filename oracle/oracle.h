@@ -110,6 +110,10 @@ uint64_t orc_count_omp(uint32_t n, const uint32_t* rowptr, const uint32_t* col);
 uint64_t orc_count_rows(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
                         uint32_t u_begin, uint32_t u_end);
 
+/* Forward counter over rows start, start+stride, ... (timed CPU samples). */
+uint64_t orc_count_rows_strided(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
+                                uint32_t start, uint32_t stride);
+
 /* Per-vertex triangle counts of the forward counter: tri[u] = sum over
  * v in N+(u) of |N+(u) & N+(v)|. */
 void orc_count_per_vertex(uint32_t n, const uint32_t* rowptr,

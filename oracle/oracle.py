@@ -53,6 +53,7 @@ _sig("orc_aorder", C.c_int, [_u32p, _u32p, C.c_uint64, C.c_uint32, _u32p, _u32p,
 _sig("orc_count", C.c_uint64, [C.c_uint32, _u32p, _u32p])
 _sig("orc_count_omp", C.c_uint64, [C.c_uint32, _u32p, _u32p])
 _sig("orc_count_rows", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32])
+_sig("orc_count_rows_strided", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32])
 _sig("orc_count_per_vertex", None, [C.c_uint32, _u32p, _u32p, _u64p])
 _sig("orc_kclique", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32])
 _sig("orc_hu_workload", None, [C.c_uint32, _u32p, _u32p, _i64p])
@@ -177,6 +178,12 @@ def count_omp(rp, col) -> int:
 def count_rows(rp, col, u_begin, u_end) -> int:
     rp = _u32(rp)
     return int(_lib.orc_count_rows(len(rp) - 1, rp, _nz(_u32(col), np.uint32), u_begin, u_end))
+
+
+def count_rows_strided(rp, col, start, stride) -> int:
+    rp = _u32(rp)
+    return int(_lib.orc_count_rows_strided(len(rp) - 1, rp, _nz(_u32(col), np.uint32), start,
+                                           stride))
 
 
 def count_per_vertex(rp, col):
