@@ -350,49 +350,47 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Small vertices (2 <= d+ <= 64): one wave per vertex, N+(u) in the wave's
-// LDS slot; lane j preloads the row bounds of the j-th out-neighbour.
-__global__ __launch_bounds__(TB) void k_vsmall(const uint32_t* __restrict__ rp,
-                                               const uint32_t* __restrict__ col,
-                                               const uint32_t* __restrict__ verts,
-                                               uint64_t nverts, unsigned long long* total) {
-  __shared__ uint32_t s_list[TB / 64][64];
-  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  uint32_t* L = s_list[wv];
-  const uint64_t wave = gtid() >> 6, nwaves = gstride() >> 6;
-  unsigned long long cnt = 0;
-  for (uint64_t i = wave; i < nverts; i += nwaves) {
-    const uint32_t u = verts[i];
-    const uint32_t ub = rp[u], du = rp[u + 1] - ub;
-    uint32_t mine = kEmpty, mvb = 0, mve = 0;
-    if (lane < du) {
-      mine = col[ub + lane];
-      mvb = rp[mine];
-      mve = rp[mine + 1];
-    }
-    L[lane] = mine;
-    wave_sync();
-    for (uint32_t j = 0; j < du; ++j) {
-      const uint32_t vb = __shfl(mvb, j, 64), ve = __shfl(mve, j, 64);
-      cnt += count_list(col, vb, ve, lane, L, du);
-    }
-    wave_sync();
+// Small vertex (2 <= d+ <= 64) on one wave: N+(u) in the wave's LDS slot L,
+// lane j preloads the row bounds of the j-th out-neighbour.
+__device__ inline uint32_t small_vertex(const uint32_t* __restrict__ rp,
+                                        const uint32_t* __restrict__ col, uint32_t u,
+                                        uint32_t lane, uint32_t* L) {
+  const uint32_t ub = rp[u], du = rp[u + 1] - ub;
+  uint32_t mine = kEmpty, mvb = 0, mve = 0;
+  if (lane < du) {
+    mine = col[ub + lane];
+    mvb = rp[mine];
+    mve = rp[mine + 1];
   }
-  flush_count(cnt, total);
+  L[lane] = mine;
+  wave_sync();
+  uint32_t c = 0;
+  for (uint32_t j = 0; j < du; ++j) {
+    const uint32_t vb = __shfl(mvb, j, 64), ve = __shfl(mve, j, 64);
+    c += count_list(col, vb, ve, lane, L, du);
+  }
+  wave_sync();
+  return c;
 }
 
-// Large vertices: a workgroup per vertex, taken from a work-sorted list
-// through an atomic cursor (persistent grid).  The block copies N+(u) into
-// LDS and stages the row bounds of its out-neighbours, so each wave's walk
-// over N+(v) lists waits only on the list loads themselves.  Lists longer
-// than the LDS copy are searched in global memory.
+// The AUTO count kernel (one launch, persistent grid):
+//  phase 1 - vertices with 64 < d+ <= CAP, in descending work order, one
+//            workgroup each via an atomic cursor.  The block copies N+(u)
+//            into LDS and stages the row bounds of its out-neighbours, so
+//            each wave's walk over N+(v) waits only on the list loads.
+//            Lists longer than CAP are searched in global memory.
+//  phase 2 - vertices with 2 <= d+ <= 64, one wave each, taken in chunks of
+//            kSmallChunk through a second cursor; these fill the tail of
+//            phase 1.
 constexpr int kStage = 1024;
+constexpr uint32_t kSmallChunk = 16;
 
 template <int CAP>
-__global__ __launch_bounds__(TB) void k_vlist(const uint32_t* __restrict__ rp,
-                                              const uint32_t* __restrict__ col,
-                                              const uint32_t* __restrict__ tasks, uint32_t ntasks,
-                                              uint32_t* cursor, unsigned long long* total) {
+__global__ __launch_bounds__(TB) void k_auto(const uint32_t* __restrict__ rp,
+                                             const uint32_t* __restrict__ col,
+                                             const uint32_t* __restrict__ large, uint32_t nlarge,
+                                             const uint32_t* __restrict__ small, uint32_t nsmall,
+                                             uint32_t* cursor, unsigned long long* total) {
   __shared__ uint32_t s_u[CAP];
   __shared__ uint32_t s_vb[kStage], s_ve[kStage];
   __shared__ uint32_t s_t;
@@ -402,8 +400,8 @@ __global__ __launch_bounds__(TB) void k_vlist(const uint32_t* __restrict__ rp,
     if (tid == 0) s_t = atomicAdd(cursor, 1u);
     __syncthreads();
     const uint32_t t = s_t;
-    if (t >= ntasks) break;
-    const uint32_t u = tasks[t];
+    if (t >= nlarge) break;
+    const uint32_t u = large[t];
     const uint32_t ub = rp[u], du = rp[u + 1] - ub;
     const bool in_lds = du <= (uint32_t)CAP;
     if (in_lds)
@@ -426,6 +424,16 @@ __global__ __launch_bounds__(TB) void k_vlist(const uint32_t* __restrict__ rp,
       }
       __syncthreads();
     }
+  }
+  // phase 2: the wave's slice of s_u holds one small list
+  uint32_t* L = s_u + wv * 64;
+  for (;;) {
+    uint32_t i0 = 0;
+    if (lane == 0) i0 = atomicAdd(cursor + 1, kSmallChunk);
+    i0 = __shfl(i0, 0, 64);
+    if (i0 >= nsmall) break;
+    const uint32_t i1 = min(nsmall, i0 + kSmallChunk);
+    for (uint32_t i = i0; i < i1; ++i) cnt += small_vertex(rp, col, small[i], lane, L);
   }
   flush_count(cnt, total);
 }
@@ -986,27 +994,20 @@ int tc_count(tc_ctx* c, tc_graph* g, const tc_sched* sched_in, uint64_t* count,
       const SchedCache& sc = g->sc;
       TCX_TRY(c->scratch.reserve(64));
       uint32_t* cursor = c->scratch.as<uint32_t>();
-      TCX_HIP(hipMemsetAsync(cursor, 0, 2 * sizeof(uint32_t), c->stream));
+      TCX_HIP(hipMemsetAsync(cursor, 0, 4 * sizeof(uint32_t), c->stream));
       const uint32_t G = s.grid_size ? s.grid_size : (uint32_t)c->num_cus * 8u;
-      if (sc.n_large) {
+      if (sc.n_large || sc.n_small) {
         TCX_TRY(lm.mark());
-        hipLaunchKernelGGL(k_vlist<2048>, dim3(std::min<uint64_t>(G, sc.n_large)), dim3(TB), 0,
-                           c->stream, rp, col, sc.auto_large.as<uint32_t>(), (uint32_t)sc.n_large,
-                           cursor, acc);
+        hipLaunchKernelGGL(k_auto<2048>, dim3(G), dim3(TB), 0, c->stream, rp, col,
+                           sc.auto_large.as<uint32_t>(), (uint32_t)sc.n_large,
+                           sc.auto_small.as<uint32_t>(), (uint32_t)sc.n_small, cursor, acc);
         tm.launches++;
       }
       if (sc.n_huge) {
         TCX_TRY(lm.mark());
-        hipLaunchKernelGGL(k_vlist<8192>, dim3(std::min<uint64_t>((uint64_t)c->num_cus * 4, sc.n_huge)),
+        hipLaunchKernelGGL(k_auto<8192>, dim3(std::min<uint64_t>((uint64_t)c->num_cus * 4, sc.n_huge)),
                            dim3(TB), 0, c->stream, rp, col, sc.auto_huge.as<uint32_t>(),
-                           (uint32_t)sc.n_huge, cursor + 1, acc);
-        tm.launches++;
-      }
-      if (sc.n_small) {
-        TCX_TRY(lm.mark());
-        unsigned grid = (unsigned)std::min<uint64_t>((uint64_t)c->num_cus * 16, (sc.n_small + 3) / 4);
-        hipLaunchKernelGGL(k_vsmall, dim3(grid), dim3(TB), 0, c->stream, rp, col,
-                           sc.auto_small.as<uint32_t>(), sc.n_small, acc);
+                           (uint32_t)sc.n_huge, (const uint32_t*)nullptr, 0u, cursor + 2, acc);
         tm.launches++;
       }
       break;

@@ -25,7 +25,7 @@ HBM_PEAK_GBPS = 8000.0
 KERNELS = ["hu", "bisson", "fox", "tricore", "auto"]
 KERNEL_SYMBOL = {
     "hu": "k_hu<2048, 1024>", "bisson": "k_bisson", "fox": "k_fox<L>",
-    "tricore": "k_tricore", "auto": "k_hu<2048, 1024> (+ k_hub)",
+    "tricore": "k_tricore", "auto": "k_auto<2048>",
 }
 
 WORKLOADS = {
@@ -82,7 +82,18 @@ def main():
     ap.add_argument("--workload", default="rmat22", choices=sorted(WORKLOADS))
     ap.add_argument("--kernels", default=",".join(KERNELS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30, "launches": 3}))
+        return
     if args.gpus != 1:
         print("bench.py: this engine measures one GPU; multi-GPU runs are not part of this "
               "build (see the sharding design for how the path would split).", file=sys.stderr)
