@@ -109,6 +109,9 @@ int tc_ctx_create(int device, tc_ctx** out) {
     c->total_mem = prop.totalGlobalMem;
     c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   }
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+    c->wall_clock_khz = khz;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     delete c;

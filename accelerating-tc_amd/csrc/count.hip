@@ -8,37 +8,13 @@
 #include <cmath>
 #include <cstring>
 
+#include "engine.h"
 #include "internal.h"
 
 namespace tcx {
 namespace {
 
-constexpr int TB = 256;
-
-__device__ inline uint64_t gtid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
-__device__ inline uint64_t gstride() { return (uint64_t)gridDim.x * blockDim.x; }
-
-__device__ inline unsigned long long wave_sum(unsigned long long v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
-
-__device__ inline void flush_count(unsigned long long cnt, unsigned long long* total) {
-  cnt = wave_sum(cnt);
-  if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(total, cnt);
-}
-
-// x in a[lo, hi) for ascending a
-__device__ inline bool contains(const uint32_t* a, uint32_t lo, uint32_t hi, uint32_t x) {
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    uint32_t v = a[mid];
-    if (v == x) return true;
-    if (v < x) lo = mid + 1;
-    else hi = mid;
-  }
-  return false;
-}
+using namespace dev;
 
 // largest e in [0, E) with pre[e] <= t  (pre ascending, pre[0] = 0 <= t)
 __device__ inline uint32_t owner_of(const uint32_t* pre, uint32_t E, uint32_t t) {
@@ -302,138 +278,6 @@ __global__ __launch_bounds__(TB) void k_hu(const uint32_t* __restrict__ rp,
     }
     __syncthreads();
     u = u2;
-  }
-  flush_count(cnt, total);
-}
-
-// ---- AUTO: vertex-centric, N+(u) as an LDS hash set ----------------------
-//
-// Multiplicative hashing with linear probing; the table holds N+(u) at load
-// <= 1/2, so a probe costs ~1.5 LDS reads on average.  Every wedge
-// u->v->w reads w from N+(v) with consecutive lanes on consecutive entries.
-constexpr uint32_t kEmpty = 0xFFFFFFFFu;
-
-// Branchless membership test in an ascending LDS list a[0, n): the loop
-// count depends on n only, so lanes of a wave never diverge.
-__device__ inline uint32_t lds_has(const uint32_t* a, uint32_t n, uint32_t x) {
-  const uint32_t* p = a;
-  uint32_t len = n;
-  while (len > 1) {
-    const uint32_t half = len >> 1;
-    p = p[half] <= x ? p + half : p;
-    len -= half;
-  }
-  return (n > 0 && *p == x) ? 1u : 0u;
-}
-
-// Streams N+(v) = col[vb, ve) with the wave's lanes, four 64-entry chunks in
-// flight per lane, and counts the entries present in the LDS list a[0, n).
-__device__ inline uint32_t count_list(const uint32_t* __restrict__ col, uint32_t vb, uint32_t ve,
-                                      uint32_t lane, const uint32_t* a, uint32_t n) {
-  uint32_t c = 0;
-  for (uint32_t k = vb + lane; k < ve; k += 256) {
-    const uint32_t w0 = col[k];
-    const uint32_t w1 = k + 64 < ve ? col[k + 64] : kEmpty;
-    const uint32_t w2 = k + 128 < ve ? col[k + 128] : kEmpty;
-    const uint32_t w3 = k + 192 < ve ? col[k + 192] : kEmpty;
-    c += lds_has(a, n, w0);
-    c += lds_has(a, n, w1);
-    c += lds_has(a, n, w2);
-    c += lds_has(a, n, w3);
-  }
-  return c;
-}
-
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Small vertex (2 <= d+ <= 64) on one wave: N+(u) in the wave's LDS slot L,
-// lane j preloads the row bounds of the j-th out-neighbour.
-__device__ inline uint32_t small_vertex(const uint32_t* __restrict__ rp,
-                                        const uint32_t* __restrict__ col, uint32_t u,
-                                        uint32_t lane, uint32_t* L) {
-  const uint32_t ub = rp[u], du = rp[u + 1] - ub;
-  uint32_t mine = kEmpty, mvb = 0, mve = 0;
-  if (lane < du) {
-    mine = col[ub + lane];
-    mvb = rp[mine];
-    mve = rp[mine + 1];
-  }
-  L[lane] = mine;
-  wave_sync();
-  uint32_t c = 0;
-  for (uint32_t j = 0; j < du; ++j) {
-    const uint32_t vb = __shfl(mvb, j, 64), ve = __shfl(mve, j, 64);
-    c += count_list(col, vb, ve, lane, L, du);
-  }
-  wave_sync();
-  return c;
-}
-
-// The AUTO count kernel (one launch, persistent grid):
-//  phase 1 - vertices with 64 < d+ <= CAP, in descending work order, one
-//            workgroup each via an atomic cursor.  The block copies N+(u)
-//            into LDS and stages the row bounds of its out-neighbours, so
-//            each wave's walk over N+(v) waits only on the list loads.
-//            Lists longer than CAP are searched in global memory.
-//  phase 2 - vertices with 2 <= d+ <= 64, one wave each, taken in chunks of
-//            kSmallChunk through a second cursor; these fill the tail of
-//            phase 1.
-constexpr int kStage = 1024;
-constexpr uint32_t kSmallChunk = 16;
-
-template <int CAP>
-__global__ __launch_bounds__(TB) void k_auto(const uint32_t* __restrict__ rp,
-                                             const uint32_t* __restrict__ col,
-                                             const uint32_t* __restrict__ large, uint32_t nlarge,
-                                             const uint32_t* __restrict__ small, uint32_t nsmall,
-                                             uint32_t* cursor, unsigned long long* total) {
-  __shared__ uint32_t s_u[CAP];
-  __shared__ uint32_t s_vb[kStage], s_ve[kStage];
-  __shared__ uint32_t s_t;
-  const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  unsigned long long cnt = 0;
-  for (;;) {
-    if (tid == 0) s_t = atomicAdd(cursor, 1u);
-    __syncthreads();
-    const uint32_t t = s_t;
-    if (t >= nlarge) break;
-    const uint32_t u = large[t];
-    const uint32_t ub = rp[u], du = rp[u + 1] - ub;
-    const bool in_lds = du <= (uint32_t)CAP;
-    if (in_lds)
-      for (uint32_t i = tid; i < du; i += TB) s_u[i] = col[ub + i];
-    for (uint32_t e0 = 0; e0 < du; e0 += kStage) {
-      const uint32_t E = min((uint32_t)kStage, du - e0);
-      for (uint32_t i = tid; i < E; i += TB) {
-        const uint32_t v = col[ub + e0 + i];
-        s_vb[i] = rp[v];
-        s_ve[i] = rp[v + 1];
-      }
-      __syncthreads();
-      for (uint32_t e = wv; e < E; e += TB / 64) {
-        const uint32_t vb = s_vb[e], ve = s_ve[e];
-        if (in_lds) {
-          cnt += count_list(col, vb, ve, lane, s_u, du);
-        } else {
-          for (uint32_t k = vb + lane; k < ve; k += 64) cnt += contains(col, ub, ub + du, col[k]);
-        }
-      }
-      __syncthreads();
-    }
-  }
-  // phase 2: the wave's slice of s_u holds one small list
-  uint32_t* L = s_u + wv * 64;
-  for (;;) {
-    uint32_t i0 = 0;
-    if (lane == 0) i0 = atomicAdd(cursor + 1, kSmallChunk);
-    i0 = __shfl(i0, 0, 64);
-    if (i0 >= nsmall) break;
-    const uint32_t i1 = min(nsmall, i0 + kSmallChunk);
-    for (uint32_t i = i0; i < i1; ++i) cnt += small_vertex(rp, col, small[i], lane, L);
   }
   flush_count(cnt, total);
 }

@@ -54,6 +54,7 @@ struct tc_ctx {
   hipEvent_t lev[17] = {};  // per-launch timing marks (created on first use)
   uint64_t total_mem = 0;
   int num_cus = 256;
+  int wall_clock_khz = 100000;  // device wall_clock64() rate
   tcx::DevBuf temp;     // rocprim temp storage
   tcx::DevBuf scratch;  // small per-call scratch (counters, bitmaps)
   tcx::DevBuf accum;    // uint64 accumulators
@@ -80,6 +81,10 @@ struct SchedCache {
   DevBuf fox_eids;                 // edge ids grouped by lanes-per-edge bin
   std::vector<uint64_t> fox_off;   // bin offsets (bins+1)
   float ms_prep = 0;
+  // learned schedule (per model id)
+  uint64_t learned_version = ~0ull, learned_model = ~0ull;
+  DevBuf l_blk, l_wav, l_huge;
+  uint64_t nl_blk = 0, nl_wav = 0, nl_huge = 0;
 };
 }  // namespace tcx
 

@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -112,6 +113,31 @@ _sig("tc_fox_bin_sizes", C.c_int, [_vp, _vp, C.c_uint32, _u64p])
 _sig("tc_count", C.c_int, [_vp, _vp, C.POINTER(Sched), C.POINTER(C.c_uint64),
                            C.POINTER(Timing)])
 _sig("tc_count_per_vertex", C.c_int, [_vp, _vp, _u64p])
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+class TrainOpts(C.Structure):
+    _fields_ = [("steps", C.c_uint32), ("batch", C.c_uint32), ("lr", C.c_float),
+                ("seed", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+
+
+_sig("tc_model_num_params", C.c_uint32, [])
+_sig("tc_model_create", C.c_int, [_vp, C.c_uint64, C.POINTER(_vp)])
+_sig("tc_model_destroy", C.c_int, [_vp])
+_sig("tc_model_set_params", C.c_int, [_vp, _f32p, C.c_uint32])
+_sig("tc_model_get_params", C.c_int, [_vp, _f32p, C.c_uint32])
+_sig("tc_model_set_norm", C.c_int, [_vp, _f32p, _f32p, _f32p, _f32p])
+_sig("tc_model_get_norm", C.c_int, [_vp, _f32p, _f32p, _f32p, _f32p])
+_sig("tc_model_forward", C.c_int, [_vp, _vp, _f32p, C.c_uint32, _f32p])
+_sig("tc_model_loss_grad", C.c_int, [_vp, _vp, _f32p, _f32p, _f32p, C.c_uint32,
+                                     C.POINTER(C.c_float), _f32p])
+_sig("tc_model_train", C.c_int, [_vp, _vp, _f32p, _f32p, _f32p, C.c_uint32,
+                                 C.POINTER(TrainOpts), C.POINTER(C.c_float)])
+_sig("tc_graph_features", C.c_int, [_vp, _vp, _f32p])
+_sig("tc_graph_measure_costs", C.c_int, [_vp, _vp, C.c_int, _f32p, C.POINTER(C.c_uint64)])
+_sig("tc_learned_assignment", C.c_int, [_vp, _vp, _vp, _u8p])
+_sig("tc_count_learned", C.c_int, [_vp, _vp, _vp, C.POINTER(C.c_uint64), C.POINTER(Timing)])
 _sig("kcl_count", C.c_int, [_vp, _vp, C.c_uint32, C.c_int, C.POINTER(C.c_uint64),
                             C.POINTER(Timing)])
 _sig("kcl_gpu_solver", C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_uint64), C.c_size_t])
@@ -133,8 +159,13 @@ EXPORTED = [
     "tc_hu_workload", "tc_hu_block_starts", "tc_fox_workload", "tc_fox_bin_sizes", "tc_count",
     "tc_count_per_vertex", "kcl_count", "kcl_gpu_solver", "tc_tricore_init_gpu",
     "tc_tricore_cal_degree", "tc_tricore_redirect_edges", "tc_tricore_cal_part_num",
-    "tc_tricore_split", "tc_tricore_tricount_gpu",
+    "tc_tricore_split", "tc_tricore_tricount_gpu", "tc_model_num_params", "tc_model_create",
+    "tc_model_destroy", "tc_model_set_params", "tc_model_get_params", "tc_model_set_norm",
+    "tc_model_get_norm", "tc_model_forward", "tc_model_loss_grad", "tc_model_train",
+    "tc_graph_features", "tc_graph_measure_costs", "tc_learned_assignment", "tc_count_learned",
 ]
+VARIANT_BLOCK, VARIANT_WAVE = 0, 1
+N_FEATURES, N_OUTPUTS = 8, 2
 
 
 class TCError(RuntimeError):
@@ -248,6 +279,8 @@ class Context:
         _check(_lib.tc_ctx_create(device, C.byref(h)), "tc_ctx_create")
         self._h = h
         self.device = device
+        # graphs and models live on this context; closing it closes them first
+        self._children = weakref.WeakSet()
 
     @property
     def handle(self):
@@ -259,7 +292,9 @@ class Context:
         return b.value
 
     def close(self):
-        if self._h:
+        if getattr(self, "_h", None):
+            for child in list(self._children):
+                child.close()
             _lib.tc_ctx_destroy(self._h)
             self._h = None
 
@@ -291,6 +326,7 @@ class Graph:
                                     dst.ctypes.data if m else None, m, n, C.byref(h)),
                "tc_graph_create")
         self._h = h
+        ctx._children.add(self)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -399,12 +435,122 @@ class Graph:
         _check(_lib.tc_count_per_vertex(self.ctx.handle, self._h, tri), "tc_count_per_vertex")
         return tri[:n]
 
+    def features(self):
+        n = self.info().n
+        X = np.zeros((max(n, 1), N_FEATURES), np.float32)
+        _check(_lib.tc_graph_features(self.ctx.handle, self._h, X.reshape(-1)), "tc_graph_features")
+        return X[:n]
+
+    def measure_costs(self, variant: int):
+        """(cost_us per vertex, exact count of the timed run)."""
+        n = self.info().n
+        c = np.zeros(max(n, 1), np.float32)
+        cnt = C.c_uint64()
+        _check(_lib.tc_graph_measure_costs(self.ctx.handle, self._h, variant, c, C.byref(cnt)),
+               "tc_graph_measure_costs")
+        return c[:n], cnt.value
+
+    def learned_assignment(self, model=None):
+        n = self.info().n
+        v = np.zeros(max(n, 1), np.uint8)
+        _check(_lib.tc_learned_assignment(self.ctx.handle, self._h,
+                                          model.handle if model else None, v),
+               "tc_learned_assignment")
+        return v[:n]
+
+    def count_learned(self, model=None) -> CountResult:
+        out = C.c_uint64()
+        t = Timing()
+        _check(_lib.tc_count_learned(self.ctx.handle, self._h, model.handle if model else None,
+                                     C.byref(out), C.byref(t)), "tc_count_learned")
+        return CountResult(out.value, t.as_dict())
+
     def kclique(self, k: int, variant: int = KCL_AUTO) -> CountResult:
         out = C.c_uint64()
         t = Timing()
         _check(_lib.kcl_count(self.ctx.handle, self._h, k, variant, C.byref(out), C.byref(t)),
                "kcl_count")
         return CountResult(out.value, t.as_dict())
+
+
+class Model:
+    """The AUTO engine's per-vertex cost model (8 -> 32 -> 32 -> 2 MLP)."""
+    n_params = int(_lib.tc_model_num_params())
+
+    def __init__(self, ctx: Context, seed: int = 0):
+        h = _vp()
+        _check(_lib.tc_model_create(ctx.handle, seed, C.byref(h)), "tc_model_create")
+        self._h = h
+        self.ctx = ctx
+        ctx._children.add(self)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.tc_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self):
+        p = np.zeros(self.n_params, np.float32)
+        _check(_lib.tc_model_get_params(self._h, p, self.n_params), "tc_model_get_params")
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(_lib.tc_model_set_params(self._h, p, len(p)), "tc_model_set_params")
+
+    @property
+    def norm(self):
+        xm, xs = np.zeros(N_FEATURES, np.float32), np.zeros(N_FEATURES, np.float32)
+        ym, ys = np.zeros(N_OUTPUTS, np.float32), np.zeros(N_OUTPUTS, np.float32)
+        _check(_lib.tc_model_get_norm(self._h, xm, xs, ym, ys), "tc_model_get_norm")
+        return xm, xs, ym, ys
+
+    @norm.setter
+    def norm(self, v):
+        xm, xs, ym, ys = (np.ascontiguousarray(a, np.float32) for a in v)
+        _check(_lib.tc_model_set_norm(self._h, xm, xs, ym, ys), "tc_model_set_norm")
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((max(len(X), 1), N_OUTPUTS), np.float32)
+        _check(_lib.tc_model_forward(self.ctx.handle, self._h, X.reshape(-1), len(X),
+                                     Y.reshape(-1)), "tc_model_forward")
+        return Y[:len(X)]
+
+    def loss_grad(self, X, T, M):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        M = np.ascontiguousarray(M, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_float()
+        _check(_lib.tc_model_loss_grad(self.ctx.handle, self._h, X.reshape(-1), T.reshape(-1),
+                                       M.reshape(-1), len(X), C.byref(loss), g),
+               "tc_model_loss_grad")
+        return float(loss.value), g
+
+    def train(self, X, T, M, steps=3000, batch=4096, lr=3e-3, seed=0) -> float:
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        M = np.ascontiguousarray(M, np.float32)
+        o = TrainOpts()
+        o.steps, o.batch, o.lr, o.seed = steps, batch, lr, seed
+        loss = C.c_float()
+        _check(_lib.tc_model_train(self.ctx.handle, self._h, X.reshape(-1), T.reshape(-1),
+                                   M.reshape(-1), len(X), C.byref(o), C.byref(loss)),
+               "tc_model_train")
+        return float(loss.value)
 
 
 def kcl_gpu_solver(path: str, k: int, n_chunk: int = 1) -> int:

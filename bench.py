@@ -22,16 +22,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "accelerating-tc_amd"))
 
 HBM_PEAK_GBPS = 8000.0
-KERNELS = ["hu", "bisson", "fox", "tricore", "auto"]
+KERNELS = ["hu", "bisson", "fox", "tricore", "auto", "learned"]
 KERNEL_SYMBOL = {
     "hu": "k_hu<2048, 1024>", "bisson": "k_bisson", "fox": "k_fox<L>",
-    "tricore": "k_tricore", "auto": "k_auto<2048>",
+    "tricore": "k_tricore", "auto": "k_auto<2048>", "learned": "k_auto<2048>",
 }
 
 WORKLOADS = {
     "rmat22": dict(kind="rmat", scale=22, ef=16, seed=22, aorder=True, orient="adir"),
     "rmat24": dict(kind="rmat", scale=24, ef=16, seed=24, aorder=True, orient="adir"),
     "er": dict(kind="er", n=65536, m=1048576, seed=1, aorder=False, orient="adir"),
+    "ba24": dict(kind="ba", n=1 << 24, k=8, seed=3, aorder=False, orient="adir"),
 }
 
 
@@ -42,7 +43,19 @@ def log(*a):
 def make_input(tcamd, w):
     if w["kind"] == "rmat":
         return tcamd.gen_rmat(w["scale"], w["ef"] << w["scale"], 0.57, 0.19, 0.19, w["seed"])
+    if w["kind"] == "ba":
+        return tcamd.gen_ba(w["n"], w["k"], w["seed"])
     return tcamd.gen_er(w["n"], w["m"], w["seed"])
+
+
+def describe(w):
+    if w["kind"] == "rmat":
+        d = f"R-MAT scale {w['scale']} ef{w['ef']} (0.57,0.19,0.19,0.05) seed {w['seed']}"
+    elif w["kind"] == "ba":
+        d = f"Barabasi-Albert n={w['n']} m={w['k']} seed {w['seed']}"
+    else:
+        d = f"Erdos-Renyi n={w['n']} m={w['m']} seed {w['seed']}"
+    return d + (", A-order + A-direction" if w["aorder"] else ", A-direction")
 
 
 def cpu_baseline(rp, col, target_s=15.0):
@@ -123,17 +136,27 @@ def main():
     log(f"[bench] n={info.n} m={m} max_out={info.max_out_degree} B_alg={info.bytes_alg/1e9:.2f} GB "
         f"prep {t_prep:.2f}s (simplify {t_simpl:.2f}s, aorder {t_aorder:.2f}s)")
 
+    model = None
+    if "learned" in kernels:
+        from tcamd import learned
+        model = learned.load(ctx) if os.path.exists(learned.DEFAULT_MODEL) else None
+        if model is None:
+            log("[bench] no trained cost model; 'learned' uses the analytic fallback")
+
+    def run(k):
+        return g.count_learned(model) if k == "learned" else g.count(k)
+
     results = {}
     counts = set()
     for k in kernels:
-        r = g.count(k)  # builds and caches the schedule
+        r = run(k)  # builds and caches the schedule
         counts.add(r.count)
         for _ in range(args.warmup):
-            g.count(k)
+            run(k)
         ev_ms, dom_ms = [], []
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            r = g.count(k)
+            r = run(k)
             ev_ms.append(r.timing["ms_count"])
             dom_ms.append(r.timing["ms_dominant"])
             counts.add(r.count)
@@ -149,7 +172,7 @@ def main():
     if len(counts) != 1:
         raise SystemExit(f"kernels disagree on the count: {sorted(counts)}")
 
-    head = "auto" if "auto" in results else kernels[0]
+    head = "learned" if "learned" in results else ("auto" if "auto" in results else kernels[0])
     hr = results[head]
     achieved = info.bytes_alg / (hr["ms_kernels"] * 1e-3) / 1e9
     traffic = None
@@ -178,10 +201,7 @@ def main():
         "dtype": "u32",
         "data": "synthetic",
         "config": {
-            "workload": (f"R-MAT scale {w['scale']} ef{w['ef']} (0.57,0.19,0.19,0.05) seed {w['seed']}"
-                         if w["kind"] == "rmat" else
-                         f"Erdos-Renyi n={w['n']} m={w['m']} seed {w['seed']}")
-                        + (", A-order + A-direction" if w["aorder"] else ", A-direction"),
+            "workload": describe(w),
             "n": info.n, "m": m, "count": hr["count"], "engine": head,
             "parallelism": "single GPU",
         },

@@ -189,6 +189,71 @@ int tc_count(tc_ctx* ctx, tc_graph* g, const tc_sched* sched, uint64_t* count,
 /* Per-vertex forward counts tri[u] (n entries), same kernels. */
 int tc_count_per_vertex(tc_ctx* ctx, tc_graph* g, uint64_t* tri);
 
+/* ---- learned scheduling of the AUTO engine ------------------------------
+ * No counterpart in the reference.  The reference's schedules come from
+ * fixed work formulas (Hu/tc.cu:22-35, Fox/tc.cu:18-33,316-331) and its
+ * orientation quality from fixed cost functions (tools/costFuncCal.cpp,
+ * tools/verification.cpp).  Here a small MLP (8 features -> 32 -> 32 -> 2,
+ * ReLU, BF16 weights, FP32 accumulate) predicts, per vertex, the log2 cost in
+ * microseconds of the two AUTO variants; it is trained on costs the engine
+ * measures on itself.  The schedule only decides WHICH variant counts a
+ * vertex and in what order; every vertex with d+ >= 2 is counted exactly
+ * once, so the count is exact whatever the model predicts. */
+#define TC_MODEL_FEATURES 8
+#define TC_MODEL_OUTPUTS 2
+#define TC_VARIANT_BLOCK 0   /* a workgroup per vertex                        */
+#define TC_VARIANT_WAVE 1    /* a wave per vertex, d+ <= 512                   */
+
+typedef struct tc_model tc_model;
+
+typedef struct tc_train_opts {
+  uint32_t steps;         /* Adam steps (default 3000)          */
+  uint32_t batch;         /* samples per step (default 4096)    */
+  float lr;               /* learning rate (default 3e-3)       */
+  uint32_t seed;          /* sample order seed                  */
+  uint32_t reserved[4];
+} tc_train_opts;
+
+uint32_t tc_model_num_params(void);
+int tc_model_create(tc_ctx* ctx, uint64_t seed, tc_model** out);
+int tc_model_destroy(tc_model* m);
+/* fp32 master parameters: W1[8][32], b1[32], W2[32][32], b2[32], W3[32][2], b3[2]
+ * (W[i][j] connects input i to output j). */
+int tc_model_set_params(tc_model* m, const float* params, uint32_t n);
+int tc_model_get_params(const tc_model* m, float* params, uint32_t n);
+/* Feature / target standardisation: x' = (x - x_mean) / x_std, y = y' * y_std + y_mean. */
+int tc_model_set_norm(tc_model* m, const float* x_mean, const float* x_std,
+                      const float* y_mean, const float* y_std);
+int tc_model_get_norm(const tc_model* m, float* x_mean, float* x_std, float* y_mean,
+                      float* y_std);
+/* Forward pass on host arrays: X[N][8] raw features -> Y[N][2] log2 cost (us). */
+int tc_model_forward(tc_ctx* ctx, tc_model* m, const float* X, uint32_t N, float* Y);
+/* Masked mean-squared error in standardised target units and its gradient
+ * with respect to the fp32 master parameters (the forward pass uses their
+ * BF16 rounding).  T[N][2] raw log2 costs, M[N][2] in {0,1}. */
+int tc_model_loss_grad(tc_ctx* ctx, tc_model* m, const float* X, const float* T,
+                       const float* M, uint32_t N, float* loss, float* grad);
+/* Sets the standardisation from the data, then trains with Adam on the GPU. */
+int tc_model_train(tc_ctx* ctx, tc_model* m, const float* X, const float* T, const float* M,
+                   uint32_t N, const tc_train_opts* opts, float* final_loss);
+/* Per-vertex features X[n][8] of the oriented CSR:
+ * log2(1+d+), log2(1+d-), log2(1+sum_v d+(v)), log2(1+max_v d+(v)),
+ * log2(1+sum_v min(d+(u),d+(v))), floor(log2(1+d+)) (degree bin id),
+ * fraction of out-neighbours with d+ < 64, log2(1+sum_v ceil(d+(v)/64)). */
+int tc_graph_features(tc_ctx* ctx, tc_graph* g, float* X);
+/* Measured per-vertex cost (microseconds) of one variant, from a timed run
+ * of that variant over every vertex with d+ >= 2 (wave: d+ <= 512);
+ * 0 where not measured.  count (optional) receives the run's exact count. */
+int tc_graph_measure_costs(tc_ctx* ctx, tc_graph* g, int variant, float* cost_us,
+                           uint64_t* count);
+/* Variant chosen for each vertex by the model (255 = not scheduled, d+ < 2);
+ * model NULL gives the fixed degree-bin rule (wave iff d+ <= 64). */
+int tc_learned_assignment(tc_ctx* ctx, tc_graph* g, tc_model* m, uint8_t* variant);
+/* Exact count with the learned schedule; model NULL falls back to the fixed
+ * degree-bin schedule of TC_KERNEL_AUTO. */
+int tc_count_learned(tc_ctx* ctx, tc_graph* g, tc_model* m, uint64_t* count,
+                     tc_timing* timing);
+
 /* ---- k-clique (k-clique/kcl.cu:16-215) -------------------------------- */
 int kcl_count(tc_ctx* ctx, tc_graph* g, uint32_t k, int variant, uint64_t* count,
               tc_timing* timing);
