@@ -1,0 +1,449 @@
+// Batched counting of many small graphs in one persistent launch.
+//
+// The batch is the disjoint union of its graphs, degree-oriented in one CSR
+// (each graph keeps its own relative order, so the union's orientation is
+// every graph's own).  The AUTO engine's block and wave variants count the
+// vertices; per-vertex counts are folded into per-graph uint64 accumulators.
+// The reset + count sequence is captured once in a hipGraph and replayed.
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <vector>
+
+#include "engine.h"
+#include "internal.h"
+
+namespace tcx {
+namespace {
+
+using namespace dev;
+
+__global__ void k_batch_reset(uint32_t* __restrict__ cursor, uint32_t ncur,
+                              unsigned long long* __restrict__ counts, uint32_t ngraphs) {
+  for (uint64_t i = gtid(); i < (uint64_t)ncur + ngraphs; i += gstride()) {
+    if (i < ncur) cursor[i] = 0;
+    else counts[i - ncur] = 0;
+  }
+}
+
+__device__ inline void flush_graph(unsigned long long& acc, uint32_t g,
+                                   unsigned long long* counts) {
+  const unsigned long long w = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0 && w) atomicAdd(counts + g, w);
+  acc = 0;
+}
+
+template <int CAP>
+__global__ __launch_bounds__(TB) void k_batch(const uint32_t* __restrict__ rp,
+                                              const uint32_t* __restrict__ col,
+                                              const uint32_t* __restrict__ gid,
+                                              const uint32_t* __restrict__ blk, uint32_t nblk,
+                                              const uint32_t* __restrict__ wav, uint32_t nwav,
+                                              uint32_t wave_chunk, uint32_t* cursor,
+                                              unsigned long long* counts) {
+  __shared__ BlockLds<CAP> S;
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (;;) {
+    if (tid == 0) S.t = atomicAdd(cursor, 1u);
+    __syncthreads();
+    const uint32_t t = S.t;
+    if (t >= nblk) break;
+    const uint32_t u = blk[t];
+    unsigned long long c = block_vertex<CAP>(rp, col, u, S);
+    flush_graph(c, gid[u], counts);
+  }
+  uint32_t* L = S.u + wv * (CAP / 4);
+  unsigned long long acc = 0;
+  uint32_t cur = 0xFFFFFFFFu;
+  for (;;) {
+    uint32_t i0 = 0;
+    if (lane == 0) i0 = atomicAdd(cursor + 1, wave_chunk);
+    i0 = __shfl(i0, 0, 64);
+    if (i0 >= nwav) break;
+    const uint32_t i1 = min(nwav, i0 + wave_chunk);
+    for (uint32_t i = i0; i < i1; ++i) {
+      const uint32_t u = wav[i];
+      const uint32_t g = gid[u];
+      if (g != cur) {
+        if (cur != 0xFFFFFFFFu) flush_graph(acc, cur, counts);
+        cur = g;
+      }
+      acc += wave_vertex(rp, col, u, lane, L);
+    }
+  }
+  if (cur != 0xFFFFFFFFu) flush_graph(acc, cur, counts);
+}
+
+__global__ void k_bflags(const uint32_t* __restrict__ rp, uint32_t n, uint32_t lo, uint32_t hi,
+                         uint8_t* __restrict__ f) {
+  for (uint64_t u = gtid(); u < n; u += gstride()) {
+    const uint32_t d = rp[u + 1] - rp[u];
+    f[u] = (d >= lo && d <= hi) ? 1 : 0;
+  }
+}
+
+__global__ void k_bwork(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+                        const uint32_t* __restrict__ verts, uint32_t nv, uint64_t* __restrict__ key,
+                        uint32_t* __restrict__ val) {
+  for (uint64_t i = gtid(); i < nv; i += gstride()) {
+    const uint32_t u = verts[i];
+    uint64_t w = rp[u + 1] - rp[u];
+    for (uint32_t k = rp[u]; k < rp[u + 1]; ++k) w += rp[col[k] + 1] - rp[col[k]];
+    key[i] = ~w;
+    val[i] = u;
+  }
+}
+
+// off[g] = first position of list whose vertex is >= voff[g] (list ascending)
+__global__ void k_graph_offsets(const uint32_t* __restrict__ list, uint32_t len,
+                                const uint32_t* __restrict__ voff, uint32_t ngraphs,
+                                uint32_t* __restrict__ off) {
+  for (uint64_t g = gtid(); g <= ngraphs; g += gstride()) {
+    const uint32_t x = voff[g];
+    uint32_t lo = 0, hi = len;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (list[mid] < x) lo = mid + 1;
+      else hi = mid;
+    }
+    off[g] = lo;
+  }
+}
+
+unsigned grid_for(uint64_t n, int tb = 256) {
+  uint64_t b = (n + tb - 1) / tb;
+  return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(b, 1u << 20));
+}
+
+struct BuildArgs {
+  const uint64_t* eoff;
+  const uint32_t* src;
+  const uint32_t* dst;
+  const uint32_t* nv;
+  const uint32_t* voff;
+  uint32_t* gs;
+  uint32_t* gd;
+  uint32_t* gidh;
+  std::atomic<int> bad;
+};
+
+void build_range(uint64_t b, uint64_t e, void* p) {
+  BuildArgs& A = *static_cast<BuildArgs*>(p);
+  for (uint64_t g = b; g < e; ++g) {
+    const uint32_t base = A.voff[g], n = A.nv[g];
+    for (uint64_t k = A.eoff[g]; k < A.eoff[g + 1]; ++k) {
+      if (A.src[k] >= n || A.dst[k] >= n) {
+        A.bad.store(1, std::memory_order_relaxed);
+        return;
+      }
+      A.gs[k] = A.src[k] + base;
+      A.gd[k] = A.dst[k] + base;
+    }
+    for (uint32_t v = 0; v < n; ++v) A.gidh[base + v] = (uint32_t)g;
+  }
+}
+
+}  // namespace
+}  // namespace tcx
+
+using namespace tcx;
+
+struct tc_batch {
+  tc_ctx* ctx = nullptr;
+  tc_graph* g = nullptr;
+  uint32_t ngraphs = 0;
+  std::vector<uint32_t> voff;
+  DevBuf d_voff, gid, counts, cursors;
+  // schedule for one wave_max_deg
+  uint32_t sched_deg = 0;
+  DevBuf blk_lpt, blk_id, wav;
+  uint64_t nblk = 0, nwav = 0;
+  DevBuf blk_off, wav_off;  // per-graph ranges of blk_id / wav
+  std::vector<uint32_t> h_blk_off, h_wav_off;
+  // captured replay
+  bool captured = false;
+  uint32_t cap_deg = 0, cap_chunk = 0, cap_grid = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void release_capture(tc_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->captured = false;
+}
+
+tc_batch_sched fill(const tc_batch_sched* s, tc_ctx* c) {
+  tc_batch_sched o;
+  memset(&o, 0, sizeof(o));
+  if (s) o = *s;
+  if (!o.wave_max_deg) o.wave_max_deg = 64;
+  if (o.wave_max_deg > kWaveMax) o.wave_max_deg = kWaveMax;
+  if (!o.wave_chunk) o.wave_chunk = 16;
+  if (!o.grid) o.grid = (uint32_t)c->num_cus * 8u;
+  return o;
+}
+
+int schedule(tc_ctx* c, tc_batch* b, uint32_t wave_max_deg) {
+  if (b->sched_deg == wave_max_deg) return TC_OK;
+  tc_graph* g = b->g;
+  const uint32_t n = g->n;
+  const uint32_t* rp = g->rowptr.as<uint32_t>();
+  DevBuf flags, key, ktmp, vtmp;
+  TCX_TRY(flags.reserve((size_t)n + 1));
+  TCX_TRY(b->wav.reserve(4 * ((size_t)n + 1)));
+  TCX_TRY(b->blk_id.reserve(4 * ((size_t)n + 1)));
+  TCX_TRY(b->blk_lpt.reserve(4 * ((size_t)n + 1)));
+  hipLaunchKernelGGL(k_bflags, dim3(grid_for(n)), dim3(256), 0, c->stream, rp, n, 2u, wave_max_deg,
+                     flags.as<uint8_t>());
+  TCX_TRY(select_indices(c, flags.as<uint8_t>(), n, b->wav.as<uint32_t>(), &b->nwav));
+  hipLaunchKernelGGL(k_bflags, dim3(grid_for(n)), dim3(256), 0, c->stream, rp, n, wave_max_deg + 1,
+                     0xFFFFFFFFu, flags.as<uint8_t>());
+  TCX_TRY(select_indices(c, flags.as<uint8_t>(), n, b->blk_id.as<uint32_t>(), &b->nblk));
+  // block tasks for the batched launch: descending work (longest first)
+  const uint64_t nb = b->nblk;
+  TCX_TRY(key.reserve(8 * (nb + 1)));
+  TCX_TRY(ktmp.reserve(8 * (nb + 1)));
+  TCX_TRY(vtmp.reserve(4 * (nb + 1)));
+  DevBuf vals;
+  TCX_TRY(vals.reserve(4 * (nb + 1)));
+  if (nb) {
+    hipLaunchKernelGGL(k_bwork, dim3(grid_for(nb)), dim3(256), 0, c->stream, rp,
+                       g->col.as<uint32_t>(), b->blk_id.as<uint32_t>(), (uint32_t)nb,
+                       key.as<uint64_t>(), vals.as<uint32_t>());
+    TCX_HIP(hipGetLastError());
+    uint64_t* sk;
+    uint32_t* sv;
+    TCX_TRY(sort_pairs_u64_u32(c, key.as<uint64_t>(), ktmp.as<uint64_t>(), vals.as<uint32_t>(),
+                               vtmp.as<uint32_t>(), nb, 64, &sk, &sv));
+    TCX_HIP(hipMemcpyAsync(b->blk_lpt.p, sv, 4 * nb, hipMemcpyDeviceToDevice, c->stream));
+  }
+  // per-graph ranges of the id-ordered lists
+  const uint32_t G = b->ngraphs;
+  TCX_TRY(b->blk_off.reserve(4 * ((size_t)G + 1)));
+  TCX_TRY(b->wav_off.reserve(4 * ((size_t)G + 1)));
+  hipLaunchKernelGGL(k_graph_offsets, dim3(grid_for(G + 1ull)), dim3(256), 0, c->stream,
+                     b->blk_id.as<uint32_t>(), (uint32_t)nb, b->d_voff.as<uint32_t>(), G,
+                     b->blk_off.as<uint32_t>());
+  hipLaunchKernelGGL(k_graph_offsets, dim3(grid_for(G + 1ull)), dim3(256), 0, c->stream,
+                     b->wav.as<uint32_t>(), (uint32_t)b->nwav, b->d_voff.as<uint32_t>(), G,
+                     b->wav_off.as<uint32_t>());
+  TCX_HIP(hipGetLastError());
+  b->h_blk_off.resize(G + 1);
+  b->h_wav_off.resize(G + 1);
+  TCX_HIP(hipMemcpyAsync(b->h_blk_off.data(), b->blk_off.p, 4 * ((size_t)G + 1),
+                         hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipMemcpyAsync(b->h_wav_off.data(), b->wav_off.p, 4 * ((size_t)G + 1),
+                         hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  b->sched_deg = wave_max_deg;
+  release_capture(b);
+  return TC_OK;
+}
+
+int enqueue_batched(tc_ctx* c, tc_batch* b, const tc_batch_sched& s) {
+  tc_graph* g = b->g;
+  hipLaunchKernelGGL(k_batch_reset, dim3(grid_for(2ull + b->ngraphs)), dim3(256), 0, c->stream,
+                     b->cursors.as<uint32_t>(), 2u, b->counts.as<unsigned long long>(), b->ngraphs);
+  hipLaunchKernelGGL(k_batch<2048>, dim3(s.grid), dim3(TB), 0, c->stream, g->rowptr.as<uint32_t>(),
+                     g->col.as<uint32_t>(), b->gid.as<uint32_t>(), b->blk_lpt.as<uint32_t>(),
+                     (uint32_t)b->nblk, b->wav.as<uint32_t>(), (uint32_t)b->nwav, s.wave_chunk,
+                     b->cursors.as<uint32_t>(), b->counts.as<unsigned long long>());
+  TCX_HIP(hipGetLastError());
+  return TC_OK;
+}
+
+int check_batch(tc_ctx* c, tc_batch* b) {
+  if (!c || !b) return fail(TC_ERR_ARG, "NULL argument");
+  if (b->ctx != c) return fail(TC_ERR_ARG, "batch belongs to another context");
+  TCX_HIP(hipSetDevice(c->device));
+  return TC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tc_batch_create(tc_ctx* c, uint32_t ngraphs, const uint64_t* eoff, const uint32_t* src,
+                    const uint32_t* dst, const uint32_t* nverts, tc_batch** out) {
+  if (!c || !out || !eoff || !nverts || ngraphs == 0) return fail(TC_ERR_ARG, "bad arguments");
+  *out = nullptr;
+  const uint64_t M = eoff[ngraphs];
+  if (M && (!src || !dst)) return fail(TC_ERR_ARG, "NULL edge arrays");
+  std::vector<uint32_t> voff(ngraphs + 1, 0);
+  uint64_t total = 0;
+  for (uint32_t g = 0; g < ngraphs; ++g) {
+    if (eoff[g + 1] < eoff[g]) return fail(TC_ERR_ARG, "edge offsets must be ascending");
+    voff[g] = (uint32_t)total;
+    total += nverts[g];
+    if (total >= 0x7FFFFFFFull) return fail(TC_ERR_RANGE, "batch exceeds 2^31 vertices");
+  }
+  voff[ngraphs] = (uint32_t)total;
+  std::vector<uint32_t> gs(std::max<uint64_t>(M, 1)), gd(std::max<uint64_t>(M, 1)),
+      gidh(std::max<uint64_t>(total, 1));
+  BuildArgs A{eoff, src, dst, nverts, voff.data(), gs.data(), gd.data(), gidh.data(), {0}};
+  parallel_for(ngraphs, build_range, &A);
+  if (A.bad.load()) return fail(TC_ERR_RANGE, "a record's id exceeds its graph's vertex count");
+  tc_batch* b = new tc_batch();
+  b->ctx = c;
+  b->ngraphs = ngraphs;
+  b->voff = voff;
+  int st = tc_graph_create(c, gs.data(), gd.data(), M, (uint32_t)total, &b->g);
+  if (!st) st = tc_graph_orient(c, b->g, TC_ORIENT_DEGREE, 0);
+  if (!st) st = b->gid.reserve(4 * std::max<uint64_t>(total, 1));
+  if (!st) st = b->d_voff.reserve(4 * ((size_t)ngraphs + 1));
+  if (!st) st = b->counts.reserve(8 * (size_t)ngraphs);
+  if (!st) st = b->cursors.reserve(8 * ((size_t)ngraphs + 2));
+  if (!st && total &&
+      hipMemcpyAsync(b->gid.p, gidh.data(), 4 * total, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    st = fail(TC_ERR_HIP, "gid upload failed");
+  if (!st && hipMemcpyAsync(b->d_voff.p, voff.data(), 4 * ((size_t)ngraphs + 1),
+                            hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    st = fail(TC_ERR_HIP, "offset upload failed");
+  if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = fail(TC_ERR_HIP, "sync failed");
+  if (st) {
+    tc_graph_destroy(b->g);
+    delete b;
+    return st;
+  }
+  *out = b;
+  return TC_OK;
+}
+
+int tc_batch_destroy(tc_batch* b) {
+  if (!b) return TC_OK;
+  if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
+  release_capture(b);
+  tc_graph_destroy(b->g);
+  delete b;
+  return TC_OK;
+}
+
+int tc_batch_info_get(const tc_batch* b, tc_graph_info* info) {
+  if (!b || !info) return fail(TC_ERR_ARG, "NULL argument");
+  return tc_graph_info_get(b->g, info);
+}
+
+int tc_batch_count(tc_ctx* c, tc_batch* b, const tc_batch_sched* sched, uint64_t* counts,
+                   tc_timing* timing) {
+  TCX_TRY(check_batch(c, b));
+  if (!counts) return fail(TC_ERR_ARG, "NULL counts");
+  const tc_batch_sched s = fill(sched, c);
+  tc_timing tm;
+  memset(&tm, 0, sizeof(tm));
+  TCX_HIP(hipEventRecord(c->ev0, c->stream));
+  TCX_TRY(schedule(c, b, s.wave_max_deg));
+  if (!b->captured || b->cap_chunk != s.wave_chunk || b->cap_grid != s.grid ||
+      b->cap_deg != s.wave_max_deg) {
+    release_capture(b);
+    TCX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int st = enqueue_batched(c, b, s);
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &gr);
+    if (st) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return st;
+    }
+    if (e != hipSuccess) return fail(TC_ERR_HIP, "stream capture failed: %s", hipGetErrorString(e));
+    b->graph = gr;
+    TCX_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->captured = true;
+    b->cap_chunk = s.wave_chunk;
+    b->cap_grid = s.grid;
+    b->cap_deg = s.wave_max_deg;
+  }
+  TCX_HIP(hipEventRecord(c->ev1, c->stream));
+  TCX_HIP(hipEventSynchronize(c->ev1));
+  TCX_HIP(hipEventElapsedTime(&tm.ms_prep, c->ev0, c->ev1));
+  TCX_HIP(hipEventRecord(c->ev0, c->stream));
+  TCX_HIP(hipGraphLaunch(b->exec, c->stream));
+  TCX_HIP(hipEventRecord(c->ev1, c->stream));
+  TCX_HIP(hipEventSynchronize(c->ev1));
+  TCX_HIP(hipEventElapsedTime(&tm.ms_count, c->ev0, c->ev1));
+  tm.ms_dominant = tm.ms_count;
+  tm.launches = 1;
+  TCX_HIP(hipMemcpyAsync(counts, b->counts.p, 8 * (size_t)b->ngraphs, hipMemcpyDeviceToHost,
+                         c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  if (timing) *timing = tm;
+  return TC_OK;
+}
+
+int tc_batch_count_pergraph(tc_ctx* c, tc_batch* b, const tc_batch_sched* sched,
+                            uint64_t* counts, tc_timing* timing) {
+  TCX_TRY(check_batch(c, b));
+  if (!counts) return fail(TC_ERR_ARG, "NULL counts");
+  const tc_batch_sched s = fill(sched, c);
+  tc_timing tm;
+  memset(&tm, 0, sizeof(tm));
+  TCX_TRY(schedule(c, b, s.wave_max_deg));
+  tc_graph* g = b->g;
+  const uint32_t G = b->ngraphs;
+  uint32_t* cur = b->cursors.as<uint32_t>();
+  hipLaunchKernelGGL(k_batch_reset, dim3(grid_for(2ull * G + G)), dim3(256), 0, c->stream, cur,
+                     2u * G, b->counts.as<unsigned long long>(), G);
+  TCX_HIP(hipGetLastError());
+  TCX_HIP(hipEventRecord(c->ev0, c->stream));
+  for (uint32_t q = 0; q < G; ++q) {
+    const uint32_t b0 = b->h_blk_off[q], b1 = b->h_blk_off[q + 1];
+    const uint32_t w0 = b->h_wav_off[q], w1 = b->h_wav_off[q + 1];
+    if (b0 == b1 && w0 == w1) continue;
+    const uint64_t need = (b1 - b0) + ((w1 - w0) + 3) / 4;
+    const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(s.grid, need));
+    hipLaunchKernelGGL(k_batch<2048>, dim3(grid), dim3(TB), 0, c->stream, g->rowptr.as<uint32_t>(),
+                       g->col.as<uint32_t>(), b->gid.as<uint32_t>(), b->blk_id.as<uint32_t>() + b0,
+                       b1 - b0, b->wav.as<uint32_t>() + w0, w1 - w0, s.wave_chunk, cur + 2 * q,
+                       b->counts.as<unsigned long long>());
+    tm.launches++;
+  }
+  TCX_HIP(hipGetLastError());
+  TCX_HIP(hipEventRecord(c->ev1, c->stream));
+  TCX_HIP(hipEventSynchronize(c->ev1));
+  TCX_HIP(hipEventElapsedTime(&tm.ms_count, c->ev0, c->ev1));
+  tm.ms_dominant = tm.ms_count;
+  TCX_HIP(hipMemcpyAsync(counts, b->counts.p, 8 * (size_t)G, hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  if (timing) *timing = tm;
+  return TC_OK;
+}
+
+int tc_batch_autotune(tc_ctx* c, tc_batch* b, tc_batch_sched* best) {
+  TCX_TRY(check_batch(c, b));
+  if (!best) return fail(TC_ERR_ARG, "NULL output");
+  std::vector<uint64_t> counts(b->ngraphs);
+  auto time_of = [&](const tc_batch_sched& s, float* ms) -> int {
+    tc_timing t;
+    float acc = 0;
+    TCX_TRY(tc_batch_count(c, b, &s, counts.data(), &t));  // capture + warm
+    for (int r = 0; r < 2; ++r) {
+      TCX_TRY(tc_batch_count(c, b, &s, counts.data(), &t));
+      acc += t.ms_count;
+    }
+    *ms = acc / 2;
+    return TC_OK;
+  };
+  tc_batch_sched s = fill(nullptr, c);
+  float best_ms = 1e30f;
+  tc_batch_sched bs = s;
+  for (uint32_t d : {16u, 32u, 64u, 128u, 256u, 512u}) {
+    s.wave_max_deg = d;
+    float ms;
+    TCX_TRY(time_of(s, &ms));
+    if (ms < best_ms) { best_ms = ms; bs = s; }
+  }
+  s = bs;
+  for (uint32_t ch : {4u, 8u, 32u, 64u}) {
+    s.wave_chunk = ch;
+    float ms;
+    TCX_TRY(time_of(s, &ms));
+    if (ms < best_ms) { best_ms = ms; bs = s; }
+  }
+  *best = bs;
+  return TC_OK;
+}
+
+}  // extern "C"

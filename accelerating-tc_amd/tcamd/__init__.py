@@ -150,6 +150,24 @@ _sig("tc_tricore_split", None, [_u64p, C.c_uint64, C.c_uint64, C.POINTER(C.POINT
 _sig("tc_tricore_tricount_gpu", C.c_uint64, [C.c_uint64, _vp, _u64p, _u32p,
                                              C.POINTER(C.POINTER(C.c_uint32))])
 
+
+class BatchSched(C.Structure):
+    _fields_ = [("wave_max_deg", C.c_uint32), ("wave_chunk", C.c_uint32), ("grid", C.c_uint32),
+                ("reserved", C.c_uint32 * 5)]
+
+    def as_dict(self):
+        return {"wave_max_deg": self.wave_max_deg, "wave_chunk": self.wave_chunk,
+                "grid": self.grid}
+
+
+_sig("tc_batch_create", C.c_int, [_vp, C.c_uint32, _u64p, _vp, _vp, _u32p, C.POINTER(_vp)])
+_sig("tc_batch_destroy", C.c_int, [_vp])
+_sig("tc_batch_info_get", C.c_int, [_vp, C.POINTER(GraphInfo)])
+_sig("tc_batch_autotune", C.c_int, [_vp, _vp, C.POINTER(BatchSched)])
+_sig("tc_batch_count", C.c_int, [_vp, _vp, C.POINTER(BatchSched), _u64p, C.POINTER(Timing)])
+_sig("tc_batch_count_pergraph", C.c_int, [_vp, _vp, C.POINTER(BatchSched), _u64p,
+                                          C.POINTER(Timing)])
+
 EXPORTED = [
     "tc_status_str", "tc_last_error", "tc_ctx_create", "tc_ctx_destroy", "tc_ctx_device_memory",
     "tc_file_edge_count", "tc_read_edges", "tc_write_edges", "tc_list2bin", "tc_bin2list",
@@ -163,6 +181,8 @@ EXPORTED = [
     "tc_model_destroy", "tc_model_set_params", "tc_model_get_params", "tc_model_set_norm",
     "tc_model_get_norm", "tc_model_forward", "tc_model_loss_grad", "tc_model_train",
     "tc_graph_features", "tc_graph_measure_costs", "tc_learned_assignment", "tc_count_learned",
+    "tc_batch_create", "tc_batch_destroy", "tc_batch_info_get", "tc_batch_autotune", "tc_batch_count",
+    "tc_batch_count_pergraph",
 ]
 VARIANT_BLOCK, VARIANT_WAVE = 0, 1
 N_FEATURES, N_OUTPUTS = 8, 2
@@ -312,6 +332,9 @@ class Context:
 
     def graph(self, src, dst, n: int = 0) -> "Graph":
         return Graph(self, src, dst, n)
+
+    def batch(self, graphs) -> "Batch":
+        return Batch(self, graphs)
 
 
 class Graph:
@@ -471,6 +494,85 @@ class Graph:
         _check(_lib.kcl_count(self.ctx.handle, self._h, k, variant, C.byref(out), C.byref(t)),
                "kcl_count")
         return CountResult(out.value, t.as_dict())
+
+
+class Batch:
+    """Many small graphs counted together: graphs is a sequence of
+    (src, dst, n) with local ids < n.  Each graph is degree-oriented."""
+
+    def __init__(self, ctx: Context, graphs):
+        graphs = list(graphs)
+        if not graphs:
+            raise ValueError("empty batch")
+        off = np.zeros(len(graphs) + 1, np.uint64)
+        nv = np.zeros(len(graphs), np.uint32)
+        for i, (s, d, n) in enumerate(graphs):
+            if len(s) != len(d):
+                raise ValueError(f"graph {i}: src and dst differ in length")
+            off[i + 1] = off[i] + len(s)
+            nv[i] = n
+        m = int(off[-1])
+        src = _u32(np.concatenate([g[0] for g in graphs])) if m else np.zeros(0, np.uint32)
+        dst = _u32(np.concatenate([g[1] for g in graphs])) if m else np.zeros(0, np.uint32)
+        h = _vp()
+        _check(_lib.tc_batch_create(ctx.handle, len(graphs), off,
+                                    src.ctypes.data if m else None,
+                                    dst.ctypes.data if m else None, nv, C.byref(h)),
+               "tc_batch_create")
+        self._h = h
+        self.ctx = ctx
+        self.ngraphs = len(graphs)
+        self.nedges = m
+        self.nverts = int(nv.sum(dtype=np.uint64))
+        ctx._children.add(self)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.tc_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _sched(sched):
+        if sched is None:
+            return None
+        if isinstance(sched, BatchSched):
+            return C.byref(sched)
+        s = BatchSched()
+        for k, v in dict(sched).items():
+            setattr(s, k, int(v))
+        return C.byref(s)
+
+    def info(self) -> GraphInfo:
+        i = GraphInfo()
+        _check(_lib.tc_batch_info_get(self._h, C.byref(i)), "tc_batch_info_get")
+        return i
+
+    def autotune(self) -> BatchSched:
+        s = BatchSched()
+        _check(_lib.tc_batch_autotune(self.ctx.handle, self._h, C.byref(s)), "tc_batch_autotune")
+        return s
+
+    def count(self, sched=None):
+        """Per-graph counts (uint64 array) and timing of one hipGraph replay."""
+        out = np.zeros(self.ngraphs, np.uint64)
+        t = Timing()
+        _check(_lib.tc_batch_count(self.ctx.handle, self._h, self._sched(sched), out,
+                                   C.byref(t)), "tc_batch_count")
+        return out, t.as_dict()
+
+    def count_pergraph(self, sched=None):
+        """Same counts with one kernel launch per graph."""
+        out = np.zeros(self.ngraphs, np.uint64)
+        t = Timing()
+        _check(_lib.tc_batch_count_pergraph(self.ctx.handle, self._h, self._sched(sched), out,
+                                            C.byref(t)), "tc_batch_count_pergraph")
+        return out, t.as_dict()
 
 
 class Model:

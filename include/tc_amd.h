@@ -254,6 +254,38 @@ int tc_learned_assignment(tc_ctx* ctx, tc_graph* g, tc_model* m, uint8_t* varian
 int tc_count_learned(tc_ctx* ctx, tc_graph* g, tc_model* m, uint64_t* count,
                      tc_timing* timing);
 
+/* ---- batched counting of many small graphs -----------------------------
+ * No counterpart in the reference (one graph per process run).  A batch is
+ * the disjoint union of its graphs in one degree-oriented CSR (orientation
+ * rules are local, so this equals orienting each graph on its own); one
+ * persistent kernel counts every graph and accumulates per-graph uint64
+ * counts.  The count sequence is captured once in a hipGraph and replayed. */
+typedef struct tc_batch_sched {
+  uint32_t wave_max_deg;  /* vertices with 2 <= d+ <= this run as wave tasks (64) */
+  uint32_t wave_chunk;    /* wave tasks fetched per cursor bump (16)              */
+  uint32_t grid;          /* persistent workgroups (0 = 8 per CU)                  */
+  uint32_t reserved[5];
+} tc_batch_sched;
+
+/* Graph g holds records [edge_off[g], edge_off[g+1]) of src/dst with local
+ * ids < nverts[g].  Orientation: TC_ORIENT_DEGREE per graph. */
+int tc_batch_create(tc_ctx* ctx, uint32_t ngraphs, const uint64_t* edge_off,
+                    const uint32_t* src, const uint32_t* dst, const uint32_t* nverts,
+                    tc_batch** out);
+int tc_batch_destroy(tc_batch* b);
+/* Sizes of the batch's union CSR (n, m+, algorithmic bytes of one count). */
+int tc_batch_info_get(const tc_batch* b, tc_graph_info* info);
+/* Tries wave_max_deg x wave_chunk candidates on the batch and returns the
+ * fastest (timed with hipEvents). */
+int tc_batch_autotune(tc_ctx* ctx, tc_batch* b, tc_batch_sched* best);
+/* Exact per-graph counts (ngraphs entries) from one replay of the captured
+ * hipGraph (capture happens on the first call per schedule). */
+int tc_batch_count(tc_ctx* ctx, tc_batch* b, const tc_batch_sched* sched, uint64_t* counts,
+                   tc_timing* timing);
+/* Same counts with one kernel launch per graph (the per-graph baseline). */
+int tc_batch_count_pergraph(tc_ctx* ctx, tc_batch* b, const tc_batch_sched* sched,
+                            uint64_t* counts, tc_timing* timing);
+
 /* ---- k-clique (k-clique/kcl.cu:16-215) -------------------------------- */
 int kcl_count(tc_ctx* ctx, tc_graph* g, uint32_t k, int variant, uint64_t* count,
               tc_timing* timing);
