@@ -238,6 +238,62 @@ int tc_list2bin(const char* text_path, const char* bin_path, uint64_t* m) {
   return TC_OK;
 }
 
+// Galois .gr binary CSR (versions 1 and 2): four uint64 header words
+// (version, edge-data size, nodes, edges), nodes uint64 row ends, then the
+// column ids (uint32 in v1, padded to 8 bytes; uint64 in v2), then edge data.
+int tc_read_gr(const char* path, uint32_t* src, uint32_t* dst, uint64_t cap, uint64_t* m,
+               uint32_t* n) {
+  if (!path || !m) return fail(TC_ERR_ARG, "NULL argument");
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(TC_ERR_IO, "cannot open %s: %s", path, strerror(errno));
+  auto done = [&](int st) {
+    fclose(f);
+    return st;
+  };
+  uint64_t hdr[4];
+  if (fread(hdr, 8, 4, f) != 4) return done(fail(TC_ERR_FORMAT, "%s: short .gr header", path));
+  const uint64_t ver = hdr[0], nodes = hdr[2], edges = hdr[3];
+  if (ver != 1 && ver != 2) return done(fail(TC_ERR_FORMAT, "%s: .gr version %llu", path,
+                                              (unsigned long long)ver));
+  if (nodes >= 0x7FFFFFFFull) return done(fail(TC_ERR_RANGE, "%s: too many nodes", path));
+  if (fseeko(f, 0, SEEK_END) != 0) return done(fail(TC_ERR_IO, "cannot seek %s", path));
+  const uint64_t size = (uint64_t)ftello(f);
+  const uint64_t colw = ver == 1 ? 4 : 8;
+  const uint64_t need = 32 + 8 * nodes + colw * edges;
+  if (size < need) return done(fail(TC_ERR_FORMAT, "%s: .gr file truncated", path));
+  *m = edges;
+  if (n) *n = (uint32_t)nodes;
+  if (!src) return done(TC_OK);  // size query
+  if (edges > cap) return done(fail(TC_ERR_RANGE, "%s holds %llu edges, capacity %llu", path,
+                                    (unsigned long long)edges, (unsigned long long)cap));
+  if (edges && !dst) return done(fail(TC_ERR_ARG, "NULL output arrays"));
+  std::vector<uint64_t> ends(nodes);
+  if (fseeko(f, 32, SEEK_SET) != 0 || fread(ends.data(), 8, nodes, f) != nodes)
+    return done(fail(TC_ERR_IO, "short read on %s", path));
+  uint64_t prev = 0;
+  for (uint64_t u = 0; u < nodes; ++u) {
+    if (ends[u] < prev || ends[u] > edges)
+      return done(fail(TC_ERR_FORMAT, "%s: row ends not ascending", path));
+    for (uint64_t k = prev; k < ends[u]; ++k) src[k] = (uint32_t)u;
+    prev = ends[u];
+  }
+  if (prev != edges) return done(fail(TC_ERR_FORMAT, "%s: row ends do not cover the edges", path));
+  const uint64_t kChunk = 1u << 20;
+  std::vector<uint64_t> buf(kChunk);
+  for (uint64_t k0 = 0; k0 < edges; k0 += kChunk) {
+    const uint64_t want = std::min(kChunk, edges - k0);
+    if (fread(buf.data(), colw, want, f) != want)
+      return done(fail(TC_ERR_IO, "short read on %s", path));
+    const uint32_t* b32 = reinterpret_cast<const uint32_t*>(buf.data());
+    for (uint64_t i = 0; i < want; ++i) {
+      const uint64_t v = ver == 1 ? b32[i] : buf[i];
+      if (v >= nodes) return done(fail(TC_ERR_FORMAT, "%s: column id beyond node count", path));
+      dst[k0 + i] = (uint32_t)v;
+    }
+  }
+  return done(TC_OK);
+}
+
 int tc_bin2list(const char* bin_path, const char* text_path, uint64_t* m) {
   if (!bin_path || !text_path) return fail(TC_ERR_ARG, "NULL argument");
   FILE* f = fopen(bin_path, "rb");

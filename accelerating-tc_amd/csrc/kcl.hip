@@ -316,13 +316,25 @@ int kcl_gpu_solver(const char* path, uint32_t k, uint64_t* total, size_t n_chunk
   (void)n_chunk;
   if (!path || !total) return fail(TC_ERR_ARG, "NULL argument");
   uint64_t m = 0;
-  TCX_TRY(tc_file_edge_count(path, &m));
-  std::vector<uint32_t> s(m), d(m);
-  TCX_TRY(tc_read_edges(path, s.data(), d.data(), m, &m));
+  uint32_t n = 0;
+  const size_t len = strlen(path);
+  const bool gr = len >= 3 && strcmp(path + len - 3, ".gr") == 0;
+  std::vector<uint32_t> s, d;
+  if (gr) {
+    TCX_TRY(tc_read_gr(path, nullptr, nullptr, 0, &m, &n));
+    s.resize(std::max<uint64_t>(m, 1));
+    d.resize(std::max<uint64_t>(m, 1));
+    TCX_TRY(tc_read_gr(path, s.data(), d.data(), m, &m, &n));
+  } else {
+    TCX_TRY(tc_file_edge_count(path, &m));
+    s.resize(std::max<uint64_t>(m, 1));
+    d.resize(std::max<uint64_t>(m, 1));
+    TCX_TRY(tc_read_edges(path, s.data(), d.data(), m, &m));
+  }
   tc_ctx* c = nullptr;
   TCX_TRY(tc_ctx_create(0, &c));
   tc_graph* g = nullptr;
-  int st = tc_graph_create(c, s.data(), d.data(), m, 0, &g);
+  int st = tc_graph_create(c, s.data(), d.data(), m, n, &g);
   if (st == TC_OK) st = tc_graph_orient(c, g, TC_ORIENT_KCORE, 0);
   if (st == TC_OK) st = kcl_count(c, g, k, KCL_VARIANT_AUTO, total, nullptr);
   tc_graph_destroy(g);

@@ -90,6 +90,8 @@ _sig("tc_read_edges", C.c_int, [C.c_char_p, _u32p, _u32p, C.c_uint64, C.POINTER(
 _sig("tc_write_edges", C.c_int, [C.c_char_p, _u32p, _u32p, C.c_uint64])
 _sig("tc_list2bin", C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_uint64)])
 _sig("tc_bin2list", C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_uint64)])
+_sig("tc_read_gr", C.c_int, [C.c_char_p, _vp, _vp, C.c_uint64, C.POINTER(C.c_uint64),
+                             C.POINTER(C.c_uint32)])
 _sig("tc_gen_er", C.c_int, [C.c_uint32, C.c_uint64, C.c_uint64, _u32p, _u32p])
 _sig("tc_gen_rmat", C.c_int, [C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_double,
                               C.c_uint64, _u32p, _u32p])
@@ -170,7 +172,7 @@ _sig("tc_batch_count_pergraph", C.c_int, [_vp, _vp, C.POINTER(BatchSched), _u64p
 
 EXPORTED = [
     "tc_status_str", "tc_last_error", "tc_ctx_create", "tc_ctx_destroy", "tc_ctx_device_memory",
-    "tc_file_edge_count", "tc_read_edges", "tc_write_edges", "tc_list2bin", "tc_bin2list",
+    "tc_file_edge_count", "tc_read_edges", "tc_write_edges", "tc_list2bin", "tc_bin2list", "tc_read_gr",
     "tc_gen_er", "tc_gen_rmat", "tc_gen_ba", "tc_gen_rgg", "tc_graph_create", "tc_graph_destroy",
     "tc_graph_simplify", "tc_graph_relabel_aorder", "tc_graph_orient", "tc_graph_info_get",
     "tc_graph_get_csr", "tc_graph_get_edges", "tc_graph_get_vertex_key", "tc_graph_device_csr",
@@ -246,6 +248,35 @@ def bin2list(bin_path: str, text_path: str) -> int:
     m = C.c_uint64()
     _check(_lib.tc_bin2list(bin_path.encode(), text_path.encode(), C.byref(m)), "tc_bin2list")
     return m.value
+
+
+def read_gr(path: str):
+    """(src, dst, n) of a Galois .gr file (one record per stored arc)."""
+    m, n = C.c_uint64(), C.c_uint32()
+    _check(_lib.tc_read_gr(path.encode(), None, None, 0, C.byref(m), C.byref(n)), "tc_read_gr")
+    s = np.empty(max(m.value, 1), np.uint32)
+    d = np.empty(max(m.value, 1), np.uint32)
+    _check(_lib.tc_read_gr(path.encode(), s.ctypes.data, d.ctypes.data, m.value, C.byref(m),
+                           C.byref(n)), "tc_read_gr")
+    return s[:m.value], d[:m.value], n.value
+
+
+def write_gr(path: str, src, dst, n: int):
+    """Writes a symmetric Galois .gr v1 file (both arcs of every record,
+    rows sorted, no edge data) - the k-clique CLI's input format."""
+    s, d = _u32(src).astype(np.int64), _u32(dst).astype(np.int64)
+    a = np.concatenate([s, d])
+    b = np.concatenate([d, s])
+    keep = a != b
+    key = np.unique(a[keep] * (1 << 32) + b[keep])
+    a, b = key >> 32, key & 0xFFFFFFFF
+    ends = np.cumsum(np.bincount(a, minlength=n)).astype(np.uint64)
+    with open(path, "wb") as f:
+        np.array([1, 0, n, len(b)], np.uint64).tofile(f)
+        ends.tofile(f)
+        b.astype(np.uint32).tofile(f)
+        if len(b) % 2:
+            np.zeros(1, np.uint32).tofile(f)
 
 
 def gen_er(n: int, m: int, seed: int):

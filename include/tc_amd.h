@@ -126,6 +126,11 @@ int tc_write_edges(const char* path, const uint32_t* src, const uint32_t* dst,
  * tools/bin2list.cpp:8-22). */
 int tc_list2bin(const char* text_path, const char* bin_path, uint64_t* m);
 int tc_bin2list(const char* bin_path, const char* text_path, uint64_t* m);
+/* Galois .gr binary CSR, versions 1 and 2 (the k-clique input,
+ * k-clique/kcl.cu:149 reads it through Galois' CSRGraph::read): one record
+ * per stored arc (u, v).  src = NULL queries *m (arcs) and *n (nodes). */
+int tc_read_gr(const char* path, uint32_t* src, uint32_t* dst, uint64_t cap, uint64_t* m,
+               uint32_t* n);
 
 /* ---- seeded synthetic generators (deterministic for any thread count) -- */
 /* Erdos-Renyi: exactly m unique undirected pairs, no self-loops. */
@@ -289,9 +294,11 @@ int tc_batch_count_pergraph(tc_ctx* ctx, tc_batch* b, const tc_batch_sched* sche
 /* ---- k-clique (k-clique/kcl.cu:16-215) -------------------------------- */
 int kcl_count(tc_ctx* ctx, tc_graph* g, uint32_t k, int variant, uint64_t* count,
               tc_timing* timing);
-/* File-level form of kcl_gpu_solver (k-clique/kcl.h:6-7): reads a list2bin
- * file, k-core orientation, counts k-cliques.  n_chunk is accepted for
- * signature parity (HBM holds every level; chunks are not needed). */
+/* File-level form of kcl_gpu_solver (k-clique/kcl.h:6-7): reads a Galois
+ * .gr file (path ending in ".gr", the input of k-clique/kcl_gpu.cpp:22-27)
+ * or a list2bin file, k-core orientation, counts k-cliques.  n_chunk is
+ * accepted for signature parity (HBM holds every level; chunks are not
+ * needed). */
 int kcl_gpu_solver(const char* path, uint32_t k, uint64_t* total, size_t n_chunk);
 
 /* ---- TriCore function API (tricore/include/tricount.h:7-17) ----------- */

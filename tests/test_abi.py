@@ -131,3 +131,48 @@ def test_context_without_gpu_fails_cleanly():
     with pytest.raises(tcamd.TCError) as e:
         tcamd.Context(0)
     assert e.value.status == tcamd.ERR_HIP
+
+
+def test_galois_gr_roundtrip(tmp_path):
+    rng = np.random.default_rng(4)
+    s = rng.integers(0, 50, 400).astype(np.uint32)
+    d = rng.integers(0, 50, 400).astype(np.uint32)
+    p = str(tmp_path / "g.gr")
+    tcamd.write_gr(p, s, d, 60)
+    rs, rd, n = tcamd.read_gr(p)
+    assert n == 60
+    a = np.concatenate([s, d]).astype(np.int64)
+    b = np.concatenate([d, s]).astype(np.int64)
+    keep = a != b
+    want = np.unique(a[keep] << 32 | b[keep])
+    np.testing.assert_array_equal(rs.astype(np.int64) << 32 | rd, want)
+    # every stored arc once per direction, rows ascending
+    assert np.all(np.diff(rs.astype(np.int64)) >= 0)
+
+
+def test_galois_gr_rejects_bad_files(tmp_path):
+    p = tmp_path / "bad.gr"
+    np.array([3, 0, 2, 1], np.uint64).tofile(p)
+    with pytest.raises(tcamd.TCError):
+        tcamd.read_gr(str(p))
+    np.array([1, 0, 2, 2, 1, 2], np.uint64).tofile(p)   # ends ok, columns missing
+    with pytest.raises(tcamd.TCError):
+        tcamd.read_gr(str(p))
+    np.concatenate([np.array([1, 0, 2, 2, 1, 2], np.uint64).view(np.uint32),
+                    np.array([1, 7], np.uint32)]).tofile(p)   # column 7 >= 2 nodes
+    with pytest.raises(tcamd.TCError):
+        tcamd.read_gr(str(p))
+
+
+def test_tool_drivers_roundtrip(tmp_path):
+    import subprocess
+    exe = os.path.join(ROOT, "accelerating-tc_amd", "bin")
+    txt = tmp_path / "g.txt"
+    txt.write_text("0 1\n1 2\n2 0\n7 3\n")
+    subprocess.run([os.path.join(exe, "list2bin"), str(txt), str(tmp_path / "g.bin")],
+                   check=True, capture_output=True)
+    s, d = tcamd.read_edges(str(tmp_path / "g.bin"))
+    assert s.tolist() == [0, 1, 2, 7] and d.tolist() == [1, 2, 0, 3]
+    subprocess.run([os.path.join(exe, "bin2list"), str(tmp_path / "g.bin"), str(tmp_path / "h.txt")],
+                   check=True, capture_output=True)
+    assert (tmp_path / "h.txt").read_text().split() == txt.read_text().split()
