@@ -68,7 +68,7 @@ __global__ __launch_bounds__(TB) void k_batch(const uint32_t* __restrict__ rp,
         if (cur != 0xFFFFFFFFu) flush_graph(acc, cur, counts);
         cur = g;
       }
-      acc += wave_vertex(rp, col, u, lane, L);
+      acc += wave_vertex(rp, col, u, lane, L, S.f[wv]);
     }
   }
   if (cur != 0xFFFFFFFFu) flush_graph(acc, cur, counts);

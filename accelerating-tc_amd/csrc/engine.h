@@ -77,12 +77,82 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave.
+__device__ inline uint32_t wave_incl_scan(uint32_t x, uint32_t lane) {
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  return x;
+}
+
+// Largest j in [0, 64) with F[j] <= f, for ascending F with F[0] <= f
+// (branchless, six LDS reads).
+__device__ inline uint32_t seg64(const uint32_t* F, uint32_t f) {
+  const uint32_t* p = F;
+  uint32_t len = 64;
+  while (len > 1) {
+    const uint32_t half = len >> 1;
+    p = p[half] <= f ? p + half : p;
+    len -= half;
+  }
+  return (uint32_t)(p - F);
+}
+
+// Each lane holds one list col[vb, ve) (vb == ve for none).  Counts the
+// entries of all 64 lists that are present in the ascending LDS list a[0, n).
+// Lists of kFlatMin entries or more are streamed one at a time by the whole
+// wave (count_list).  The shorter ones are concatenated: the wave walks the
+// concatenation 64 entries per step (two steps in flight), and each lane
+// finds the list its entry belongs to by a search over the exclusive prefix
+// of the lengths, kept with the list starts in the wave's LDS slot F[128].
+// Short lists thus fill every lane instead of len/64 of them.
+constexpr uint32_t kFlatMin = 128;
+
+__device__ inline uint32_t count_lists64(const uint32_t* __restrict__ col, uint32_t vb,
+                                         uint32_t ve, uint32_t lane, const uint32_t* a,
+                                         uint32_t n, uint32_t* F) {
+  const uint32_t len = ve - vb;
+  const bool lng = len >= kFlatMin;
+  uint32_t c = 0;
+  uint64_t mask = __ballot(lng);
+  while (mask) {
+    const uint32_t j = (uint32_t)__builtin_ctzll(mask);
+    mask &= mask - 1;
+    c += count_list(col, __shfl(vb, j, 64), __shfl(ve, j, 64), lane, a, n);
+  }
+  const uint32_t sl = lng ? 0u : len;
+  const uint32_t incl = wave_incl_scan(sl, lane);
+  const uint32_t total = __shfl(incl, 63, 64);
+  if (total == 0) return c;
+  F[lane] = incl - sl;
+  F[64 + lane] = vb;
+  wave_sync();
+  for (uint32_t f0 = 0; f0 < total; f0 += 128) {
+    const uint32_t fa = f0 + lane, fb = fa + 64;
+    uint32_t xa = kEmpty, xb = kEmpty;
+    if (fa < total) {
+      const uint32_t j = seg64(F, fa);
+      xa = col[F[64 + j] + (fa - F[j])];
+    }
+    if (fb < total) {
+      const uint32_t j = seg64(F, fb);
+      xb = col[F[64 + j] + (fb - F[j])];
+    }
+    c += lds_has(a, n, xa);
+    c += lds_has(a, n, xb);
+  }
+  wave_sync();  // F is rewritten by the next call
+  return c;
+}
+
 // Wave variant: one wave counts the wedges of u (d+(u) <= kWaveMax) with
-// N+(u) in the wave's LDS slot L; lane j preloads the row bounds of the
-// j-th out-neighbour of each 64-neighbour group.
+// N+(u) in the wave's LDS slot L; lane j takes the j-th out-neighbour of
+// each 64-neighbour group, whose lists go through count_lists64 (F: the
+// wave's 128-word slot).
 __device__ inline uint32_t wave_vertex(const uint32_t* __restrict__ rp,
                                        const uint32_t* __restrict__ col, uint32_t u,
-                                       uint32_t lane, uint32_t* L) {
+                                       uint32_t lane, uint32_t* L, uint32_t* F) {
   const uint32_t ub = rp[u], du = rp[u + 1] - ub;
   for (uint32_t i = lane; i < du; i += 64) L[i] = col[ub + i];
   wave_sync();
@@ -94,11 +164,7 @@ __device__ inline uint32_t wave_vertex(const uint32_t* __restrict__ rp,
       mvb = rp[v];
       mve = rp[v + 1];
     }
-    const uint32_t jn = min(64u, du - j0);
-    for (uint32_t j = 0; j < jn; ++j) {
-      const uint32_t vb = __shfl(mvb, j, 64), ve = __shfl(mve, j, 64);
-      c += count_list(col, vb, ve, lane, L, du);
-    }
+    c += count_lists64(col, mvb, mve, lane, L, du, F);
   }
   wave_sync();
   return c;
@@ -109,13 +175,15 @@ struct BlockLds {
   uint32_t u[CAP];        // N+(u); reused as 4 wave slots of CAP/4 in wave phases
   uint32_t vb[kStage];
   uint32_t ve[kStage];
+  uint32_t f[TB / 64][128];  // per-wave slots of count_lists64
   uint32_t t;
 };
 
 // Block variant: the whole workgroup counts the wedges of u.  N+(u) is copied
 // into LDS (when it fits), the row bounds of the out-neighbours are staged in
-// LDS, and wave w walks the lists of neighbours w, w+4, ...  Lists longer
-// than CAP are searched in global memory.  Every thread must call it.
+// LDS, and wave w takes the staged neighbours w, w+4, w+8, ... 64 at a time
+// (count_lists64).  Lists longer than CAP are searched in global memory.
+// Every thread must call it.
 template <int CAP>
 __device__ inline uint32_t block_vertex(const uint32_t* __restrict__ rp,
                                         const uint32_t* __restrict__ col, uint32_t u,
@@ -134,11 +202,21 @@ __device__ inline uint32_t block_vertex(const uint32_t* __restrict__ rp,
       S.ve[i] = rp[v + 1];
     }
     __syncthreads();
-    for (uint32_t e = wv; e < E; e += TB / 64) {
-      const uint32_t vb = S.vb[e], ve = S.ve[e];
-      if (in_lds) {
-        c += count_list(col, vb, ve, lane, S.u, du);
-      } else {
+    if (in_lds) {
+      // wave wv owns staged neighbours e = 4 * i + wv; 64 of them per group
+      const uint32_t mine = E > wv ? (E - wv + 3) / 4 : 0;
+      for (uint32_t g0 = 0; g0 < mine; g0 += 64) {
+        uint32_t vb = 0, ve = 0;
+        if (g0 + lane < mine) {
+          const uint32_t e = 4 * (g0 + lane) + wv;
+          vb = S.vb[e];
+          ve = S.ve[e];
+        }
+        c += count_lists64(col, vb, ve, lane, S.u, du, S.f[wv]);
+      }
+    } else {
+      for (uint32_t e = wv; e < E; e += TB / 64) {
+        const uint32_t vb = S.vb[e], ve = S.ve[e];
         for (uint32_t k = vb + lane; k < ve; k += 64) c += contains(col, ub, ub + du, col[k]);
       }
     }
@@ -176,7 +254,7 @@ __global__ __launch_bounds__(TB) void k_auto(const uint32_t* __restrict__ rp,
     i0 = __shfl(i0, 0, 64);
     if (i0 >= nwav) break;
     const uint32_t i1 = min(nwav, i0 + kWaveChunk);
-    for (uint32_t i = i0; i < i1; ++i) cnt += wave_vertex(rp, col, wav[i], lane, L);
+    for (uint32_t i = i0; i < i1; ++i) cnt += wave_vertex(rp, col, wav[i], lane, L, S.f[wv]);
   }
   flush_count(cnt, total);
 }
