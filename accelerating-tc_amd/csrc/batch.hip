@@ -43,6 +43,7 @@ __global__ __launch_bounds__(TB) void k_batch(const uint32_t* __restrict__ rp,
                                               unsigned long long* counts) {
   __shared__ BlockLds<CAP> S;
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  clear_bitmap(S);
   for (;;) {
     if (tid == 0) S.t = atomicAdd(cursor, 1u);
     __syncthreads();
@@ -52,7 +53,7 @@ __global__ __launch_bounds__(TB) void k_batch(const uint32_t* __restrict__ rp,
     unsigned long long c = block_vertex<CAP>(rp, col, u, S);
     flush_graph(c, gid[u], counts);
   }
-  uint32_t* L = S.u + wv * (CAP / 4);
+  const WaveSlot W = wave_slot(S, wv);
   unsigned long long acc = 0;
   uint32_t cur = 0xFFFFFFFFu;
   for (;;) {
@@ -68,7 +69,7 @@ __global__ __launch_bounds__(TB) void k_batch(const uint32_t* __restrict__ rp,
         if (cur != 0xFFFFFFFFu) flush_graph(acc, cur, counts);
         cur = g;
       }
-      acc += wave_vertex(rp, col, u, lane, L, S.f[wv]);
+      acc += wave_vertex(rp, col, u, lane, W);
     }
   }
   if (cur != 0xFFFFFFFFu) flush_graph(acc, cur, counts);

@@ -1,5 +1,18 @@
 // Device building blocks of the AUTO engine (vertex-centric, N+(u) in LDS),
-// shared by the count kernels and the cost-measurement kernels.
+// shared by the count kernels, the batched kernel and the cost-measurement
+// kernels.
+//
+// For a vertex u the work is the multiset X = concatenation of N+(v) over
+// v in N+(u); the count is |{x in X : x in N+(u)}|.  A wave
+//  1. generates the entries of X 64 at a time (whole lists streamed by the
+//     wave, or short lists concatenated across lanes),
+//  2. tests each entry against a hashed bitmap of N+(u) in LDS (one LDS read;
+//     most entries are not in N+(u) and stop here),
+//  3. appends the survivors to a per-wave queue in LDS (ballot + mbcnt), and
+//  4. once 64 survivors are queued, resolves them with full lanes by a
+//     branchless binary search of N+(u).
+// Step 2 replaces a log2(d+(u))-deep chain of dependent LDS reads per entry
+// by one read; step 3 keeps the searches at full lane occupancy.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -11,9 +24,11 @@ namespace dev {
 
 constexpr int TB = 256;                 // threads per workgroup (4 waves)
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
-constexpr int kStage = 1024;            // out-neighbour row bounds staged per pass
 constexpr uint32_t kWaveMax = 512;      // longest N+(u) the wave variant holds in LDS
 constexpr uint32_t kWaveChunk = 16;     // wave tasks taken per cursor bump
+constexpr uint32_t kQueue = 320;        // per-wave survivor queue (< 64 + 4 * 64 held)
+constexpr uint32_t kFlatMin = 64;       // lists this long or longer are streamed whole
+constexpr uint32_t kFilterMin = 16;     // wave variant: bitmap only for d+(u) above this
 
 __device__ inline uint64_t gtid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ inline uint64_t gstride() { return (uint64_t)gridDim.x * blockDim.x; }
@@ -53,24 +68,6 @@ __device__ inline uint32_t lds_has(const uint32_t* a, uint32_t n, uint32_t x) {
   return (n > 0 && *p == x) ? 1u : 0u;
 }
 
-// Streams N+(v) = col[vb, ve) with the wave's lanes, four 64-entry chunks in
-// flight per lane, and counts the entries present in the LDS list a[0, n).
-__device__ inline uint32_t count_list(const uint32_t* __restrict__ col, uint32_t vb, uint32_t ve,
-                                      uint32_t lane, const uint32_t* a, uint32_t n) {
-  uint32_t c = 0;
-  for (uint32_t k = vb + lane; k < ve; k += 256) {
-    const uint32_t w0 = col[k];
-    const uint32_t w1 = k + 64 < ve ? col[k + 64] : kEmpty;
-    const uint32_t w2 = k + 128 < ve ? col[k + 128] : kEmpty;
-    const uint32_t w3 = k + 192 < ve ? col[k + 192] : kEmpty;
-    c += lds_has(a, n, w0);
-    c += lds_has(a, n, w1);
-    c += lds_has(a, n, w2);
-    c += lds_has(a, n, w3);
-  }
-  return c;
-}
-
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -86,6 +83,12 @@ __device__ inline uint32_t wave_incl_scan(uint32_t x, uint32_t lane) {
   return x;
 }
 
+// Number of set bits of m below this lane.
+__device__ inline uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // Largest j in [0, 64) with F[j] <= f, for ascending F with F[0] <= f
 // (branchless, six LDS reads).
 __device__ inline uint32_t seg64(const uint32_t* F, uint32_t f) {
@@ -99,19 +102,88 @@ __device__ inline uint32_t seg64(const uint32_t* F, uint32_t f) {
   return (uint32_t)(p - F);
 }
 
-// Each lane holds one list col[vb, ve) (vb == ve for none).  Counts the
-// entries of all 64 lists that are present in the ascending LDS list a[0, n).
-// Lists of kFlatMin entries or more are streamed one at a time by the whole
-// wave (count_list).  The shorter ones are concatenated: the wave walks the
-// concatenation 64 entries per step (two steps in flight), and each lane
-// finds the list its entry belongs to by a search over the exclusive prefix
-// of the lengths, kept with the list starts in the wave's LDS slot F[128].
-// Short lists thus fill every lane instead of len/64 of them.
-constexpr uint32_t kFlatMin = 128;
+// Multiplicative hash of a vertex id to (32 - shift) bits.
+__device__ inline uint32_t fhash(uint32_t x, uint32_t shift) { return (x * 0x9E3779B1u) >> shift; }
 
-__device__ inline uint32_t count_lists64(const uint32_t* __restrict__ col, uint32_t vb,
-                                         uint32_t ve, uint32_t lane, const uint32_t* a,
-                                         uint32_t n, uint32_t* F) {
+__device__ inline void bm_set(uint32_t* bm, uint32_t x, uint32_t shift) {
+  const uint32_t h = fhash(x, shift);
+  atomicOr(bm + (h >> 5), 1u << (h & 31));
+}
+
+// Per-wave probe state for one vertex u (all fields wave-uniform).
+struct Probe {
+  const uint32_t* a;   // N+(u), ascending: LDS, or global when `global`
+  uint32_t n;          // d+(u)
+  const uint32_t* bm;  // hashed bitmap of N+(u) (LDS), used when `filt`
+  uint32_t shift;      // 32 - log2(bitmap bits)
+  uint32_t* Q;         // kQueue-entry survivor queue (LDS)
+  uint32_t qn;         // queued survivors
+  bool filt;
+  bool global;
+};
+
+__device__ inline uint32_t probe_search(const Probe& P, uint32_t x) {
+  if (!P.global) return lds_has(P.a, P.n, x);
+  return (x != kEmpty && contains(P.a, 0, P.n, x)) ? 1u : 0u;
+}
+
+// Appends x (kEmpty: nothing) to the queue when it passes the filter.
+__device__ inline void push(Probe& P, uint32_t x) {
+  bool pass = x != kEmpty;
+  if (P.filt && pass) {
+    const uint32_t h = fhash(x, P.shift);
+    pass = (P.bm[h >> 5] >> (h & 31)) & 1u;
+  }
+  const uint64_t m = __ballot(pass);
+  if (pass) P.Q[P.qn + lanes_below(m)] = x;
+  P.qn += (uint32_t)__popcll(m);
+}
+
+// Resolves queued survivors 64 at a time; `all` also resolves a partial tail.
+__device__ inline uint32_t drain(Probe& P, uint32_t lane, bool all) {
+  uint32_t c = 0;
+  if (P.qn >= 64 || (all && P.qn)) {
+    wave_sync();
+    while (P.qn >= 64) {
+      P.qn -= 64;
+      c += probe_search(P, P.Q[P.qn + lane]);
+    }
+    if (all && P.qn) {
+      c += probe_search(P, lane < P.qn ? P.Q[lane] : kEmpty);
+      P.qn = 0;
+    }
+    wave_sync();
+  }
+  return c;
+}
+
+// Streams col[vb, ve) with the whole wave, four 64-entry chunks in flight.
+__device__ inline uint32_t stream_list(const uint32_t* __restrict__ col, uint32_t vb, uint32_t ve,
+                                       uint32_t lane, Probe& P) {
+  uint32_t c = 0;
+  for (uint32_t k0 = vb; k0 < ve; k0 += 256) {
+    const uint32_t k = k0 + lane;
+    const uint32_t w0 = k < ve ? col[k] : kEmpty;
+    const uint32_t w1 = k + 64 < ve ? col[k + 64] : kEmpty;
+    const uint32_t w2 = k + 128 < ve ? col[k + 128] : kEmpty;
+    const uint32_t w3 = k + 192 < ve ? col[k + 192] : kEmpty;
+    push(P, w0);
+    push(P, w1);
+    push(P, w2);
+    push(P, w3);
+    c += drain(P, lane, false);
+  }
+  return c;
+}
+
+// Each lane holds one list col[vb, ve) (vb == ve for none).  Lists of
+// kFlatMin entries or more are streamed one at a time by the whole wave; the
+// shorter ones are concatenated and walked 64 entries per step (two steps in
+// flight), each lane finding the list of its entry by a search over the
+// exclusive prefix of the lengths, kept with the list starts in the wave's
+// LDS slot F[128].
+__device__ inline uint32_t lists64(const uint32_t* __restrict__ col, uint32_t vb, uint32_t ve,
+                                   uint32_t lane, uint32_t* F, Probe& P) {
   const uint32_t len = ve - vb;
   const bool lng = len >= kFlatMin;
   uint32_t c = 0;
@@ -119,7 +191,7 @@ __device__ inline uint32_t count_lists64(const uint32_t* __restrict__ col, uint3
   while (mask) {
     const uint32_t j = (uint32_t)__builtin_ctzll(mask);
     mask &= mask - 1;
-    c += count_list(col, __shfl(vb, j, 64), __shfl(ve, j, 64), lane, a, n);
+    c += stream_list(col, __shfl(vb, j, 64), __shfl(ve, j, 64), lane, P);
   }
   const uint32_t sl = lng ? 0u : len;
   const uint32_t incl = wave_incl_scan(sl, lane);
@@ -139,98 +211,127 @@ __device__ inline uint32_t count_lists64(const uint32_t* __restrict__ col, uint3
       const uint32_t j = seg64(F, fb);
       xb = col[F[64 + j] + (fb - F[j])];
     }
-    c += lds_has(a, n, xa);
-    c += lds_has(a, n, xb);
+    push(P, xa);
+    push(P, xb);
+    c += drain(P, lane, false);
   }
   wave_sync();  // F is rewritten by the next call
   return c;
 }
 
-// Wave variant: one wave counts the wedges of u (d+(u) <= kWaveMax) with
-// N+(u) in the wave's LDS slot L; lane j takes the j-th out-neighbour of
-// each 64-neighbour group, whose lists go through count_lists64 (F: the
-// wave's 128-word slot).
+// LDS of one wave for the wave variant.
+struct WaveSlot {
+  uint32_t* L;   // N+(u), kWaveMax entries
+  uint32_t* bm;  // hashed bitmap, 2^(32 - shift) bits, all zero between vertices
+  uint32_t shift;
+  uint32_t* F;   // 128 words
+  uint32_t* Q;   // kQueue words
+};
+
+// Wave variant: one wave counts the wedges of u (d+(u) <= kWaveMax).  Lane j
+// takes the j-th out-neighbour of each 64-neighbour group.  The bitmap is
+// built for d+(u) > kFilterMin and its words are cleared again at the end.
 __device__ inline uint32_t wave_vertex(const uint32_t* __restrict__ rp,
                                        const uint32_t* __restrict__ col, uint32_t u,
-                                       uint32_t lane, uint32_t* L, uint32_t* F) {
+                                       uint32_t lane, const WaveSlot& W) {
   const uint32_t ub = rp[u], du = rp[u + 1] - ub;
-  for (uint32_t i = lane; i < du; i += 64) L[i] = col[ub + i];
+  const bool filt = du > kFilterMin;
+  for (uint32_t i = lane; i < du; i += 64) {
+    const uint32_t x = col[ub + i];
+    W.L[i] = x;
+    if (filt) bm_set(W.bm, x, W.shift);
+  }
   wave_sync();
+  Probe P{W.L, du, W.bm, W.shift, W.Q, 0u, filt, false};
   uint32_t c = 0;
   for (uint32_t j0 = 0; j0 < du; j0 += 64) {
     uint32_t mvb = 0, mve = 0;
     if (j0 + lane < du) {
-      const uint32_t v = L[j0 + lane];
+      const uint32_t v = W.L[j0 + lane];
       mvb = rp[v];
       mve = rp[v + 1];
     }
-    c += count_lists64(col, mvb, mve, lane, L, du, F);
+    c += lists64(col, mvb, mve, lane, W.F, P);
   }
+  c += drain(P, lane, true);
+  if (filt)
+    for (uint32_t i = lane; i < du; i += 64) W.bm[fhash(W.L[i], W.shift) >> 5] = 0;
   wave_sync();
   return c;
 }
 
+// Workgroup LDS: N+(u) and its bitmap (CAP * 16 bits) for block tasks; in
+// wave phases both split into four per-wave slots.
 template <int CAP>
 struct BlockLds {
-  uint32_t u[CAP];        // N+(u); reused as 4 wave slots of CAP/4 in wave phases
-  uint32_t vb[kStage];
-  uint32_t ve[kStage];
-  uint32_t f[TB / 64][128];  // per-wave slots of count_lists64
+  uint32_t u[CAP];
+  uint32_t bm[CAP / 2];
+  uint32_t f[TB / 64][128];
+  uint32_t q[TB / 64][kQueue];
   uint32_t t;
+  static constexpr uint32_t kBlockShift = 32u - (uint32_t)__builtin_ctz((unsigned)CAP * 16u);
+  static constexpr uint32_t kWaveShift = 32u - (uint32_t)__builtin_ctz((unsigned)CAP * 4u);
+  static_assert(CAP / 4 >= (int)kWaveMax, "wave slots must hold kWaveMax entries");
 };
 
+template <int CAP>
+__device__ inline WaveSlot wave_slot(BlockLds<CAP>& S, uint32_t wv) {
+  return WaveSlot{S.u + wv * (CAP / 4), S.bm + wv * (CAP / 8), BlockLds<CAP>::kWaveShift,
+                  S.f[wv], S.q[wv]};
+}
+
+// Zeroes the bitmap region (once per workgroup, before the first vertex).
+template <int CAP>
+__device__ inline void clear_bitmap(BlockLds<CAP>& S) {
+  for (uint32_t i = threadIdx.x; i < (uint32_t)CAP / 2; i += TB) S.bm[i] = 0;
+  __syncthreads();
+}
+
 // Block variant: the whole workgroup counts the wedges of u.  N+(u) is copied
-// into LDS (when it fits), the row bounds of the out-neighbours are staged in
-// LDS, and wave w takes the staged neighbours w, w+4, w+8, ... 64 at a time
-// (count_lists64).  Lists longer than CAP are searched in global memory.
-// Every thread must call it.
+// into LDS (when it fits; longer lists are searched in global memory) and
+// hashed into the bitmap; wave w takes the out-neighbours w, w+4, w+8, ...
+// 64 at a time.  Every thread must call it; it ends with the bitmap clear
+// and a barrier.
 template <int CAP>
 __device__ inline uint32_t block_vertex(const uint32_t* __restrict__ rp,
                                         const uint32_t* __restrict__ col, uint32_t u,
                                         BlockLds<CAP>& S) {
+  constexpr uint32_t shift = BlockLds<CAP>::kBlockShift;
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const uint32_t ub = rp[u], du = rp[u + 1] - ub;
   const bool in_lds = du <= (uint32_t)CAP;
-  uint32_t c = 0;
-  if (in_lds)
-    for (uint32_t i = tid; i < du; i += TB) S.u[i] = col[ub + i];
-  for (uint32_t e0 = 0; e0 < du; e0 += kStage) {
-    const uint32_t E = min((uint32_t)kStage, du - e0);
-    for (uint32_t i = tid; i < E; i += TB) {
-      const uint32_t v = col[ub + e0 + i];
-      S.vb[i] = rp[v];
-      S.ve[i] = rp[v + 1];
-    }
-    __syncthreads();
-    if (in_lds) {
-      // wave wv owns staged neighbours e = 4 * i + wv; 64 of them per group
-      const uint32_t mine = E > wv ? (E - wv + 3) / 4 : 0;
-      for (uint32_t g0 = 0; g0 < mine; g0 += 64) {
-        uint32_t vb = 0, ve = 0;
-        if (g0 + lane < mine) {
-          const uint32_t e = 4 * (g0 + lane) + wv;
-          vb = S.vb[e];
-          ve = S.ve[e];
-        }
-        c += count_lists64(col, vb, ve, lane, S.u, du, S.f[wv]);
-      }
-    } else {
-      for (uint32_t e = wv; e < E; e += TB / 64) {
-        const uint32_t vb = S.vb[e], ve = S.ve[e];
-        for (uint32_t k = vb + lane; k < ve; k += 64) c += contains(col, ub, ub + du, col[k]);
-      }
-    }
-    __syncthreads();
+  for (uint32_t i = tid; i < du; i += TB) {
+    const uint32_t x = col[ub + i];
+    if (in_lds) S.u[i] = x;
+    bm_set(S.bm, x, shift);
   }
+  __syncthreads();
+  const uint32_t* a = in_lds ? S.u : col + ub;
+  Probe P{a, du, S.bm, shift, S.q[wv], 0u, true, !in_lds};
+  uint32_t c = 0;
+  const uint32_t mine = du > wv ? (du - wv + 3) / 4 : 0;
+  for (uint32_t g0 = 0; g0 < mine; g0 += 64) {
+    uint32_t vb = 0, ve = 0;
+    if (g0 + lane < mine) {
+      const uint32_t v = a[4 * (g0 + lane) + wv];
+      vb = rp[v];
+      ve = rp[v + 1];
+    }
+    c += lists64(col, vb, ve, lane, S.f[wv], P);
+  }
+  c += drain(P, lane, true);
+  __syncthreads();
+  for (uint32_t i = tid; i < du; i += TB) S.bm[fhash(a[i], shift) >> 5] = 0;
+  __syncthreads();
   return c;
 }
 
 // The AUTO count kernel (one launch, persistent grid):
 //  phase 1 - block tasks (vertex list `blk`, in the schedule's order), one
 //            workgroup each via cursor[0];
-//  phase 2 - wave tasks (list `wav`, d+ <= min(kWaveMax, CAP/4)), one wave
-//            each, kWaveChunk at a time via cursor[1]; they fill the tail
-//            left by phase 1.
+//  phase 2 - wave tasks (list `wav`, d+ <= kWaveMax), one wave each,
+//            kWaveChunk at a time via cursor[1]; they fill the tail left by
+//            phase 1.
 template <int CAP>
 __global__ __launch_bounds__(TB) void k_auto(const uint32_t* __restrict__ rp,
                                              const uint32_t* __restrict__ col,
@@ -239,6 +340,7 @@ __global__ __launch_bounds__(TB) void k_auto(const uint32_t* __restrict__ rp,
                                              uint32_t* cursor, unsigned long long* total) {
   __shared__ BlockLds<CAP> S;
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  clear_bitmap(S);
   unsigned long long cnt = 0;
   for (;;) {
     if (tid == 0) S.t = atomicAdd(cursor, 1u);
@@ -247,14 +349,14 @@ __global__ __launch_bounds__(TB) void k_auto(const uint32_t* __restrict__ rp,
     if (t >= nblk) break;
     cnt += block_vertex<CAP>(rp, col, blk[t], S);
   }
-  uint32_t* L = S.u + wv * (CAP / 4);
+  const WaveSlot W = wave_slot(S, wv);
   for (;;) {
     uint32_t i0 = 0;
     if (lane == 0) i0 = atomicAdd(cursor + 1, kWaveChunk);
     i0 = __shfl(i0, 0, 64);
     if (i0 >= nwav) break;
     const uint32_t i1 = min(nwav, i0 + kWaveChunk);
-    for (uint32_t i = i0; i < i1; ++i) cnt += wave_vertex(rp, col, wav[i], lane, L, S.f[wv]);
+    for (uint32_t i = i0; i < i1; ++i) cnt += wave_vertex(rp, col, wav[i], lane, W);
   }
   flush_count(cnt, total);
 }

@@ -252,6 +252,7 @@ __global__ __launch_bounds__(TB) void k_time_block(const uint32_t* __restrict__ 
                                                    unsigned long long* total) {
   __shared__ BlockLds<CAP> S;
   const uint32_t tid = threadIdx.x;
+  clear_bitmap(S);
   unsigned long long cnt = 0;
   for (;;) {
     // the task's cost includes fetching it (cursor atomic + barrier): for
@@ -275,9 +276,10 @@ __global__ __launch_bounds__(TB) void k_time_wave(const uint32_t* __restrict__ r
                                                   const uint32_t* __restrict__ verts, uint32_t n,
                                                   uint32_t* cursor, float* __restrict__ cost,
                                                   unsigned long long* total) {
-  __shared__ uint32_t slots[TB / 64][kWaveMax];
-  __shared__ uint32_t fslots[TB / 64][128];
+  __shared__ BlockLds<2048> S;
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  clear_bitmap(S);
+  const WaveSlot W = wave_slot(S, wv);
   unsigned long long cnt = 0;
   // tasks are fetched kWaveChunk at a time, as the count kernel does; each
   // task's cost carries its share of the fetch
@@ -289,7 +291,7 @@ __global__ __launch_bounds__(TB) void k_time_wave(const uint32_t* __restrict__ r
     if (i0 >= n) break;
     const uint32_t i1 = min(n, i0 + kWaveChunk);
     for (uint32_t i = i0; i < i1; ++i) {
-      cnt += wave_vertex(rp, col, verts[i], lane, slots[wv], fslots[wv]);
+      cnt += wave_vertex(rp, col, verts[i], lane, W);
       const uint64_t t1 = wall_clock64();
       if (lane == 0) cost[verts[i]] = (float)(t1 - t0);
       t0 = t1;
