@@ -591,15 +591,15 @@ static int gather_flags(tc_ctx* c, const uint8_t* in, const uint32_t* idx, uint3
   return TC_OK;
 }
 
-static int auto_schedule(tc_ctx* c, tc_graph* g) {
+static int auto_schedule(tc_ctx* c, tc_graph* g, uint32_t wmax, uint32_t hmin) {
   SchedCache& sc = g->sc;
-  if (sc.auto_ok) return TC_OK;
+  if (sc.auto_ok && sc.auto_wmax == wmax && sc.auto_hmin == hmin) return TC_OK;
   const uint32_t n = g->n;
   const uint32_t* rp = g->rowptr.as<uint32_t>();
   DevBuf flags, key, ktmp, ids, itmp;
   TCX_TRY(flags.reserve((size_t)n + 1));
   TCX_TRY(sc.auto_small.reserve(4 * ((size_t)n + 1)));
-  hipLaunchKernelGGL(k_row_flags, dim3(grid_for(n)), dim3(TB), 0, c->stream, rp, n, 2u, 64u,
+  hipLaunchKernelGGL(k_row_flags, dim3(grid_for(n)), dim3(TB), 0, c->stream, rp, n, 2u, wmax,
                      flags.as<uint8_t>());
   TCX_HIP(hipGetLastError());
   TCX_TRY(select_indices(c, flags.as<uint8_t>(), n, sc.auto_small.as<uint32_t>(), &sc.n_small));
@@ -621,20 +621,22 @@ static int auto_schedule(tc_ctx* c, tc_graph* g) {
   TCX_TRY(dflag.reserve((size_t)n + 1));
   TCX_TRY(sc.auto_large.reserve(4 * ((size_t)n + 1)));
   TCX_TRY(sc.auto_huge.reserve(4 * ((size_t)n + 1)));
-  hipLaunchKernelGGL(k_row_flags, dim3(grid_for(n)), dim3(TB), 0, c->stream, rp, n, 65u, 2048u,
-                     flags.as<uint8_t>());
+  hipLaunchKernelGGL(k_row_flags, dim3(grid_for(n)), dim3(TB), 0, c->stream, rp, n,
+                     std::max(wmax + 1, 2u), hmin, flags.as<uint8_t>());
   TCX_HIP(hipGetLastError());
   // flags are indexed by vertex; gather them into sorted order
   TCX_TRY(gather_flags(c, flags.as<uint8_t>(), sv, n, dflag.as<uint8_t>()));
   TCX_TRY(select_flagged_u32(c, sv, dflag.as<uint8_t>(), sc.auto_large.as<uint32_t>(), n,
                              &sc.n_large));
-  hipLaunchKernelGGL(k_row_flags, dim3(grid_for(n)), dim3(TB), 0, c->stream, rp, n, 2049u,
-                     0xFFFFFFFFu, flags.as<uint8_t>());
+  hipLaunchKernelGGL(k_row_flags, dim3(grid_for(n)), dim3(TB), 0, c->stream, rp, n,
+                     std::max(hmin + 1, 2u), 0xFFFFFFFFu, flags.as<uint8_t>());
   TCX_HIP(hipGetLastError());
   TCX_TRY(gather_flags(c, flags.as<uint8_t>(), sv, n, dflag.as<uint8_t>()));
   TCX_TRY(select_flagged_u32(c, sv, dflag.as<uint8_t>(), sc.auto_huge.as<uint32_t>(), n,
                              &sc.n_huge));
   sc.auto_ok = true;
+  sc.auto_wmax = wmax;
+  sc.auto_hmin = hmin;
   return TC_OK;
 }
 
@@ -744,9 +746,14 @@ int tc_count(tc_ctx* c, tc_graph* g, const tc_sched* sched_in, uint64_t* count,
       TCX_TRY(fox_schedule(c, g, L));
       break;
     }
-    case TC_KERNEL_AUTO:
-      TCX_TRY(auto_schedule(c, g));
+    case TC_KERNEL_AUTO: {
+      const uint32_t wmax = s.auto_thresholds[0] ? s.auto_thresholds[0] : 64u;
+      const uint32_t hmin = s.auto_thresholds[1] ? s.auto_thresholds[1] : 2048u;
+      if (wmax > kWaveMax) return fail(TC_ERR_ARG, "auto_thresholds[0] must be <= %u", kWaveMax);
+      if (hmin < wmax) return fail(TC_ERR_ARG, "auto_thresholds[1] must be >= auto_thresholds[0]");
+      TCX_TRY(auto_schedule(c, g, wmax, hmin));
       break;
+    }
     default:
       break;
   }

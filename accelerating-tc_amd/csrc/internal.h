@@ -70,6 +70,7 @@ struct SchedCache {
   uint32_t hu_nblocks = 0;
   DevBuf hu_start;                 // int32[nblocks+1]
   bool auto_ok = false;
+  uint32_t auto_wmax = 0, auto_hmin = 0;  // thresholds the AUTO lists were built for
   DevBuf auto_small;               // vertices with 2 <= d+ <= 64, id order
   DevBuf auto_large;               // d+ in (64, 2048], by descending work
   DevBuf auto_huge;                // d+ > 2048, by descending work

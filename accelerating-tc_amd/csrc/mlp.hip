@@ -323,28 +323,67 @@ __global__ void k_predicted_work(const uint32_t* __restrict__ rp, uint32_t n,
   if ((threadIdx.x & 63) == 0) atomicAdd(total, acc);
 }
 
-// wave_cap: log2 of the longest wave task (us) the schedule accepts; a
-// longer one would outlast the other waves and leave the machine idle.
+// A degree bin leaves the fixed rule's variant only for a predicted saving
+// above this fraction of its wave-time.
+constexpr float kSwitchMargin = 0.25f;
+
+__global__ void k_fill_f32(float* __restrict__ p, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.f;
+}
+
+// Per degree bin b = floor(log2 d+) (2 <= d+ <= kWaveMax): predicted total
+// wave-time (us) of the bin's vertices under each variant.  A block task
+// holds four waves for its duration, a wave task one.
+__global__ void k_bin_costs(const uint32_t* __restrict__ rp, uint32_t n,
+                            const float* __restrict__ Y, float* __restrict__ sums /*32 x 2*/) {
+  __shared__ float acc[32][2];
+  if (threadIdx.x < 64) acc[threadIdx.x >> 1][threadIdx.x & 1] = 0.f;
+  __syncthreads();
+  for (uint64_t u = gtid(); u < n; u += gstride()) {
+    const uint32_t d = rp[u + 1] - rp[u];
+    if (d < 2 || d > kWaveMax) continue;
+    const uint32_t b = 31u - (uint32_t)__builtin_clz(d);
+    atomicAdd(&acc[b][TC_VARIANT_BLOCK], exp2f(fminf(Y[u * O + TC_VARIANT_BLOCK] + 2.f, 40.f)));
+    atomicAdd(&acc[b][TC_VARIANT_WAVE], exp2f(fminf(Y[u * O + TC_VARIANT_WAVE], 40.f)));
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const float v = acc[threadIdx.x >> 1][threadIdx.x & 1];
+    if (v != 0.f) atomicAdd(sums + threadIdx.x, v);
+  }
+}
+
+// Variant per vertex: block unless its degree bin is in `wave_bins` (bins
+// whose vertices cost less wave-time as wave tasks) and its own predicted
+// wave time stays under `guard` (log2 us): a longer wave task would outlast
+// the other waves and leave the machine idle.  key: block tasks by
+// descending predicted cost.  wkey: wave tasks predicted above `big` first
+// (descending cost), then the rest in id order (neighbouring rows share
+// cache lines); other vertices sort last.
 __global__ void k_assign(const uint32_t* __restrict__ rp, uint32_t n, const float* __restrict__ Y,
-                         float wave_cap, uint8_t* __restrict__ var, uint64_t* __restrict__ key,
-                         uint32_t* ids) {
+                         uint32_t wave_bins, float guard, float big, uint8_t* __restrict__ var,
+                         uint64_t* __restrict__ key, uint64_t* __restrict__ wkey, uint32_t* ids,
+                         uint32_t* ids2) {
   for (uint64_t u = gtid(); u < n; u += gstride()) {
     const uint32_t d = rp[u + 1] - rp[u];
     uint8_t v = 255;
     float c = 0.f;
     if (d >= 2) {
-      // predicted log2 latencies -> log2 wave-time: a block task holds four
-      // waves for its duration, a wave task one
       const float cb = Y[u * O + TC_VARIANT_BLOCK] + 2.f, cw = Y[u * O + TC_VARIANT_WAVE];
-      const bool wave = d <= kWaveMax && cw < cb && cw <= wave_cap;
+      const uint32_t b = 31u - (uint32_t)__builtin_clz(d);
+      const bool wave = d <= kWaveMax && ((wave_bins >> b) & 1u) && cw <= guard;
       v = wave ? TC_VARIANT_WAVE : TC_VARIANT_BLOCK;
       c = wave ? cw : cb;
     }
     var[u] = v;
     // predicted log2 cost -> order-preserving integer, descending after sort
     const float clamped = fminf(fmaxf(c, -30.f), 60.f);
-    key[u] = ~(uint64_t)(uint32_t)((clamped + 32.f) * 16777216.f);
+    const uint64_t desc = ~(uint64_t)(uint32_t)((clamped + 32.f) * 16777216.f);
+    key[u] = desc;
+    wkey[u] = v != TC_VARIANT_WAVE ? ~0ull
+              : c > big ? (desc & ~(1ull << 63)) : ((1ull << 63) | (uint64_t)u);
     ids[u] = (uint32_t)u;
+    ids2[u] = (uint32_t)u;
   }
 }
 
@@ -478,10 +517,13 @@ int graph_features(tc_ctx* c, tc_graph* g, DevBuf& X) {
 }
 
 // per-vertex predicted [block, wave] log2 cost, or the fixed rule
-int assignment(tc_ctx* c, tc_graph* g, tc_model* md, DevBuf& var, DevBuf& key, DevBuf& ids) {
+int assignment(tc_ctx* c, tc_graph* g, tc_model* md, DevBuf& var, DevBuf& key, DevBuf& wkey,
+               DevBuf& ids, DevBuf& ids2) {
   const uint32_t n = g->n;
   TCX_TRY(var.reserve((size_t)n + 1));
   TCX_TRY(key.reserve(8 * ((size_t)n + 1)));
+  TCX_TRY(wkey.reserve(8 * ((size_t)n + 1)));
+  TCX_TRY(ids2.reserve(4 * ((size_t)n + 1)));
   TCX_TRY(ids.reserve(4 * ((size_t)n + 1)));
   DevBuf Y;
   TCX_TRY(Y.reserve(4ull * O * ((size_t)n + 1)));
@@ -498,24 +540,50 @@ int assignment(tc_ctx* c, tc_graph* g, tc_model* md, DevBuf& var, DevBuf& key, D
     hipLaunchKernelGGL(k_fixed_costs, dim3(grid_for(n)), dim3(256), 0, c->stream,
                        g->rowptr.as<uint32_t>(), X.as<float>(), n, Y.as<float>());
   }
-  // expected kernel time = predicted wave-time / resident waves; a wave task
-  // (fetched kWaveChunk at a time) may take at most 1/kWaveChunk of it
+  // per-bin variant: the one with the smaller predicted total wave-time
+  TCX_TRY(c->prim_out.reserve(64 * 4 + 64));
+  float* dsum = c->prim_out.as<float>();
+  hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, c->stream, dsum, 65u);
+  hipLaunchKernelGGL(k_bin_costs, dim3(std::min<unsigned>(grid_for(n), 2048)), dim3(256), 0,
+                     c->stream, g->rowptr.as<uint32_t>(), n, Y.as<float>(), dsum);
+  uint32_t wave_bins = 0;
   float wave_cap = 1e30f;
+  {
+    float sums[64];
+    TCX_HIP(hipMemcpyAsync(sums, dsum, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+    TCX_HIP(hipStreamSynchronize(c->stream));
+    // the fixed rule (wave iff d+ <= 64, bins 1..5 and d+ = 64 of bin 6) is
+    // the prior: a bin changes variant only when the prediction says the
+    // other one saves more than kSwitchMargin of the bin's wave-time
+    for (int b = 0; b < 32; ++b) {
+      const float w = sums[2 * b + TC_VARIANT_WAVE], bl = sums[2 * b + TC_VARIANT_BLOCK];
+      if (!(w > 0.f)) continue;
+      const bool prior_wave = b <= 5;
+      const bool wave = prior_wave ? !(bl < (1.f - kSwitchMargin) * w)
+                                   : (w < (1.f - kSwitchMargin) * bl);
+      if (wave) wave_bins |= 1u << b;
+    }
+  }
+  // per-wave share of the predicted wave-time (log2 us): a wave task may take
+  // at most half of it; tasks above 1/16 of it (a fetch holds kWaveChunk
+  // tasks) go first so that they cannot end up in the tail
+  float guard = 1e30f, big = 1e30f;
   if (md) {
-    TCX_TRY(c->prim_out.reserve(64));
-    float* dtot = c->prim_out.as<float>();
-    TCX_HIP(hipMemsetAsync(dtot, 0, 4, c->stream));
+    float* dtot = c->prim_out.as<float>() + 64;
     hipLaunchKernelGGL(k_predicted_work, dim3(std::min<unsigned>(grid_for(n), 2048)), dim3(256), 0,
                        c->stream, g->rowptr.as<uint32_t>(), n, Y.as<float>(), dtot);
     float tot = 0;
     TCX_HIP(hipMemcpyAsync(&tot, dtot, 4, hipMemcpyDeviceToHost, c->stream));
     TCX_HIP(hipStreamSynchronize(c->stream));
     const double waves = (double)c->num_cus * 32.0;
-    wave_cap = (float)std::log2(std::max(1e-3, tot / waves / kWaveChunk));
+    const float share = (float)std::log2(std::max(1e-3, tot / waves));
+    guard = share - 1.f;
+    big = share - std::log2((float)kWaveChunk);
   }
   hipLaunchKernelGGL(k_assign, dim3(grid_for(n)), dim3(256), 0, c->stream,
-                     g->rowptr.as<uint32_t>(), n, Y.as<float>(), wave_cap, var.as<uint8_t>(),
-                     key.as<uint64_t>(), ids.as<uint32_t>());
+                     g->rowptr.as<uint32_t>(), n, Y.as<float>(), wave_bins, guard, big,
+                     var.as<uint8_t>(), key.as<uint64_t>(), wkey.as<uint64_t>(), ids.as<uint32_t>(),
+                     ids2.as<uint32_t>());
   TCX_HIP(hipGetLastError());
   TCX_HIP(hipStreamSynchronize(c->stream));
   return TC_OK;
@@ -526,8 +594,8 @@ int learned_schedule(tc_ctx* c, tc_graph* g, tc_model* md) {
   const uint64_t mid = md ? md->id : 0;
   if (L.learned_version == g->sched_version && L.learned_model == mid) return TC_OK;
   const uint32_t n = g->n;
-  DevBuf var, key, ids, ktmp, itmp, flag;
-  TCX_TRY(assignment(c, g, md, var, key, ids));
+  DevBuf var, key, wkey, ids, ids2, ktmp, itmp, flag;
+  TCX_TRY(assignment(c, g, md, var, key, wkey, ids, ids2));
   TCX_TRY(ktmp.reserve(8 * ((size_t)n + 1)));
   TCX_TRY(itmp.reserve(4 * ((size_t)n + 1)));
   TCX_TRY(flag.reserve((size_t)n + 1));
@@ -542,14 +610,22 @@ int learned_schedule(tc_ctx* c, tc_graph* g, tc_model* md) {
   hipLaunchKernelGGL(k_pick, dim3(grid_for(n)), dim3(256), 0, c->stream, var.as<uint8_t>(), order,
                      rp, n, (uint8_t)TC_VARIANT_BLOCK, 2048u, flag.as<uint8_t>());
   TCX_TRY(select_flagged_u32(c, order, flag.as<uint8_t>(), L.l_blk.as<uint32_t>(), n, &L.nl_blk));
-  // wave tasks stay in id order: many short tasks gain more from the
-  // locality of neighbouring rows than from a cost order
+  // wave tasks: the predicted-long ones first, the rest in id order (many
+  // short tasks gain more from the locality of neighbouring rows than from a
+  // cost order); the count comes from the flags, the order from wkey
   hipLaunchKernelGGL(k_pick_by_id, dim3(grid_for(n)), dim3(256), 0, c->stream, var.as<uint8_t>(),
                      rp, n, (uint8_t)TC_VARIANT_WAVE, kWaveMax, flag.as<uint8_t>());
   TCX_TRY(select_indices(c, flag.as<uint8_t>(), n, L.l_wav.as<uint32_t>(), &L.nl_wav));
   hipLaunchKernelGGL(k_pick_huge, dim3(grid_for(n)), dim3(256), 0, c->stream, var.as<uint8_t>(),
                      order, rp, n, 2048u, flag.as<uint8_t>());
   TCX_TRY(select_flagged_u32(c, order, flag.as<uint8_t>(), L.l_huge.as<uint32_t>(), n, &L.nl_huge));
+  // (after the last use of `order`: this sort reuses its temporaries)
+  uint64_t* swk;
+  uint32_t* worder;
+  TCX_TRY(sort_pairs_u64_u32(c, wkey.as<uint64_t>(), ktmp.as<uint64_t>(), ids2.as<uint32_t>(),
+                             itmp.as<uint32_t>(), n, 64, &swk, &worder));
+  if (L.nl_wav)
+    TCX_HIP(hipMemcpyAsync(L.l_wav.p, worder, 4 * L.nl_wav, hipMemcpyDeviceToDevice, c->stream));
   TCX_HIP(hipGetLastError());
   L.learned_version = g->sched_version;
   L.learned_model = mid;
@@ -837,8 +913,8 @@ int tc_learned_assignment(tc_ctx* c, tc_graph* g, tc_model* md, uint8_t* variant
   if (g->orient_mode < 0) return fail(TC_ERR_STATE, "graph must be oriented");
   TCX_HIP(hipSetDevice(c->device));
   if (g->n == 0) return TC_OK;
-  DevBuf var, key, ids;
-  TCX_TRY(assignment(c, g, md, var, key, ids));
+  DevBuf var, key, wkey, ids, ids2;
+  TCX_TRY(assignment(c, g, md, var, key, wkey, ids, ids2));
   TCX_HIP(hipMemcpyAsync(variant, var.p, g->n, hipMemcpyDeviceToHost, c->stream));
   TCX_HIP(hipStreamSynchronize(c->stream));
   return TC_OK;

@@ -14,6 +14,7 @@ unbuilt library raises at import time.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import weakref
@@ -324,6 +325,21 @@ class CountResult:
     timing: dict
 
 
+# Contexts still open when the interpreter exits are closed (children first)
+# by an atexit hook, while the library and the HIP runtime are intact:
+# __del__ during interpreter teardown runs in no defined order, if at all.
+_live_contexts = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_contexts():
+    for c in list(_live_contexts):
+        try:
+            c.close()
+        except Exception:
+            pass
+
+
 class Context:
     def __init__(self, device: int = 0):
         h = _vp()
@@ -332,6 +348,7 @@ class Context:
         self.device = device
         # graphs and models live on this context; closing it closes them first
         self._children = weakref.WeakSet()
+        _live_contexts.add(self)
 
     @property
     def handle(self):

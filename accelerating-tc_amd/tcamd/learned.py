@@ -26,15 +26,24 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_MODEL = os.path.join(HERE, "cost_model.json")
 
 # Training graphs: small and mid-size members of the benchmark families
-# (R-MAT with several skews, Barabasi-Albert, Erdos-Renyi, random geometric).
+# (R-MAT with several skews, Barabasi-Albert, Erdos-Renyi, random geometric),
+# prepared as the benchmarks prepare theirs (R-MAT: A-order relabelling).
 TRAIN_GRAPHS = [
-    ("rmat", dict(scale=18, ef=16, abc=(0.57, 0.19, 0.19), seed=101)),
-    ("rmat", dict(scale=19, ef=16, abc=(0.57, 0.19, 0.19), seed=102)),
-    ("rmat", dict(scale=18, ef=32, abc=(0.45, 0.22, 0.22), seed=103)),
-    ("rmat", dict(scale=17, ef=8, abc=(0.65, 0.15, 0.15), seed=104)),
+    ("rmat", dict(scale=18, ef=16, abc=(0.57, 0.19, 0.19), seed=101, aorder=True)),
+    ("rmat", dict(scale=19, ef=16, abc=(0.57, 0.19, 0.19), seed=102, aorder=True)),
+    ("rmat", dict(scale=20, ef=16, abc=(0.57, 0.19, 0.19), seed=108, aorder=True)),
+    ("rmat", dict(scale=21, ef=16, abc=(0.57, 0.19, 0.19), seed=109, aorder=True)),
+    ("rmat", dict(scale=22, ef=16, abc=(0.57, 0.19, 0.19), seed=122, aorder=True)),
+    ("rmat", dict(scale=23, ef=16, abc=(0.57, 0.19, 0.19), seed=123, aorder=True)),
+    ("rmat", dict(scale=18, ef=32, abc=(0.45, 0.22, 0.22), seed=103, aorder=True)),
+    ("rmat", dict(scale=17, ef=8, abc=(0.65, 0.15, 0.15), seed=104, aorder=True)),
+    ("rmat", dict(scale=19, ef=16, abc=(0.57, 0.19, 0.19), seed=110, aorder=False)),
     ("ba", dict(n=1 << 18, k=8, seed=105)),
+    ("ba", dict(n=1 << 21, k=8, seed=111)),
     ("er", dict(n=1 << 17, m=1 << 21, seed=106)),
+    ("er", dict(n=1 << 20, m=1 << 24, seed=112)),
     ("rgg", dict(n=1 << 17, deg=32, seed=107)),
+    ("rgg", dict(n=1 << 20, deg=48, seed=113)),
 ]
 
 
@@ -93,6 +102,8 @@ def collect(ctx, graphs=TRAIN_GRAPHS, per_graph=60000, log=print):
         s, d = make_graph(kind, p)
         g = ctx.graph(s, d)
         g.simplify()
+        if p.get("aorder"):
+            g.relabel_aorder()
         g.orient(tcamd.ORIENT_ADIR)
         X, T, M = samples_of(g, per_graph, seed=i)
         log(f"[learned] {kind} {p}: {len(X)} samples")

@@ -82,15 +82,18 @@ typedef struct tc_graph_info {
 /* Per-call schedule.  Zero fields take the defaults listed. */
 typedef struct tc_sched {
   int32_t kernel;           /* TC_KERNEL_*                                       */
-  uint32_t block_size;      /* threads per workgroup (default 256)               */
-  uint32_t grid_size;       /* Hu: work-balanced blocks (default 8192);
-                               Bisson/TriCore: resident workgroups               */
-  uint32_t lds_slice;       /* Hu/AUTO: adjacency entries staged in LDS (2048)  */
+  uint32_t block_size;      /* reserved (workgroups are 256 threads)             */
+  uint32_t grid_size;       /* Hu: work-balanced blocks (default 30000);
+                               Bisson/TriCore/AUTO: resident workgroups          */
+  uint32_t lds_slice;       /* reserved                                          */
   uint32_t bitmap_min_deg;  /* Bisson: d+ above which the bitmap path runs (64)  */
-  uint32_t fox_max_lanes;   /* Fox: widest lanes-per-edge bin, 32 or 64 (64)     */
+  uint32_t fox_max_lanes;   /* Fox: widest lanes-per-edge bin, 1..64 (64)        */
   uint32_t tricore_parts;   /* TriCore: node-range partitions (1)                */
-  uint32_t auto_bins;       /* AUTO: 0 = fixed bins, 1 = learned (model loaded)  */
-  uint32_t auto_thresholds[4]; /* AUTO: d+ bin edges (default 8, 64, 512, 4096) */
+  uint32_t auto_bins;       /* reserved                                          */
+  uint32_t auto_thresholds[4]; /* AUTO: [0] largest d+ run as a wave task (64,
+                               at most 512); [1] largest d+ run by the 2048-entry
+                               block kernel, above it the 8192-entry one (2048);
+                               [2], [3] reserved                                 */
   uint32_t reserved[4];
 } tc_sched;
 

@@ -175,6 +175,9 @@ def test_per_vertex_counts(ctx, case):
     dict(kernel="hu", grid_size=1),
     dict(kernel="hu", grid_size=977),
     dict(kernel="auto", grid_size=3),
+    dict(kernel="auto", auto_thresholds=[2, 2]),
+    dict(kernel="auto", auto_thresholds=[512, 512], grid_size=5),
+    dict(kernel="auto", auto_thresholds=[16, 40]),
 ])
 @pytest.mark.parametrize("case", ["rmat12", "hub", "k6", "messy"])
 def test_schedule_variants(ctx, case, sched):
@@ -185,6 +188,34 @@ def test_schedule_variants(ctx, case, sched):
         g = ctx.graph(s, d).orient(orient)
         assert g.count(k, **sched).count == truth(case)
         g.close()
+
+
+def test_auto_hub_beyond_lds(ctx):
+    """A row longer than the 8192-entry LDS slot is searched in global memory."""
+    n = 12000
+    hub = np.full(n - 1, n - 1, np.uint32)
+    spokes = np.arange(n - 1, dtype=np.uint32)
+    ring_s = np.arange(n - 1, dtype=np.uint32)
+    ring_d = np.roll(ring_s, -1)
+    s = np.concatenate([hub, ring_s])
+    d = np.concatenate([spokes, ring_d])
+    expect = orc.pipeline_count(s, d, "id")[0]
+    assert expect == n - 1
+    g = ctx.graph(s, d).orient(tcamd.ORIENT_ID)
+    assert g.info().max_out_degree == n - 1
+    for th in ([64, 2048], [512, 600], [2, 2]):
+        assert g.count("auto", auto_thresholds=th).count == expect
+    g.close()
+
+
+def test_auto_rejects_bad_thresholds(ctx):
+    s, d = CASES["k6"]()
+    g = ctx.graph(s, d).orient(tcamd.ORIENT_DEGREE)
+    with pytest.raises(tcamd.TCError):
+        g.count("auto", auto_thresholds=[1024, 2048])
+    with pytest.raises(tcamd.TCError):
+        g.count("auto", auto_thresholds=[64, 32])
+    g.close()
 
 
 def test_empty_and_tiny(ctx):
