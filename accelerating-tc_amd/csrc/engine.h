@@ -43,6 +43,49 @@ __device__ inline void flush_count(unsigned long long cnt, unsigned long long* t
   if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(total, cnt);
 }
 
+// Histogram of ids: h[x] += 1 for every x of a[0, m) (and of b[0, m) when b
+// is not null).  Ids below HOT are counted in LDS first: skewed inputs
+// (R-MAT, Barabasi-Albert) put their hubs at small ids, whose global atomics
+// would otherwise serialise in one L2 channel.
+template <int HOT>
+__global__ __launch_bounds__(256) void k_count_ids(const uint32_t* __restrict__ a,
+                                                  const uint32_t* __restrict__ b, uint64_t m,
+                                                  uint32_t n, uint32_t* __restrict__ h) {
+  __shared__ uint32_t hot[HOT];
+  for (uint32_t i = threadIdx.x; i < (uint32_t)HOT; i += blockDim.x) hot[i] = 0;
+  __syncthreads();
+  for (uint64_t i = gtid(); i < m; i += gstride()) {
+    const uint32_t x = a[i];
+    if (x < (uint32_t)HOT) atomicAdd(hot + x, 1u);
+    else atomicAdd(h + x, 1u);
+    if (b) {
+      const uint32_t y = b[i];
+      if (y < (uint32_t)HOT) atomicAdd(hot + y, 1u);
+      else atomicAdd(h + y, 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < (uint32_t)HOT && i < n; i += blockDim.x)
+    if (hot[i]) atomicAdd(h + i, hot[i]);
+}
+
+// rowptr[r] = first position of the ascending (src << 32 | dst) keys whose
+// src is >= r, for r in [0, n]: one independent search per row.
+template <typename Key>
+__global__ void k_rowptr_search(const Key* __restrict__ keys, uint64_t m, uint32_t n,
+                                uint32_t* __restrict__ rowptr) {
+  for (uint64_t r = gtid(); r <= n; r += gstride()) {
+    const Key x = (Key)(r << 32);
+    uint64_t lo = 0, hi = m;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (keys[mid] < x) lo = mid + 1;
+      else hi = mid;
+    }
+    rowptr[r] = (uint32_t)lo;
+  }
+}
+
 // x in a[lo, hi) for ascending a (global memory, early exit)
 __device__ inline bool contains(const uint32_t* a, uint32_t lo, uint32_t hi, uint32_t x) {
   while (lo < hi) {

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "engine.h"
 #include "internal.h"
 
 namespace tcx {
@@ -36,14 +37,6 @@ __global__ void k_fill_i32(int32_t* p, int32_t v, uint64_t n) {
 
 // Degree over the raw records (duplicates and self-loops counted),
 // Hu/preTC.cu:119-123, Bisson/preTC.cu:65-76, tricount_gpu.cu:37-47.
-__global__ void k_degree_raw(const uint32_t* __restrict__ s, const uint32_t* __restrict__ d,
-                             uint64_t m, uint32_t* deg) {
-  for (uint64_t i = gtid(); i < m; i += gstride()) {
-    atomicAdd(deg + s[i], 1u);
-    atomicAdd(deg + d[i], 1u);
-  }
-}
-
 // (prune) flag[v] = deg[v] <= 1, Hu/preTC.cu:139-144.
 __global__ void k_low_degree_flag(const uint32_t* deg, uint32_t n, uint32_t* flag) {
   for (uint64_t v = gtid(); v < n; v += gstride()) flag[v] = deg[v] <= 1 ? 1u : 0u;
@@ -119,22 +112,15 @@ __global__ void k_unzip(const uint64_t* __restrict__ keys, uint64_t m, uint32_t*
 // Row offsets + targets + sources from keys sorted by (src,dst); one thread
 // per key plus one for the tail (tricount_gpu.cu:127-137 builds the same
 // offsets from a sorted first-endpoint array).
-__global__ void k_csr_from_sorted(const uint64_t* __restrict__ keys, uint64_t m, uint32_t n,
-                                  uint32_t* __restrict__ rowptr, uint32_t* __restrict__ col,
-                                  uint32_t* __restrict__ esrc) {
-  for (uint64_t i = gtid(); i <= m; i += gstride()) {
-    int64_t prev = i > 0 ? (int64_t)(keys[i - 1] >> 32) : -1;
-    int64_t cur = i < m ? (int64_t)(keys[i] >> 32) : (int64_t)n;
-    for (int64_t r = prev + 1; r <= cur; ++r) rowptr[r] = (uint32_t)i;
-    if (i < m) {
-      col[i] = (uint32_t)keys[i];
-      if (esrc) esrc[i] = (uint32_t)(keys[i] >> 32);
-    }
+// col (and esrc) from sorted (src << 32 | dst) keys; rowptr comes from
+// dev::k_rowptr_search, so runs of empty rows cost nothing extra.
+__global__ void k_csr_cols(const uint64_t* __restrict__ keys, uint64_t m,
+                           uint32_t* __restrict__ col, uint32_t* __restrict__ esrc) {
+  for (uint64_t i = gtid(); i < m; i += gstride()) {
+    const uint64_t k = keys[i];
+    col[i] = (uint32_t)k;
+    if (esrc) esrc[i] = (uint32_t)(k >> 32);
   }
-}
-
-__global__ void k_indegree(const uint32_t* __restrict__ col, uint64_t m, uint32_t* indeg) {
-  for (uint64_t i = gtid(); i < m; i += gstride()) atomicAdd(indeg + col[i], 1u);
 }
 
 __device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
@@ -300,6 +286,28 @@ __global__ void k_aorder_apply(uint32_t* s, uint32_t* d, uint64_t m, const uint3
   }
 }
 
+// rowptr + col (+ esrc) of the CSR of m sorted, unique keys over n rows.
+int csr_launch(tc_ctx* c, const uint64_t* keys, uint64_t m, uint32_t n, uint32_t* rowptr,
+               uint32_t* col, uint32_t* esrc) {
+  if (m)
+    hipLaunchKernelGGL(k_csr_cols, dim3(grid_for(m)), dim3(TB), 0, c->stream, keys, m, col, esrc);
+  hipLaunchKernelGGL(dev::k_rowptr_search<uint64_t>, dim3(grid_for((uint64_t)n + 1)), dim3(TB), 0,
+                     c->stream, keys, m, n, rowptr);
+  TCX_HIP(hipGetLastError());
+  return TC_OK;
+}
+
+// h[x] += occurrences of x in a[0, m) (and b[0, m) when b is not null); h
+// must be zeroed.
+int count_ids(tc_ctx* c, const uint32_t* a, const uint32_t* b, uint64_t m, uint32_t n,
+              uint32_t* h) {
+  if (m == 0) return TC_OK;
+  const unsigned G = (unsigned)std::min<uint64_t>(grid_for(m), (uint64_t)c->num_cus * 4u);
+  hipLaunchKernelGGL(dev::k_count_ids<8192>, dim3(G), dim3(256), 0, c->stream, a, b, m, n, h);
+  TCX_HIP(hipGetLastError());
+  return TC_OK;
+}
+
 int read_u32(tc_ctx* c, const uint32_t* dptr, uint32_t* out) {
   TCX_HIP(hipMemcpyAsync(out, dptr, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
   TCX_HIP(hipStreamSynchronize(c->stream));
@@ -336,10 +344,8 @@ static int csr_from_keys(tc_ctx* c, tc_graph* g, DevBuf& keys, DevBuf& tmp, uint
   TCX_TRY(g->rowptr.reserve(sizeof(uint32_t) * ((size_t)n_out + 1)));
   TCX_TRY(g->col.reserve(sizeof(uint32_t) * std::max<uint64_t>(nuniq, 1)));
   TCX_TRY(g->esrc.reserve(sizeof(uint32_t) * std::max<uint64_t>(nuniq, 1)));
-  hipLaunchKernelGGL(k_csr_from_sorted, dim3(grid_for(nuniq + 1)), dim3(TB), 0, c->stream, uniq,
-                     nuniq, n_out, g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(),
-                     g->esrc.as<uint32_t>());
-  TCX_HIP(hipGetLastError());
+  TCX_TRY(csr_launch(c, uniq, nuniq, n_out, g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(),
+                     g->esrc.as<uint32_t>()));
   g->sched_version++;
   return compute_csr_stats(c, g);
 }
@@ -352,8 +358,7 @@ int compute_csr_stats(tc_ctx* c, tc_graph* g) {
   TCX_HIP(hipMemsetAsync(acc, 0, 3 * sizeof(unsigned long long), c->stream));
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(g->n)), dim3(TB), 0, c->stream,
                      indeg.as<uint32_t>(), 0u, (uint64_t)g->n);
-  hipLaunchKernelGGL(k_indegree, dim3(grid_for(g->m_or)), dim3(TB), 0, c->stream,
-                     g->col.as<uint32_t>(), g->m_or, indeg.as<uint32_t>());
+  TCX_TRY(count_ids(c, g->col.as<uint32_t>(), nullptr, g->m_or, g->n, indeg.as<uint32_t>()));
   hipLaunchKernelGGL(k_csr_stats, dim3(grid_for(g->n)), dim3(TB), 0, c->stream,
                      g->rowptr.as<uint32_t>(), indeg.as<uint32_t>(), g->n, acc);
   TCX_HIP(hipGetLastError());
@@ -397,9 +402,7 @@ static int sym_csr(tc_ctx* c, const uint64_t* keys, uint64_t m2, uint32_t n, Dev
                    DevBuf& col) {
   TCX_TRY(rp.reserve(sizeof(uint32_t) * ((size_t)n + 1)));
   TCX_TRY(col.reserve(sizeof(uint32_t) * std::max<uint64_t>(m2, 1)));
-  hipLaunchKernelGGL(k_csr_from_sorted, dim3(grid_for(m2 + 1)), dim3(TB), 0, c->stream, keys, m2,
-                     n, rp.as<uint32_t>(), col.as<uint32_t>(), (uint32_t*)nullptr);
-  TCX_HIP(hipGetLastError());
+  TCX_TRY(csr_launch(c, keys, m2, n, rp.as<uint32_t>(), col.as<uint32_t>(), nullptr));
   return TC_OK;
 }
 
@@ -616,8 +619,8 @@ int tc_graph_relabel_aorder(tc_ctx* c, tc_graph* g, uint32_t* perm_out) {
                      0u, (uint64_t)n);
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(TB), 0, c->stream, rtd.as<uint32_t>(),
                      0u, (uint64_t)n);
-  hipLaunchKernelGGL(k_degree_raw, dim3(grid_for(g->m_raw)), dim3(TB), 0, c->stream,
-                     g->src.as<uint32_t>(), g->dst.as<uint32_t>(), g->m_raw, deg.as<uint32_t>());
+  TCX_TRY(count_ids(c, g->src.as<uint32_t>(), g->dst.as<uint32_t>(), g->m_raw, n,
+                    deg.as<uint32_t>()));
   hipLaunchKernelGGL(k_aorder_outdeg, dim3(grid_for(g->m_raw)), dim3(TB), 0, c->stream,
                      g->src.as<uint32_t>(), g->dst.as<uint32_t>(), g->m_raw, deg.as<uint32_t>(),
                      rtd.as<uint32_t>());
@@ -654,8 +657,7 @@ int tc_graph_orient(tc_ctx* c, tc_graph* g, int mode, uint32_t flags) {
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n + 1)), dim3(TB), 0, c->stream,
                        deg.as<uint32_t>(), 0u, (uint64_t)n + 1);
     if (mode != TC_ORIENT_ID)
-      hipLaunchKernelGGL(k_degree_raw, dim3(grid_for(m)), dim3(TB), 0, c->stream,
-                         g->src.as<uint32_t>(), g->dst.as<uint32_t>(), m, deg.as<uint32_t>());
+      TCX_TRY(count_ids(c, g->src.as<uint32_t>(), g->dst.as<uint32_t>(), m, n, deg.as<uint32_t>()));
     TCX_HIP(hipGetLastError());
     uint32_t n_out = n;
     const uint32_t* pruned_before = nullptr;

@@ -212,10 +212,6 @@ __global__ void k_to_bf16(const float* __restrict__ w, uint16_t* __restrict__ wb
 }
 
 // ---- features ------------------------------------------------------------
-__global__ void k_indeg(const uint32_t* __restrict__ col, uint64_t m, uint32_t* indeg) {
-  for (uint64_t i = gtid(); i < m; i += gstride()) atomicAdd(indeg + col[i], 1u);
-}
-
 __global__ void k_features(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                            const uint32_t* __restrict__ indeg, uint32_t n, float* __restrict__ X) {
   for (uint64_t u = gtid(); u < n; u += gstride()) {
@@ -506,8 +502,10 @@ int graph_features(tc_ctx* c, tc_graph* g, DevBuf& X) {
   TCX_TRY(indeg.reserve(4 * ((size_t)n + 1)));
   TCX_TRY(X.reserve(4ull * F * ((size_t)n + 1)));
   TCX_HIP(hipMemsetAsync(indeg.p, 0, 4 * ((size_t)n + 1), c->stream));
-  hipLaunchKernelGGL(k_indeg, dim3(grid_for(g->m_or)), dim3(256), 0, c->stream,
-                     g->col.as<uint32_t>(), g->m_or, indeg.as<uint32_t>());
+  if (g->m_or)
+    hipLaunchKernelGGL(k_count_ids<8192>, dim3(std::min<uint64_t>(grid_for(g->m_or), (uint64_t)c->num_cus * 4u)),
+                       dim3(256), 0, c->stream, g->col.as<uint32_t>(), (const uint32_t*)nullptr,
+                       g->m_or, n, indeg.as<uint32_t>());
   hipLaunchKernelGGL(k_features, dim3(grid_for(n)), dim3(256), 0, c->stream,
                      g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(), indeg.as<uint32_t>(), n,
                      X.as<float>());

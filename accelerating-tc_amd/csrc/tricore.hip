@@ -59,16 +59,24 @@ __global__ void k_tri_part_keys(const uint64_t* __restrict__ e, uint64_t m, uint
   }
 }
 
-// node_index[j] for j in [0, count] from sorted FIRST values (tricount_gpu.cu:127-137)
+// node_index[j] for j in [0, count] from the sorted keys (tricount_gpu.cu:127-137):
+// the first position whose FIRST is >= start + j, one search per entry.
 __global__ void k_tri_node_index(const uint64_t* __restrict__ keys, uint64_t m, uint32_t start,
-                                 uint32_t count, uint32_t* __restrict__ index,
-                                 uint32_t* __restrict__ adj) {
-  for (uint64_t i = gtid(); i <= m; i += gstride()) {
-    int64_t prev = i > 0 ? (int64_t)((uint32_t)(keys[i - 1] >> 32) - start) : -1;
-    int64_t next = i < m ? (int64_t)((uint32_t)(keys[i] >> 32) - start) : (int64_t)count;
-    for (int64_t j = prev + 1; j <= next; ++j) index[j] = (uint32_t)i;
-    if (i < m) adj[i] = (uint32_t)keys[i];
+                                 uint32_t count, uint32_t* __restrict__ index) {
+  for (uint64_t j = gtid(); j <= count; j += gstride()) {
+    const uint64_t x = ((uint64_t)start + j) << 32;
+    uint64_t lo = 0, hi = m;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (keys[mid] < x) lo = mid + 1;
+      else hi = mid;
+    }
+    index[j] = (uint32_t)lo;
   }
+}
+
+__global__ void k_tri_adj(const uint64_t* __restrict__ keys, uint64_t m, uint32_t* __restrict__ adj) {
+  for (uint64_t i = gtid(); i < m; i += gstride()) adj[i] = (uint32_t)keys[i];
 }
 
 }  // namespace
@@ -173,8 +181,11 @@ void tc_tricore_split(uint64_t* edges, uint64_t m, uint64_t part_num, uint32_t**
     if (unique_u64(c, sorted, other, np, &nu) != TC_OK) return;
     uint32_t count = hi - lo + 1;  // tricount_gpu.cu:182
     if (idx.reserve(4 * ((size_t)count + 1))) return;
-    hipLaunchKernelGGL(k_tri_node_index, dim3(grid_for(nu + 1)), dim3(256), 0, c->stream, other, nu,
-                       lo, count, idx.as<uint32_t>(), adj.as<uint32_t>());
+    hipLaunchKernelGGL(k_tri_node_index, dim3(grid_for((uint64_t)count + 1)), dim3(256), 0,
+                       c->stream, other, nu, lo, count, idx.as<uint32_t>());
+    if (nu)
+      hipLaunchKernelGGL(k_tri_adj, dim3(grid_for(nu)), dim3(256), 0, c->stream, other, nu,
+                         adj.as<uint32_t>());
     node_index[p] = (uint32_t*)malloc(4 * ((size_t)count + 1));
     if (!node_index[p]) { set_error("split: host allocation failed"); return; }
     (void)hipMemcpyAsync(node_index[p], idx.p, 4 * ((size_t)count + 1), hipMemcpyDeviceToHost, c->stream);
