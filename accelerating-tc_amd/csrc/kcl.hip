@@ -214,6 +214,179 @@ __global__ __launch_bounds__(64) void k_kcl_merge(const uint32_t* __restrict__ r
   if (lane == 0 && cnt) atomicAdd(total, cnt);
 }
 
+// ---- PIVOT path ---------------------------------------------------------------
+// Pivoting over the succinct clique tree (the counting scheme of Jain and
+// Seshadhri, "The Power of Pivoting for Exact Clique Counting", WSDM 2020),
+// one wave per root u.  The sub-graph induced by N+(u) is held as a symmetric
+// d x d bit matrix in LDS.  A tree node (C, h, p) has candidate set C, h
+// held vertices (u and the non-pivot branch vertices) and p pivots; every
+// k-clique of the sub-tree is the held set plus S from the pivots plus a
+// clique of C.  A node picks the pivot x of C with most neighbours in C and
+// branches on x and on the vertices of C outside N(x), each branch removing
+// its vertex from C for the later ones; a leaf (C empty) holds
+// binom(p, k - h) k-cliques.  For the one k asked:
+//   h > k or h + p + |C| < k  -> nothing below,
+//   k - h = 0, 1, 2          -> closed forms 1, p + |C|,
+//                               binom(p, 2) + p |C| + edges(C),
+// so the tree is only walked where at least three more vertices are needed.
+// The walk's stack (C and the branches still to take, per level) lives in
+// the wave's slice of `stk`: d + 1 levels of 2W + 1 words.
+__device__ inline unsigned long long binom_small(uint32_t n, uint32_t r) {
+  if (r > n) return 0;
+  unsigned __int128 c = 1;
+  for (uint32_t i = 1; i <= r; ++i) c = c * (n - r + i) / i;
+  return (unsigned long long)c;
+}
+
+__device__ inline uint64_t wave_or_u64(uint64_t v) {
+  for (int off = 32; off > 0; off >>= 1) v |= __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ inline uint32_t wave_sum_u32(uint32_t v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ inline uint32_t wave_max_u32(uint32_t v) {
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+template <int W>
+__global__ __launch_bounds__(64) void k_kcl_pivot(const uint32_t* __restrict__ rp,
+                                                  const uint32_t* __restrict__ col,
+                                                  const uint32_t* __restrict__ roots,
+                                                  uint64_t nroots, uint32_t k,
+                                                  uint64_t* __restrict__ stk_all,
+                                                  unsigned long long* total) {
+  constexpr uint32_t D = 64 * W;
+  constexpr uint32_t LW = 2 * W + 1;  // stack words per level
+  __shared__ uint32_t s_list[D];
+  __shared__ uint64_t s_M[D * W];
+  __shared__ uint64_t s_C[W];
+  const uint32_t lane = threadIdx.x;
+  uint64_t* stk = stk_all + (uint64_t)blockIdx.x * (D + 1) * LW;
+  unsigned long long cnt = 0;  // lane 0 accumulates
+  for (uint64_t ri = blockIdx.x; ri < nroots; ri += gridDim.x) {
+    const uint32_t u = roots[ri];
+    const uint32_t base = rp[u], d = rp[u + 1] - base;
+    for (uint32_t i = lane; i < d; i += 64) s_list[i] = col[base + i];
+    for (uint32_t i = lane; i < d * W; i += 64) s_M[i] = 0;
+    __syncthreads();
+    // symmetric adjacency of N+(u): bit (i, j) for every arc c_i -> c_j
+    for (uint32_t i = 0; i < d; ++i) {
+      const uint32_t ci = s_list[i];
+      const uint32_t e1 = rp[ci + 1];
+      for (uint32_t e = rp[ci] + lane; e < e1; e += 64) {
+        const uint32_t x = col[e];
+        uint32_t lo = 0, len = d;
+        while (len > 1) {
+          const uint32_t half = len >> 1;
+          lo = s_list[lo + half] <= x ? lo + half : lo;
+          len -= half;
+        }
+        if (s_list[lo] == x) {
+          atomicOr((unsigned long long*)&s_M[i * W + (lo >> 6)], 1ull << (lo & 63));
+          atomicOr((unsigned long long*)&s_M[lo * W + (i >> 6)], 1ull << (i & 63));
+        }
+      }
+    }
+    __syncthreads();
+
+    // enter(): the node (s_C, h, p); counts it or pushes a level
+    int L = -1;
+    auto enter = [&](uint32_t h, uint32_t p) {
+      // |C| and, when needed, edges(C) or the pivot
+      uint32_t cs = 0;
+      for (int w = 0; w < W; ++w) cs += (uint32_t)__popcll(s_C[w]);
+      const uint32_t r = k - h;
+      if (cs == 0) {
+        if (lane == 0) cnt += binom_small(p, r);
+        return;
+      }
+      if (h + p + cs < k) return;
+      if (r == 0) {
+        if (lane == 0) cnt += 1;
+        return;
+      }
+      if (r == 1) {
+        if (lane == 0) cnt += p + cs;
+        return;
+      }
+      // scores: |N(x) & C| for every x in C (lane x mod 64, word x / 64)
+      uint32_t best = 0, esum = 0;
+      for (int t = 0; t < W; ++t) {
+        const uint64_t cw = s_C[t];
+        const uint32_t x = 64u * t + lane;
+        if ((cw >> lane) & 1ull) {
+          uint32_t sc = 0;
+          for (int w = 0; w < W; ++w) sc += (uint32_t)__popcll(s_M[x * W + w] & s_C[w]);
+          esum += sc;
+          // (score, smallest index) packed for a max-reduction
+          const uint32_t key = (sc << 16) | (0xFFFFu - x);
+          best = max(best, key);
+        }
+      }
+      if (r == 2) {
+        const uint32_t e2 = wave_sum_u32(esum);
+        if (lane == 0) cnt += binom_small(p, 2) + (unsigned long long)p * cs + e2 / 2;
+        return;
+      }
+      best = wave_max_u32(best);
+      const uint32_t piv = 0xFFFFu - (best & 0xFFFFu);
+      ++L;
+      uint64_t* lv = stk + (uint64_t)L * LW;
+      if (lane < (uint32_t)W) {
+        const uint64_t c = s_C[lane];
+        uint64_t todo = c & ~s_M[piv * W + lane];
+        if (lane == (piv >> 6)) todo |= 1ull << (piv & 63);
+        lv[lane] = c;
+        lv[W + lane] = todo;
+      }
+      if (lane == 0) lv[2 * W] = (uint64_t)piv | ((uint64_t)h << 16) | ((uint64_t)p << 32);
+      __syncthreads();
+    };
+
+    for (int w = lane; w < W; w += 64) {
+      const uint32_t lo = 64u * w;
+      s_C[w] = d >= lo + 64 ? ~0ull : (d > lo ? ((1ull << (d - lo)) - 1) : 0ull);
+    }
+    __syncthreads();
+    enter(1, 0);
+    while (L >= 0) {
+      uint64_t* lv = stk + (uint64_t)L * LW;
+      const uint64_t todo_w = lane < (uint32_t)W ? lv[W + lane] : 0ull;
+      const uint64_t nz = __ballot(todo_w != 0);
+      if (nz == 0) {
+        --L;
+        continue;
+      }
+      const uint32_t fw = (uint32_t)__builtin_ctzll(nz);
+      const uint64_t tw = __shfl(todo_w, fw, 64);
+      const uint32_t v = fw * 64u + (uint32_t)__builtin_ctzll(tw);
+      const uint64_t meta = lv[2 * W];
+      const uint32_t piv = (uint32_t)(meta & 0xFFFFu), h = (uint32_t)((meta >> 16) & 0xFFFFu),
+                     p = (uint32_t)(meta >> 32);
+      // take v: child C = C & N(v); v leaves C and the branch list
+      if (lane < (uint32_t)W) {
+        const uint64_t c = lv[lane];
+        const uint64_t vb = lane == fw ? (1ull << (v & 63)) : 0ull;
+        s_C[lane] = c & s_M[v * W + lane];
+        lv[lane] = c & ~vb;
+        lv[W + lane] = todo_w & ~vb;
+      }
+      __syncthreads();
+      const bool is_piv = v == piv;
+      const uint32_t h2 = h + (is_piv ? 0u : 1u), p2 = p + (is_piv ? 1u : 0u);
+      if (h2 > k) continue;
+      enter(h2, p2);
+    }
+    __syncthreads();
+  }
+  if (lane == 0 && cnt) atomicAdd(total, cnt);
+}
+
 }  // namespace
 }  // namespace tcx
 
@@ -234,7 +407,7 @@ int kcl_count(tc_ctx* c, tc_graph* g, uint32_t k, int variant, uint64_t* count,
   if (g->ctx != c) return fail(TC_ERR_ARG, "graph belongs to another context");
   if (g->orient_mode < 0) return fail(TC_ERR_STATE, "graph must be oriented before counting");
   if (k == 0 || k > (uint32_t)kMaxK) return fail(TC_ERR_RANGE, "k must be in 1..%d", kMaxK);
-  if (variant < KCL_VARIANT_AUTO || variant > KCL_VARIANT_MERGE)
+  if (variant < KCL_VARIANT_AUTO || variant > KCL_VARIANT_PIVOT)
     return fail(TC_ERR_ARG, "unknown k-clique variant %d", variant);
   TCX_HIP(hipSetDevice(c->device));
   tc_timing tm;
@@ -246,17 +419,24 @@ int kcl_count(tc_ctx* c, tc_graph* g, uint32_t k, int variant, uint64_t* count,
   const uint32_t* rp = g->rowptr.as<uint32_t>();
   const uint32_t* col = g->col.as<uint32_t>();
   // root buckets: [lo, hi] of d+ and the path that takes them
-  struct Bucket { uint32_t lo, hi; int w; };  // w = bitmap words, 0 = merge
+  struct Bucket { uint32_t lo, hi; int w; bool pivot; };  // w = bitmap words, 0 = merge
   std::vector<Bucket> buckets;
   const uint32_t need = k - 1;  // a root needs k-1 out-neighbours
-  if (variant == KCL_VARIANT_MERGE || k > (uint32_t)kBitmapMaxK) {
-    buckets.push_back({need, 0xFFFFFFFFu, 0});
+  if (variant == KCL_VARIANT_AUTO || variant == KCL_VARIANT_PIVOT) {
+    buckets.push_back({need, 64, 1, true});
+    buckets.push_back({65, 128, 2, true});
+    buckets.push_back({129, 256, 4, true});
+    buckets.push_back({257, 512, 8, true});
+    buckets.push_back({513, 1024, 16, true});
+    buckets.push_back({1025, 0xFFFFFFFFu, 0, false});
+  } else if (variant == KCL_VARIANT_MERGE || k > (uint32_t)kBitmapMaxK) {
+    buckets.push_back({need, 0xFFFFFFFFu, 0, false});
   } else {
-    buckets.push_back({need, 64, 1});
-    buckets.push_back({65, 128, 2});
-    buckets.push_back({129, 256, 4});
-    buckets.push_back({257, 512, 8});
-    buckets.push_back({513, 0xFFFFFFFFu, 0});
+    buckets.push_back({need, 64, 1, false});
+    buckets.push_back({65, 128, 2, false});
+    buckets.push_back({129, 256, 4, false});
+    buckets.push_back({257, 512, 8, false});
+    buckets.push_back({513, 0xFFFFFFFFu, 0, false});
   }
   DevBuf flags, roots, scratch;
   TCX_TRY(flags.reserve((size_t)g->n + 1));
@@ -282,6 +462,20 @@ int kcl_count(tc_ctx* c, tc_graph* g, uint32_t k, int variant, uint64_t* count,
       TCX_TRY(scratch.reserve(4ull * G * slots * maxd));
       hipLaunchKernelGGL(k_kcl_merge, dim3(G), dim3(64), 0, c->stream, rp, col,
                          roots.as<uint32_t>(), nr, k, maxd, slots, scratch.as<uint32_t>(), acc);
+    } else if (b.pivot) {
+      // waves resident per CU: LDS-bound for the wide matrices
+      const uint64_t per_cu = b.w <= 4 ? 16 : (b.w == 8 ? 4 : 1);
+      const uint32_t G = (uint32_t)std::min<uint64_t>(nr, (uint64_t)c->num_cus * per_cu);
+      const uint64_t D = 64ull * b.w;
+      TCX_TRY(scratch.reserve(8ull * G * (D + 1) * (2ull * b.w + 1)));
+      uint64_t* stk = scratch.as<uint64_t>();
+      switch (b.w) {
+        case 1: hipLaunchKernelGGL(k_kcl_pivot<1>, dim3(G), dim3(64), 0, c->stream, rp, col, roots.as<uint32_t>(), nr, k, stk, acc); break;
+        case 2: hipLaunchKernelGGL(k_kcl_pivot<2>, dim3(G), dim3(64), 0, c->stream, rp, col, roots.as<uint32_t>(), nr, k, stk, acc); break;
+        case 4: hipLaunchKernelGGL(k_kcl_pivot<4>, dim3(G), dim3(64), 0, c->stream, rp, col, roots.as<uint32_t>(), nr, k, stk, acc); break;
+        case 8: hipLaunchKernelGGL(k_kcl_pivot<8>, dim3(G), dim3(64), 0, c->stream, rp, col, roots.as<uint32_t>(), nr, k, stk, acc); break;
+        default: hipLaunchKernelGGL(k_kcl_pivot<16>, dim3(G), dim3(64), 0, c->stream, rp, col, roots.as<uint32_t>(), nr, k, stk, acc); break;
+      }
     } else {
       uint32_t G = (uint32_t)std::min<uint64_t>(nr, (uint64_t)c->num_cus * 16);
       switch (b.w) {

@@ -41,7 +41,7 @@ FLAG_PRUNE_DEG1 = 1
 KERNEL_HU, KERNEL_BISSON, KERNEL_FOX, KERNEL_TRICORE, KERNEL_AUTO = 0, 1, 2, 3, 4
 KERNEL_NAMES = {"hu": KERNEL_HU, "bisson": KERNEL_BISSON, "fox": KERNEL_FOX,
                 "tricore": KERNEL_TRICORE, "auto": KERNEL_AUTO}
-KCL_AUTO, KCL_BITMAP, KCL_MERGE = 0, 1, 2
+KCL_AUTO, KCL_BITMAP, KCL_MERGE, KCL_PIVOT = 0, 1, 2, 3
 
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")

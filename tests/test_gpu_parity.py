@@ -235,7 +235,8 @@ def test_empty_and_tiny(ctx):
 
 
 @pytest.mark.parametrize("case", ["er", "rmat12", "rgg", "k6", "hub"])
-@pytest.mark.parametrize("variant", [tcamd.KCL_AUTO, tcamd.KCL_MERGE])
+@pytest.mark.parametrize("variant", [tcamd.KCL_AUTO, tcamd.KCL_MERGE, tcamd.KCL_BITMAP,
+                                     tcamd.KCL_PIVOT])
 def test_kclique_matches_oracle(ctx, case, variant):
     s, d = CASES[case]()
     rp_o, col_o = oracle_csr(s, d, "kcore")
@@ -255,7 +256,35 @@ def test_kclique_dense_bitmap_widths(ctx):
         g = ctx.graph(s, d).orient(tcamd.ORIENT_KCORE)
         for k in (3, 4, 5):
             assert g.kclique(k).count == comb(n, k), (n, k)
+            assert g.kclique(k, tcamd.KCL_BITMAP).count == comb(n, k), (n, k)
         g.close()
+
+
+def test_kclique_pivot_deep_and_wide(ctx):
+    """Pivot path: cliques of up to 1024 vertices (every bit-matrix width, the
+    deepest clique trees) and k up to 12; K_n holds C(n, k) k-cliques, and a
+    disjoint union adds."""
+    from math import comb
+    for n in (12, 64, 65, 129, 300, 513, 1000):
+        s, d = clique(n)
+        g = ctx.graph(s, d).orient(tcamd.ORIENT_DEGREE)
+        for k in (3, 4, 7, 12):
+            if comb(n, k) < 2 ** 63:
+                assert g.kclique(k, tcamd.KCL_PIVOT).count == comb(n, k), (n, k)
+        g.close()
+    # K_40 + K_25 + a sparse random part sharing no vertex
+    a, b = clique(40)
+    c, e = clique(25)
+    rs, rd = er(2000, 12000, 9)
+    s = np.concatenate([a, c + 40, rs + 65]).astype(np.uint32)
+    d = np.concatenate([b, e + 40, rd + 65]).astype(np.uint32)
+    rp_o, col_o = oracle_csr(rs, rd, "degree")
+    g = ctx.graph(s, d).orient(tcamd.ORIENT_ADIR)
+    for k in (3, 5, 8, 11):
+        expect = comb(40, k) + comb(25, k) + orc.kclique(rp_o, col_o, k)
+        assert g.kclique(k, tcamd.KCL_PIVOT).count == expect, k
+        assert g.kclique(k).count == expect, k
+    g.close()
 
 
 def test_tricore_function_api(ctx):
