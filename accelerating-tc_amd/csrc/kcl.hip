@@ -416,6 +416,13 @@ int kcl_count(tc_ctx* c, tc_graph* g, uint32_t k, int variant, uint64_t* count,
   if (k == 1) { *count = g->n; if (timing) *timing = tm; return TC_OK; }
   if (k == 2) { *count = g->m_or; if (timing) *timing = tm; return TC_OK; }
   if (g->m_or == 0) { if (timing) *timing = tm; return TC_OK; }
+  if (k == 3 && variant == KCL_VARIANT_AUTO) {
+    // 3-cliques are triangles: the AUTO triangle engine counts them directly
+    tc_sched s;
+    memset(&s, 0, sizeof(s));
+    s.kernel = TC_KERNEL_AUTO;
+    return tc_count(c, g, &s, count, timing);
+  }
   const uint32_t* rp = g->rowptr.as<uint32_t>();
   const uint32_t* col = g->col.as<uint32_t>();
   // root buckets: [lo, hi] of d+ and the path that takes them

@@ -7,7 +7,8 @@ already resident in HBM.  Default workload: R-MAT scale 22, edge factor 16,
 A-order relabelling + A-direction orientation; every method (Hu, Bisson,
 Fox, TriCore) and the degree-binned AUTO engine are timed, counts must agree.
 
-    python bench.py [--steps K] [--warmup W] [--workload rmat22|rmat24|er|ba24|batched]
+    python bench.py [--steps K] [--warmup W]
+                    [--workload rmat22|rmat24|er|ba24|batched|kclique_rgg|kclique_rmat20]
 
 Prints ONE JSON line.  `value` = TC edges/s of the head engine (the learned
 schedule when a trained cost model is present, else AUTO): oriented edges /
@@ -15,7 +16,9 @@ count time, wall clock over K steps.  The `batched` workload counts 4,096
 small graphs (vertex counts log-uniform in [64, 65536], average degree
 uniform in [4, 64], 1/3 each Erdos-Renyi / R-MAT / random geometric, seed 5)
 with one hipGraph replay per step and reports the one-launch-per-graph
-baseline beside it.
+baseline beside it.  The k-clique workloads count k-cliques (k = 4..7 on a
+random geometric graph, n = 10^6, average degree 48; k = 4, 5 on R-MAT scale
+20, edge factor 32) after checking that 3-cliques equal the triangle count.
 """
 import argparse
 import json
@@ -39,6 +42,8 @@ WORKLOADS = {
     "er": dict(kind="er", n=65536, m=1048576, seed=1, aorder=False, orient="adir"),
     "ba24": dict(kind="ba", n=1 << 24, k=8, seed=3, aorder=False, orient="adir"),
     "batched": dict(kind="batched", count=4096, seed=5),
+    "kclique_rgg": dict(kind="kclique", graph="rgg", n=1000000, deg=48, seed=4, ks=(4, 5, 6, 7)),
+    "kclique_rmat20": dict(kind="kclique", graph="rmat", scale=20, ef=32, seed=20, ks=(4, 5)),
 }
 
 
@@ -197,6 +202,87 @@ def main_batched(args, w):
     ctx.close()
 
 
+def main_kclique(args, w):
+    import numpy as np
+    import tcamd
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    t0 = time.perf_counter()
+    if w["graph"] == "rgg":
+        src, dst = tcamd.gen_rgg(w["n"], tcamd.rgg_radius(w["n"], w["deg"]), w["seed"])
+        desc = f"random geometric n={w['n']} average degree {w['deg']} seed {w['seed']}"
+    else:
+        src, dst = tcamd.gen_rmat(w["scale"], w["ef"] << w["scale"], 0.57, 0.19, 0.19, w["seed"])
+        desc = (f"R-MAT scale {w['scale']} ef{w['ef']} (0.57,0.19,0.19,0.05) seed {w['seed']}")
+    t_gen = time.perf_counter() - t0
+    ctx = tcamd.Context(0)
+    t0 = time.perf_counter()
+    g = ctx.graph(src, dst)
+    g.simplify()
+    g.orient(tcamd.ORIENT_ADIR)
+    t_prep = time.perf_counter() - t0
+    info = g.info()
+    m = info.m_oriented
+    log(f"[bench] {desc}: n={info.n} m={m} max_out={info.max_out_degree} prep {t_prep:.2f}s")
+    tri = g.count(tcamd.KERNEL_AUTO).count
+    k3 = g.kclique(3, tcamd.KCL_PIVOT).count
+    if k3 != tri:
+        raise SystemExit(f"3-cliques {k3} != triangles {tri}")
+    per_k = {}
+    for k in w["ks"]:
+        ref = g.kclique(k).count  # also the warm-up
+        ms = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            r = g.kclique(k)
+            if r.count != ref:
+                raise SystemExit(f"k={k}: count changed between steps")
+            ms.append(r.timing["ms_count"])
+        wall = (time.perf_counter() - t0) / args.steps
+        per_k[k] = dict(count=r.count, ms_wall=wall * 1e3, ms_kernels=float(np.mean(ms)),
+                        edges_per_s=m / wall)
+        log(f"[bench] k={k}: {r.count} cliques, {wall*1e3:.2f} ms ({m / wall:.3e} edges/s)")
+    cpu = None
+    if not args.no_cpu:
+        import oracle as orc
+        rp, col = g.csr()
+        k = w["ks"][0]
+        stride = max(1, info.n // 512)
+        t0 = time.perf_counter()
+        orc.kclique_rows_strided(rp, col, k, 0, stride)
+        t1 = max(time.perf_counter() - t0, 1e-4)
+        stride2 = max(1, int(round(t1 * stride / 15.0)))
+        t0 = time.perf_counter()
+        orc.kclique_rows_strided(rp, col, k, 0, stride2)
+        t = time.perf_counter() - t0
+        deg = np.diff(rp.astype(np.int64))
+        edges = int(deg[0::stride2].sum())
+        cpu = {"value": edges / t, "unit": "edges/s", "cores": 1, "kind": "port",
+               "sample": (f"oracle depth-first {k}-clique counter over every {stride2}-th root "
+                          f"of the same oriented CSR: {edges} of {m} edges, {t:.1f} s")}
+        log(f"[bench] cpu baseline {cpu['value']:.3e} edges/s ({cpu['sample']})")
+    total_wall = sum(v["ms_wall"] for v in per_k.values()) * 1e-3
+    line = {
+        "metric": "TC edges/s + HBM GB/s % peak; learned/batched vs plain HIP, exact counts",
+        "value": m * len(per_k) / total_wall,
+        "unit": "edges/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": total_wall * 1e3 / len(per_k),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"k-clique k={','.join(map(str, w['ks']))} on {desc}, "
+                               f"A-direction orientation", "n": info.n, "m": m,
+                   "triangles": tri, "engine": "k-clique pivot", "parallelism": "single GPU"},
+        "roofline": None,
+        "cpu_baseline": cpu,
+        "kclique": {str(k): v for k, v in per_k.items()},
+        "end_to_end_s": {"generate": t_gen, "preprocess": t_prep},
+    }
+    print(json.dumps(line), flush=True)
+    g.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -225,6 +311,8 @@ def main():
     w = WORKLOADS[args.workload]
     if w["kind"] == "batched":
         return main_batched(args, w)
+    if w["kind"] == "kclique":
+        return main_kclique(args, w)
 
     import numpy as np
     import tcamd

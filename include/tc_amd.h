@@ -58,7 +58,7 @@ extern "C" {
 #define TC_KERNEL_AUTO 4        /* degree-binned engine (fixed or learned bins) */
 
 /* k-clique variant. */
-#define KCL_VARIANT_AUTO 0      /* pivot path for d+ <= 1024, merge path above      */
+#define KCL_VARIANT_AUTO 0      /* k = 3: triangle engine; else pivot (d+ <= 1024) or merge */
 #define KCL_VARIANT_BITMAP 1    /* LDS bitmap depth-first path (d+ <= 512, k <= 9)  */
 #define KCL_VARIANT_MERGE 2     /* sorted-list intersection path                   */
 #define KCL_VARIANT_PIVOT 3     /* pivoting over the clique tree, LDS bit matrix   */

@@ -532,6 +532,19 @@ uint64_t orc_kclique(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
   return total;
 }
 
+uint64_t orc_kclique_rows_strided(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
+                                  uint32_t k, uint32_t start, uint32_t stride) {
+  if (k < 2 || stride == 0) return 0;
+  uint64_t total = 0;
+  std::vector<uint32_t> S;
+  for (uint64_t u = start; u < n; u += stride) {
+    S.assign(col + rowptr[u], col + rowptr[u + 1]);
+    if (S.size() + 1 < k) continue;
+    total += clique_rec(rowptr, col, S, k - 1);
+  }
+  return total;
+}
+
 void orc_hu_workload(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
                      int64_t* w) {
   for (uint32_t u = 0; u < n; ++u) {  // Hu/tc.cu:22-35

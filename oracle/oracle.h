@@ -124,6 +124,10 @@ void orc_count_per_vertex(uint32_t n, const uint32_t* rowptr,
  * pinned: C(n,k) on K_n, Turan graphs, networkx clique enumeration. */
 uint64_t orc_kclique(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
                      uint32_t k);
+/* The same count restricted to the roots start, start + stride, ... (a
+ * bounded sample for timing; the rows partition the cliques by root). */
+uint64_t orc_kclique_rows_strided(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
+                                  uint32_t k, uint32_t start, uint32_t stride);
 
 /* Hu work model, Hu/tc.cu:22-35: W(u) = (sum_{v in N+(u)} d+(v)) *
  * (unsigned)log2(d+(u)); 0 when d+(u) <= 1.  pinned: closed form. */

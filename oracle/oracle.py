@@ -56,6 +56,8 @@ _sig("orc_count_rows", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_ui
 _sig("orc_count_rows_strided", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32])
 _sig("orc_count_per_vertex", None, [C.c_uint32, _u32p, _u32p, _u64p])
 _sig("orc_kclique", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32])
+_sig("orc_kclique_rows_strided", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32,
+                                               C.c_uint32])
 _sig("orc_hu_workload", None, [C.c_uint32, _u32p, _u32p, _i64p])
 _sig("orc_hu_block_starts", None, [C.c_uint32, _i64p, C.c_uint32, _i32p])
 _sig("orc_fox_workload", None, [C.c_uint64, _u32p, _u32p, _u32p, _i32p])
@@ -197,6 +199,12 @@ def count_per_vertex(rp, col):
 def kclique(rp, col, k) -> int:
     rp = _u32(rp)
     return int(_lib.orc_kclique(len(rp) - 1, rp, _nz(_u32(col), np.uint32), k))
+
+
+def kclique_rows_strided(rp, col, k, start, stride) -> int:
+    rp = _u32(rp)
+    return int(_lib.orc_kclique_rows_strided(len(rp) - 1, rp, _nz(_u32(col), np.uint32), k,
+                                             start, stride))
 
 
 def hu_workload(rp, col):

@@ -392,3 +392,15 @@ def test_list2bin_bin2list_match_reference(tmp_path):
     _run_ref("bin2list", [str(tmp_path / "ref.bin"), str(tmp_path / "ref.txt")], str(tmp_path))
     tcamd.bin2list(str(tmp_path / "ours.bin"), str(tmp_path / "ours.txt"))
     assert open(tmp_path / "ref.txt").read() == open(tmp_path / "ours.txt").read()
+
+
+def test_kclique_rows_strided_partitions_the_count():
+    s, d = rmat(9, 8, 3)
+    n = orc.num_nodes(s, d)
+    deg = orc.degree_raw(s, d, n)
+    os_, od = orc.orient_simple(s, d, deg, orc.ORIENT_DEGREE)
+    rp, col = orc.build_csr(os_, od, n)
+    for k in (3, 4):
+        total = orc.kclique(rp, col, k)
+        parts = sum(orc.kclique_rows_strided(rp, col, k, r, 3) for r in range(3))
+        assert parts == total
