@@ -253,29 +253,45 @@ __device__ inline uint32_t wave_max_u32(uint32_t v) {
   return v;
 }
 
-template <int W>
-__global__ __launch_bounds__(64) void k_kcl_pivot(const uint32_t* __restrict__ rp,
-                                                  const uint32_t* __restrict__ col,
-                                                  const uint32_t* __restrict__ roots,
-                                                  uint64_t nroots, uint32_t k,
-                                                  uint64_t* __restrict__ stk_all,
-                                                  unsigned long long* total) {
+__device__ inline void wave_sync_kcl() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// NW waves share one root: they build its bit matrix together, then wave w
+// walks the sub-trees of the root's branches w, w + NW, ... (each branch's
+// candidate set is the root's C minus the earlier branch vertices, so the
+// branches are independent).  Every wave has its own LDS slot for the
+// current C and its own stack slice; a stack word is only read back by the
+// lane that wrote it (lane w holds word w of C and of the branch set).
+template <int W, int NW>
+__global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restrict__ rp,
+                                                       const uint32_t* __restrict__ col,
+                                                       const uint32_t* __restrict__ roots,
+                                                       uint64_t nroots, uint32_t copies,
+                                                       uint32_t k,
+                                                       uint64_t* __restrict__ stk_all,
+                                                       unsigned long long* total) {
   constexpr uint32_t D = 64 * W;
   constexpr uint32_t LW = 2 * W + 1;  // stack words per level
   __shared__ uint32_t s_list[D];
   __shared__ uint64_t s_M[D * W];
-  __shared__ uint64_t s_C[W];
-  const uint32_t lane = threadIdx.x;
-  uint64_t* stk = stk_all + (uint64_t)blockIdx.x * (D + 1) * LW;
-  unsigned long long cnt = 0;  // lane 0 accumulates
-  for (uint64_t ri = blockIdx.x; ri < nroots; ri += gridDim.x) {
-    const uint32_t u = roots[ri];
+  __shared__ uint64_t s_Cs[NW][W];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  uint64_t* s_C = s_Cs[wv];
+  uint64_t* stk = stk_all + ((uint64_t)blockIdx.x * NW + wv) * (D + 1) * LW;
+  unsigned long long cnt = 0;  // lane 0 of each wave accumulates
+  const uint32_t Q = copies * NW;  // walkers per root
+  for (uint64_t ti = blockIdx.x; ti < nroots * copies; ti += gridDim.x) {
+    const uint32_t u = roots[ti / copies];
+    const uint32_t q = (uint32_t)(ti % copies) * NW + wv;  // this wave's walker number
     const uint32_t base = rp[u], d = rp[u + 1] - base;
-    for (uint32_t i = lane; i < d; i += 64) s_list[i] = col[base + i];
-    for (uint32_t i = lane; i < d * W; i += 64) s_M[i] = 0;
+    for (uint32_t i = tid; i < d; i += 64 * NW) s_list[i] = col[base + i];
+    for (uint32_t i = tid; i < d * W; i += 64 * NW) s_M[i] = 0;
     __syncthreads();
     // symmetric adjacency of N+(u): bit (i, j) for every arc c_i -> c_j
-    for (uint32_t i = 0; i < d; ++i) {
+    for (uint32_t i = wv; i < d; i += NW) {
       const uint32_t ci = s_list[i];
       const uint32_t e1 = rp[ci + 1];
       for (uint32_t e = rp[ci] + lane; e < e1; e += 64) {
@@ -294,93 +310,154 @@ __global__ __launch_bounds__(64) void k_kcl_pivot(const uint32_t* __restrict__ r
     }
     __syncthreads();
 
-    // enter(): the node (s_C, h, p); counts it or pushes a level
-    int L = -1;
-    auto enter = [&](uint32_t h, uint32_t p) {
-      // |C| and, when needed, edges(C) or the pivot
+    // Node (s_C, h, p): counted in closed form (push = false / no pivot
+    // needed), or its pivot is returned for the caller to branch on.
+    // Returns true when the node needs branching; piv and its branch set
+    // (lanes < W, word `lane`) are written to *piv_out / *todo_out.
+    auto node = [&](uint32_t h, uint32_t p, uint32_t* piv_out, uint64_t* todo_out) -> bool {
+      uint64_t cr[W];  // C, the same in every lane
       uint32_t cs = 0;
-      for (int w = 0; w < W; ++w) cs += (uint32_t)__popcll(s_C[w]);
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        cr[w] = s_C[w];
+        cs += (uint32_t)__popcll(cr[w]);
+      }
       const uint32_t r = k - h;
       if (cs == 0) {
         if (lane == 0) cnt += binom_small(p, r);
-        return;
+        return false;
       }
-      if (h + p + cs < k) return;
+      if (h + p + cs < k) return false;
       if (r == 0) {
         if (lane == 0) cnt += 1;
-        return;
+        return false;
       }
       if (r == 1) {
         if (lane == 0) cnt += p + cs;
-        return;
-      }
-      // scores: |N(x) & C| for every x in C (lane x mod 64, word x / 64)
-      uint32_t best = 0, esum = 0;
-      for (int t = 0; t < W; ++t) {
-        const uint64_t cw = s_C[t];
-        const uint32_t x = 64u * t + lane;
-        if ((cw >> lane) & 1ull) {
-          uint32_t sc = 0;
-          for (int w = 0; w < W; ++w) sc += (uint32_t)__popcll(s_M[x * W + w] & s_C[w]);
-          esum += sc;
-          // (score, smallest index) packed for a max-reduction
-          const uint32_t key = (sc << 16) | (0xFFFFu - x);
-          best = max(best, key);
-        }
+        return false;
       }
       if (r == 2) {
+        // edges(C) = half the sum over x in C of |N(x) & C| (lane x mod 64,
+        // word x / 64); empty words of C hold no x
+        uint32_t esum = 0;
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+          if (cr[t] == 0) continue;
+          const uint32_t x = 64u * t + lane;
+          if ((cr[t] >> lane) & 1ull) {
+#pragma unroll
+            for (int w = 0; w < W; ++w) esum += (uint32_t)__popcll(s_M[x * W + w] & cr[w]);
+          }
+        }
         const uint32_t e2 = wave_sum_u32(esum);
         if (lane == 0) cnt += binom_small(p, 2) + (unsigned long long)p * cs + e2 / 2;
-        return;
+        return false;
+      }
+      // pivot: the member of C's first non-empty 64-word with most
+      // neighbours in C (any member gives exact counts; this one keeps the
+      // choice at W reads per lane)
+      uint32_t best = 0;
+      int t0 = 0;
+#pragma unroll
+      for (int t = W - 1; t >= 0; --t)
+        if (cr[t]) t0 = t;
+      {
+        uint64_t ct = 0;
+#pragma unroll
+        for (int t = 0; t < W; ++t)
+          if (t == t0) ct = cr[t];
+        const uint32_t x = 64u * (uint32_t)t0 + lane;
+        if ((ct >> lane) & 1ull) {
+          uint32_t sc = 0;
+#pragma unroll
+          for (int w = 0; w < W; ++w) sc += (uint32_t)__popcll(s_M[x * W + w] & cr[w]);
+          best = (sc << 16) | (0xFFFFu - x);  // (score, smallest index)
+        }
       }
       best = wave_max_u32(best);
       const uint32_t piv = 0xFFFFu - (best & 0xFFFFu);
-      ++L;
-      uint64_t* lv = stk + (uint64_t)L * LW;
+      *piv_out = piv;
       if (lane < (uint32_t)W) {
-        const uint64_t c = s_C[lane];
-        uint64_t todo = c & ~s_M[piv * W + lane];
+        uint64_t todo = s_C[lane] & ~s_M[piv * W + lane];  // C outside N(pivot)
         if (lane == (piv >> 6)) todo |= 1ull << (piv & 63);
-        lv[lane] = c;
-        lv[W + lane] = todo;
+        *todo_out = todo;
       }
-      if (lane == 0) lv[2 * W] = (uint64_t)piv | ((uint64_t)h << 16) | ((uint64_t)p << 32);
-      __syncthreads();
+      return true;
     };
 
-    for (int w = lane; w < W; w += 64) {
-      const uint32_t lo = 64u * w;
-      s_C[w] = d >= lo + 64 ? ~0ull : (d > lo ? ((1ull << (d - lo)) - 1) : 0ull);
+    // Every walker runs the same depth-first walk from the root.  Nodes with
+    // h = 1 (the root and its pivot chain) are walked by all of them and
+    // counted by walker 0; the sub-trees entered through a first non-pivot
+    // branch (h = 2) are numbered in walk order and walker q takes those
+    // numbered q mod Q.  One heavy root thus spreads over Q waves.
+    if (lane < (uint32_t)W) {
+      const uint32_t lo = 64u * lane;
+      s_C[lane] = d >= lo + 64 ? ~0ull : (d > lo ? ((1ull << (d - lo)) - 1) : 0ull);
     }
-    __syncthreads();
-    enter(1, 0);
+    wave_sync_kcl();
+    uint32_t seq = 0;
+    int L = -1;
+    uint32_t piv = 0;
+    uint64_t todo = 0;
+    {
+      const unsigned long long before = cnt;
+      const bool br = node(1, 0, &piv, &todo);
+      if (q != 0) cnt = before;
+      if (br) {
+        L = 0;
+        if (lane < (uint32_t)W) {
+          stk[lane] = s_C[lane];
+          stk[W + lane] = todo;
+        }
+        if (lane == 0) stk[2 * W] = (uint64_t)piv | (1ull << 16);
+      }
+    }
+    wave_sync_kcl();
     while (L >= 0) {
       uint64_t* lv = stk + (uint64_t)L * LW;
       const uint64_t todo_w = lane < (uint32_t)W ? lv[W + lane] : 0ull;
-      const uint64_t nz = __ballot(todo_w != 0);
-      if (nz == 0) {
+      const uint64_t nzl = __ballot(todo_w != 0);
+      if (nzl == 0) {
         --L;
         continue;
       }
-      const uint32_t fw = (uint32_t)__builtin_ctzll(nz);
-      const uint64_t tw = __shfl(todo_w, fw, 64);
-      const uint32_t v = fw * 64u + (uint32_t)__builtin_ctzll(tw);
-      const uint64_t meta = lv[2 * W];
-      const uint32_t piv = (uint32_t)(meta & 0xFFFFu), h = (uint32_t)((meta >> 16) & 0xFFFFu),
+      const uint32_t fwl = (uint32_t)__builtin_ctzll(nzl);
+      const uint64_t twl = __shfl(todo_w, fwl, 64);
+      const uint32_t x = fwl * 64u + (uint32_t)__builtin_ctzll(twl);
+      const uint64_t meta = __shfl(lane == 0 ? lv[2 * W] : 0ull, 0, 64);
+      const uint32_t lpiv = (uint32_t)(meta & 0xFFFFu), h = (uint32_t)((meta >> 16) & 0xFFFFu),
                      p = (uint32_t)(meta >> 32);
-      // take v: child C = C & N(v); v leaves C and the branch list
+      const bool xp = x == lpiv;
+      const uint32_t h2 = h + (xp ? 0u : 1u), p2 = p + (xp ? 1u : 0u);
+      // take x: child C = C & N(x); x leaves C and the branch set
+      bool skip = h2 > k;
+      if (h == 1 && h2 == 2) {
+        const bool mine = (seq++ % Q) == q;
+        skip = skip || !mine;
+      }
       if (lane < (uint32_t)W) {
         const uint64_t c = lv[lane];
-        const uint64_t vb = lane == fw ? (1ull << (v & 63)) : 0ull;
-        s_C[lane] = c & s_M[v * W + lane];
-        lv[lane] = c & ~vb;
-        lv[W + lane] = todo_w & ~vb;
+        const uint64_t xb = lane == fwl ? (1ull << (x & 63)) : 0ull;
+        if (!skip) s_C[lane] = c & s_M[x * W + lane];
+        lv[lane] = c & ~xb;
+        lv[W + lane] = todo_w & ~xb;
       }
-      __syncthreads();
-      const bool is_piv = v == piv;
-      const uint32_t h2 = h + (is_piv ? 0u : 1u), p2 = p + (is_piv ? 1u : 0u);
-      if (h2 > k) continue;
-      enter(h2, p2);
+      wave_sync_kcl();
+      if (skip) continue;
+      // an h = 1 child is walked by every walker: walker 0 keeps its count
+      const unsigned long long before = cnt;
+      const bool br = node(h2, p2, &piv, &todo);
+      if (h2 == 1 && q != 0) cnt = before;
+      if (br) {
+        ++L;
+        uint64_t* nv = stk + (uint64_t)L * LW;
+        if (lane < (uint32_t)W) {
+          nv[lane] = s_C[lane];
+          nv[W + lane] = todo;
+        }
+        if (lane == 0) nv[2 * W] = (uint64_t)piv | ((uint64_t)h2 << 16) | ((uint64_t)p2 << 32);
+      }
+      wave_sync_kcl();
     }
     __syncthreads();
   }
@@ -470,18 +547,26 @@ int kcl_count(tc_ctx* c, tc_graph* g, uint32_t k, int variant, uint64_t* count,
       hipLaunchKernelGGL(k_kcl_merge, dim3(G), dim3(64), 0, c->stream, rp, col,
                          roots.as<uint32_t>(), nr, k, maxd, slots, scratch.as<uint32_t>(), acc);
     } else if (b.pivot) {
-      // waves resident per CU: LDS-bound for the wide matrices
+      // workgroups resident per CU (LDS-bound for the wide matrices); the
+      // wide ones run four walkers per root, and a root is repeated
+      // `copies` times so that the heaviest roots spread over many CUs
       const uint64_t per_cu = b.w <= 4 ? 16 : (b.w == 8 ? 4 : 1);
-      const uint32_t G = (uint32_t)std::min<uint64_t>(nr, (uint64_t)c->num_cus * per_cu);
+      const uint64_t nw = b.w <= 4 ? 1 : 4;
+      const uint64_t slots = (uint64_t)c->num_cus * per_cu;
+      const uint64_t max_copies = b.w <= 4 ? 1 : (b.w == 8 ? 8 : 32);
+      const uint32_t copies =
+          (uint32_t)std::min<uint64_t>(max_copies, std::max<uint64_t>(1, (4 * slots + nr - 1) / nr));
+      const uint32_t G = (uint32_t)std::min<uint64_t>(nr * copies, slots);
       const uint64_t D = 64ull * b.w;
-      TCX_TRY(scratch.reserve(8ull * G * (D + 1) * (2ull * b.w + 1)));
+      TCX_TRY(scratch.reserve(8ull * G * nw * (D + 1) * (2ull * b.w + 1)));
       uint64_t* stk = scratch.as<uint64_t>();
+      const uint32_t* rts = roots.as<uint32_t>();
       switch (b.w) {
-        case 1: hipLaunchKernelGGL(k_kcl_pivot<1>, dim3(G), dim3(64), 0, c->stream, rp, col, roots.as<uint32_t>(), nr, k, stk, acc); break;
-        case 2: hipLaunchKernelGGL(k_kcl_pivot<2>, dim3(G), dim3(64), 0, c->stream, rp, col, roots.as<uint32_t>(), nr, k, stk, acc); break;
-        case 4: hipLaunchKernelGGL(k_kcl_pivot<4>, dim3(G), dim3(64), 0, c->stream, rp, col, roots.as<uint32_t>(), nr, k, stk, acc); break;
-        case 8: hipLaunchKernelGGL(k_kcl_pivot<8>, dim3(G), dim3(64), 0, c->stream, rp, col, roots.as<uint32_t>(), nr, k, stk, acc); break;
-        default: hipLaunchKernelGGL(k_kcl_pivot<16>, dim3(G), dim3(64), 0, c->stream, rp, col, roots.as<uint32_t>(), nr, k, stk, acc); break;
+        case 1: hipLaunchKernelGGL((k_kcl_pivot<1, 1>), dim3(G), dim3(64), 0, c->stream, rp, col, rts, nr, copies, k, stk, acc); break;
+        case 2: hipLaunchKernelGGL((k_kcl_pivot<2, 1>), dim3(G), dim3(64), 0, c->stream, rp, col, rts, nr, copies, k, stk, acc); break;
+        case 4: hipLaunchKernelGGL((k_kcl_pivot<4, 1>), dim3(G), dim3(64), 0, c->stream, rp, col, rts, nr, copies, k, stk, acc); break;
+        case 8: hipLaunchKernelGGL((k_kcl_pivot<8, 4>), dim3(G), dim3(256), 0, c->stream, rp, col, rts, nr, copies, k, stk, acc); break;
+        default: hipLaunchKernelGGL((k_kcl_pivot<16, 4>), dim3(G), dim3(256), 0, c->stream, rp, col, rts, nr, copies, k, stk, acc); break;
       }
     } else {
       uint32_t G = (uint32_t)std::min<uint64_t>(nr, (uint64_t)c->num_cus * 16);
