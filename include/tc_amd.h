@@ -255,8 +255,12 @@ int tc_graph_features(tc_ctx* ctx, tc_graph* g, float* X);
  * 0 where not measured.  count (optional) receives the run's exact count. */
 int tc_graph_measure_costs(tc_ctx* ctx, tc_graph* g, int variant, float* cost_us,
                            uint64_t* count);
-/* Variant chosen for each vertex by the model (255 = not scheduled, d+ < 2);
- * model NULL gives the fixed degree-bin rule (wave iff d+ <= 64). */
+/* Variant chosen for each vertex (255 = not scheduled, d+ < 2).  Per degree
+ * bin floor(log2 d+), the model's summed wave-time of the two variants picks
+ * the bin's variant, with the fixed rule (wave iff d+ <= 64) as the prior: a
+ * bin switches only for a predicted saving above 25 %.  A wave task
+ * predicted to take more than half of a wave's share goes to the block
+ * variant.  model NULL gives the fixed rule. */
 int tc_learned_assignment(tc_ctx* ctx, tc_graph* g, tc_model* m, uint8_t* variant);
 /* Exact count with the learned schedule; model NULL falls back to the fixed
  * degree-bin schedule of TC_KERNEL_AUTO. */
