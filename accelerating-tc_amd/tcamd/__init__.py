@@ -125,6 +125,8 @@ class TrainOpts(C.Structure):
                 ("seed", C.c_uint32), ("reserved", C.c_uint32 * 4)]
 
 
+_sig("tc_cost_functions", C.c_int, [_vp, _vp, C.c_int, _f32p])
+_sig("tc_approx_ratio", C.c_int, [_vp, _vp, C.POINTER(C.c_double)])
 _sig("tc_model_num_params", C.c_uint32, [])
 _sig("tc_model_create", C.c_int, [_vp, C.c_uint64, C.POINTER(_vp)])
 _sig("tc_model_destroy", C.c_int, [_vp])
@@ -178,7 +180,7 @@ EXPORTED = [
     "tc_graph_simplify", "tc_graph_relabel_aorder", "tc_graph_orient", "tc_graph_info_get",
     "tc_graph_get_csr", "tc_graph_get_edges", "tc_graph_get_vertex_key", "tc_graph_device_csr",
     "tc_hu_workload", "tc_hu_block_starts", "tc_fox_workload", "tc_fox_bin_sizes", "tc_count",
-    "tc_count_per_vertex", "kcl_count", "kcl_gpu_solver", "tc_tricore_init_gpu",
+    "tc_count_per_vertex", "tc_cost_functions", "tc_approx_ratio", "kcl_count", "kcl_gpu_solver", "tc_tricore_init_gpu",
     "tc_tricore_cal_degree", "tc_tricore_redirect_edges", "tc_tricore_cal_part_num",
     "tc_tricore_split", "tc_tricore_tricount_gpu", "tc_model_num_params", "tc_model_create",
     "tc_model_destroy", "tc_model_set_params", "tc_model_get_params", "tc_model_set_norm",
@@ -499,6 +501,18 @@ class Graph:
         _check(_lib.tc_count(self.ctx.handle, self._h, C.byref(s), C.byref(out), C.byref(t)),
                "tc_count")
         return CountResult(out.value, t.as_dict())
+
+    def cost_functions(self, k: int = 1):
+        """(idPartCost, degreePartCost, lpPartCost, f_lp, f_degree) of the raw
+        records: tools/costFuncCal.cpp and tools/verification.cpp."""
+        out = np.zeros(5, np.float32)
+        _check(_lib.tc_cost_functions(self.ctx.handle, self._h, k, out), "tc_cost_functions")
+        return out
+
+    def approx_ratio(self) -> float:
+        r = C.c_double()
+        _check(_lib.tc_approx_ratio(self.ctx.handle, self._h, C.byref(r)), "tc_approx_ratio")
+        return r.value
 
     def count_per_vertex(self):
         n = self.info().n

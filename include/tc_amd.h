@@ -189,6 +189,17 @@ int tc_fox_workload(tc_ctx* ctx, tc_graph* g, int32_t* w);
  * bins per Fox/tc.cu:316-331. */
 int tc_fox_bin_sizes(tc_ctx* ctx, tc_graph* g, uint32_t max_lanes, uint64_t* count);
 
+/* ---- orientation cost functions of the raw records -------------------
+ * The values the reference's analysis tools print, from the raw records
+ * (ids as read, nodes = max id + 1):
+ *   out[0..2] = idPartCost, degreePartCost, lpPartCost of
+ *               tools/costFuncCal.cpp:40-88 with threshold factor k;
+ *   out[3..4] = f_lp, f_degree of tools/verification.cpp:36-60.
+ * Counts run on the GPU; the float sums follow the tools' order exactly. */
+int tc_cost_functions(tc_ctx* ctx, tc_graph* g, int k, float* out /*5*/);
+/* Approximation ratio bound of tools/approximateRatio.cpp:37-94. */
+int tc_approx_ratio(tc_ctx* ctx, tc_graph* g, double* ratio);
+
 /* ---- L4/L5: counting --------------------------------------------------- */
 /* Exact triangle count of the oriented CSR (uint64; the reference prints it as
  * "There are N triangles in the input graph.", Hu/tc.cu:337).  sched may be

@@ -320,3 +320,19 @@ def test_larger_rmat_all_kernels(ctx):
     for k in KERNELS:
         assert g.count(k).count == expect, k
     g.close()
+
+
+@pytest.mark.parametrize("case", ["er", "rmat12", "ba", "rgg", "hub", "messy", "k6"])
+def test_cost_functions_match_oracle(ctx, case):
+    """tools/costFuncCal.cpp, tools/verification.cpp, tools/approximateRatio.cpp:
+    the same float / double values, bit for bit."""
+    s, d = CASES[case]()
+    g = ctx.graph(s, d)
+    for k in (0, 1, 3):
+        got = g.cost_functions(k)
+        ref = np.concatenate([np.asarray(orc.cost_func(s, d, k), np.float32),
+                              np.asarray(orc.verification(s, d), np.float32)])
+        np.testing.assert_array_equal(got, ref)
+    assert g.approx_ratio() == orc.approx_ratio(s, d) or (
+        np.isnan(g.approx_ratio()) and np.isnan(orc.approx_ratio(s, d)))
+    g.close()
