@@ -275,6 +275,55 @@ __global__ void k_aorder_outdeg(const uint32_t* __restrict__ s, const uint32_t* 
   }
 }
 
+// ---- id relabellings of the format tools ----------------------------------
+__global__ void k_present_flags(const uint32_t* __restrict__ deg, uint32_t n,
+                                uint32_t* __restrict__ f) {
+  for (uint64_t v = gtid(); v < n; v += gstride()) f[v] = deg[v] ? 1u : 0u;
+}
+
+__global__ void k_fill_u64(uint64_t* p, uint64_t v, uint64_t n) {
+  for (uint64_t i = gtid(); i < n; i += gstride()) p[i] = v;
+}
+
+// first[v] = smallest record index whose source is v
+__global__ void k_first_src(const uint32_t* __restrict__ s, uint64_t m,
+                            unsigned long long* first) {
+  for (uint64_t i = gtid(); i < m; i += gstride()) atomicMin(first + s[i], (unsigned long long)i);
+}
+
+// adjust2tc order: ids by first appearance as a source, then the others by id
+__global__ void k_first_keys(const uint64_t* __restrict__ first, uint32_t n, uint64_t m,
+                             uint64_t* __restrict__ key, uint32_t* __restrict__ ids) {
+  for (uint64_t v = gtid(); v < n; v += gstride()) {
+    key[v] = first[v] != ~0ull ? first[v] : m + v;
+    ids[v] = (uint32_t)v;
+  }
+}
+
+// directionByReorder order: (degree, id) ascending
+__global__ void k_degree_keys(const uint32_t* __restrict__ deg, uint32_t n,
+                              uint64_t* __restrict__ key, uint32_t* __restrict__ ids) {
+  for (uint64_t v = gtid(); v < n; v += gstride()) {
+    key[v] = ((uint64_t)deg[v] << 32) | v;
+    ids[v] = (uint32_t)v;
+  }
+}
+
+// perm[order[j]] = j, or n - 1 - j when reversed
+__global__ void k_rank_perm(const uint32_t* __restrict__ order, uint32_t n, bool reversed,
+                            uint32_t* __restrict__ perm) {
+  for (uint64_t j = gtid(); j < n; j += gstride())
+    perm[order[j]] = reversed ? (uint32_t)(n - 1 - j) : (uint32_t)j;
+}
+
+__global__ void k_apply_perm(uint32_t* s, uint32_t* d, uint64_t m,
+                             const uint32_t* __restrict__ perm) {
+  for (uint64_t i = gtid(); i < m; i += gstride()) {
+    s[i] = perm[s[i]];
+    d[i] = perm[d[i]];
+  }
+}
+
 __global__ void k_aorder_apply(uint32_t* s, uint32_t* d, uint64_t m, const uint32_t* __restrict__ deg,
                                const uint32_t* __restrict__ perm) {
   for (uint64_t i = gtid(); i < m; i += gstride()) {
@@ -636,6 +685,80 @@ int tc_graph_relabel_aorder(tc_ctx* c, tc_graph* g, uint32_t* perm_out) {
   TCX_HIP(hipGetLastError());
   TCX_HIP(hipStreamSynchronize(c->stream));
   if (perm_out) std::copy(h_perm.begin(), h_perm.end(), perm_out);
+  g->orient_mode = -1;  // any CSR built before is stale
+  return TC_OK;
+}
+
+int tc_graph_relabel(tc_ctx* c, tc_graph* g, int mode, uint32_t* perm_out) {
+  TCX_TRY(check_graph(g));
+  if (c != g->ctx) return fail(TC_ERR_ARG, "graph belongs to another context");
+  if (mode < TC_RELABEL_COMPACT || mode > TC_RELABEL_DEGREE)
+    return fail(TC_ERR_ARG, "unknown relabel mode %d", mode);
+  const uint64_t m = g->m_raw;
+  uint32_t* s = g->src.as<uint32_t>();
+  uint32_t* d = g->dst.as<uint32_t>();
+  uint32_t n = 1;  // max id + 1 over the records, as the tools compute it
+  if (m) {
+    uint32_t ms = 0, md = 0;
+    TCX_TRY(max_u32(c, s, m, &ms));
+    TCX_TRY(max_u32(c, d, m, &md));
+    n = std::max(ms, md) + 1;
+  }
+  DevBuf deg, perm, key, ktmp, ids, itmp;
+  TCX_TRY(perm.reserve(4ull * n));
+  uint32_t n_new = n;
+  if (mode == TC_RELABEL_COMPACT || mode == TC_RELABEL_DEGREE) {
+    TCX_TRY(deg.reserve(4ull * n));
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(TB), 0, c->stream,
+                       deg.as<uint32_t>(), 0u, (uint64_t)n);
+    TCX_TRY(count_ids(c, s, d, m, n, deg.as<uint32_t>()));
+  }
+  if (mode == TC_RELABEL_COMPACT) {
+    // new id = number of present ids below (tools/serialize.cpp:32-44)
+    DevBuf flags;
+    TCX_TRY(flags.reserve(4ull * n));
+    hipLaunchKernelGGL(k_present_flags, dim3(grid_for(n)), dim3(TB), 0, c->stream,
+                       deg.as<uint32_t>(), n, flags.as<uint32_t>());
+    TCX_TRY(exclusive_scan_u32(c, flags.as<uint32_t>(), perm.as<uint32_t>(), n));
+    uint32_t last_perm = 0, last_flag = 0;
+    TCX_TRY(read_u32(c, perm.as<uint32_t>() + n - 1, &last_perm));
+    TCX_TRY(read_u32(c, flags.as<uint32_t>() + n - 1, &last_flag));
+    n_new = m ? last_perm + last_flag : 1;
+  } else {
+    TCX_TRY(key.reserve(8ull * n));
+    TCX_TRY(ktmp.reserve(8ull * n));
+    TCX_TRY(ids.reserve(4ull * n));
+    TCX_TRY(itmp.reserve(4ull * n));
+    if (mode == TC_RELABEL_FIRST_SRC) {
+      DevBuf first;
+      TCX_TRY(first.reserve(8ull * n));
+      hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n)), dim3(TB), 0, c->stream,
+                         first.as<uint64_t>(), ~0ull, (uint64_t)n);
+      if (m)
+        hipLaunchKernelGGL(k_first_src, dim3(grid_for(m)), dim3(TB), 0, c->stream, s, m,
+                           first.as<unsigned long long>());
+      hipLaunchKernelGGL(k_first_keys, dim3(grid_for(n)), dim3(TB), 0, c->stream,
+                         first.as<uint64_t>(), n, m, key.as<uint64_t>(), ids.as<uint32_t>());
+    } else {
+      hipLaunchKernelGGL(k_degree_keys, dim3(grid_for(n)), dim3(TB), 0, c->stream,
+                         deg.as<uint32_t>(), n, key.as<uint64_t>(), ids.as<uint32_t>());
+    }
+    TCX_HIP(hipGetLastError());
+    uint64_t* sk;
+    uint32_t* order;
+    TCX_TRY(sort_pairs_u64_u32(c, key.as<uint64_t>(), ktmp.as<uint64_t>(), ids.as<uint32_t>(),
+                               itmp.as<uint32_t>(), n, 64, &sk, &order));
+    hipLaunchKernelGGL(k_rank_perm, dim3(grid_for(n)), dim3(TB), 0, c->stream, order, n,
+                       mode == TC_RELABEL_DEGREE, perm.as<uint32_t>());
+  }
+  if (m)
+    hipLaunchKernelGGL(k_apply_perm, dim3(grid_for(m)), dim3(TB), 0, c->stream, s, d, m,
+                       perm.as<uint32_t>());
+  TCX_HIP(hipGetLastError());
+  if (perm_out)
+    TCX_HIP(hipMemcpyAsync(perm_out, perm.p, 4ull * n, hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  g->n = n_new;
   g->orient_mode = -1;  // any CSR built before is stale
   return TC_OK;
 }

@@ -42,6 +42,7 @@ KERNEL_HU, KERNEL_BISSON, KERNEL_FOX, KERNEL_TRICORE, KERNEL_AUTO = 0, 1, 2, 3, 
 KERNEL_NAMES = {"hu": KERNEL_HU, "bisson": KERNEL_BISSON, "fox": KERNEL_FOX,
                 "tricore": KERNEL_TRICORE, "auto": KERNEL_AUTO}
 KCL_AUTO, KCL_BITMAP, KCL_MERGE, KCL_PIVOT = 0, 1, 2, 3
+RELABEL_COMPACT, RELABEL_FIRST_SRC, RELABEL_DEGREE = 0, 1, 2
 
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -105,6 +106,7 @@ _sig("tc_graph_destroy", C.c_int, [_vp])
 _sig("tc_graph_simplify", C.c_int, [_vp, _vp])
 _sig("tc_graph_relabel_aorder", C.c_int, [_vp, _vp, _vp])
 _sig("tc_graph_orient", C.c_int, [_vp, _vp, C.c_int, C.c_uint32])
+_sig("tc_graph_relabel", C.c_int, [_vp, _vp, C.c_int, _vp])
 _sig("tc_graph_info_get", C.c_int, [_vp, C.POINTER(GraphInfo)])
 _sig("tc_graph_get_csr", C.c_int, [_vp, _vp, _vp])
 _sig("tc_graph_get_edges", C.c_int, [_vp, _vp, _vp])
@@ -177,7 +179,7 @@ EXPORTED = [
     "tc_status_str", "tc_last_error", "tc_ctx_create", "tc_ctx_destroy", "tc_ctx_device_memory",
     "tc_file_edge_count", "tc_read_edges", "tc_write_edges", "tc_list2bin", "tc_bin2list", "tc_read_gr",
     "tc_gen_er", "tc_gen_rmat", "tc_gen_ba", "tc_gen_rgg", "tc_graph_create", "tc_graph_destroy",
-    "tc_graph_simplify", "tc_graph_relabel_aorder", "tc_graph_orient", "tc_graph_info_get",
+    "tc_graph_simplify", "tc_graph_relabel_aorder", "tc_graph_relabel", "tc_graph_orient", "tc_graph_info_get",
     "tc_graph_get_csr", "tc_graph_get_edges", "tc_graph_get_vertex_key", "tc_graph_device_csr",
     "tc_hu_workload", "tc_hu_block_starts", "tc_fox_workload", "tc_fox_bin_sizes", "tc_count",
     "tc_count_per_vertex", "tc_cost_functions", "tc_approx_ratio", "kcl_count", "kcl_gpu_solver", "tc_tricore_init_gpu",
@@ -422,6 +424,17 @@ class Graph:
     def simplify(self):
         _check(_lib.tc_graph_simplify(self.ctx.handle, self._h), "tc_graph_simplify")
         return self
+
+    def relabel(self, mode: int):
+        """Id relabelling of tools/serialize.cpp (RELABEL_COMPACT),
+        tools/adjust2tc.cpp (RELABEL_FIRST_SRC) or
+        tools/directionByReorder.cpp (RELABEL_DEGREE); returns old -> new."""
+        s, d = self.edges()
+        nids = int(max(s.max(), d.max())) + 1 if len(s) else 1
+        perm = np.empty(nids, np.uint32)
+        _check(_lib.tc_graph_relabel(self.ctx.handle, self._h, mode, perm.ctypes.data),
+               "tc_graph_relabel")
+        return perm
 
     def relabel_aorder(self):
         n = self.info().n

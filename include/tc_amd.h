@@ -164,6 +164,21 @@ int tc_graph_simplify(tc_ctx* ctx, tc_graph* g);
  * the records are rewritten with new ids.  perm (optional, n entries) gets
  * old->new.  TC_ERR_UNSUPPORTED outside the reference's defined range. */
 int tc_graph_relabel_aorder(tc_ctx* ctx, tc_graph* g, uint32_t* perm);
+/* Id relabellings of the reference's format tools, applied to the raw
+ * records on the GPU (records keep their order).  perm (optional) gets
+ * old -> new for ids 0 .. max id (max id + 1 entries, max over the records
+ * before the call).  The graph's n becomes the largest new id + 1.
+ *   TC_RELABEL_COMPACT   tools/serialize.cpp:17-47: ids that occur keep their
+ *                        order, ids that never occur are squeezed out
+ *   TC_RELABEL_FIRST_SRC tools/adjust2tc.cpp:13-35: numbered by first
+ *                        appearance as a source, then the rest by id
+ *   TC_RELABEL_DEGREE    tools/directionByReorder.cpp:38-70 (and Bisson's
+ *                        chooseIndex != 0, Bisson/preTC.cu:146-161): sorted
+ *                        by (degree, id), the highest degree gets id 0 */
+#define TC_RELABEL_COMPACT 0
+#define TC_RELABEL_FIRST_SRC 1
+#define TC_RELABEL_DEGREE 2
+int tc_graph_relabel(tc_ctx* ctx, tc_graph* g, int mode, uint32_t* perm);
 /* Orients the raw records and builds the oriented CSR in HBM
  * (Hu/preTC.cu:116-291, Fox/preTC.cu:88-203, tricore/src/tricount_gpu.cu:37-203,
  * edge_directing/A-direction-seri.cpp:35-273, edge_directing/kcore_direction.cpp:25-146). */

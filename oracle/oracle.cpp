@@ -668,6 +668,52 @@ void orc_verification(const uint32_t* src, const uint32_t* dst, uint64_t m,
   out[1] = f_degree;
 }
 
+void orc_relabel_compact(const uint32_t* src, const uint32_t* dst, uint64_t m, uint32_t* perm) {
+  // tools/serialize.cpp:17-47: present ids keep their order, absent ids are
+  // squeezed out (new = id - number of absent ids below it)
+  uint32_t mx = 0;
+  for (uint64_t i = 0; i < m; ++i) mx = std::max(mx, std::max(src[i], dst[i]));
+  std::vector<uint8_t> present((size_t)mx + 1, 0);
+  for (uint64_t i = 0; i < m; ++i) present[src[i]] = present[dst[i]] = 1;
+  uint32_t absent = 0;
+  for (uint32_t i = 0; i <= mx; ++i) {
+    perm[i] = i - absent;
+    if (!present[i]) ++absent;
+  }
+}
+
+void orc_relabel_first_src(const uint32_t* src, const uint32_t* dst, uint64_t m,
+                           uint32_t* perm) {
+  // tools/adjust2tc.cpp:13-35: ids numbered in order of first appearance as
+  // a source, then every other id up to the largest in id order
+  uint32_t mx = 0;
+  for (uint64_t i = 0; i < m; ++i) mx = std::max(mx, std::max(src[i], dst[i]));
+  std::vector<int64_t> f((size_t)mx + 1, -1);
+  int64_t count = 0;
+  for (uint64_t i = 0; i < m; ++i)
+    if (f[src[i]] == -1) f[src[i]] = count++;
+  for (uint32_t i = 0; i <= mx; ++i)
+    if (f[i] == -1) f[i] = count++;
+  for (uint32_t i = 0; i <= mx; ++i) perm[i] = (uint32_t)f[i];
+}
+
+void orc_relabel_degree(const uint32_t* src, const uint32_t* dst, uint64_t m, uint32_t* perm) {
+  // tools/directionByReorder.cpp:38-70: sort (degree, id) ascending; the
+  // i-th gets id n - 1 - i, so the highest degree becomes 0
+  uint32_t mx = 0;
+  for (uint64_t i = 0; i < m; ++i) mx = std::max(mx, std::max(src[i], dst[i]));
+  const uint64_t n = (uint64_t)mx + 1;
+  std::vector<uint32_t> deg(n, 0);
+  for (uint64_t i = 0; i < m; ++i) {
+    deg[src[i]]++;
+    deg[dst[i]]++;
+  }
+  std::vector<std::pair<uint32_t, uint32_t>> u(n);
+  for (uint64_t i = 0; i < n; ++i) u[i] = {deg[i], (uint32_t)i};
+  std::sort(u.begin(), u.end());
+  for (uint64_t i = 0; i < n; ++i) perm[u[i].second] = (uint32_t)(n - 1 - i);
+}
+
 double orc_approx_ratio(const uint32_t* src, const uint32_t* dst, uint64_t m) {
   // tools/approximateRatio.cpp:16-94
   int maxNodeId = 0;

@@ -158,6 +158,15 @@ void orc_cost_func(const uint32_t* src, const uint32_t* dst, uint64_t m,
 /* tools/verification.cpp:11-60 -> out[0] = f_lp, out[1] = f_degree. */
 void orc_verification(const uint32_t* src, const uint32_t* dst, uint64_t m,
                       float* out);
+/* Id relabellings of the format tools; perm (max id + 1 entries) gets
+ * old -> new.  pinned: the reference tools built in oracle/_ref.
+ *   tools/serialize.cpp:17-47           present ids compacted in order
+ *   tools/adjust2tc.cpp:13-35           first appearance as a source first
+ *   tools/directionByReorder.cpp:38-70  by (degree, id), highest degree -> 0 */
+void orc_relabel_compact(const uint32_t* src, const uint32_t* dst, uint64_t m, uint32_t* perm);
+void orc_relabel_first_src(const uint32_t* src, const uint32_t* dst, uint64_t m,
+                           uint32_t* perm);
+void orc_relabel_degree(const uint32_t* src, const uint32_t* dst, uint64_t m, uint32_t* perm);
 /* tools/approximateRatio.cpp:14-94 -> approximation ratio. */
 double orc_approx_ratio(const uint32_t* src, const uint32_t* dst, uint64_t m);
 

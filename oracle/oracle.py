@@ -59,6 +59,8 @@ _sig("orc_kclique", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32])
 _sig("orc_kclique_rows_strided", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32,
                                                C.c_uint32])
 _sig("orc_hu_workload", None, [C.c_uint32, _u32p, _u32p, _i64p])
+for _name in ("orc_relabel_compact", "orc_relabel_first_src", "orc_relabel_degree"):
+    _sig(_name, None, [_u32p, _u32p, C.c_uint64, _u32p])
 _sig("orc_hu_block_starts", None, [C.c_uint32, _i64p, C.c_uint32, _i32p])
 _sig("orc_fox_workload", None, [C.c_uint64, _u32p, _u32p, _u32p, _i32p])
 _sig("orc_fox_bins", None, [_i32p, C.c_uint32, C.c_uint32, _u32p])
@@ -247,6 +249,18 @@ def verification(src, dst):
     out = np.zeros(2, np.float32)
     _lib.orc_verification(_u32(src), _u32(dst), len(src), out)
     return out
+
+
+def relabel(src, dst, mode):
+    """old -> new id map of tools/serialize.cpp ("compact"),
+    tools/adjust2tc.cpp ("first_src") or tools/directionByReorder.cpp ("degree")."""
+    src, dst = _u32(src), _u32(dst)
+    n = num_nodes(src, dst)
+    perm = np.zeros(max(n, 1), np.uint32)
+    fn = {"compact": _lib.orc_relabel_compact, "first_src": _lib.orc_relabel_first_src,
+          "degree": _lib.orc_relabel_degree}[mode]
+    fn(_nz(src, np.uint32), _nz(dst, np.uint32), len(src), perm)
+    return perm[:n]
 
 
 def approx_ratio(src, dst) -> float:

@@ -120,3 +120,26 @@ def test_kclique_driver_reads_gr(tmp_path, k):
     out = run("kcl", "-symmetricGraph", "-simpleGraph", gr, f"-k={k}")
     assert int(re.search(r"total_num_cliques = (\d+)", out).group(1)) == \
         [56, 70, 56][k - 3]
+
+
+def test_relabel_drivers(graph_file, tmp_path):
+    _, s, d, t = graph_file
+    s = (s.astype(np.int64) * 2 + 1).astype(np.uint32)
+    d = (d.astype(np.int64) * 2 + 1).astype(np.uint32)
+    p = str(tmp_path / "holes.bin")
+    tcamd.write_edges(p, s, d)
+    for exe, mode in (("serialize", "compact"), ("direction_by_reorder", "degree")):
+        out = str(tmp_path / f"{exe}.bin")
+        run(exe, p, out)
+        rs, rd = tcamd.read_edges(out)
+        perm = orc.relabel(s, d, mode)
+        np.testing.assert_array_equal(rs, perm[s])
+        np.testing.assert_array_equal(rd, perm[d])
+        assert triangles(run("tc", "-f", out)) == t
+    txt = tmp_path / "g.txt"
+    txt.write_text("".join(f"{a} {b}\n" for a, b in zip(s.tolist(), d.tolist())))
+    run("adjust2tc", str(txt), str(tmp_path / "adj.txt"))
+    got = np.loadtxt(str(tmp_path / "adj.txt"), dtype=np.int64).reshape(-1, 2)
+    perm = orc.relabel(s, d, "first_src")
+    np.testing.assert_array_equal(got[:, 0], perm[s])
+    np.testing.assert_array_equal(got[:, 1], perm[d])

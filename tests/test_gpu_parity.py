@@ -336,3 +336,29 @@ def test_cost_functions_match_oracle(ctx, case):
     assert g.approx_ratio() == orc.approx_ratio(s, d) or (
         np.isnan(g.approx_ratio()) and np.isnan(orc.approx_ratio(s, d)))
     g.close()
+
+
+@pytest.mark.parametrize("case", ["er", "rmat12", "messy", "hub", "k6"])
+def test_relabel_tools_match_oracle(ctx, case):
+    """tools/serialize.cpp, adjust2tc.cpp, directionByReorder.cpp relabellings
+    (the oracle is pinned against those tools' own builds)."""
+    s, d = CASES[case]()
+    s = (s.astype(np.int64) * 3 + 2).astype(np.uint32)   # holes in the id range
+    d = (d.astype(np.int64) * 3 + 2).astype(np.uint32)
+    for mode, name in ((tcamd.RELABEL_COMPACT, "compact"), (tcamd.RELABEL_FIRST_SRC, "first_src"),
+                       (tcamd.RELABEL_DEGREE, "degree")):
+        g = ctx.graph(s, d)
+        perm = g.relabel(mode)
+        ref = orc.relabel(s, d, name)
+        np.testing.assert_array_equal(perm, ref)
+        rs, rd = g.edges()
+        np.testing.assert_array_equal(rs, ref[s])
+        np.testing.assert_array_equal(rd, ref[d])
+        assert g.info().n == int(ref.max()) + 1
+        g.orient(tcamd.ORIENT_DEGREE)
+        assert g.count("auto").count == truth(case)
+        g.close()
+    e = np.zeros(0, np.uint32)
+    g = ctx.graph(e, e)
+    assert g.relabel(tcamd.RELABEL_DEGREE).tolist() == [0]
+    g.close()

@@ -404,3 +404,45 @@ def test_kclique_rows_strided_partitions_the_count():
         total = orc.kclique(rp, col, k)
         parts = sum(orc.kclique_rows_strided(rp, col, k, r, 3) for r in range(3))
         assert parts == total
+
+
+@needs_ref
+@pytest.mark.parametrize("make", [lambda: er(300, 2000, 4), messy, lambda: rmat(9, 8, 2)])
+def test_relabel_tools_match_reference(tmp_path, make):
+    """tools/serialize.cpp, tools/adjust2tc.cpp, tools/directionByReorder.cpp
+    pin the oracle's relabellings (records rewritten through perm)."""
+    s, d = make()
+    # leave holes in the id range so that serialize has ids to squeeze out
+    s = (s.astype(np.int64) * 3 + 1).astype(np.uint32)
+    d = (d.astype(np.int64) * 3 + 1).astype(np.uint32)
+    p = str(tmp_path / "g.bin")
+    tcamd.write_edges(p, s, d)
+    for tool, mode in (("serialize", "compact"), ("directionByReorder", "degree")):
+        out = str(tmp_path / f"{tool}.bin")
+        _run_ref(tool, [p, out], str(tmp_path))
+        rs, rd = tcamd.read_edges(out)
+        perm = orc.relabel(s, d, mode)
+        np.testing.assert_array_equal(rs, perm[s])
+        np.testing.assert_array_equal(rd, perm[d])
+    txt = str(tmp_path / "g.txt")
+    with open(txt, "w") as f:
+        for a, b in zip(s.tolist(), d.tolist()):
+            f.write(f"{a} {b}\n")
+    _run_ref("adjust2tc", [txt, str(tmp_path / "adj.txt")], str(tmp_path))
+    got = np.loadtxt(str(tmp_path / "adj.txt"), dtype=np.int64).reshape(-1, 2)
+    perm = orc.relabel(s, d, "first_src")
+    np.testing.assert_array_equal(got[:, 0], perm[s])
+    np.testing.assert_array_equal(got[:, 1], perm[d])
+
+
+def test_relabel_oracle_properties():
+    s, d = rmat(8, 8, 5)
+    n = orc.num_nodes(s, d)
+    for mode in ("compact", "first_src", "degree"):
+        perm = orc.relabel(s, d, mode)
+        assert len(perm) == n
+        if mode != "compact":
+            assert sorted(perm.tolist()) == list(range(n))  # a bijection on [0, n)
+    deg = np.bincount(np.concatenate([s, d]), minlength=n)
+    perm = orc.relabel(s, d, "degree")
+    assert perm[np.argmax(deg)] == 0 or deg[perm == 0][0] == deg.max()
