@@ -28,7 +28,7 @@ constexpr uint32_t kWaveMax = 512;      // longest N+(u) the wave variant holds 
 constexpr uint32_t kWaveChunk = 16;     // wave tasks taken per cursor bump
 constexpr uint32_t kQueue = 320;        // per-wave survivor queue (< 64 + 4 * 64 held)
 constexpr uint32_t kFlatMin = 64;       // lists this long or longer are streamed whole
-constexpr uint32_t kFilterMin = 16;     // wave variant: bitmap only for d+(u) above this
+constexpr uint32_t kFilterMin = 1;      // wave variant: bitmap for every d+(u) above this
 
 __device__ inline uint64_t gtid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ inline uint64_t gstride() { return (uint64_t)gridDim.x * blockDim.x; }
@@ -145,8 +145,11 @@ __device__ inline uint32_t seg64(const uint32_t* F, uint32_t f) {
   return (uint32_t)(p - F);
 }
 
-// Multiplicative hash of a vertex id to (32 - shift) bits.
-__device__ inline uint32_t fhash(uint32_t x, uint32_t shift) { return (x * 0x9E3779B1u) >> shift; }
+// Hash of a vertex id to (32 - shift) bits: the low bits with the bits above
+// bit 15 folded in (neighbouring ids, common after A-order, stay spread).
+__device__ inline uint32_t fhash(uint32_t x, uint32_t shift) {
+  return (x ^ (x >> 15)) & (0xFFFFFFFFu >> shift);
+}
 
 __device__ inline void bm_set(uint32_t* bm, uint32_t x, uint32_t shift) {
   const uint32_t h = fhash(x, shift);
@@ -200,20 +203,21 @@ __device__ inline uint32_t drain(Probe& P, uint32_t lane, bool all) {
   return c;
 }
 
-// Streams col[vb, ve) with the whole wave, four 64-entry chunks in flight.
+// Streams col[vb, ve) with the whole wave, eight 64-entry chunks in flight
+// (the queue is drained after every four).
 __device__ inline uint32_t stream_list(const uint32_t* __restrict__ col, uint32_t vb, uint32_t ve,
                                        uint32_t lane, Probe& P) {
   uint32_t c = 0;
-  for (uint32_t k0 = vb; k0 < ve; k0 += 256) {
+  for (uint32_t k0 = vb; k0 < ve; k0 += 512) {
     const uint32_t k = k0 + lane;
-    const uint32_t w0 = k < ve ? col[k] : kEmpty;
-    const uint32_t w1 = k + 64 < ve ? col[k + 64] : kEmpty;
-    const uint32_t w2 = k + 128 < ve ? col[k + 128] : kEmpty;
-    const uint32_t w3 = k + 192 < ve ? col[k + 192] : kEmpty;
-    push(P, w0);
-    push(P, w1);
-    push(P, w2);
-    push(P, w3);
+    uint32_t w[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) w[j] = k + 64 * j < ve ? col[k + 64 * j] : kEmpty;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) push(P, w[j]);
+    c += drain(P, lane, false);
+#pragma unroll
+    for (int j = 4; j < 8; ++j) push(P, w[j]);
     c += drain(P, lane, false);
   }
   return c;
