@@ -33,7 +33,7 @@ __device__ inline void flush_graph(unsigned long long& acc, uint32_t g,
   acc = 0;
 }
 
-template <int CAP>
+template <int CAP, uint32_t FMIN>
 __global__ __launch_bounds__(TB) void k_batch(const uint32_t* __restrict__ rp,
                                               const uint32_t* __restrict__ col,
                                               const uint32_t* __restrict__ gid,
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(TB) void k_batch(const uint32_t* __restrict__ rp,
     const uint32_t t = S.t;
     if (t >= nblk) break;
     const uint32_t u = blk[t];
-    unsigned long long c = block_vertex<CAP>(rp, col, u, S);
+    unsigned long long c = block_vertex<CAP, 1>(rp, col, u, S);
     flush_graph(c, gid[u], counts);
   }
   const WaveSlot W = wave_slot(S, wv);
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(TB) void k_batch(const uint32_t* __restrict__ rp,
         if (cur != 0xFFFFFFFFu) flush_graph(acc, cur, counts);
         cur = g;
       }
-      acc += wave_vertex(rp, col, u, lane, W);
+      acc += wave_vertex<FMIN, 1>(rp, col, u, lane, W);
     }
   }
   if (cur != 0xFFFFFFFFu) flush_graph(acc, cur, counts);
@@ -163,7 +163,7 @@ struct tc_batch {
   std::vector<uint32_t> h_blk_off, h_wav_off;
   // captured replay
   bool captured = false;
-  uint32_t cap_deg = 0, cap_chunk = 0, cap_grid = 0;
+  uint32_t cap_deg = 0, cap_chunk = 0, cap_grid = 0, cap_fmin = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -186,7 +186,17 @@ tc_batch_sched fill(const tc_batch_sched* s, tc_ctx* c) {
   if (o.wave_max_deg > kWaveMax) o.wave_max_deg = kWaveMax;
   if (!o.wave_chunk) o.wave_chunk = 16;
   if (!o.grid) o.grid = (uint32_t)c->num_cus * 8u;
+  o.filter_min = o.filter_min == 1 ? 1u : 16u;
   return o;
+}
+
+// k_batch for the schedule's wave-variant filter threshold (1 or 16)
+template <typename... A>
+void launch_batch(const tc_batch_sched& s, dim3 grid, hipStream_t st, A... args) {
+  if (s.filter_min == 1)
+    hipLaunchKernelGGL((k_batch<2048, 1>), grid, dim3(TB), 0, st, args...);
+  else
+    hipLaunchKernelGGL((k_batch<2048, 16>), grid, dim3(TB), 0, st, args...);
 }
 
 int schedule(tc_ctx* c, tc_batch* b, uint32_t wave_max_deg) {
@@ -250,10 +260,10 @@ int enqueue_batched(tc_ctx* c, tc_batch* b, const tc_batch_sched& s) {
   tc_graph* g = b->g;
   hipLaunchKernelGGL(k_batch_reset, dim3(grid_for(2ull + b->ngraphs)), dim3(256), 0, c->stream,
                      b->cursors.as<uint32_t>(), 2u, b->counts.as<unsigned long long>(), b->ngraphs);
-  hipLaunchKernelGGL(k_batch<2048>, dim3(s.grid), dim3(TB), 0, c->stream, g->rowptr.as<uint32_t>(),
-                     g->col.as<uint32_t>(), b->gid.as<uint32_t>(), b->blk_lpt.as<uint32_t>(),
-                     (uint32_t)b->nblk, b->wav.as<uint32_t>(), (uint32_t)b->nwav, s.wave_chunk,
-                     b->cursors.as<uint32_t>(), b->counts.as<unsigned long long>());
+  launch_batch(s, dim3(s.grid), c->stream, g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(),
+               b->gid.as<uint32_t>(), b->blk_lpt.as<uint32_t>(), (uint32_t)b->nblk,
+               b->wav.as<uint32_t>(), (uint32_t)b->nwav, s.wave_chunk, b->cursors.as<uint32_t>(),
+               b->counts.as<unsigned long long>());
   TCX_HIP(hipGetLastError());
   return TC_OK;
 }
@@ -339,7 +349,7 @@ int tc_batch_count(tc_ctx* c, tc_batch* b, const tc_batch_sched* sched, uint64_t
   TCX_HIP(hipEventRecord(c->ev0, c->stream));
   TCX_TRY(schedule(c, b, s.wave_max_deg));
   if (!b->captured || b->cap_chunk != s.wave_chunk || b->cap_grid != s.grid ||
-      b->cap_deg != s.wave_max_deg) {
+      b->cap_deg != s.wave_max_deg || b->cap_fmin != s.filter_min) {
     release_capture(b);
     TCX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int st = enqueue_batched(c, b, s);
@@ -356,6 +366,7 @@ int tc_batch_count(tc_ctx* c, tc_batch* b, const tc_batch_sched* sched, uint64_t
     b->cap_chunk = s.wave_chunk;
     b->cap_grid = s.grid;
     b->cap_deg = s.wave_max_deg;
+    b->cap_fmin = s.filter_min;
   }
   TCX_HIP(hipEventRecord(c->ev1, c->stream));
   TCX_HIP(hipEventSynchronize(c->ev1));
@@ -395,10 +406,10 @@ int tc_batch_count_pergraph(tc_ctx* c, tc_batch* b, const tc_batch_sched* sched,
     if (b0 == b1 && w0 == w1) continue;
     const uint64_t need = (b1 - b0) + ((w1 - w0) + 3) / 4;
     const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(s.grid, need));
-    hipLaunchKernelGGL(k_batch<2048>, dim3(grid), dim3(TB), 0, c->stream, g->rowptr.as<uint32_t>(),
-                       g->col.as<uint32_t>(), b->gid.as<uint32_t>(), b->blk_id.as<uint32_t>() + b0,
-                       b1 - b0, b->wav.as<uint32_t>() + w0, w1 - w0, s.wave_chunk, cur + 2 * q,
-                       b->counts.as<unsigned long long>());
+    launch_batch(s, dim3(grid), c->stream, g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(),
+                 b->gid.as<uint32_t>(), b->blk_id.as<uint32_t>() + b0, b1 - b0,
+                 b->wav.as<uint32_t>() + w0, w1 - w0, s.wave_chunk, cur + 2 * q,
+                 b->counts.as<unsigned long long>());
     tm.launches++;
   }
   TCX_HIP(hipGetLastError());
@@ -432,6 +443,13 @@ int tc_batch_autotune(tc_ctx* c, tc_batch* b, tc_batch_sched* best) {
   tc_batch_sched bs = s;
   for (uint32_t d : {16u, 32u, 64u, 128u, 256u, 512u}) {
     s.wave_max_deg = d;
+    float ms;
+    TCX_TRY(time_of(s, &ms));
+    if (ms < best_ms) { best_ms = ms; bs = s; }
+  }
+  s = bs;
+  s.filter_min = 1;
+  {
     float ms;
     TCX_TRY(time_of(s, &ms));
     if (ms < best_ms) { best_ms = ms; bs = s; }

@@ -203,22 +203,24 @@ __device__ inline uint32_t drain(Probe& P, uint32_t lane, bool all) {
   return c;
 }
 
-// Streams col[vb, ve) with the whole wave, eight 64-entry chunks in flight
-// (the queue is drained after every four).
+// Streams col[vb, ve) with the whole wave, 4 * U 64-entry chunks in flight
+// (the queue is drained after every four).  U = 2 suits long lists (R-MAT
+// hubs); U = 1 wastes fewer lanes on short ones (many small graphs).
+template <int U = 2>
 __device__ inline uint32_t stream_list(const uint32_t* __restrict__ col, uint32_t vb, uint32_t ve,
                                        uint32_t lane, Probe& P) {
   uint32_t c = 0;
-  for (uint32_t k0 = vb; k0 < ve; k0 += 512) {
+  for (uint32_t k0 = vb; k0 < ve; k0 += 256 * U) {
     const uint32_t k = k0 + lane;
-    uint32_t w[8];
+    uint32_t w[4 * U];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) w[j] = k + 64 * j < ve ? col[k + 64 * j] : kEmpty;
+    for (int j = 0; j < 4 * U; ++j) w[j] = k + 64 * j < ve ? col[k + 64 * j] : kEmpty;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) push(P, w[j]);
-    c += drain(P, lane, false);
+    for (int h = 0; h < U; ++h) {
 #pragma unroll
-    for (int j = 4; j < 8; ++j) push(P, w[j]);
-    c += drain(P, lane, false);
+      for (int j = 0; j < 4; ++j) push(P, w[4 * h + j]);
+      c += drain(P, lane, false);
+    }
   }
   return c;
 }
@@ -229,6 +231,7 @@ __device__ inline uint32_t stream_list(const uint32_t* __restrict__ col, uint32_
 // flight), each lane finding the list of its entry by a search over the
 // exclusive prefix of the lengths, kept with the list starts in the wave's
 // LDS slot F[128].
+template <int U = 2>
 __device__ inline uint32_t lists64(const uint32_t* __restrict__ col, uint32_t vb, uint32_t ve,
                                    uint32_t lane, uint32_t* F, Probe& P) {
   const uint32_t len = ve - vb;
@@ -238,7 +241,7 @@ __device__ inline uint32_t lists64(const uint32_t* __restrict__ col, uint32_t vb
   while (mask) {
     const uint32_t j = (uint32_t)__builtin_ctzll(mask);
     mask &= mask - 1;
-    c += stream_list(col, __shfl(vb, j, 64), __shfl(ve, j, 64), lane, P);
+    c += stream_list<U>(col, __shfl(vb, j, 64), __shfl(ve, j, 64), lane, P);
   }
   const uint32_t sl = lng ? 0u : len;
   const uint32_t incl = wave_incl_scan(sl, lane);
@@ -277,12 +280,14 @@ struct WaveSlot {
 
 // Wave variant: one wave counts the wedges of u (d+(u) <= kWaveMax).  Lane j
 // takes the j-th out-neighbour of each 64-neighbour group.  The bitmap is
-// built for d+(u) > kFilterMin and its words are cleared again at the end.
+// built for d+(u) > FMIN and its words are cleared again at the end (below
+// that, every entry goes straight to the search).
+template <uint32_t FMIN = kFilterMin, int U = 2>
 __device__ inline uint32_t wave_vertex(const uint32_t* __restrict__ rp,
                                        const uint32_t* __restrict__ col, uint32_t u,
                                        uint32_t lane, const WaveSlot& W) {
   const uint32_t ub = rp[u], du = rp[u + 1] - ub;
-  const bool filt = du > kFilterMin;
+  const bool filt = du > FMIN;
   for (uint32_t i = lane; i < du; i += 64) {
     const uint32_t x = col[ub + i];
     W.L[i] = x;
@@ -298,7 +303,7 @@ __device__ inline uint32_t wave_vertex(const uint32_t* __restrict__ rp,
       mvb = rp[v];
       mve = rp[v + 1];
     }
-    c += lists64(col, mvb, mve, lane, W.F, P);
+    c += lists64<U>(col, mvb, mve, lane, W.F, P);
   }
   c += drain(P, lane, true);
   if (filt)
@@ -339,7 +344,7 @@ __device__ inline void clear_bitmap(BlockLds<CAP>& S) {
 // hashed into the bitmap; wave w takes the out-neighbours w, w+4, w+8, ...
 // 64 at a time.  Every thread must call it; it ends with the bitmap clear
 // and a barrier.
-template <int CAP>
+template <int CAP, int U = 2>
 __device__ inline uint32_t block_vertex(const uint32_t* __restrict__ rp,
                                         const uint32_t* __restrict__ col, uint32_t u,
                                         BlockLds<CAP>& S) {
@@ -364,7 +369,7 @@ __device__ inline uint32_t block_vertex(const uint32_t* __restrict__ rp,
       vb = rp[v];
       ve = rp[v + 1];
     }
-    c += lists64(col, vb, ve, lane, S.f[wv], P);
+    c += lists64<U>(col, vb, ve, lane, S.f[wv], P);
   }
   c += drain(P, lane, true);
   __syncthreads();

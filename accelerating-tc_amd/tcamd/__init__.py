@@ -160,11 +160,11 @@ _sig("tc_tricore_tricount_gpu", C.c_uint64, [C.c_uint64, _vp, _u64p, _u32p,
 
 class BatchSched(C.Structure):
     _fields_ = [("wave_max_deg", C.c_uint32), ("wave_chunk", C.c_uint32), ("grid", C.c_uint32),
-                ("reserved", C.c_uint32 * 5)]
+                ("filter_min", C.c_uint32), ("reserved", C.c_uint32 * 4)]
 
     def as_dict(self):
         return {"wave_max_deg": self.wave_max_deg, "wave_chunk": self.wave_chunk,
-                "grid": self.grid}
+                "grid": self.grid, "filter_min": self.filter_min}
 
 
 _sig("tc_batch_create", C.c_int, [_vp, C.c_uint32, _u64p, _vp, _vp, _u32p, C.POINTER(_vp)])

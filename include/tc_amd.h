@@ -303,7 +303,9 @@ typedef struct tc_batch_sched {
   uint32_t wave_max_deg;  /* vertices with 2 <= d+ <= this run as wave tasks (64) */
   uint32_t wave_chunk;    /* wave tasks fetched per cursor bump (16)              */
   uint32_t grid;          /* persistent workgroups (0 = 8 per CU)                  */
-  uint32_t reserved[5];
+  uint32_t filter_min;    /* wave tasks hash N+(u) into a bitmap filter above this
+                             d+: 1 or 16 (0 = 16)                                 */
+  uint32_t reserved[4];
 } tc_batch_sched;
 
 /* Graph g holds records [edge_off[g], edge_off[g+1]) of src/dst with local
@@ -314,8 +316,8 @@ int tc_batch_create(tc_ctx* ctx, uint32_t ngraphs, const uint64_t* edge_off,
 int tc_batch_destroy(tc_batch* b);
 /* Sizes of the batch's union CSR (n, m+, algorithmic bytes of one count). */
 int tc_batch_info_get(const tc_batch* b, tc_graph_info* info);
-/* Tries wave_max_deg x wave_chunk candidates on the batch and returns the
- * fastest (timed with hipEvents). */
+/* Tries wave_max_deg, then filter_min, then wave_chunk candidates on the
+ * batch and returns the fastest (timed with hipEvents). */
 int tc_batch_autotune(tc_ctx* ctx, tc_batch* b, tc_batch_sched* best);
 /* Exact per-graph counts (ngraphs entries) from one replay of the captured
  * hipGraph (capture happens on the first call per schedule). */
