@@ -143,3 +143,20 @@ def test_relabel_drivers(graph_file, tmp_path):
     perm = orc.relabel(s, d, "first_src")
     np.testing.assert_array_equal(got[:, 0], perm[s])
     np.testing.assert_array_equal(got[:, 1], perm[d])
+
+
+def test_kcore_direction_driver(graph_file, tmp_path):
+    p, s, d, _ = graph_file
+    out = str(tmp_path / "kcore.bin")
+    run("kcore_direction", p, out)
+    got = np.fromfile(out, dtype=np.uint64)
+    # the file holds records {src, dst}: as uint64 FIRST = dst, SECOND = src
+    a = d.astype(np.int64)
+    b = s.astype(np.int64)
+    n = orc.num_nodes(s, d)
+    srp, scol = orc.build_sym(s, d, n)
+    core = orc.kcore(n, srp, scol).astype(np.int64)
+    swap = (core[a] > core[b]) | ((core[a] == core[b]) & (a > b))
+    lo = np.where(swap, b, a).astype(np.uint64)
+    hi = np.where(swap, a, b).astype(np.uint64)
+    np.testing.assert_array_equal(got, lo << np.uint64(32) | hi)
