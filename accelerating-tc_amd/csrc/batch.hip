@@ -283,6 +283,7 @@ int tc_batch_create(tc_ctx* c, uint32_t ngraphs, const uint64_t* eoff, const uin
                     const uint32_t* dst, const uint32_t* nverts, tc_batch** out) {
   if (!c || !out || !eoff || !nverts || ngraphs == 0) return fail(TC_ERR_ARG, "bad arguments");
   *out = nullptr;
+  if (eoff[0] != 0) return fail(TC_ERR_ARG, "edge_off[0] must be 0");
   const uint64_t M = eoff[ngraphs];
   if (M && (!src || !dst)) return fail(TC_ERR_ARG, "NULL edge arrays");
   std::vector<uint32_t> voff(ngraphs + 1, 0);

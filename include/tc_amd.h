@@ -309,7 +309,8 @@ typedef struct tc_batch_sched {
 } tc_batch_sched;
 
 /* Graph g holds records [edge_off[g], edge_off[g+1]) of src/dst with local
- * ids < nverts[g].  Orientation: TC_ORIENT_DEGREE per graph. */
+ * ids < nverts[g]; edge_off has ngraphs + 1 ascending entries, edge_off[0] = 0.
+ * Orientation: TC_ORIENT_DEGREE per graph. */
 int tc_batch_create(tc_ctx* ctx, uint32_t ngraphs, const uint64_t* edge_off,
                     const uint32_t* src, const uint32_t* dst, const uint32_t* nverts,
                     tc_batch** out);

@@ -110,3 +110,11 @@ def test_rejects_ids_beyond_graph(ctx):
     d = np.array([1, 5], np.uint32)
     with pytest.raises(tcamd.TCError):
         ctx.batch([(s, d, 5)])
+    # offsets must start at 0 and ascend
+    import ctypes as C
+    h = C.c_void_p()
+    nv = np.array([6], np.uint32)
+    for off in ([1, 2], [2, 1]):
+        st = tcamd._lib.tc_batch_create(ctx.handle, 1, np.array(off, np.uint64), s.ctypes.data,
+                                        d.ctypes.data, nv, C.byref(h))
+        assert st == tcamd.ERR_ARG
