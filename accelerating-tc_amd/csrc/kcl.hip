@@ -353,6 +353,40 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
         if (lane == 0) cnt += binom_small(p, 2) + (unsigned long long)p * cs + e2 / 2;
         return false;
       }
+      if (r == 3) {
+        // binom(p,3) + binom(p,2)|C| + p edges(C) + triangles(C).  Lane x
+        // holds R = N(x) & C; for every y in R above x, |R & N(y)| counts
+        // the triangles through edge (x, y), each triangle once per edge.
+        uint64_t esum = 0, tsum = 0;
+        for (int t = 0; t < W; ++t) {
+          if (cr[t] == 0) continue;
+          const uint32_t x = 64u * t + lane;
+          if (!((cr[t] >> lane) & 1ull)) continue;
+          uint64_t rx[W];
+#pragma unroll
+          for (int w = 0; w < W; ++w) {
+            rx[w] = s_M[x * W + w] & cr[w];
+            esum += (uint32_t)__popcll(rx[w]);
+          }
+#pragma unroll
+          for (int w = 0; w < W; ++w) {
+            uint64_t above = w < t ? 0ull : (w > t ? rx[w] : (rx[w] & ~((2ull << lane) - 1)));
+            while (above) {
+              const uint32_t y = 64u * w + (uint32_t)__builtin_ctzll(above);
+              above &= above - 1;
+              uint32_t c3 = 0;
+#pragma unroll
+              for (int w2 = 0; w2 < W; ++w2) c3 += (uint32_t)__popcll(rx[w2] & s_M[y * W + w2]);
+              tsum += c3;
+            }
+          }
+        }
+        const unsigned long long e2 = wave_sum(esum), t3 = wave_sum(tsum);
+        if (lane == 0)
+          cnt += binom_small(p, 3) + binom_small(p, 2) * cs + (unsigned long long)p * (e2 / 2) +
+                 t3 / 3;
+        return false;
+      }
       // pivot: the member of C's first non-empty 64-word with most
       // neighbours in C (any member gives exact counts; this one keeps the
       // choice at W reads per lane)
