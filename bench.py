@@ -228,9 +228,12 @@ def main_kclique(args, w):
     k3 = g.kclique(3, tcamd.KCL_PIVOT).count
     if k3 != tri:
         raise SystemExit(f"3-cliques {k3} != triangles {tri}")
+    ks = tuple(int(x) for x in args.ks.split(",") if x) or w["ks"]
     per_k = {}
-    for k in w["ks"]:
+    for k in ks:
+        t0 = time.perf_counter()
         ref = g.kclique(k).count  # also the warm-up
+        log(f"[bench] k={k}: reference count {ref} in {time.perf_counter() - t0:.2f} s")
         ms = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
@@ -246,7 +249,7 @@ def main_kclique(args, w):
     if not args.no_cpu:
         import oracle as orc
         rp, col = g.csr()
-        k = w["ks"][0]
+        k = ks[0]
         stride = max(1, info.n // 512)
         t0 = time.perf_counter()
         orc.kclique_rows_strided(rp, col, k, 0, stride)
@@ -270,7 +273,7 @@ def main_kclique(args, w):
         "ms_per_step": total_wall * 1e3 / len(per_k),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u32", "data": "synthetic",
-        "config": {"workload": f"k-clique k={','.join(map(str, w['ks']))} on {desc}, "
+        "config": {"workload": f"k-clique k={','.join(map(str, ks))} on {desc}, "
                                f"A-direction orientation", "n": info.n, "m": m,
                    "triangles": tri, "engine": "k-clique pivot", "parallelism": "single GPU"},
         "roofline": None,
@@ -291,6 +294,7 @@ def main():
     ap.add_argument("--workload", default="rmat22", choices=sorted(WORKLOADS))
     ap.add_argument("--kernels", default=",".join(KERNELS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--ks", default="", help="k-clique workloads: comma-separated k values")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
