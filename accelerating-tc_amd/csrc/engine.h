@@ -185,6 +185,33 @@ __device__ inline void push(Probe& P, uint32_t x) {
   P.qn += (uint32_t)__popcll(m);
 }
 
+// push() of four entries, with the four bitmap probes issued before any
+// queue write so that their LDS reads overlap.  Faster for long streamed
+// lists; short-list kernels keep the one-entry form, which needs fewer
+// registers.
+__device__ inline void push4(Probe& P, uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3) {
+  bool p0 = x0 != kEmpty, p1 = x1 != kEmpty, p2 = x2 != kEmpty, p3 = x3 != kEmpty;
+  if (P.filt) {
+    const uint32_t h0 = fhash(x0, P.shift), h1 = fhash(x1, P.shift);
+    const uint32_t h2 = fhash(x2, P.shift), h3 = fhash(x3, P.shift);
+    const uint32_t b0 = P.bm[h0 >> 5], b1 = P.bm[h1 >> 5], b2 = P.bm[h2 >> 5], b3 = P.bm[h3 >> 5];
+    p0 = p0 && ((b0 >> (h0 & 31)) & 1u);
+    p1 = p1 && ((b1 >> (h1 & 31)) & 1u);
+    p2 = p2 && ((b2 >> (h2 & 31)) & 1u);
+    p3 = p3 && ((b3 >> (h3 & 31)) & 1u);
+  }
+  const uint64_t m0 = __ballot(p0), m1 = __ballot(p1), m2 = __ballot(p2), m3 = __ballot(p3);
+  uint32_t q = P.qn;
+  if (p0) P.Q[q + lanes_below(m0)] = x0;
+  q += (uint32_t)__popcll(m0);
+  if (p1) P.Q[q + lanes_below(m1)] = x1;
+  q += (uint32_t)__popcll(m1);
+  if (p2) P.Q[q + lanes_below(m2)] = x2;
+  q += (uint32_t)__popcll(m2);
+  if (p3) P.Q[q + lanes_below(m3)] = x3;
+  P.qn = q + (uint32_t)__popcll(m3);
+}
+
 // Resolves queued survivors 64 at a time; `all` also resolves a partial tail.
 __device__ inline uint32_t drain(Probe& P, uint32_t lane, bool all) {
   uint32_t c = 0;
@@ -217,8 +244,12 @@ __device__ inline uint32_t stream_list(const uint32_t* __restrict__ col, uint32_
     for (int j = 0; j < 4 * U; ++j) w[j] = k + 64 * j < ve ? col[k + 64 * j] : kEmpty;
 #pragma unroll
     for (int h = 0; h < U; ++h) {
+      if constexpr (U >= 2) {
+        push4(P, w[4 * h], w[4 * h + 1], w[4 * h + 2], w[4 * h + 3]);
+      } else {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) push(P, w[4 * h + j]);
+        for (int j = 0; j < 4; ++j) push(P, w[4 * h + j]);
+      }
       c += drain(P, lane, false);
     }
   }
