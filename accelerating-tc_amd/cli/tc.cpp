@@ -1,5 +1,5 @@
 // General driver: tc -f <edges.bin> [--kernel hu|bisson|fox|tricore|auto]
-//                    [--orient id|degree|adir|kcore|tricore] [--aorder] [--simplify]
+//                    [--orient id|degree|adir|kcore|degeneracy|tricore] [--aorder] [--simplify]
 // Any method under any orientation; prints the reference's triangle line.
 #include <cstring>
 
@@ -20,7 +20,7 @@ int main(int argc, const char* argv[]) {
       const std::string v = argv[++i];
       orient = v == "id" ? TC_ORIENT_ID : v == "degree" ? TC_ORIENT_DEGREE
              : v == "kcore" ? TC_ORIENT_KCORE : v == "tricore" ? TC_ORIENT_DEGREE_TRICORE
-             : TC_ORIENT_ADIR;
+             : v == "degeneracy" ? TC_ORIENT_DEGENERACY : TC_ORIENT_ADIR;
     } else if (a == "--aorder") aorder = true;
     else if (a == "--simplify") simplify = true;
   }

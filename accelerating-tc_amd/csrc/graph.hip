@@ -220,11 +220,14 @@ __global__ void k_core_init(const uint32_t* __restrict__ rp, uint32_t n, int32_t
   }
 }
 
+// `step` (optional) receives the index of the frontier that removes each
+// vertex: every frontier is one step, numbered over the whole peel.
 __global__ void k_core_scan(uint32_t n, const int32_t* __restrict__ deg, int32_t* core, int32_t k,
-                            uint32_t* frontier, uint32_t* fcount) {
+                            uint32_t* frontier, uint32_t* fcount, int32_t* step, int32_t t) {
   for (uint64_t v = gtid(); v < n; v += gstride()) {
     if (core[v] == -1 && deg[v] <= k) {
       core[v] = k;
+      if (step) step[v] = t;
       frontier[atomicAdd(fcount, 1u)] = (uint32_t)v;
     }
   }
@@ -233,7 +236,7 @@ __global__ void k_core_scan(uint32_t n, const int32_t* __restrict__ deg, int32_t
 __global__ void k_core_expand(const uint32_t* __restrict__ frontier, uint32_t fsize,
                               const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                               int32_t* deg, int32_t* core, int32_t k, uint32_t* next,
-                              uint32_t* ncount) {
+                              uint32_t* ncount, int32_t* step, int32_t t) {
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t wave = gtid() >> 6, nwaves = gstride() >> 6;
   for (uint64_t f = wave; f < fsize; f += nwaves) {
@@ -244,13 +247,15 @@ __global__ void k_core_expand(const uint32_t* __restrict__ frontier, uint32_t fs
       int32_t old = atomicSub(deg + y, 1);
       if (old == k + 1) {
         core[y] = k;
+        if (step) step[y] = t;
         next[atomicAdd(ncount, 1u)] = y;
       }
     }
   }
 }
 
-// keep[e] for a sorted symmetric key list: ADIR (mode 2) or KCORE (mode 3).
+// keep[e] for a sorted symmetric key list: ADIR (mode 2), or KCORE (mode 3)
+// and DEGENERACY (mode 5), whose keys both point up from the lower (key, id).
 __global__ void k_sym_keep(const uint64_t* __restrict__ keys, uint64_t m2,
                            const int32_t* __restrict__ key, int mode, uint8_t* keep) {
   for (uint64_t e = gtid(); e < m2; e += gstride()) {
@@ -515,8 +520,12 @@ static int adir_peel(tc_ctx* c, uint32_t n, uint64_t m2, const uint32_t* rp, con
   return TC_OK;
 }
 
+// Core numbers into `core`; with `step` non-NULL, also the removal step of
+// every vertex.  A vertex removed at step t has at most core[v] neighbours
+// removed at step t or later (its remaining degree when its frontier formed),
+// so orienting every edge from the lower (step, id) bounds d+(v) by core[v].
 static int kcore_peel(tc_ctx* c, uint32_t n, const uint32_t* rp, const uint32_t* col,
-                      int32_t* core) {
+                      int32_t* core, int32_t* step = nullptr) {
   DevBuf deg, fa, fb;
   TCX_TRY(deg.reserve(sizeof(int32_t) * ((size_t)n + 1)));
   TCX_TRY(fa.reserve(sizeof(uint32_t) * ((size_t)n + 1)));
@@ -527,26 +536,28 @@ static int kcore_peel(tc_ctx* c, uint32_t n, const uint32_t* rp, const uint32_t*
                      deg.as<int32_t>(), core);
   TCX_HIP(hipGetLastError());
   uint64_t removed = 0;
-  int32_t k = 0;
+  int32_t k = 0, t = 0;
   uint32_t* F = fa.as<uint32_t>();
   uint32_t* N = fb.as<uint32_t>();
   while (removed < n) {
     TCX_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), c->stream));
     hipLaunchKernelGGL(k_core_scan, dim3(grid_for(n)), dim3(TB), 0, c->stream, n,
-                       deg.as<int32_t>(), core, k, F, cnt);
+                       deg.as<int32_t>(), core, k, F, cnt, step, t);
     TCX_HIP(hipGetLastError());
     uint32_t fs = 0;
     TCX_TRY(read_u32(c, cnt, &fs));
     removed += fs;
+    if (fs) ++t;
     while (fs > 0) {
       TCX_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), c->stream));
       hipLaunchKernelGGL(k_core_expand, dim3(std::min<unsigned>(4096, grid_for(fs * 64ull))),
                          dim3(TB), 0, c->stream, F, fs, rp, col, deg.as<int32_t>(), core, k, N,
-                         cnt);
+                         cnt, step, t);
       TCX_HIP(hipGetLastError());
       uint32_t ns = 0;
       TCX_TRY(read_u32(c, cnt, &ns));
       removed += ns;
+      if (ns) ++t;
       std::swap(F, N);
       fs = ns;
     }
@@ -766,7 +777,7 @@ int tc_graph_relabel(tc_ctx* c, tc_graph* g, int mode, uint32_t* perm_out) {
 int tc_graph_orient(tc_ctx* c, tc_graph* g, int mode, uint32_t flags) {
   TCX_TRY(check_graph(g));
   if (c != g->ctx) return fail(TC_ERR_ARG, "graph belongs to another context");
-  if (mode < TC_ORIENT_ID || mode > TC_ORIENT_DEGREE_TRICORE)
+  if (mode < TC_ORIENT_ID || mode > TC_ORIENT_DEGENERACY)
     return fail(TC_ERR_ARG, "unknown orientation mode %d", mode);
   if ((flags & TC_FLAG_PRUNE_DEG1) && mode != TC_ORIENT_DEGREE)
     return fail(TC_ERR_ARG, "degree<=1 pruning applies to TC_ORIENT_DEGREE only");
@@ -818,10 +829,16 @@ int tc_graph_orient(tc_ctx* c, tc_graph* g, int mode, uint32_t flags) {
   TCX_TRY(sym_csr(c, skeys, m2, n, srp, scol));
   TCX_TRY(g->vkey.reserve(4 * ((size_t)n + 1)));
   if (n > 0) {
-    if (mode == TC_ORIENT_ADIR)
+    if (mode == TC_ORIENT_ADIR) {
       TCX_TRY(adir_peel(c, n, m2, srp.as<uint32_t>(), scol.as<uint32_t>(), g->vkey.as<int32_t>()));
-    else
+    } else if (mode == TC_ORIENT_KCORE) {
       TCX_TRY(kcore_peel(c, n, srp.as<uint32_t>(), scol.as<uint32_t>(), g->vkey.as<int32_t>()));
+    } else {
+      DevBuf core;
+      TCX_TRY(core.reserve(4 * ((size_t)n + 1)));
+      TCX_TRY(kcore_peel(c, n, srp.as<uint32_t>(), scol.as<uint32_t>(), core.as<int32_t>(),
+                         g->vkey.as<int32_t>()));
+    }
   }
   TCX_TRY(keep.reserve(std::max<uint64_t>(m2, 1)));
   hipLaunchKernelGGL(k_sym_keep, dim3(grid_for(m2)), dim3(TB), 0, c->stream, skeys, m2,
@@ -880,8 +897,9 @@ int tc_graph_get_edges(const tc_graph* g, uint32_t* src, uint32_t* dst) {
 
 int tc_graph_get_vertex_key(const tc_graph* g, int32_t* key) {
   if (!g || !g->ctx || !key) return fail(TC_ERR_ARG, "NULL argument");
-  if (g->orient_mode != TC_ORIENT_ADIR && g->orient_mode != TC_ORIENT_KCORE)
-    return fail(TC_ERR_STATE, "vertex keys exist after ADIR or KCORE orientation only");
+  if (g->orient_mode != TC_ORIENT_ADIR && g->orient_mode != TC_ORIENT_KCORE &&
+      g->orient_mode != TC_ORIENT_DEGENERACY)
+    return fail(TC_ERR_STATE, "vertex keys exist after ADIR, KCORE or DEGENERACY orientation only");
   TCX_HIP(hipSetDevice(g->ctx->device));
   if (g->n)
     TCX_HIP(hipMemcpyAsync(key, g->vkey.p, 4 * (size_t)g->n, hipMemcpyDeviceToHost, g->ctx->stream));

@@ -655,7 +655,9 @@ int kcl_gpu_solver(const char* path, uint32_t k, uint64_t* total, size_t n_chunk
   TCX_TRY(tc_ctx_create(0, &c));
   tc_graph* g = nullptr;
   int st = tc_graph_create(c, s.data(), d.data(), m, n, &g);
-  if (st == TC_OK) st = tc_graph_orient(c, g, TC_ORIENT_KCORE, 0);
+  // degeneracy order: every out-degree is at most the degeneracy, which
+  // keeps the roots on the LDS pivot path
+  if (st == TC_OK) st = tc_graph_orient(c, g, TC_ORIENT_DEGENERACY, 0);
   if (st == TC_OK) st = kcl_count(c, g, k, KCL_VARIANT_AUTO, total, nullptr);
   tc_graph_destroy(g);
   tc_ctx_destroy(c);

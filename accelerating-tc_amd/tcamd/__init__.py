@@ -37,6 +37,7 @@ ERR_ARG, ERR_IO, ERR_FORMAT, ERR_RANGE, ERR_NOMEM, ERR_HIP, ERR_UNSUPPORTED, ERR
     -1, -2, -3, -4, -5, -6, -7, -8)
 
 ORIENT_ID, ORIENT_DEGREE, ORIENT_ADIR, ORIENT_KCORE, ORIENT_DEGREE_TRICORE = 0, 1, 2, 3, 4
+ORIENT_DEGENERACY = 5
 FLAG_PRUNE_DEG1 = 1
 KERNEL_HU, KERNEL_BISSON, KERNEL_FOX, KERNEL_TRICORE, KERNEL_AUTO = 0, 1, 2, 3, 4
 KERNEL_NAMES = {"hu": KERNEL_HU, "bisson": KERNEL_BISSON, "fox": KERNEL_FOX,

@@ -42,8 +42,10 @@ WORKLOADS = {
     "er": dict(kind="er", n=65536, m=1048576, seed=1, aorder=False, orient="adir"),
     "ba24": dict(kind="ba", n=1 << 24, k=8, seed=3, aorder=False, orient="adir"),
     "batched": dict(kind="batched", count=4096, seed=5),
-    "kclique_rgg": dict(kind="kclique", graph="rgg", n=1000000, deg=48, seed=4, ks=(4, 5, 6, 7)),
-    "kclique_rmat20": dict(kind="kclique", graph="rmat", scale=20, ef=32, seed=20, ks=(4, 5)),
+    "kclique_rgg": dict(kind="kclique", graph="rgg", n=1000000, deg=48, seed=4, ks=(4, 5, 6, 7),
+                        orient="degeneracy"),
+    "kclique_rmat20": dict(kind="kclique", graph="rmat", scale=20, ef=32, seed=20, ks=(4, 5),
+                           orient="degeneracy"),
 }
 
 
@@ -206,6 +208,11 @@ def main_kclique(args, w):
     import numpy as np
     import tcamd
 
+    KCL_ORIENTS = {"degeneracy": (tcamd.ORIENT_DEGENERACY, "degeneracy (k-core peel order)"),
+                   "adir": (tcamd.ORIENT_ADIR, "A-direction"),
+                   "degree": (tcamd.ORIENT_DEGREE, "degree"),
+                   "kcore": (tcamd.ORIENT_KCORE, "k-core (core number, id)")}
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     t0 = time.perf_counter()
     if w["graph"] == "rgg":
@@ -219,11 +226,13 @@ def main_kclique(args, w):
     t0 = time.perf_counter()
     g = ctx.graph(src, dst)
     g.simplify()
-    g.orient(tcamd.ORIENT_ADIR)
+    orient = args.orient or w["orient"]
+    g.orient(KCL_ORIENTS[orient][0])
     t_prep = time.perf_counter() - t0
     info = g.info()
     m = info.m_oriented
-    log(f"[bench] {desc}: n={info.n} m={m} max_out={info.max_out_degree} prep {t_prep:.2f}s")
+    log(f"[bench] {desc}: {orient} orientation n={info.n} m={m} max_out={info.max_out_degree} "
+        f"prep {t_prep:.2f}s")
     tri = g.count(tcamd.KERNEL_AUTO).count
     k3 = g.kclique(3, tcamd.KCL_PIVOT).count
     if k3 != tri:
@@ -249,20 +258,25 @@ def main_kclique(args, w):
     if not args.no_cpu:
         import oracle as orc
         rp, col = g.csr()
-        k = ks[0]
-        stride = max(1, info.n // 512)
-        t0 = time.perf_counter()
-        orc.kclique_rows_strided(rp, col, k, 0, stride)
-        t1 = max(time.perf_counter() - t0, 1e-4)
-        stride2 = max(1, int(round(t1 * stride / 15.0)))
-        t0 = time.perf_counter()
-        orc.kclique_rows_strided(rp, col, k, 0, stride2)
-        t = time.perf_counter() - t0
+        k = min(ks)
         deg = np.diff(rp.astype(np.int64))
-        edges = int(deg[0::stride2].sum())
+        # roots in a seeded random order, one oracle call each, until about
+        # 15 s have passed (a root's cost grows steeply with its out-degree,
+        # so a fixed stride can land on the densest roots)
+        order = np.random.default_rng(0).permutation(info.n)
+        edges = roots = 0
+        t0 = time.perf_counter()
+        for r in order:
+            orc.kclique_rows_strided(rp, col, k, int(r), info.n)
+            edges += int(deg[r])
+            roots += 1
+            if time.perf_counter() - t0 > 15.0:
+                break
+        t = time.perf_counter() - t0
         cpu = {"value": edges / t, "unit": "edges/s", "cores": 1, "kind": "port",
-               "sample": (f"oracle depth-first {k}-clique counter over every {stride2}-th root "
-                          f"of the same oriented CSR: {edges} of {m} edges, {t:.1f} s")}
+               "sample": (f"oracle depth-first {k}-clique counter over {roots} roots drawn at "
+                          f"random (seed 0) from the same oriented CSR: {edges} of {m} edges, "
+                          f"{t:.1f} s")}
         log(f"[bench] cpu baseline {cpu['value']:.3e} edges/s ({cpu['sample']})")
     total_wall = sum(v["ms_wall"] for v in per_k.values()) * 1e-3
     line = {
@@ -274,7 +288,8 @@ def main_kclique(args, w):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u32", "data": "synthetic",
         "config": {"workload": f"k-clique k={','.join(map(str, ks))} on {desc}, "
-                               f"A-direction orientation", "n": info.n, "m": m,
+                               f"{KCL_ORIENTS[orient][1]} orientation", "n": info.n, "m": m,
+                   "max_out_degree": info.max_out_degree,
                    "triangles": tri, "engine": "k-clique pivot", "parallelism": "single GPU"},
         "roofline": None,
         "cpu_baseline": cpu,
@@ -295,6 +310,8 @@ def main():
     ap.add_argument("--kernels", default=",".join(KERNELS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--ks", default="", help="k-clique workloads: comma-separated k values")
+    ap.add_argument("--orient", default="", choices=["", "degeneracy", "adir", "degree", "kcore"],
+                    help="k-clique workloads: orientation (default: degeneracy)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()

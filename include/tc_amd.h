@@ -46,6 +46,12 @@ extern "C" {
 #define TC_ORIENT_ADIR 2        /* A-direction peel: edge_directing/A-direction-seri.cpp:35-153,252-273 */
 #define TC_ORIENT_KCORE 3       /* edge_directing/kcore_direction.cpp:91-146 */
 #define TC_ORIENT_DEGREE_TRICORE 4 /* tricore/src/tricount_gpu.cu:69-80 (ties from the smaller id) */
+/* Degeneracy order: the k-core peel of edge_directing/kcore_direction.cpp:91-126
+ * run level-synchronously, every edge from the lower (removal step, id).  The
+ * reference orients by (core number, id) instead (:137-143); refining the key
+ * by the removal step bounds every out-degree by the vertex's core number,
+ * hence by the degeneracy (the k-clique path's best orientation). */
+#define TC_ORIENT_DEGENERACY 5
 
 /* Orientation flags. */
 #define TC_FLAG_PRUNE_DEG1 1u   /* Hu/preTC.cu:135-169: drop degree<=1 vertices, compact ids */
@@ -187,7 +193,8 @@ int tc_graph_info_get(const tc_graph* g, tc_graph_info* info);
 /* Copies of device state (host arrays of the documented sizes). */
 int tc_graph_get_csr(const tc_graph* g, uint32_t* rowptr /*n+1*/, uint32_t* col /*m+*/);
 int tc_graph_get_edges(const tc_graph* g, uint32_t* src, uint32_t* dst /*m_raw*/);
-/* ADIR: peel priorities (A-direction-seri.cpp numbering); KCORE: core numbers. */
+/* ADIR: peel priorities (A-direction-seri.cpp numbering); KCORE: core numbers;
+ * DEGENERACY: removal steps. */
 int tc_graph_get_vertex_key(const tc_graph* g, int32_t* key /*n*/);
 /* Device addresses of the CSR (for callers sharing HBM buffers). */
 int tc_graph_device_csr(const tc_graph* g, const uint32_t** rowptr,
@@ -333,9 +340,9 @@ int kcl_count(tc_ctx* ctx, tc_graph* g, uint32_t k, int variant, uint64_t* count
               tc_timing* timing);
 /* File-level form of kcl_gpu_solver (k-clique/kcl.h:6-7): reads a Galois
  * .gr file (path ending in ".gr", the input of k-clique/kcl_gpu.cpp:22-27)
- * or a list2bin file, k-core orientation, counts k-cliques.  n_chunk is
- * accepted for signature parity (HBM holds every level; chunks are not
- * needed). */
+ * or a list2bin file, degeneracy orientation (TC_ORIENT_DEGENERACY),
+ * counts k-cliques.  n_chunk is accepted for signature parity (HBM holds
+ * every level; chunks are not needed). */
 int kcl_gpu_solver(const char* path, uint32_t k, uint64_t* total, size_t n_chunk);
 
 /* ---- TriCore function API (tricore/include/tricount.h:7-17) ----------- */

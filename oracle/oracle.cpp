@@ -408,6 +408,46 @@ uint64_t orc_kcore_orient(uint32_t n, const uint32_t* sym_rowptr,
   return k;
 }
 
+void orc_degeneracy(uint32_t n, const uint32_t* sym_rowptr, const uint32_t* sym_col,
+                    uint32_t* step, uint32_t* core) {
+  const uint32_t kNone = 0xFFFFFFFFu;
+  std::vector<uint32_t> deg(n);
+  for (uint32_t v = 0; v < n; ++v) {
+    deg[v] = sym_rowptr[v + 1] - sym_rowptr[v];
+    step[v] = kNone;
+  }
+  uint32_t removed = 0, k = 0, t = 0;
+  std::vector<uint32_t> frontier;
+  while (removed < n) {
+    // every remaining vertex of degree <= k, then their remaining
+    // neighbours that drop to <= k, one frontier per step
+    frontier.clear();
+    for (uint32_t v = 0; v < n; ++v)
+      if (step[v] == kNone && deg[v] <= k) frontier.push_back(v);
+    while (!frontier.empty()) {
+      for (uint32_t v : frontier) {
+        step[v] = t;
+        core[v] = k;
+      }
+      removed += (uint32_t)frontier.size();
+      std::vector<uint32_t> next;
+      for (uint32_t v : frontier)
+        for (uint32_t e = sym_rowptr[v]; e < sym_rowptr[v + 1]; ++e) {
+          const uint32_t y = sym_col[e];
+          if (step[y] != kNone) continue;
+          if (--deg[y] == k) next.push_back(y);
+        }
+      ++t;
+      frontier.swap(next);
+    }
+    if (removed == n) break;
+    uint32_t mind = kNone;
+    for (uint32_t v = 0; v < n; ++v)
+      if (step[v] == kNone) mind = std::min(mind, deg[v]);
+    k = std::max(k + 1, mind);
+  }
+}
+
 int orc_aorder(const uint32_t* src, const uint32_t* dst, uint64_t m, uint32_t n,
                uint32_t* perm, uint32_t* osrc, uint32_t* odst) {
   const int BUCKETSIZE = 40;  // A-order.cpp:14

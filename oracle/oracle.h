@@ -87,6 +87,20 @@ uint64_t orc_kcore_orient(uint32_t n, const uint32_t* sym_rowptr,
                           const uint32_t* sym_col, const uint32_t* core,
                           uint32_t* osrc, uint32_t* odst);
 
+/* Removal steps of the level-synchronous k-core peel (the engine's
+ * degeneracy order; edge_directing/kcore_direction.cpp:91-126 peels the same
+ * levels one vertex at a time).  Level k starts at 0 and each level then
+ * begins at max(k + 1, least remaining degree).  Within a level, step t
+ * removes every remaining vertex whose degree among the remaining vertices
+ * is <= k; steps repeat until none is left at that degree.  Steps are
+ * numbered 0, 1, ... over the whole peel; core[v] gets the level.  Orient
+ * with orc_kcore_orient(step) (lower (step, id) is the source).
+ * pinned: core == orc_kcore (networkx.core_number), out-degree <= core on
+ * every vertex, hand-traced paths / stars / cliques; the step numbering is
+ * this engine's own (no reference counterpart). */
+void orc_degeneracy(uint32_t n, const uint32_t* sym_rowptr, const uint32_t* sym_col,
+                    uint32_t* step, uint32_t* core);
+
 /* A-order relabelling, vertex_ordering/A-order/A-order.cpp:62-230.
  * Inputs: raw edge list.  Outputs: perm[old] = new id (the badLookUpTable of
  * :206-210) and the oriented edges (the "normal" file, :219-224).

@@ -49,6 +49,7 @@ _sig("orc_adir_priority", None, [C.c_uint32, _u32p, _u32p, _i32p])
 _sig("orc_adir_orient", C.c_uint64, [C.c_uint32, _u32p, _u32p, _i32p, _u32p, _u32p])
 _sig("orc_kcore", None, [C.c_uint32, _u32p, _u32p, _u32p])
 _sig("orc_kcore_orient", C.c_uint64, [C.c_uint32, _u32p, _u32p, _u32p, _u32p, _u32p])
+_sig("orc_degeneracy", None, [C.c_uint32, _u32p, _u32p, _u32p, _u32p])
 _sig("orc_aorder", C.c_int, [_u32p, _u32p, C.c_uint64, C.c_uint32, _u32p, _u32p, _u32p])
 _sig("orc_count", C.c_uint64, [C.c_uint32, _u32p, _u32p])
 _sig("orc_count_omp", C.c_uint64, [C.c_uint32, _u32p, _u32p])
@@ -154,6 +155,14 @@ def kcore_orient(n, srp, scol, core):
     k = _lib.orc_kcore_orient(n, _u32(srp), _nz(_u32(scol), np.uint32),
                               _nz(_u32(core), np.uint32), os_, od)
     return os_[:k], od[:k]
+
+
+def degeneracy(n, srp, scol):
+    """(removal step, core number) per vertex of the level-synchronous peel."""
+    step = np.zeros(max(n, 1), np.uint32)
+    core = np.zeros(max(n, 1), np.uint32)
+    _lib.orc_degeneracy(n, _u32(srp), _nz(_u32(scol), np.uint32), step, core)
+    return step[:n], core[:n]
 
 
 def aorder(src, dst, n):

@@ -15,6 +15,7 @@ ORIENTS = {
     "tricore": tcamd.ORIENT_DEGREE_TRICORE,
     "adir": tcamd.ORIENT_ADIR,
     "kcore": tcamd.ORIENT_KCORE,
+    "degeneracy": tcamd.ORIENT_DEGENERACY,
 }
 KERNELS = ["hu", "bisson", "fox", "tricore", "auto"]
 
@@ -39,8 +40,11 @@ def oracle_csr(s, d, mode):
     if mode == "adir":
         key = orc.adir_priority(n, srp, scol)
         os_, od = orc.adir_orient(n, srp, scol, key)
-    else:
+    elif mode == "kcore":
         key = orc.kcore(n, srp, scol)
+        os_, od = orc.kcore_orient(n, srp, scol, key)
+    else:
+        key, _ = orc.degeneracy(n, srp, scol)
         os_, od = orc.kcore_orient(n, srp, scol, key)
     return orc.build_csr(os_, od, n)
 
@@ -95,6 +99,19 @@ def test_kcore_numbers_exact(ctx, case):
     expect = orc.kcore(n, srp, scol)
     g = ctx.graph(s, d).orient(tcamd.ORIENT_KCORE)
     np.testing.assert_array_equal(g.vertex_key(), expect.astype(np.int32))
+    g.close()
+
+
+@pytest.mark.parametrize("case", ["er", "rmat12", "ba", "rgg", "messy"])
+def test_degeneracy_steps_exact(ctx, case):
+    s, d = CASES[case]()
+    n = orc.num_nodes(s, d)
+    srp, scol = orc.build_sym(s, d, n)
+    step, core = orc.degeneracy(n, srp, scol)
+    g = ctx.graph(s, d).orient(tcamd.ORIENT_DEGENERACY)
+    np.testing.assert_array_equal(g.vertex_key(), step.astype(np.int32))
+    rp, _ = g.csr()
+    assert np.all(np.diff(rp.astype(np.int64)) <= core)
     g.close()
 
 

@@ -151,13 +151,14 @@ def _is_dag(rp, col):
     return nx.is_directed_acyclic_graph(G)
 
 
-@pytest.mark.parametrize("mode", MODES + ["kcore"])
+@pytest.mark.parametrize("mode", MODES + ["kcore", "degeneracy"])
 def test_orientations_are_dags_covering_each_edge_once(mode):
     s, d = rmat(9, 8, 11)
     n = orc.num_nodes(s, d)
-    if mode == "kcore":
+    if mode in ("kcore", "degeneracy"):
         srp, scol = orc.build_sym(s, d, n)
-        os_, od = orc.kcore_orient(n, srp, scol, orc.kcore(n, srp, scol))
+        key = orc.kcore(n, srp, scol) if mode == "kcore" else orc.degeneracy(n, srp, scol)[0]
+        os_, od = orc.kcore_orient(n, srp, scol, key)
         rp, col = orc.build_csr(os_, od, n)
     else:
         _, rp, col = orc.pipeline_count(s, d, mode)
@@ -218,6 +219,53 @@ def test_kcore_vs_networkx():
         G.add_nodes_from(range(n))
         ref = nx.core_number(G)
         assert [int(core[v]) for v in range(n)] == [ref[v] for v in range(n)]
+
+
+# ---- degeneracy order (level-synchronous k-core peel) -----------------------
+
+def _degen(s, d):
+    s, d = np.array(s, np.uint32), np.array(d, np.uint32)
+    n = orc.num_nodes(s, d)
+    srp, scol = orc.build_sym(s, d, n)
+    step, core = orc.degeneracy(n, srp, scol)
+    return step.tolist(), core.tolist()
+
+
+def test_degeneracy_hand_traces():
+    # path 0-1-2-3: level 1 removes both ends (step 0), then the middle pair,
+    # whose degrees fall to 1 (step 1)
+    assert _degen([0, 1, 2], [1, 2, 3]) == ([0, 1, 1, 0], [1, 1, 1, 1])
+    # path of five: the removal runs inwards one step at a time
+    assert _degen([0, 1, 2, 3], [1, 2, 3, 4]) == ([0, 1, 2, 1, 0], [1] * 5)
+    # star, centre 0: leaves at step 0, the centre drops to 1 at step 1
+    assert _degen([0, 0, 0, 0], [1, 2, 3, 4]) == ([1, 0, 0, 0, 0], [1] * 5)
+    # K4: nothing below level 3, then all four at once
+    s, d = clique(4)
+    assert _degen(s, d) == ([0] * 4, [3] * 4)
+    # triangle with pendant 3 on vertex 0: the pendant goes at level 1; vertex
+    # 0 falls to 2, which is not <= 1; level 2 takes the triangle in one step
+    assert _degen([0, 1, 2, 0], [1, 2, 0, 3]) == ([1, 1, 1, 0], [2, 2, 2, 1])
+
+
+@pytest.mark.parametrize("graph", ["rmat", "er", "messy"])
+def test_degeneracy_order_bounds_out_degree_by_core(graph):
+    s, d = {"rmat": lambda: rmat(11, 16, 3), "er": lambda: er(2000, 30000, 4),
+            "messy": messy}[graph]()
+    n = orc.num_nodes(s, d)
+    srp, scol = orc.build_sym(s, d, n)
+    step, core = orc.degeneracy(n, srp, scol)
+    # the levels are the core numbers (networkx-pinned orc_kcore)
+    np.testing.assert_array_equal(core, orc.kcore(n, srp, scol))
+    os_, od = orc.kcore_orient(n, srp, scol, step)
+    rp, col = orc.build_csr(os_, od, n)
+    dp = np.diff(rp.astype(np.int64))
+    assert np.all(dp <= core) and dp.max() <= core.max()
+    # with the (core, id) key of kcore_direction.cpp the bound fails on R-MAT
+    rk, _ = orc.build_csr(*orc.kcore_orient(n, srp, scol, core), n)
+    if graph == "rmat":
+        assert np.any(np.diff(rk.astype(np.int64)) > core)
+    # every orientation keeps the triangle count
+    assert orc.count_rows(rp, col, 0, n) == orc.count_rows(rk, _, 0, n)
 
 
 # ---- k-clique ---------------------------------------------------------------
