@@ -51,6 +51,15 @@ __device__ inline bool contains(const uint32_t* a, uint32_t lo, uint32_t hi, uin
   return false;
 }
 
+// keys[i] = hi - d+(roots[i]): an ascending sort puts the widest roots first.
+__global__ void k_root_keys(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ roots,
+                            uint64_t nr, uint32_t hi, uint32_t* __restrict__ keys) {
+  for (uint64_t i = gtid(); i < nr; i += gstride()) {
+    const uint32_t u = roots[i];
+    keys[i] = hi - (rp[u + 1] - rp[u]);
+  }
+}
+
 __global__ void k_deg_flags(const uint32_t* __restrict__ rp, uint32_t n, uint32_t lo, uint32_t hi,
                             uint8_t* __restrict__ f) {
   for (uint64_t u = gtid(); u < n; u += gstride()) {
@@ -270,9 +279,10 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
                                                        const uint32_t* __restrict__ col,
                                                        const uint32_t* __restrict__ roots,
                                                        uint64_t nroots, uint32_t copies,
-                                                       uint32_t k,
+                                                       uint32_t chunk, uint32_t k,
                                                        uint64_t* __restrict__ stk_all,
-                                                       unsigned long long* total) {
+                                                       unsigned long long* total,
+                                                       unsigned long long* claim) {
   constexpr uint32_t D = 64 * W;
   constexpr uint32_t LW = 2 * W + 1;  // stack words per level
   __shared__ uint32_t s_list[D];
@@ -283,217 +293,237 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
   uint64_t* stk = stk_all + ((uint64_t)blockIdx.x * NW + wv) * (D + 1) * LW;
   unsigned long long cnt = 0;  // lane 0 of each wave accumulates
   const uint32_t Q = copies * NW;  // walkers per root
-  for (uint64_t ti = blockIdx.x; ti < nroots * copies; ti += gridDim.x) {
-    const uint32_t u = roots[ti / copies];
-    const uint32_t q = (uint32_t)(ti % copies) * NW + wv;  // this wave's walker number
-    const uint32_t base = rp[u], d = rp[u + 1] - base;
-    for (uint32_t i = tid; i < d; i += 64 * NW) s_list[i] = col[base + i];
-    for (uint32_t i = tid; i < d * W; i += 64 * NW) s_M[i] = 0;
+  // (root, copy) tasks are claimed `chunk` at a time from a counter, in the
+  // order of `roots` (heaviest first), so no workgroup is left holding a
+  // fixed share of the heavy roots
+  const uint64_t ntask = nroots * copies;
+  __shared__ unsigned long long s_ti;
+  for (;;) {
+    if (tid == 0) s_ti = atomicAdd(claim, (unsigned long long)chunk);
     __syncthreads();
-    // symmetric adjacency of N+(u): bit (i, j) for every arc c_i -> c_j
-    for (uint32_t i = wv; i < d; i += NW) {
-      const uint32_t ci = s_list[i];
-      const uint32_t e1 = rp[ci + 1];
-      for (uint32_t e = rp[ci] + lane; e < e1; e += 64) {
-        const uint32_t x = col[e];
-        uint32_t lo = 0, len = d;
-        while (len > 1) {
-          const uint32_t half = len >> 1;
-          lo = s_list[lo + half] <= x ? lo + half : lo;
-          len -= half;
-        }
-        if (s_list[lo] == x) {
-          atomicOr((unsigned long long*)&s_M[i * W + (lo >> 6)], 1ull << (lo & 63));
-          atomicOr((unsigned long long*)&s_M[lo * W + (i >> 6)], 1ull << (i & 63));
+    const uint64_t tb = s_ti;
+    if (tb >= ntask) break;
+    const uint64_t te = tb + chunk < ntask ? tb + chunk : ntask;
+    for (uint64_t ti = tb; ti < te; ++ti) {
+      const uint32_t u = roots[ti / copies];
+      const uint32_t q = (uint32_t)(ti % copies) * NW + wv;  // this wave's walker number
+      const uint32_t base = rp[u], d = rp[u + 1] - base;
+      for (uint32_t i = tid; i < d; i += 64 * NW) s_list[i] = col[base + i];
+      for (uint32_t i = tid; i < d * W; i += 64 * NW) s_M[i] = 0;
+      __syncthreads();
+      // symmetric adjacency of N+(u): bit (i, j) for every arc c_i -> c_j
+      for (uint32_t i = wv; i < d; i += NW) {
+        const uint32_t ci = s_list[i];
+        const uint32_t e1 = rp[ci + 1];
+        for (uint32_t e = rp[ci] + lane; e < e1; e += 64) {
+          const uint32_t x = col[e];
+          uint32_t lo = 0, len = d;
+          while (len > 1) {
+            const uint32_t half = len >> 1;
+            lo = s_list[lo + half] <= x ? lo + half : lo;
+            len -= half;
+          }
+          if (s_list[lo] == x) {
+            atomicOr((unsigned long long*)&s_M[i * W + (lo >> 6)], 1ull << (lo & 63));
+            atomicOr((unsigned long long*)&s_M[lo * W + (i >> 6)], 1ull << (i & 63));
+          }
         }
       }
-    }
-    __syncthreads();
+      __syncthreads();
 
-    // Node (s_C, h, p): counted in closed form (push = false / no pivot
-    // needed), or its pivot is returned for the caller to branch on.
-    // Returns true when the node needs branching; piv and its branch set
-    // (lanes < W, word `lane`) are written to *piv_out / *todo_out.
-    auto node = [&](uint32_t h, uint32_t p, uint32_t* piv_out, uint64_t* todo_out) -> bool {
-      uint64_t cr[W];  // C, the same in every lane
-      uint32_t cs = 0;
-#pragma unroll
-      for (int w = 0; w < W; ++w) {
-        cr[w] = s_C[w];
-        cs += (uint32_t)__popcll(cr[w]);
-      }
-      const uint32_t r = k - h;
-      if (cs == 0) {
-        if (lane == 0) cnt += binom_small(p, r);
-        return false;
-      }
-      if (h + p + cs < k) return false;
-      if (r == 0) {
-        if (lane == 0) cnt += 1;
-        return false;
-      }
-      if (r == 1) {
-        if (lane == 0) cnt += p + cs;
-        return false;
-      }
-      if (r == 2) {
-        // edges(C) = half the sum over x in C of |N(x) & C| (lane x mod 64,
-        // word x / 64); empty words of C hold no x
-        uint32_t esum = 0;
-#pragma unroll
-        for (int t = 0; t < W; ++t) {
-          if (cr[t] == 0) continue;
-          const uint32_t x = 64u * t + lane;
-          if ((cr[t] >> lane) & 1ull) {
-#pragma unroll
-            for (int w = 0; w < W; ++w) esum += (uint32_t)__popcll(s_M[x * W + w] & cr[w]);
-          }
+      // Node (s_C, h, p): counted in closed form (push = false / no pivot
+      // needed), or its pivot is returned for the caller to branch on.
+      // Returns true when the node needs branching; piv and its branch set
+      // (lanes < W, word `lane`) are written to *piv_out / *todo_out.
+      // Nodes with h = 1 are walked by every walker: their single terms are
+      // counted by walker 0 (`lead`), and the sums of the r = 2, 3 closed
+      // forms are split over the walkers by 64-vertex word of C.  Those sums
+      // count every edge of C once (at its smaller end) and every triangle
+      // once (at its smallest vertex), so each walker's share is an integer.
+      auto node = [&](uint32_t h, uint32_t p, uint32_t* piv_out, uint64_t* todo_out) -> bool {
+        const bool lead = h != 1 || q == 0;
+        const bool part = h == 1 && Q > 1;
+        uint64_t cr[W];  // C, the same in every lane
+        uint32_t cs = 0;
+  #pragma unroll
+        for (int w = 0; w < W; ++w) {
+          cr[w] = s_C[w];
+          cs += (uint32_t)__popcll(cr[w]);
         }
-        const uint32_t e2 = wave_sum_u32(esum);
-        if (lane == 0) cnt += binom_small(p, 2) + (unsigned long long)p * cs + e2 / 2;
-        return false;
-      }
-      if (r == 3) {
-        // binom(p,3) + binom(p,2)|C| + p edges(C) + triangles(C).  Lane x
-        // holds R = N(x) & C; for every y in R above x, |R & N(y)| counts
-        // the triangles through edge (x, y), each triangle once per edge.
-        uint64_t esum = 0, tsum = 0;
-        for (int t = 0; t < W; ++t) {
-          if (cr[t] == 0) continue;
-          const uint32_t x = 64u * t + lane;
-          if (!((cr[t] >> lane) & 1ull)) continue;
-          uint64_t rx[W];
-#pragma unroll
-          for (int w = 0; w < W; ++w) {
-            rx[w] = s_M[x * W + w] & cr[w];
-            esum += (uint32_t)__popcll(rx[w]);
-          }
-#pragma unroll
-          for (int w = 0; w < W; ++w) {
-            uint64_t above = w < t ? 0ull : (w > t ? rx[w] : (rx[w] & ~((2ull << lane) - 1)));
-            while (above) {
-              const uint32_t y = 64u * w + (uint32_t)__builtin_ctzll(above);
-              above &= above - 1;
-              uint32_t c3 = 0;
-#pragma unroll
-              for (int w2 = 0; w2 < W; ++w2) c3 += (uint32_t)__popcll(rx[w2] & s_M[y * W + w2]);
-              tsum += c3;
+        const uint32_t r = k - h;
+        if (cs == 0) {
+          if (lane == 0 && lead) cnt += binom_small(p, r);
+          return false;
+        }
+        if (h + p + cs < k) return false;
+        if (r == 0) {
+          if (lane == 0 && lead) cnt += 1;
+          return false;
+        }
+        if (r == 1) {
+          if (lane == 0 && lead) cnt += p + cs;
+          return false;
+        }
+        if (r == 2) {
+          // binom(p,2) + p|C| + edges(C); lane x (word t) counts its
+          // neighbours in C above x
+          uint32_t esum = 0;
+  #pragma unroll
+          for (int t = 0; t < W; ++t) {
+            if (cr[t] == 0 || (part && (uint32_t)t % Q != q)) continue;
+            const uint32_t x = 64u * t + lane;
+            if ((cr[t] >> lane) & 1ull) {
+  #pragma unroll
+              for (int w = 0; w < W; ++w) {
+                const uint64_t above = w < t ? 0ull : (w > t ? ~0ull : ~((2ull << lane) - 1));
+                esum += (uint32_t)__popcll(s_M[x * W + w] & cr[w] & above);
+              }
             }
           }
+          const unsigned long long e = wave_sum_u32(esum);
+          if (lane == 0)
+            cnt += (lead ? binom_small(p, 2) + (unsigned long long)p * cs : 0ull) + e;
+          return false;
         }
-        const unsigned long long e2 = wave_sum(esum), t3 = wave_sum(tsum);
-        if (lane == 0)
-          cnt += binom_small(p, 3) + binom_small(p, 2) * cs + (unsigned long long)p * (e2 / 2) +
-                 t3 / 3;
-        return false;
-      }
-      // pivot: the member of C's first non-empty 64-word with most
-      // neighbours in C (any member gives exact counts; this one keeps the
-      // choice at W reads per lane)
-      uint32_t best = 0;
-      int t0 = 0;
-#pragma unroll
-      for (int t = W - 1; t >= 0; --t)
-        if (cr[t]) t0 = t;
-      {
-        uint64_t ct = 0;
-#pragma unroll
-        for (int t = 0; t < W; ++t)
-          if (t == t0) ct = cr[t];
-        const uint32_t x = 64u * (uint32_t)t0 + lane;
-        if ((ct >> lane) & 1ull) {
-          uint32_t sc = 0;
-#pragma unroll
-          for (int w = 0; w < W; ++w) sc += (uint32_t)__popcll(s_M[x * W + w] & cr[w]);
-          best = (sc << 16) | (0xFFFFu - x);  // (score, smallest index)
+        if (r == 3) {
+          // binom(p,3) + binom(p,2)|C| + p edges(C) + triangles(C).  Lane x
+          // holds A = N(x) & C above x: |A| edges start at x, and the pairs
+          // (y, z) of A with y ~ z, summed over y in A, count each triangle
+          // whose smallest vertex is x twice.
+          uint64_t esum = 0, tsum = 0;
+          for (int t = 0; t < W; ++t) {
+            if (cr[t] == 0 || (part && (uint32_t)t % Q != q)) continue;
+            const uint32_t x = 64u * t + lane;
+            if (!((cr[t] >> lane) & 1ull)) continue;
+            uint64_t ax[W];
+  #pragma unroll
+            for (int w = 0; w < W; ++w) {
+              const uint64_t above = w < t ? 0ull : (w > t ? ~0ull : ~((2ull << lane) - 1));
+              ax[w] = s_M[x * W + w] & cr[w] & above;
+              esum += (uint32_t)__popcll(ax[w]);
+            }
+  #pragma unroll
+            for (int w = 0; w < W; ++w) {
+              uint64_t ys = ax[w];
+              while (ys) {
+                const uint32_t y = 64u * w + (uint32_t)__builtin_ctzll(ys);
+                ys &= ys - 1;
+                uint32_t c3 = 0;
+  #pragma unroll
+                for (int w2 = 0; w2 < W; ++w2) c3 += (uint32_t)__popcll(ax[w2] & s_M[y * W + w2]);
+                tsum += c3;
+              }
+            }
+          }
+          const unsigned long long e = wave_sum(esum), t2 = wave_sum(tsum);
+          if (lane == 0)
+            cnt += (lead ? binom_small(p, 3) + binom_small(p, 2) * cs : 0ull) +
+                   (unsigned long long)p * e + t2 / 2;
+          return false;
         }
-      }
-      best = wave_max_u32(best);
-      const uint32_t piv = 0xFFFFu - (best & 0xFFFFu);
-      *piv_out = piv;
-      if (lane < (uint32_t)W) {
-        uint64_t todo = s_C[lane] & ~s_M[piv * W + lane];  // C outside N(pivot)
-        if (lane == (piv >> 6)) todo |= 1ull << (piv & 63);
-        *todo_out = todo;
-      }
-      return true;
-    };
+        // pivot: the member of C's first non-empty 64-word with most
+        // neighbours in C (any member gives exact counts; this one keeps the
+        // choice at W reads per lane)
+        uint32_t best = 0;
+        int t0 = 0;
+  #pragma unroll
+        for (int t = W - 1; t >= 0; --t)
+          if (cr[t]) t0 = t;
+        {
+          uint64_t ct = 0;
+  #pragma unroll
+          for (int t = 0; t < W; ++t)
+            if (t == t0) ct = cr[t];
+          const uint32_t x = 64u * (uint32_t)t0 + lane;
+          if ((ct >> lane) & 1ull) {
+            uint32_t sc = 0;
+  #pragma unroll
+            for (int w = 0; w < W; ++w) sc += (uint32_t)__popcll(s_M[x * W + w] & cr[w]);
+            best = (sc << 16) | (0xFFFFu - x);  // (score, smallest index)
+          }
+        }
+        best = wave_max_u32(best);
+        const uint32_t piv = 0xFFFFu - (best & 0xFFFFu);
+        *piv_out = piv;
+        if (lane < (uint32_t)W) {
+          uint64_t todo = s_C[lane] & ~s_M[piv * W + lane];  // C outside N(pivot)
+          if (lane == (piv >> 6)) todo |= 1ull << (piv & 63);
+          *todo_out = todo;
+        }
+        return true;
+      };
 
-    // Every walker runs the same depth-first walk from the root.  Nodes with
-    // h = 1 (the root and its pivot chain) are walked by all of them and
-    // counted by walker 0; the sub-trees entered through a first non-pivot
-    // branch (h = 2) are numbered in walk order and walker q takes those
-    // numbered q mod Q.  One heavy root thus spreads over Q waves.
-    if (lane < (uint32_t)W) {
-      const uint32_t lo = 64u * lane;
-      s_C[lane] = d >= lo + 64 ? ~0ull : (d > lo ? ((1ull << (d - lo)) - 1) : 0ull);
-    }
-    wave_sync_kcl();
-    uint32_t seq = 0;
-    int L = -1;
-    uint32_t piv = 0;
-    uint64_t todo = 0;
-    {
-      const unsigned long long before = cnt;
-      const bool br = node(1, 0, &piv, &todo);
-      if (q != 0) cnt = before;
-      if (br) {
-        L = 0;
-        if (lane < (uint32_t)W) {
-          stk[lane] = s_C[lane];
-          stk[W + lane] = todo;
-        }
-        if (lane == 0) stk[2 * W] = (uint64_t)piv | (1ull << 16);
-      }
-    }
-    wave_sync_kcl();
-    while (L >= 0) {
-      uint64_t* lv = stk + (uint64_t)L * LW;
-      const uint64_t todo_w = lane < (uint32_t)W ? lv[W + lane] : 0ull;
-      const uint64_t nzl = __ballot(todo_w != 0);
-      if (nzl == 0) {
-        --L;
-        continue;
-      }
-      const uint32_t fwl = (uint32_t)__builtin_ctzll(nzl);
-      const uint64_t twl = __shfl(todo_w, fwl, 64);
-      const uint32_t x = fwl * 64u + (uint32_t)__builtin_ctzll(twl);
-      const uint64_t meta = __shfl(lane == 0 ? lv[2 * W] : 0ull, 0, 64);
-      const uint32_t lpiv = (uint32_t)(meta & 0xFFFFu), h = (uint32_t)((meta >> 16) & 0xFFFFu),
-                     p = (uint32_t)(meta >> 32);
-      const bool xp = x == lpiv;
-      const uint32_t h2 = h + (xp ? 0u : 1u), p2 = p + (xp ? 1u : 0u);
-      // take x: child C = C & N(x); x leaves C and the branch set
-      bool skip = h2 > k;
-      if (h == 1 && h2 == 2) {
-        const bool mine = (seq++ % Q) == q;
-        skip = skip || !mine;
-      }
+      // Every walker runs the same depth-first walk from the root.  Nodes with
+      // h = 1 (the root and its pivot chain) are walked by all of them and
+      // counted as node() describes; the sub-trees entered through a first non-pivot
+      // branch (h = 2) are numbered in walk order and walker q takes those
+      // numbered q mod Q.  One heavy root thus spreads over Q waves.
       if (lane < (uint32_t)W) {
-        const uint64_t c = lv[lane];
-        const uint64_t xb = lane == fwl ? (1ull << (x & 63)) : 0ull;
-        if (!skip) s_C[lane] = c & s_M[x * W + lane];
-        lv[lane] = c & ~xb;
-        lv[W + lane] = todo_w & ~xb;
+        const uint32_t lo = 64u * lane;
+        s_C[lane] = d >= lo + 64 ? ~0ull : (d > lo ? ((1ull << (d - lo)) - 1) : 0ull);
       }
       wave_sync_kcl();
-      if (skip) continue;
-      // an h = 1 child is walked by every walker: walker 0 keeps its count
-      const unsigned long long before = cnt;
-      const bool br = node(h2, p2, &piv, &todo);
-      if (h2 == 1 && q != 0) cnt = before;
-      if (br) {
-        ++L;
-        uint64_t* nv = stk + (uint64_t)L * LW;
-        if (lane < (uint32_t)W) {
-          nv[lane] = s_C[lane];
-          nv[W + lane] = todo;
+      uint32_t seq = 0;
+      int L = -1;
+      uint32_t piv = 0;
+      uint64_t todo = 0;
+      {
+        const bool br = node(1, 0, &piv, &todo);
+        if (br) {
+          L = 0;
+          if (lane < (uint32_t)W) {
+            stk[lane] = s_C[lane];
+            stk[W + lane] = todo;
+          }
+          if (lane == 0) stk[2 * W] = (uint64_t)piv | (1ull << 16);
         }
-        if (lane == 0) nv[2 * W] = (uint64_t)piv | ((uint64_t)h2 << 16) | ((uint64_t)p2 << 32);
       }
       wave_sync_kcl();
+      while (L >= 0) {
+        uint64_t* lv = stk + (uint64_t)L * LW;
+        const uint64_t todo_w = lane < (uint32_t)W ? lv[W + lane] : 0ull;
+        const uint64_t nzl = __ballot(todo_w != 0);
+        if (nzl == 0) {
+          --L;
+          continue;
+        }
+        const uint32_t fwl = (uint32_t)__builtin_ctzll(nzl);
+        const uint64_t twl = __shfl(todo_w, fwl, 64);
+        const uint32_t x = fwl * 64u + (uint32_t)__builtin_ctzll(twl);
+        const uint64_t meta = __shfl(lane == 0 ? lv[2 * W] : 0ull, 0, 64);
+        const uint32_t lpiv = (uint32_t)(meta & 0xFFFFu), h = (uint32_t)((meta >> 16) & 0xFFFFu),
+                       p = (uint32_t)(meta >> 32);
+        const bool xp = x == lpiv;
+        const uint32_t h2 = h + (xp ? 0u : 1u), p2 = p + (xp ? 1u : 0u);
+        // take x: child C = C & N(x); x leaves C and the branch set
+        bool skip = h2 > k;
+        if (h == 1 && h2 == 2) {
+          const bool mine = (seq++ % Q) == q;
+          skip = skip || !mine;
+        }
+        if (lane < (uint32_t)W) {
+          const uint64_t c = lv[lane];
+          const uint64_t xb = lane == fwl ? (1ull << (x & 63)) : 0ull;
+          if (!skip) s_C[lane] = c & s_M[x * W + lane];
+          lv[lane] = c & ~xb;
+          lv[W + lane] = todo_w & ~xb;
+        }
+        wave_sync_kcl();
+        if (skip) continue;
+        const bool br = node(h2, p2, &piv, &todo);
+        if (br) {
+          ++L;
+          uint64_t* nv = stk + (uint64_t)L * LW;
+          if (lane < (uint32_t)W) {
+            nv[lane] = s_C[lane];
+            nv[W + lane] = todo;
+          }
+          if (lane == 0) nv[2 * W] = (uint64_t)piv | ((uint64_t)h2 << 16) | ((uint64_t)p2 << 32);
+        }
+        wave_sync_kcl();
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
   if (lane == 0 && cnt) atomicAdd(total, cnt);
 }
@@ -556,9 +586,10 @@ int kcl_count(tc_ctx* c, tc_graph* g, uint32_t k, int variant, uint64_t* count,
     buckets.push_back({257, 512, 8, false});
     buckets.push_back({513, 0xFFFFFFFFu, 0, false});
   }
-  DevBuf flags, roots, scratch;
+  DevBuf flags, roots, scratch, sortbuf, claimbuf;
   TCX_TRY(flags.reserve((size_t)g->n + 1));
   TCX_TRY(roots.reserve(4 * ((size_t)g->n + 1)));
+  TCX_TRY(claimbuf.reserve(64));
   TCX_TRY(c->accum.reserve(64));
   unsigned long long* acc = c->accum.as<unsigned long long>();
   TCX_HIP(hipMemsetAsync(acc, 0, 8, c->stream));
@@ -581,11 +612,11 @@ int kcl_count(tc_ctx* c, tc_graph* g, uint32_t k, int variant, uint64_t* count,
       hipLaunchKernelGGL(k_kcl_merge, dim3(G), dim3(64), 0, c->stream, rp, col,
                          roots.as<uint32_t>(), nr, k, maxd, slots, scratch.as<uint32_t>(), acc);
     } else if (b.pivot) {
-      // workgroups resident per CU (LDS-bound for the wide matrices); the
-      // wide ones run four walkers per root, and a root is repeated
-      // `copies` times so that the heaviest roots spread over many CUs
-      const uint64_t per_cu = b.w <= 4 ? 16 : (b.w == 8 ? 4 : 1);
-      const uint64_t nw = b.w <= 4 ? 1 : 4;
+      // workgroups resident per CU (LDS- and register-bound for the wide
+      // matrices); the wide ones run 8 or 16 walkers per root, and a root
+      // is repeated `copies` times when few roots would leave CUs idle
+      const uint64_t per_cu = b.w <= 4 ? 16 : (b.w == 8 ? 2 : 1);
+      const uint64_t nw = b.w <= 4 ? 1 : (b.w == 8 ? 8 : 16);
       const uint64_t slots = (uint64_t)c->num_cus * per_cu;
       const uint64_t max_copies = b.w <= 4 ? 1 : (b.w == 8 ? 8 : 32);
       const uint32_t copies =
@@ -594,13 +625,26 @@ int kcl_count(tc_ctx* c, tc_graph* g, uint32_t k, int variant, uint64_t* count,
       const uint64_t D = 64ull * b.w;
       TCX_TRY(scratch.reserve(8ull * G * nw * (D + 1) * (2ull * b.w + 1)));
       uint64_t* stk = scratch.as<uint64_t>();
-      const uint32_t* rts = roots.as<uint32_t>();
+      // widest roots first (stable: ties stay in id order)
+      TCX_TRY(sortbuf.reserve(12 * (nr + 1)));
+      uint32_t* keys = sortbuf.as<uint32_t>();
+      uint32_t* ktmp = keys + nr + 1;
+      uint32_t* vtmp = ktmp + nr + 1;
+      hipLaunchKernelGGL(k_root_keys, dim3(grid_for(nr)), dim3(256), 0, c->stream, rp,
+                         roots.as<uint32_t>(), nr, b.hi, keys);
+      TCX_HIP(hipGetLastError());
+      uint32_t *sk = nullptr, *rts = nullptr;
+      TCX_TRY(sort_pairs_u32_u32(c, keys, ktmp, roots.as<uint32_t>(), vtmp, nr, 11, &sk, &rts));
+      unsigned long long* claim = claimbuf.as<unsigned long long>();
+      TCX_HIP(hipMemsetAsync(claim, 0, 8, c->stream));
+      // narrow roots are cheap: claim several per atomic
+      const uint32_t chunk = b.w == 1 ? 16 : (b.w == 2 ? 8 : (b.w == 4 ? 4 : 1));
       switch (b.w) {
-        case 1: hipLaunchKernelGGL((k_kcl_pivot<1, 1>), dim3(G), dim3(64), 0, c->stream, rp, col, rts, nr, copies, k, stk, acc); break;
-        case 2: hipLaunchKernelGGL((k_kcl_pivot<2, 1>), dim3(G), dim3(64), 0, c->stream, rp, col, rts, nr, copies, k, stk, acc); break;
-        case 4: hipLaunchKernelGGL((k_kcl_pivot<4, 1>), dim3(G), dim3(64), 0, c->stream, rp, col, rts, nr, copies, k, stk, acc); break;
-        case 8: hipLaunchKernelGGL((k_kcl_pivot<8, 4>), dim3(G), dim3(256), 0, c->stream, rp, col, rts, nr, copies, k, stk, acc); break;
-        default: hipLaunchKernelGGL((k_kcl_pivot<16, 4>), dim3(G), dim3(256), 0, c->stream, rp, col, rts, nr, copies, k, stk, acc); break;
+        case 1: hipLaunchKernelGGL((k_kcl_pivot<1, 1>), dim3(G), dim3(64), 0, c->stream, rp, col, rts, nr, copies, chunk, k, stk, acc, claim); break;
+        case 2: hipLaunchKernelGGL((k_kcl_pivot<2, 1>), dim3(G), dim3(64), 0, c->stream, rp, col, rts, nr, copies, chunk, k, stk, acc, claim); break;
+        case 4: hipLaunchKernelGGL((k_kcl_pivot<4, 1>), dim3(G), dim3(64), 0, c->stream, rp, col, rts, nr, copies, chunk, k, stk, acc, claim); break;
+        case 8: hipLaunchKernelGGL((k_kcl_pivot<8, 8>), dim3(G), dim3(512), 0, c->stream, rp, col, rts, nr, copies, chunk, k, stk, acc, claim); break;
+        default: hipLaunchKernelGGL((k_kcl_pivot<16, 16>), dim3(G), dim3(1024), 0, c->stream, rp, col, rts, nr, copies, chunk, k, stk, acc, claim); break;
       }
     } else {
       uint32_t G = (uint32_t)std::min<uint64_t>(nr, (uint64_t)c->num_cus * 16);
