@@ -298,6 +298,7 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
   // fixed share of the heavy roots
   const uint64_t ntask = nroots * copies;
   __shared__ unsigned long long s_ti;
+  __shared__ uint32_t s_next;
   for (;;) {
     if (tid == 0) s_ti = atomicAdd(claim, (unsigned long long)chunk);
     __syncthreads();
@@ -310,6 +311,7 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
       const uint32_t base = rp[u], d = rp[u + 1] - base;
       for (uint32_t i = tid; i < d; i += 64 * NW) s_list[i] = col[base + i];
       for (uint32_t i = tid; i < d * W; i += 64 * NW) s_M[i] = 0;
+      if (tid == 0) s_next = NW;
       __syncthreads();
       // symmetric adjacency of N+(u): bit (i, j) for every arc c_i -> c_j
       for (uint32_t i = wv; i < d; i += NW) {
@@ -345,7 +347,7 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
         const bool part = h == 1 && Q > 1;
         uint64_t cr[W];  // C, the same in every lane
         uint32_t cs = 0;
-  #pragma unroll
+#pragma unroll
         for (int w = 0; w < W; ++w) {
           cr[w] = s_C[w];
           cs += (uint32_t)__popcll(cr[w]);
@@ -368,12 +370,12 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
           // binom(p,2) + p|C| + edges(C); lane x (word t) counts its
           // neighbours in C above x
           uint32_t esum = 0;
-  #pragma unroll
+#pragma unroll
           for (int t = 0; t < W; ++t) {
             if (cr[t] == 0 || (part && (uint32_t)t % Q != q)) continue;
             const uint32_t x = 64u * t + lane;
             if ((cr[t] >> lane) & 1ull) {
-  #pragma unroll
+#pragma unroll
               for (int w = 0; w < W; ++w) {
                 const uint64_t above = w < t ? 0ull : (w > t ? ~0ull : ~((2ull << lane) - 1));
                 esum += (uint32_t)__popcll(s_M[x * W + w] & cr[w] & above);
@@ -396,20 +398,20 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
             const uint32_t x = 64u * t + lane;
             if (!((cr[t] >> lane) & 1ull)) continue;
             uint64_t ax[W];
-  #pragma unroll
+#pragma unroll
             for (int w = 0; w < W; ++w) {
               const uint64_t above = w < t ? 0ull : (w > t ? ~0ull : ~((2ull << lane) - 1));
               ax[w] = s_M[x * W + w] & cr[w] & above;
               esum += (uint32_t)__popcll(ax[w]);
             }
-  #pragma unroll
+#pragma unroll
             for (int w = 0; w < W; ++w) {
               uint64_t ys = ax[w];
               while (ys) {
                 const uint32_t y = 64u * w + (uint32_t)__builtin_ctzll(ys);
                 ys &= ys - 1;
                 uint32_t c3 = 0;
-  #pragma unroll
+#pragma unroll
                 for (int w2 = 0; w2 < W; ++w2) c3 += (uint32_t)__popcll(ax[w2] & s_M[y * W + w2]);
                 tsum += c3;
               }
@@ -421,25 +423,20 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
                    (unsigned long long)p * e + t2 / 2;
           return false;
         }
-        // pivot: the member of C's first non-empty 64-word with most
-        // neighbours in C (any member gives exact counts; this one keeps the
-        // choice at W reads per lane)
+        // pivot: the member of C with most neighbours in C, the smallest such
+        // index on ties (any member gives exact counts; the best one leaves
+        // the fewest branches)
         uint32_t best = 0;
-        int t0 = 0;
-  #pragma unroll
-        for (int t = W - 1; t >= 0; --t)
-          if (cr[t]) t0 = t;
-        {
-          uint64_t ct = 0;
-  #pragma unroll
-          for (int t = 0; t < W; ++t)
-            if (t == t0) ct = cr[t];
-          const uint32_t x = 64u * (uint32_t)t0 + lane;
-          if ((ct >> lane) & 1ull) {
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+          if (cr[t] == 0) continue;
+          const uint32_t x = 64u * t + lane;
+          if ((cr[t] >> lane) & 1ull) {
             uint32_t sc = 0;
-  #pragma unroll
+#pragma unroll
             for (int w = 0; w < W; ++w) sc += (uint32_t)__popcll(s_M[x * W + w] & cr[w]);
-            best = (sc << 16) | (0xFFFFu - x);  // (score, smallest index)
+            const uint32_t key = (sc << 16) | (0xFFFFu - x);  // (score, smallest index)
+            best = key > best ? key : best;
           }
         }
         best = wave_max_u32(best);
@@ -453,17 +450,18 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
         return true;
       };
 
-      // Every walker runs the same depth-first walk from the root.  Nodes with
-      // h = 1 (the root and its pivot chain) are walked by all of them and
-      // counted as node() describes; the sub-trees entered through a first non-pivot
-      // branch (h = 2) are numbered in walk order and walker q takes those
-      // numbered q mod Q.  One heavy root thus spreads over Q waves.
+      // Every walker runs the same depth-first walk from the root.  Nodes
+      // with h = 1 (the root and its pivot chain) are walked by all of them
+      // and counted as node() describes; the sub-trees entered through a
+      // first non-pivot branch (h = 2) are numbered in walk order and each
+      // is taken by one walker (below).  One heavy root thus spreads over Q
+      // waves.
       if (lane < (uint32_t)W) {
         const uint32_t lo = 64u * lane;
         s_C[lane] = d >= lo + 64 ? ~0ull : (d > lo ? ((1ull << (d - lo)) - 1) : 0ull);
       }
       wave_sync_kcl();
-      uint32_t seq = 0;
+      uint32_t seq = 0, mine = q;
       int L = -1;
       uint32_t piv = 0;
       uint64_t todo = 0;
@@ -498,8 +496,23 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
         // take x: child C = C & N(x); x leaves C and the branch set
         bool skip = h2 > k;
         if (h == 1 && h2 == 2) {
-          const bool mine = (seq++ % Q) == q;
-          skip = skip || !mine;
+          // with one copy per root the walkers of the workgroup claim
+          // sub-trees from an LDS counter as they reach them (each holds one
+          // claim, always ahead of its own walk); otherwise walker q takes
+          // every Q-th
+          bool take;
+          if (NW > 1 && copies == 1) {
+            take = seq == mine;
+            if (take) {
+              uint32_t cl = 0;
+              if (lane == 0) cl = atomicAdd(&s_next, 1u);
+              mine = __shfl(cl, 0, 64);
+            }
+          } else {
+            take = (seq % Q) == q;
+          }
+          ++seq;
+          skip = skip || !take;
         }
         if (lane < (uint32_t)W) {
           const uint64_t c = lv[lane];
