@@ -457,6 +457,7 @@ __global__ __launch_bounds__(TB) void k_per_vertex(const uint32_t* __restrict__ 
 
 // ---- schedule builders ------------------------------------------------------
 
+
 static int ensure_version(tc_graph* g) {
   SchedCache& sc = g->sc;
   if (sc.version != g->sched_version) {
@@ -474,6 +475,15 @@ static unsigned grid_for(uint64_t n, int tb = TB) {
   if (b < 1) b = 1;
   if (b > (1u << 20)) b = 1u << 20;
   return (unsigned)b;
+}
+
+int per_vertex_counts(tc_ctx* c, tc_graph* g, unsigned long long* tri) {
+  if (g->n == 0) return TC_OK;
+  hipLaunchKernelGGL(k_per_vertex, dim3(std::min<uint64_t>(grid_for(g->n * 64ull), 65536)),
+                     dim3(TB), 0, c->stream, g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(), g->n,
+                     tri);
+  TCX_HIP(hipGetLastError());
+  return TC_OK;
 }
 
 // start[] over a per-vertex work array with the Hu partition rule.
@@ -881,10 +891,7 @@ int tc_count_per_vertex(tc_ctx* c, tc_graph* g, uint64_t* tri) {
   if (g->n == 0) return TC_OK;
   DevBuf d;
   TCX_TRY(d.reserve(8 * (size_t)g->n));
-  hipLaunchKernelGGL(k_per_vertex, dim3(std::min<uint64_t>(grid_for(g->n * 64ull), 65536)),
-                     dim3(TB), 0, c->stream, g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(), g->n,
-                     d.as<unsigned long long>());
-  TCX_HIP(hipGetLastError());
+  TCX_TRY(tcx::per_vertex_counts(c, g, d.as<unsigned long long>()));
   TCX_HIP(hipMemcpyAsync(tri, d.p, 8 * (size_t)g->n, hipMemcpyDeviceToHost, c->stream));
   TCX_HIP(hipStreamSynchronize(c->stream));
   return TC_OK;

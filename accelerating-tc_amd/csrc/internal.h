@@ -86,6 +86,10 @@ struct SchedCache {
   uint64_t learned_version = ~0ull, learned_model = ~0ull;
   DevBuf l_blk, l_wav, l_huge;
   uint64_t nl_blk = 0, nl_wav = 0, nl_huge = 0;
+  // learned k-clique plan (per k and model id)
+  uint64_t kcl_version = ~0ull, kcl_model = ~0ull;
+  uint32_t kcl_k = 0;
+  DevBuf kcl_path, kcl_key;
 };
 }  // namespace tcx
 
@@ -139,6 +143,16 @@ int build_csr_from_keys(tc_ctx* c, tc_graph* g, uint64_t* keys, uint64_t nkeys,
                         int end_bit);
 int compute_csr_stats(tc_ctx* c, tc_graph* g);
 int build_sym(tc_ctx* c, tc_graph* g, DevBuf& rowptr, DevBuf& col, uint64_t* m2);
+
+// ---- cost model and per-vertex counts (mlp.hip, count.hip) ---------------
+// Predicted log2 costs Y[n][2] for device features X[n][8]; fails when the
+// model belongs to another context.
+int model_predict(tc_ctx* c, tc_model* m, const float* dX, uint32_t n, float* dY);
+// Identity of the model's current parameters (changes with every update).
+uint64_t model_id(const tc_model* m);
+// tri[u] = |{v -> w : v, w in N+(u)}|, the edges of the sub-graph of N+(u)
+// (uint64 per vertex, device array of n entries).
+int per_vertex_counts(tc_ctx* c, tc_graph* g, unsigned long long* tri);
 
 // ---- A-order (aorder.cpp, host heap) -------------------------------------
 int aorder_perm_from_outdeg(const uint32_t* rtd, uint32_t n, uint32_t* perm);

@@ -11,6 +11,7 @@
 // Every k-clique has a unique source in the DAG and a unique chain order,
 // so it is counted exactly once on either path.
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 
 #include "internal.h"
@@ -19,6 +20,8 @@ namespace tcx {
 namespace {
 
 constexpr int kMaxK = 12;
+constexpr int kKclFeatures = TC_MODEL_FEATURES;
+constexpr uint32_t kPivotMax = 1024;  // widest bit matrix in LDS (16 words per row)
 
 __device__ inline uint64_t gtid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ inline uint64_t gstride() { return (uint64_t)gridDim.x * blockDim.x; }
@@ -60,11 +63,86 @@ __global__ void k_root_keys(const uint32_t* __restrict__ rp, const uint32_t* __r
   }
 }
 
+// f[u] = lo <= d+(u) <= hi, and (with a plan) the plan sends u to `want`
 __global__ void k_deg_flags(const uint32_t* __restrict__ rp, uint32_t n, uint32_t lo, uint32_t hi,
+                            const uint8_t* __restrict__ path, uint8_t want,
                             uint8_t* __restrict__ f) {
   for (uint64_t u = gtid(); u < n; u += gstride()) {
     uint32_t d = rp[u + 1] - rp[u];
-    f[u] = (d >= lo && d <= hi) ? 1 : 0;
+    f[u] = (d >= lo && d <= hi && (!path || path[u] == want)) ? 1 : 0;
+  }
+}
+
+__global__ void k_plan_keys(const uint32_t* __restrict__ roots, uint64_t nr,
+                            const uint32_t* __restrict__ key, uint32_t* __restrict__ keys) {
+  for (uint64_t i = gtid(); i < nr; i += gstride()) keys[i] = key[roots[i]];
+}
+
+// Per-root features X[n][8] for k (roots with d+ >= k-1; others zero):
+// log2(1+d+), log2(1+m) with m the edges of the sub-graph of N+(u),
+// density m / C(d+,2), k, log2(1+sum_v d+(v)), log2 of the k-1 cliques a
+// random graph of that density would hold (C(d+,k-1) p^C(k-1,2)), the
+// bit-matrix width class (0: d+ <= 64 ... 4: <= 1024, 5: above),
+// log2(1+max_v d+(v)).
+__global__ void k_kcl_features(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+                               const unsigned long long* __restrict__ tri, uint32_t n, uint32_t k,
+                               float* __restrict__ X) {
+  for (uint64_t u = gtid(); u < n; u += gstride()) {
+    const uint32_t b = rp[u], d = rp[u + 1] - b;
+    float* x = X + u * kKclFeatures;
+    if (d + 1 < k) {
+      for (int j = 0; j < kKclFeatures; ++j) x[j] = 0.f;
+      continue;
+    }
+    uint64_t s2 = 0;
+    uint32_t mx = 0;
+    for (uint32_t e = b; e < b + d; ++e) {
+      const uint32_t v = col[e];
+      const uint32_t dv = rp[v + 1] - rp[v];
+      s2 += dv;
+      mx = dv > mx ? dv : mx;
+    }
+    const double m = (double)tri[u];
+    const double pairs = 0.5 * (double)d * (double)(d - 1);
+    const double dens = pairs > 0 ? m / pairs : 0.0;
+    const double r = (double)(k - 1);
+    double est = -30.0;
+    if (dens > 0) {
+      const double lc = (lgamma((double)d + 1) - lgamma(r + 1) - lgamma((double)d - r + 1)) / M_LN2;
+      est = fmax(-30.0, lc + 0.5 * r * (r - 1) * log2(dens));
+    }
+    x[0] = (float)log2(1.0 + d);
+    x[1] = (float)log2(1.0 + m);
+    x[2] = (float)dens;
+    x[3] = (float)k;
+    x[4] = (float)log2(1.0 + (double)s2);
+    x[5] = (float)est;
+    x[6] = d <= 64 ? 0.f : d <= 128 ? 1.f : d <= 256 ? 2.f : d <= 512 ? 3.f : d <= 1024 ? 4.f : 5.f;
+    x[7] = (float)log2(1.0 + mx);
+  }
+}
+
+// Learned plan from predicted log2 costs Y[n][2] (pivot, merge).  The fixed
+// rule (pivot iff d+ <= 1024) is the prior: a root leaves it only for a
+// predicted saving above kPlanMargin.  Each launch takes its roots widest
+// first, as kcl_count does (ordering by predicted cost measured slower).
+constexpr float kPlanMargin = 0.25f;
+__global__ void k_kcl_plan(const uint32_t* __restrict__ rp, uint32_t n, uint32_t need,
+                           const float* __restrict__ Y, uint8_t* __restrict__ path,
+                           uint32_t* __restrict__ key) {
+  const float lm = log2f(1.f - kPlanMargin);
+  for (uint64_t u = gtid(); u < n; u += gstride()) {
+    const uint32_t d = rp[u + 1] - rp[u];
+    if (d < need) {
+      path[u] = 255;
+      key[u] = 0xFFFFFFFFu;
+      continue;
+    }
+    const float yp = Y[2 * u], ym = Y[2 * u + 1];
+    bool pivot = d <= kPivotMax;
+    if (pivot && ym < yp + lm) pivot = false;
+    path[u] = pivot ? 0 : 1;
+    key[u] = 0xFFFFFFFFu - d;
   }
 }
 
@@ -167,7 +245,8 @@ __global__ __launch_bounds__(64) void k_kcl_merge(const uint32_t* __restrict__ r
                                                   const uint32_t* __restrict__ roots,
                                                   uint64_t nroots, uint32_t k, uint32_t max_d,
                                                   uint32_t slots, uint32_t* __restrict__ scratch,
-                                                  unsigned long long* total) {
+                                                  unsigned long long* total,
+                                                  float* __restrict__ cost, float cost_scale) {
   __shared__ uint32_t s_size[kMaxK];
   __shared__ uint32_t s_idx[kMaxK];
   const uint32_t lane = threadIdx.x;
@@ -176,6 +255,7 @@ __global__ __launch_bounds__(64) void k_kcl_merge(const uint32_t* __restrict__ r
   const int last = (int)k - 2;
   for (uint64_t ri = blockIdx.x; ri < nroots; ri += gridDim.x) {
     const uint32_t u = roots[ri];
+    const uint64_t t_root = cost ? wall_clock64() : 0;
     const uint32_t ub = rp[u], d = rp[u + 1] - ub;
     s_size[1] = d;
     s_idx[1] = 0;
@@ -218,6 +298,7 @@ __global__ __launch_bounds__(64) void k_kcl_merge(const uint32_t* __restrict__ r
       }
     }
     __syncthreads();
+    if (cost && lane == 0) cost[u] = (float)(wall_clock64() - t_root) * cost_scale;
   }
   cnt = wave_sum(cnt);
   if (lane == 0 && cnt) atomicAdd(total, cnt);
@@ -282,7 +363,8 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
                                                        uint32_t chunk, uint32_t k,
                                                        uint64_t* __restrict__ stk_all,
                                                        unsigned long long* total,
-                                                       unsigned long long* claim) {
+                                                       unsigned long long* claim,
+                                                       float* __restrict__ cost, float cost_scale) {
   constexpr uint32_t D = 64 * W;
   constexpr uint32_t LW = 2 * W + 1;  // stack words per level
   __shared__ uint32_t s_list[D];
@@ -307,6 +389,7 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
     const uint64_t te = tb + chunk < ntask ? tb + chunk : ntask;
     for (uint64_t ti = tb; ti < te; ++ti) {
       const uint32_t u = roots[ti / copies];
+      const uint64_t t_task = (cost && tid == 0) ? wall_clock64() : 0;
       const uint32_t q = (uint32_t)(ti % copies) * NW + wv;  // this wave's walker number
       const uint32_t base = rp[u], d = rp[u + 1] - base;
       for (uint32_t i = tid; i < d; i += 64 * NW) s_list[i] = col[base + i];
@@ -536,6 +619,7 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
         wave_sync_kcl();
       }
       __syncthreads();
+      if (cost && tid == 0) atomicAdd(cost + u, (float)(wall_clock64() - t_task) * cost_scale);
     }
   }
   if (lane == 0 && cnt) atomicAdd(total, cnt);
@@ -553,43 +637,56 @@ static unsigned grid_for(uint64_t n, int tb = 256) {
   return (unsigned)b;
 }
 
-extern "C" {
+namespace {
 
-int kcl_count(tc_ctx* c, tc_graph* g, uint32_t k, int variant, uint64_t* count,
-              tc_timing* timing) {
-  if (!c || !g || !count) return fail(TC_ERR_ARG, "NULL argument");
-  if (g->ctx != c) return fail(TC_ERR_ARG, "graph belongs to another context");
-  if (g->orient_mode < 0) return fail(TC_ERR_STATE, "graph must be oriented before counting");
-  if (k == 0 || k > (uint32_t)kMaxK) return fail(TC_ERR_RANGE, "k must be in 1..%d", kMaxK);
-  if (variant < KCL_VARIANT_AUTO || variant > KCL_VARIANT_PIVOT)
-    return fail(TC_ERR_ARG, "unknown k-clique variant %d", variant);
-  TCX_HIP(hipSetDevice(c->device));
+struct KclPlan {
+  const uint8_t* path;  // 0 pivot, 1 merge, 255 none
+  const uint32_t* key;  // ascending = launched first
+};
+
+// Fixed plan for cost measurement: every root with need <= d+ <= hi goes to
+// `want`, widest first.
+__global__ void k_fixed_plan(const uint32_t* __restrict__ rp, uint32_t n, uint32_t need,
+                             uint32_t hi, uint8_t want, uint8_t* __restrict__ path,
+                             uint32_t* __restrict__ key) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n;
+       u += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t d = rp[u + 1] - rp[u];
+    path[u] = (d >= need && d <= hi) ? want : 255;
+    key[u] = 0xFFFFFFFFu - d;
+  }
+}
+
+// Counts k-cliques over the roots with d+ >= k-1 (k >= 3, m+ > 0).  plan
+// (optional): per-root path and launch order.  cost (optional, device, n
+// floats, zeroed by the caller): receives each root's measured
+// CU-microseconds on the pivot and merge paths.
+int kcl_run(tc_ctx* c, tc_graph* g, uint32_t k, int variant, const KclPlan* plan, float* cost,
+            uint64_t* count, tc_timing* timing) {
   tc_timing tm;
   memset(&tm, 0, sizeof(tm));
-  *count = 0;
-  if (k == 1) { *count = g->n; if (timing) *timing = tm; return TC_OK; }
-  if (k == 2) { *count = g->m_or; if (timing) *timing = tm; return TC_OK; }
-  if (g->m_or == 0) { if (timing) *timing = tm; return TC_OK; }
-  if (k == 3 && variant == KCL_VARIANT_AUTO) {
-    // 3-cliques are triangles: the AUTO triangle engine counts them directly
-    tc_sched s;
-    memset(&s, 0, sizeof(s));
-    s.kernel = TC_KERNEL_AUTO;
-    return tc_count(c, g, &s, count, timing);
-  }
   const uint32_t* rp = g->rowptr.as<uint32_t>();
   const uint32_t* col = g->col.as<uint32_t>();
   // root buckets: [lo, hi] of d+ and the path that takes them
   struct Bucket { uint32_t lo, hi; int w; bool pivot; };  // w = bitmap words, 0 = merge
   std::vector<Bucket> buckets;
   const uint32_t need = k - 1;  // a root needs k-1 out-neighbours
-  if (variant == KCL_VARIANT_AUTO || variant == KCL_VARIANT_PIVOT) {
+  if (plan) {
+    // learned: pivot buckets hold the roots the plan sends to the pivot
+    // path, one merge launch takes the rest
     buckets.push_back({need, 64, 1, true});
     buckets.push_back({65, 128, 2, true});
     buckets.push_back({129, 256, 4, true});
     buckets.push_back({257, 512, 8, true});
-    buckets.push_back({513, 1024, 16, true});
-    buckets.push_back({1025, 0xFFFFFFFFu, 0, false});
+    buckets.push_back({513, kPivotMax, 16, true});
+    buckets.push_back({need, 0xFFFFFFFFu, 0, false});
+  } else if (variant == KCL_VARIANT_AUTO || variant == KCL_VARIANT_PIVOT) {
+    buckets.push_back({need, 64, 1, true});
+    buckets.push_back({65, 128, 2, true});
+    buckets.push_back({129, 256, 4, true});
+    buckets.push_back({257, 512, 8, true});
+    buckets.push_back({513, kPivotMax, 16, true});
+    buckets.push_back({kPivotMax + 1, 0xFFFFFFFFu, 0, false});
   } else if (variant == KCL_VARIANT_MERGE || k > (uint32_t)kBitmapMaxK) {
     buckets.push_back({need, 0xFFFFFFFFu, 0, false});
   } else {
@@ -600,6 +697,7 @@ int kcl_count(tc_ctx* c, tc_graph* g, uint32_t k, int variant, uint64_t* count,
     buckets.push_back({513, 0xFFFFFFFFu, 0, false});
   }
   DevBuf flags, roots, scratch, sortbuf, claimbuf;
+  const float us_per_tick = 1000.f / (float)c->wall_clock_khz;
   TCX_TRY(flags.reserve((size_t)g->n + 1));
   TCX_TRY(roots.reserve(4 * ((size_t)g->n + 1)));
   TCX_TRY(claimbuf.reserve(64));
@@ -610,20 +708,49 @@ int kcl_count(tc_ctx* c, tc_graph* g, uint32_t k, int variant, uint64_t* count,
   for (const Bucket& b : buckets) {
     uint32_t lo = std::max(b.lo, need);
     if (lo > b.hi) continue;
+    const uint8_t* path = plan ? plan->path : nullptr;
+    const uint8_t want = b.w == 0 ? 1 : 0;
     hipLaunchKernelGGL(k_deg_flags, dim3(grid_for(g->n)), dim3(256), 0, c->stream, rp, g->n, lo,
-                       b.hi, flags.as<uint8_t>());
+                       b.hi, path, want, flags.as<uint8_t>());
     TCX_HIP(hipGetLastError());
     uint64_t nr = 0;
     TCX_TRY(select_indices(c, flags.as<uint8_t>(), g->n, roots.as<uint32_t>(), &nr));
     if (nr == 0) continue;
+    // each launch takes its roots in the plan's order, or (pivot launches
+    // without a plan) widest first; sorted before the timed region
+    const uint32_t* order = roots.as<uint32_t>();
+    if (!plan && b.pivot) {
+      TCX_TRY(sortbuf.reserve(12 * (nr + 1)));
+      uint32_t* keys = sortbuf.as<uint32_t>();
+      uint32_t* ktmp = keys + nr + 1;
+      uint32_t* vtmp = ktmp + nr + 1;
+      hipLaunchKernelGGL(k_root_keys, dim3(grid_for(nr)), dim3(256), 0, c->stream, rp,
+                         roots.as<uint32_t>(), nr, b.hi, keys);
+      TCX_HIP(hipGetLastError());
+      uint32_t *sk = nullptr, *sv = nullptr;
+      TCX_TRY(sort_pairs_u32_u32(c, keys, ktmp, roots.as<uint32_t>(), vtmp, nr, 11, &sk, &sv));
+      order = sv;
+    } else if (plan) {
+      TCX_TRY(sortbuf.reserve(12 * (nr + 1)));
+      uint32_t* keys = sortbuf.as<uint32_t>();
+      uint32_t* ktmp = keys + nr + 1;
+      uint32_t* vtmp = ktmp + nr + 1;
+      hipLaunchKernelGGL(k_plan_keys, dim3(grid_for(nr)), dim3(256), 0, c->stream,
+                         roots.as<uint32_t>(), nr, plan->key, keys);
+      TCX_HIP(hipGetLastError());
+      uint32_t *sk = nullptr, *sv = nullptr;
+      TCX_TRY(sort_pairs_u32_u32(c, keys, ktmp, roots.as<uint32_t>(), vtmp, nr, 32, &sk, &sv));
+      order = sv;
+    }
     TCX_HIP(hipEventRecord(c->ev0, c->stream));
     if (b.w == 0) {
       uint32_t G = (uint32_t)std::min<uint64_t>(nr, (uint64_t)c->num_cus * 8);
       uint32_t maxd = std::max<uint32_t>(g->max_out, 1);
       uint32_t slots = k > 3 ? k - 3 : 1;
       TCX_TRY(scratch.reserve(4ull * G * slots * maxd));
-      hipLaunchKernelGGL(k_kcl_merge, dim3(G), dim3(64), 0, c->stream, rp, col,
-                         roots.as<uint32_t>(), nr, k, maxd, slots, scratch.as<uint32_t>(), acc);
+      // cost: CU-microseconds (one wave of the num_cus * 8 holds 1/8 of a CU)
+      hipLaunchKernelGGL(k_kcl_merge, dim3(G), dim3(64), 0, c->stream, rp, col, order, nr, k,
+                         maxd, slots, scratch.as<uint32_t>(), acc, cost, us_per_tick / 8.f);
     } else if (b.pivot) {
       // workgroups resident per CU (LDS- and register-bound for the wide
       // matrices); the wide ones run 8 or 16 walkers per root, and a root
@@ -638,26 +765,19 @@ int kcl_count(tc_ctx* c, tc_graph* g, uint32_t k, int variant, uint64_t* count,
       const uint64_t D = 64ull * b.w;
       TCX_TRY(scratch.reserve(8ull * G * nw * (D + 1) * (2ull * b.w + 1)));
       uint64_t* stk = scratch.as<uint64_t>();
-      // widest roots first (stable: ties stay in id order)
-      TCX_TRY(sortbuf.reserve(12 * (nr + 1)));
-      uint32_t* keys = sortbuf.as<uint32_t>();
-      uint32_t* ktmp = keys + nr + 1;
-      uint32_t* vtmp = ktmp + nr + 1;
-      hipLaunchKernelGGL(k_root_keys, dim3(grid_for(nr)), dim3(256), 0, c->stream, rp,
-                         roots.as<uint32_t>(), nr, b.hi, keys);
-      TCX_HIP(hipGetLastError());
-      uint32_t *sk = nullptr, *rts = nullptr;
-      TCX_TRY(sort_pairs_u32_u32(c, keys, ktmp, roots.as<uint32_t>(), vtmp, nr, 11, &sk, &rts));
+      const uint32_t* rts = order;  // widest first (stable: ties in id order)
+      // cost: CU-microseconds (a workgroup holds 1/per_cu of a CU)
+      const float cs = us_per_tick / (float)per_cu;
       unsigned long long* claim = claimbuf.as<unsigned long long>();
       TCX_HIP(hipMemsetAsync(claim, 0, 8, c->stream));
       // narrow roots are cheap: claim several per atomic
       const uint32_t chunk = b.w == 1 ? 16 : (b.w == 2 ? 8 : (b.w == 4 ? 4 : 1));
       switch (b.w) {
-        case 1: hipLaunchKernelGGL((k_kcl_pivot<1, 1>), dim3(G), dim3(64), 0, c->stream, rp, col, rts, nr, copies, chunk, k, stk, acc, claim); break;
-        case 2: hipLaunchKernelGGL((k_kcl_pivot<2, 1>), dim3(G), dim3(64), 0, c->stream, rp, col, rts, nr, copies, chunk, k, stk, acc, claim); break;
-        case 4: hipLaunchKernelGGL((k_kcl_pivot<4, 1>), dim3(G), dim3(64), 0, c->stream, rp, col, rts, nr, copies, chunk, k, stk, acc, claim); break;
-        case 8: hipLaunchKernelGGL((k_kcl_pivot<8, 8>), dim3(G), dim3(512), 0, c->stream, rp, col, rts, nr, copies, chunk, k, stk, acc, claim); break;
-        default: hipLaunchKernelGGL((k_kcl_pivot<16, 16>), dim3(G), dim3(1024), 0, c->stream, rp, col, rts, nr, copies, chunk, k, stk, acc, claim); break;
+        case 1: hipLaunchKernelGGL((k_kcl_pivot<1, 1>), dim3(G), dim3(64), 0, c->stream, rp, col, rts, nr, copies, chunk, k, stk, acc, claim, cost, cs); break;
+        case 2: hipLaunchKernelGGL((k_kcl_pivot<2, 1>), dim3(G), dim3(64), 0, c->stream, rp, col, rts, nr, copies, chunk, k, stk, acc, claim, cost, cs); break;
+        case 4: hipLaunchKernelGGL((k_kcl_pivot<4, 1>), dim3(G), dim3(64), 0, c->stream, rp, col, rts, nr, copies, chunk, k, stk, acc, claim, cost, cs); break;
+        case 8: hipLaunchKernelGGL((k_kcl_pivot<8, 8>), dim3(G), dim3(512), 0, c->stream, rp, col, rts, nr, copies, chunk, k, stk, acc, claim, cost, cs); break;
+        default: hipLaunchKernelGGL((k_kcl_pivot<16, 16>), dim3(G), dim3(1024), 0, c->stream, rp, col, rts, nr, copies, chunk, k, stk, acc, claim, cost, cs); break;
       }
     } else {
       uint32_t G = (uint32_t)std::min<uint64_t>(nr, (uint64_t)c->num_cus * 16);
@@ -687,6 +807,163 @@ int kcl_count(tc_ctx* c, tc_graph* g, uint32_t k, int variant, uint64_t* count,
   tm.ms_count = total_ms;
   if (timing) *timing = tm;
   return TC_OK;
+}
+
+int kcl_check(tc_ctx* c, tc_graph* g, uint32_t k) {
+  if (!c || !g) return fail(TC_ERR_ARG, "NULL argument");
+  if (g->ctx != c) return fail(TC_ERR_ARG, "graph belongs to another context");
+  if (g->orient_mode < 0) return fail(TC_ERR_STATE, "graph must be oriented before counting");
+  if (k < 3 || k > (uint32_t)kMaxK) return fail(TC_ERR_RANGE, "k must be in 3..%d", kMaxK);
+  TCX_HIP(hipSetDevice(c->device));
+  return TC_OK;
+}
+
+// per-root features (device, n x 8) for k
+int root_features(tc_ctx* c, tc_graph* g, uint32_t k, DevBuf& X) {
+  const uint32_t n = g->n;
+  DevBuf tri;
+  TCX_TRY(tri.reserve(8 * ((size_t)n + 1)));
+  TCX_TRY(X.reserve(4ull * kKclFeatures * ((size_t)n + 1)));
+  TCX_TRY(per_vertex_counts(c, g, tri.as<unsigned long long>()));
+  hipLaunchKernelGGL(k_kcl_features, dim3(grid_for(n)), dim3(256), 0, c->stream,
+                     g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(),
+                     tri.as<unsigned long long>(), n, k, X.as<float>());
+  TCX_HIP(hipGetLastError());
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  return TC_OK;
+}
+
+// The learned plan of (graph, k, model), built once and kept with the
+// graph's schedules (rebuilt when the CSR, k or the model's parameters
+// change).  prep_ms (optional) receives the build time, 0 when cached.
+int cached_plan(tc_ctx* c, tc_graph* g, uint32_t k, tc_model* m, float* prep_ms) {
+  SchedCache& sc = g->sc;
+  if (prep_ms) *prep_ms = 0;
+  if (sc.kcl_version == g->sched_version && sc.kcl_k == k && sc.kcl_model == model_id(m))
+    return TC_OK;
+  const uint32_t n = g->n;
+  const auto t0 = std::chrono::steady_clock::now();
+  DevBuf X, Y;
+  TCX_TRY(root_features(c, g, k, X));
+  TCX_TRY(Y.reserve(8 * ((size_t)n + 1)));
+  TCX_TRY(model_predict(c, m, X.as<float>(), n, Y.as<float>()));
+  TCX_TRY(sc.kcl_path.reserve((size_t)n + 1));
+  TCX_TRY(sc.kcl_key.reserve(4 * ((size_t)n + 1)));
+  hipLaunchKernelGGL(k_kcl_plan, dim3(grid_for(n)), dim3(256), 0, c->stream,
+                     g->rowptr.as<uint32_t>(), n, k - 1, Y.as<float>(),
+                     sc.kcl_path.as<uint8_t>(), sc.kcl_key.as<uint32_t>());
+  TCX_HIP(hipGetLastError());
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  sc.kcl_version = g->sched_version;
+  sc.kcl_k = k;
+  sc.kcl_model = model_id(m);
+  if (prep_ms)
+    *prep_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0)
+                   .count();
+  return TC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kcl_root_features(tc_ctx* c, tc_graph* g, uint32_t k, float* X) {
+  TCX_TRY(kcl_check(c, g, k));
+  if (!X) return fail(TC_ERR_ARG, "NULL output");
+  if (g->n == 0) return TC_OK;
+  DevBuf dX;
+  TCX_TRY(root_features(c, g, k, dX));
+  TCX_HIP(hipMemcpyAsync(X, dX.p, 4ull * kKclFeatures * g->n, hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  return TC_OK;
+}
+
+int kcl_measure_costs(tc_ctx* c, tc_graph* g, uint32_t k, int variant, float* cost_us,
+                      uint64_t* count) {
+  TCX_TRY(kcl_check(c, g, k));
+  if (!cost_us) return fail(TC_ERR_ARG, "NULL output");
+  if (variant != KCL_VARIANT_PIVOT && variant != KCL_VARIANT_MERGE)
+    return fail(TC_ERR_ARG, "costs are measured for KCL_VARIANT_PIVOT or KCL_VARIANT_MERGE");
+  const uint32_t n = g->n;
+  if (count) *count = 0;
+  if (n == 0) return TC_OK;
+  DevBuf path, key, cost;
+  TCX_TRY(path.reserve((size_t)n + 1));
+  TCX_TRY(key.reserve(4 * ((size_t)n + 1)));
+  TCX_TRY(cost.reserve(4 * ((size_t)n + 1)));
+  TCX_HIP(hipMemsetAsync(cost.p, 0, 4 * (size_t)n, c->stream));
+  const bool pivot = variant == KCL_VARIANT_PIVOT;
+  hipLaunchKernelGGL(k_fixed_plan, dim3(grid_for(n)), dim3(256), 0, c->stream,
+                     g->rowptr.as<uint32_t>(), n, k - 1, pivot ? kPivotMax : 0xFFFFFFFFu,
+                     (uint8_t)(pivot ? 0 : 1), path.as<uint8_t>(), key.as<uint32_t>());
+  TCX_HIP(hipGetLastError());
+  uint64_t total = 0;
+  if (g->m_or) {
+    const KclPlan plan{path.as<uint8_t>(), key.as<uint32_t>()};
+    TCX_TRY(kcl_run(c, g, k, variant, &plan, cost.as<float>(), &total, nullptr));
+  }
+  TCX_HIP(hipMemcpyAsync(cost_us, cost.p, 4 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  if (count) *count = total;
+  return TC_OK;
+}
+
+int kcl_learned_plan(tc_ctx* c, tc_graph* g, uint32_t k, tc_model* m, uint8_t* path) {
+  TCX_TRY(kcl_check(c, g, k));
+  if (!m || !path) return fail(TC_ERR_ARG, "NULL argument");
+  const uint32_t n = g->n;
+  if (n == 0) return TC_OK;
+  TCX_TRY(cached_plan(c, g, k, m, nullptr));
+  TCX_HIP(hipMemcpyAsync(path, g->sc.kcl_path.p, n, hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  return TC_OK;
+}
+
+int kcl_count_learned(tc_ctx* c, tc_graph* g, uint32_t k, tc_model* m, uint64_t* count,
+                      tc_timing* timing) {
+  if (!count) return fail(TC_ERR_ARG, "NULL argument");
+  if (!m) return kcl_count(c, g, k, KCL_VARIANT_AUTO, count, timing);
+  TCX_TRY(kcl_check(c, g, k));
+  *count = 0;
+  tc_timing tm;
+  memset(&tm, 0, sizeof(tm));
+  const uint32_t n = g->n;
+  if (g->m_or == 0 || n == 0) {
+    if (timing) *timing = tm;
+    return TC_OK;
+  }
+  float prep_ms = 0;
+  TCX_TRY(cached_plan(c, g, k, m, &prep_ms));
+  const KclPlan plan{g->sc.kcl_path.as<uint8_t>(), g->sc.kcl_key.as<uint32_t>()};
+  TCX_TRY(kcl_run(c, g, k, KCL_VARIANT_AUTO, &plan, nullptr, count, &tm));
+  tm.ms_prep = prep_ms;
+  if (timing) *timing = tm;
+  return TC_OK;
+}
+
+int kcl_count(tc_ctx* c, tc_graph* g, uint32_t k, int variant, uint64_t* count,
+              tc_timing* timing) {
+  if (!c || !g || !count) return fail(TC_ERR_ARG, "NULL argument");
+  if (g->ctx != c) return fail(TC_ERR_ARG, "graph belongs to another context");
+  if (g->orient_mode < 0) return fail(TC_ERR_STATE, "graph must be oriented before counting");
+  if (k == 0 || k > (uint32_t)kMaxK) return fail(TC_ERR_RANGE, "k must be in 1..%d", kMaxK);
+  if (variant < KCL_VARIANT_AUTO || variant > KCL_VARIANT_PIVOT)
+    return fail(TC_ERR_ARG, "unknown k-clique variant %d", variant);
+  TCX_HIP(hipSetDevice(c->device));
+  tc_timing tm;
+  memset(&tm, 0, sizeof(tm));
+  *count = 0;
+  if (k == 1) { *count = g->n; if (timing) *timing = tm; return TC_OK; }
+  if (k == 2) { *count = g->m_or; if (timing) *timing = tm; return TC_OK; }
+  if (g->m_or == 0) { if (timing) *timing = tm; return TC_OK; }
+  if (k == 3 && variant == KCL_VARIANT_AUTO) {
+    // 3-cliques are triangles: the AUTO triangle engine counts them directly
+    tc_sched s;
+    memset(&s, 0, sizeof(s));
+    s.kernel = TC_KERNEL_AUTO;
+    return tc_count(c, g, &s, count, timing);
+  }
+  return kcl_run(c, g, k, variant, nullptr, nullptr, count, timing);
 }
 
 int kcl_gpu_solver(const char* path, uint32_t k, uint64_t* total, size_t n_chunk) {

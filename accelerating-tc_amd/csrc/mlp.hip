@@ -545,7 +545,6 @@ int assignment(tc_ctx* c, tc_graph* g, tc_model* md, DevBuf& var, DevBuf& key, D
   hipLaunchKernelGGL(k_bin_costs, dim3(std::min<unsigned>(grid_for(n), 2048)), dim3(256), 0,
                      c->stream, g->rowptr.as<uint32_t>(), n, Y.as<float>(), dsum);
   uint32_t wave_bins = 0;
-  float wave_cap = 1e30f;
   {
     float sums[64];
     TCX_HIP(hipMemcpyAsync(sums, dsum, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
@@ -631,6 +630,19 @@ int learned_schedule(tc_ctx* c, tc_graph* g, tc_model* md) {
 }
 
 }  // namespace
+
+namespace tcx {
+uint64_t model_id(const tc_model* md) { return md ? md->id : 0; }
+
+int model_predict(tc_ctx* c, tc_model* md, const float* dX, uint32_t n, float* dY) {
+  TCX_TRY(check_model(c, md));
+  if (n == 0) return TC_OK;
+  hipLaunchKernelGGL(k_mlp_forward, dim3(std::min<unsigned>(grid_for(n), 8192)), dim3(256), 0,
+                     c->stream, dX, n, md->wbf.as<uint16_t>(), md->nm, dY);
+  TCX_HIP(hipGetLastError());
+  return TC_OK;
+}
+}  // namespace tcx
 
 extern "C" {
 

@@ -149,6 +149,11 @@ _sig("tc_count_learned", C.c_int, [_vp, _vp, _vp, C.POINTER(C.c_uint64), C.POINT
 _sig("kcl_count", C.c_int, [_vp, _vp, C.c_uint32, C.c_int, C.POINTER(C.c_uint64),
                             C.POINTER(Timing)])
 _sig("kcl_gpu_solver", C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_uint64), C.c_size_t])
+_sig("kcl_root_features", C.c_int, [_vp, _vp, C.c_uint32, _f32p])
+_sig("kcl_measure_costs", C.c_int, [_vp, _vp, C.c_uint32, C.c_int, _f32p, C.POINTER(C.c_uint64)])
+_sig("kcl_learned_plan", C.c_int, [_vp, _vp, C.c_uint32, _vp, _u8p])
+_sig("kcl_count_learned", C.c_int, [_vp, _vp, C.c_uint32, _vp, C.POINTER(C.c_uint64),
+                                    C.POINTER(Timing)])
 _sig("tc_tricore_init_gpu", C.c_uint64, [])
 _sig("tc_tricore_cal_degree", None, [_u64p, C.c_uint64, _u32p, C.c_uint32])
 _sig("tc_tricore_redirect_edges", None, [_u64p, C.c_uint64, _u32p, C.c_uint32])
@@ -183,7 +188,9 @@ EXPORTED = [
     "tc_graph_simplify", "tc_graph_relabel_aorder", "tc_graph_relabel", "tc_graph_orient", "tc_graph_info_get",
     "tc_graph_get_csr", "tc_graph_get_edges", "tc_graph_get_vertex_key", "tc_graph_device_csr",
     "tc_hu_workload", "tc_hu_block_starts", "tc_fox_workload", "tc_fox_bin_sizes", "tc_count",
-    "tc_count_per_vertex", "tc_cost_functions", "tc_approx_ratio", "kcl_count", "kcl_gpu_solver", "tc_tricore_init_gpu",
+    "tc_count_per_vertex", "tc_cost_functions", "tc_approx_ratio", "kcl_count", "kcl_gpu_solver",
+    "kcl_root_features", "kcl_measure_costs", "kcl_learned_plan", "kcl_count_learned",
+    "tc_tricore_init_gpu",
     "tc_tricore_cal_degree", "tc_tricore_redirect_edges", "tc_tricore_cal_part_num",
     "tc_tricore_split", "tc_tricore_tricount_gpu", "tc_model_num_params", "tc_model_create",
     "tc_model_destroy", "tc_model_set_params", "tc_model_get_params", "tc_model_set_norm",
@@ -569,6 +576,39 @@ class Graph:
         t = Timing()
         _check(_lib.kcl_count(self.ctx.handle, self._h, k, variant, C.byref(out), C.byref(t)),
                "kcl_count")
+        return CountResult(out.value, t.as_dict())
+
+    def kclique_features(self, k: int):
+        """Per-root features (n x 8) of the learned k-clique plan."""
+        n = self.info().n
+        X = np.zeros((max(n, 1), N_FEATURES), np.float32)
+        _check(_lib.kcl_root_features(self.ctx.handle, self._h, k, X.reshape(-1)),
+               "kcl_root_features")
+        return X[:n]
+
+    def kclique_costs(self, k: int, variant: int):
+        """(CU-microseconds per root on KCL_PIVOT or KCL_MERGE, k-cliques of
+        the measured roots)."""
+        n = self.info().n
+        c = np.zeros(max(n, 1), np.float32)
+        cnt = C.c_uint64()
+        _check(_lib.kcl_measure_costs(self.ctx.handle, self._h, k, variant, c, C.byref(cnt)),
+               "kcl_measure_costs")
+        return c[:n], cnt.value
+
+    def kclique_plan(self, k: int, model):
+        """Path per root: 0 pivot, 1 merge, 255 none."""
+        n = self.info().n
+        out = np.zeros(max(n, 1), np.uint8)
+        _check(_lib.kcl_learned_plan(self.ctx.handle, self._h, k, model.handle, out),
+               "kcl_learned_plan")
+        return out[:n]
+
+    def kclique_learned(self, k: int, model=None) -> CountResult:
+        out = C.c_uint64()
+        t = Timing()
+        _check(_lib.kcl_count_learned(self.ctx.handle, self._h, k, model.handle if model else None,
+                                      C.byref(out), C.byref(t)), "kcl_count_learned")
         return CountResult(out.value, t.as_dict())
 
 

@@ -238,6 +238,10 @@ def main_kclique(args, w):
     if k3 != tri:
         raise SystemExit(f"3-cliques {k3} != triangles {tri}")
     ks = tuple(int(x) for x in args.ks.split(",") if x) or w["ks"]
+    kmodel = None
+    from tcamd import kclique_learned as kl
+    if os.path.exists(kl.DEFAULT_MODEL):
+        kmodel = kl.load(ctx)
     per_k = {}
     for k in ks:
         t0 = time.perf_counter()
@@ -254,6 +258,20 @@ def main_kclique(args, w):
         per_k[k] = dict(count=r.count, ms_wall=wall * 1e3, ms_kernels=float(np.mean(ms)),
                         edges_per_s=m / wall)
         log(f"[bench] k={k}: {r.count} cliques, {wall*1e3:.2f} ms ({m / wall:.3e} edges/s)")
+        if kmodel is not None:
+            rl = g.kclique_learned(k, kmodel)  # builds and keeps the plan
+            prep = rl.timing["ms_prep"]
+            lm = []
+            for _ in range(args.steps):
+                rl = g.kclique_learned(k, kmodel)
+                if rl.count != ref:
+                    raise SystemExit(f"k={k}: learned plan count {rl.count} != {ref}")
+                lm.append(rl.timing["ms_count"])
+            merge = int((g.kclique_plan(k, kmodel) == 1).sum())
+            per_k[k]["learned"] = dict(ms_kernels=float(np.mean(lm)), ms_plan=prep,
+                                       merge_roots=merge)
+            log(f"[bench] k={k}: learned plan {np.mean(lm):.2f} ms (plan {prep:.1f} ms once, "
+                f"{merge} roots on the merge path)")
     cpu = None
     if not args.no_cpu:
         import oracle as orc

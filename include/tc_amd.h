@@ -345,6 +345,38 @@ int kcl_count(tc_ctx* ctx, tc_graph* g, uint32_t k, int variant, uint64_t* count
  * every level; chunks are not needed). */
 int kcl_gpu_solver(const char* path, uint32_t k, uint64_t* total, size_t n_chunk);
 
+/* ---- learned k-clique plan --------------------------------------------
+ * No counterpart in the reference (k-clique/kcl.cu runs one path for every
+ * root).  The cost model of tc_model_* (8 features -> 2 log2 costs) is
+ * trained on per-root costs measured by the kernels themselves and picks,
+ * per root, the LDS bit-matrix pivot path or the merge path.  The count is
+ * exact whatever the model predicts. */
+/* Per-root features X[n][8] for k (zero for roots with d+ < k-1):
+ * log2(1+d+), log2(1+m) with m the edges of the sub-graph of N+(u),
+ * density m / C(d+,2), k, log2(1+sum_v d+(v)), log2(C(d+,k-1) p^C(k-1,2))
+ * (the (k-1)-cliques of a random graph of density p), the bit-matrix width
+ * class (0: d+ <= 64, 1: <= 128, 2: <= 256, 3: <= 512, 4: <= 1024, 5: above),
+ * log2(1+max_v d+(v)). */
+int kcl_root_features(tc_ctx* ctx, tc_graph* g, uint32_t k, float* X);
+/* Measured cost of every root on one path, in CU-microseconds (the root's
+ * time times the share of a CU its workgroup holds); 0 where not measured.
+ * KCL_VARIANT_PIVOT measures the roots with k-1 <= d+ <= 1024,
+ * KCL_VARIANT_MERGE every root with d+ >= k-1; count (optional) receives
+ * the k-cliques of the measured roots. */
+int kcl_measure_costs(tc_ctx* ctx, tc_graph* g, uint32_t k, int variant, float* cost_us,
+                      uint64_t* count);
+/* Path per root (0 pivot, 1 merge, 255 none: d+ < k-1).  The fixed rule
+ * (pivot iff d+ <= 1024) is the prior: a root moves to the merge path only
+ * for a predicted saving above 25 %. */
+int kcl_learned_plan(tc_ctx* ctx, tc_graph* g, uint32_t k, tc_model* m, uint8_t* path);
+/* Exact k-clique count (3 <= k <= 12) with the learned plan; each launch
+ * takes its roots widest first, as kcl_count.  The plan is built once per
+ * (graph, k, model parameters) and kept with the graph.  m NULL runs
+ * kcl_count(KCL_VARIANT_AUTO).  timing->ms_prep holds the plan's build time
+ * (0 when it was kept). */
+int kcl_count_learned(tc_ctx* ctx, tc_graph* g, uint32_t k, tc_model* m, uint64_t* count,
+                      tc_timing* timing);
+
 /* ---- TriCore function API (tricore/include/tricount.h:7-17) ----------- */
 /* Same signatures and meaning as the reference; edges are the uint64 view of
  * the record, FIRST = high word (tricore/include/util.h:14-16).  Failures are
