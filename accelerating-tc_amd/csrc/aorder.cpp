@@ -3,8 +3,10 @@
 // Follows vertex_ordering/A-order/A-order.cpp:100-210.  The GPU computes the
 // oriented out-degree (runTimeDegree, :86-99); this file runs the greedy
 // bucket fill, which is a strictly sequential heap walk, and returns the
-// permutation old id -> new id.  The bandwidth model and the heap steps are
-// kept exactly as in the reference because they define the permutation.
+// permutation old id -> new id.  The heap steps are kept exactly as in the
+// reference because they define the permutation; the bandwidth model is a
+// parameter whose default is the reference's (bit for bit), and which can be
+// replaced by one measured on this GPU (tc_measure_list_bandwidth).
 #include <algorithm>
 #include <cmath>
 
@@ -14,17 +16,15 @@ namespace {
 
 constexpr int kBucket = 40;  // A-order.cpp:14
 
-// Bandwidth lookup model, A-order.cpp:20-36.
-const float kBw[10] = {0, 0, 271, 310, 399, 419, 458, 478, 524, 539};
-const float kLambda[10] = {0, 0, 145.51, 121.98, 53.26, 23.92, 16, 14, 12.94, 10.24};
-
-float bw_of(int len) {
-  if (len < 10) return kBw[len];
-  return 384.1 * std::pow(len, 0.1491);
+// The model's arithmetic follows A-order.cpp:23-36: the power law in double,
+// rounded to float; the equal-bandwidth term in float.
+float bw_of(const tc_bw_model& m, int len) {
+  if (len < 10) return m.table[len];
+  return (float)(m.a * std::pow((double)len, m.b));
 }
-float equal_bw_of(int len) {
-  if (len < 10) return kLambda[len] * std::sqrt(bw_of(len)) / std::sqrt((float)len);
-  return 12 * std::sqrt(bw_of(len)) / std::sqrt((float)len);
+float equal_bw_of(const tc_bw_model& m, int len) {
+  if (len < 10) return m.lambda[len] * std::sqrt(bw_of(m, len)) / std::sqrt((float)len);
+  return m.lambda_tail * std::sqrt(bw_of(m, len)) / std::sqrt((float)len);
 }
 
 struct Slot {
@@ -50,8 +50,22 @@ struct ByIdDesc {  // A-order.cpp:56-60
 
 namespace tcx {
 
+// The reference's bandwidth lookup model, A-order.cpp:20-36.
+void bw_model_reference(tc_bw_model* m) {
+  const float bw[10] = {0, 0, 271, 310, 399, 419, 458, 478, 524, 539};
+  const float lambda[10] = {0, 0, 145.51f, 121.98f, 53.26f, 23.92f, 16, 14, 12.94f, 10.24f};
+  for (int i = 0; i < 10; ++i) {
+    m->table[i] = bw[i];
+    m->lambda[i] = lambda[i];
+  }
+  m->a = 384.1;
+  m->b = 0.1491;
+  m->lambda_tail = 12;
+}
+
 // rtd: oriented out-degree per vertex (n entries).  perm: old -> new.
-int aorder_perm_from_outdeg(const uint32_t* rtd, uint32_t n, uint32_t* perm) {
+int aorder_perm_from_outdeg(const uint32_t* rtd, uint32_t n, uint32_t* perm,
+                            const tc_bw_model& model) {
   uint32_t maxd = 0;
   for (uint32_t v = 0; v < n; ++v) maxd = std::max(maxd, rtd[v]);
   // vertices grouped by out-degree, ascending id inside a group
@@ -78,8 +92,8 @@ int aorder_perm_from_outdeg(const uint32_t* rtd, uint32_t n, uint32_t* perm) {
 
   std::vector<float> bw(maxd + 1), ebw(maxd + 1);
   for (uint32_t d = 0; d <= maxd; ++d) {
-    bw[d] = bw_of((int)d);
-    ebw[d] = equal_bw_of((int)d);
+    bw[d] = bw_of(model, (int)d);
+    ebw[d] = equal_bw_of(model, (int)d);
   }
   std::vector<Slot> heap(buckets);
   std::vector<std::vector<uint32_t>> members(buckets);

@@ -667,6 +667,14 @@ int tc_graph_simplify(tc_ctx* c, tc_graph* g) {
 }
 
 int tc_graph_relabel_aorder(tc_ctx* c, tc_graph* g, uint32_t* perm_out) {
+  tc_bw_model ref;
+  bw_model_reference(&ref);
+  return tc_graph_relabel_aorder_model(c, g, &ref, perm_out);
+}
+
+int tc_graph_relabel_aorder_model(tc_ctx* c, tc_graph* g, const tc_bw_model* model,
+                                  uint32_t* perm_out) {
+  if (!model) return fail(TC_ERR_ARG, "NULL bandwidth model");
   TCX_TRY(check_graph(g));
   if (c != g->ctx) return fail(TC_ERR_ARG, "graph belongs to another context");
   uint32_t n = g->n;
@@ -688,7 +696,7 @@ int tc_graph_relabel_aorder(tc_ctx* c, tc_graph* g, uint32_t* perm_out) {
   std::vector<uint32_t> h_rtd(n), h_perm(n);
   TCX_HIP(hipMemcpyAsync(h_rtd.data(), rtd.p, 4 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
   TCX_HIP(hipStreamSynchronize(c->stream));
-  TCX_TRY(aorder_perm_from_outdeg(h_rtd.data(), n, h_perm.data()));
+  TCX_TRY(aorder_perm_from_outdeg(h_rtd.data(), n, h_perm.data(), *model));
   TCX_HIP(hipMemcpyAsync(perm.p, h_perm.data(), 4 * (size_t)n, hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_aorder_apply, dim3(grid_for(g->m_raw)), dim3(TB), 0, c->stream,
                      g->src.as<uint32_t>(), g->dst.as<uint32_t>(), g->m_raw, deg.as<uint32_t>(),

@@ -155,7 +155,9 @@ uint64_t model_id(const tc_model* m);
 int per_vertex_counts(tc_ctx* c, tc_graph* g, unsigned long long* tri);
 
 // ---- A-order (aorder.cpp, host heap) -------------------------------------
-int aorder_perm_from_outdeg(const uint32_t* rtd, uint32_t n, uint32_t* perm);
+int aorder_perm_from_outdeg(const uint32_t* rtd, uint32_t n, uint32_t* perm,
+                            const tc_bw_model& model);
+void bw_model_reference(tc_bw_model* m);
 
 // ---- host parallel-for ----------------------------------------------------
 void parallel_for(uint64_t n, void (*fn)(uint64_t begin, uint64_t end, void* arg),

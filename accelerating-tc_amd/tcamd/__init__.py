@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import atexit
 import ctypes as C
+import json
 import os
 import weakref
 from dataclasses import dataclass
@@ -106,6 +107,31 @@ _sig("tc_graph_create", C.c_int, [_vp, _vp, _vp, C.c_uint64, C.c_uint32, C.POINT
 _sig("tc_graph_destroy", C.c_int, [_vp])
 _sig("tc_graph_simplify", C.c_int, [_vp, _vp])
 _sig("tc_graph_relabel_aorder", C.c_int, [_vp, _vp, _vp])
+
+
+class BwModel(C.Structure):
+    """A-order's list-read bandwidth model (tc_bw_model)."""
+    _fields_ = [("table", C.c_float * 10), ("a", C.c_double), ("b", C.c_double),
+                ("lambda_", C.c_float * 10), ("lambda_tail", C.c_float)]
+
+    def as_dict(self):
+        return {"table": list(self.table), "a": self.a, "b": self.b,
+                "lambda": list(self.lambda_), "lambda_tail": self.lambda_tail}
+
+    @classmethod
+    def from_dict(cls, d):
+        m = cls()
+        m.table[:] = [float(x) for x in d["table"]]
+        m.a, m.b = float(d["a"]), float(d["b"])
+        m.lambda_[:] = [float(x) for x in d["lambda"]]
+        m.lambda_tail = float(d["lambda_tail"])
+        return m
+
+
+_sig("tc_graph_relabel_aorder_model", C.c_int, [_vp, _vp, C.POINTER(BwModel), _vp])
+_sig("tc_bw_model_reference", C.c_int, [C.POINTER(BwModel)])
+_sig("tc_measure_list_bandwidth", C.c_int, [_vp, _vp, C.c_uint32, _vp])
+_sig("tc_bw_model_fit", C.c_int, [_vp, _vp, C.c_uint32, C.POINTER(BwModel)])
 _sig("tc_graph_orient", C.c_int, [_vp, _vp, C.c_int, C.c_uint32])
 _sig("tc_graph_relabel", C.c_int, [_vp, _vp, C.c_int, _vp])
 _sig("tc_graph_info_get", C.c_int, [_vp, C.POINTER(GraphInfo)])
@@ -186,6 +212,8 @@ EXPORTED = [
     "tc_file_edge_count", "tc_read_edges", "tc_write_edges", "tc_list2bin", "tc_bin2list", "tc_read_gr",
     "tc_gen_er", "tc_gen_rmat", "tc_gen_ba", "tc_gen_rgg", "tc_graph_create", "tc_graph_destroy",
     "tc_graph_simplify", "tc_graph_relabel_aorder", "tc_graph_relabel", "tc_graph_orient", "tc_graph_info_get",
+    "tc_graph_relabel_aorder_model", "tc_bw_model_reference", "tc_measure_list_bandwidth",
+    "tc_bw_model_fit",
     "tc_graph_get_csr", "tc_graph_get_edges", "tc_graph_get_vertex_key", "tc_graph_device_csr",
     "tc_hu_workload", "tc_hu_block_starts", "tc_fox_workload", "tc_fox_bin_sizes", "tc_count",
     "tc_count_per_vertex", "tc_cost_functions", "tc_approx_ratio", "kcl_count", "kcl_gpu_solver",
@@ -324,6 +352,38 @@ def gen_rgg(n: int, r: float, seed: int):
     return s[:m.value], d[:m.value]
 
 
+def bw_model_reference() -> BwModel:
+    m = BwModel()
+    _check(_lib.tc_bw_model_reference(C.byref(m)), "tc_bw_model_reference")
+    return m
+
+
+def measure_list_bandwidth(ctx: "Context", lens) -> np.ndarray:
+    """GB/s of list entries read, one wave per list of each length."""
+    lens = np.ascontiguousarray(lens, np.uint32)
+    out = np.zeros(max(len(lens), 1), np.float32)
+    _check(_lib.tc_measure_list_bandwidth(ctx.handle, lens.ctypes.data, len(lens),
+                                          out.ctypes.data), "tc_measure_list_bandwidth")
+    return out[:len(lens)]
+
+
+def fit_bw_model(lens, gbps) -> BwModel:
+    lens = np.ascontiguousarray(lens, np.uint32)
+    gbps = np.ascontiguousarray(gbps, np.float32)
+    m = BwModel()
+    _check(_lib.tc_bw_model_fit(lens.ctypes.data, gbps.ctypes.data, len(lens), C.byref(m)),
+           "tc_bw_model_fit")
+    return m
+
+
+BW_MODEL_MI355X = os.path.join(_HERE, "aorder_bw_mi355x.json")
+
+
+def load_bw_model(path: str = BW_MODEL_MI355X) -> BwModel:
+    with open(path) as f:
+        return BwModel.from_dict(json.load(f)["model"])
+
+
 def rgg_radius(n: int, avg_degree: float) -> float:
     """Radius giving the target average degree in the unit square."""
     return float(np.sqrt(avg_degree / (np.pi * (n - 1))))
@@ -444,11 +504,17 @@ class Graph:
                "tc_graph_relabel")
         return perm
 
-    def relabel_aorder(self):
+    def relabel_aorder(self, model: "BwModel | None" = None):
+        """A-order; model (optional) replaces the reference's bandwidth model."""
         n = self.info().n
         perm = np.empty(max(n, 1), np.uint32)
-        _check(_lib.tc_graph_relabel_aorder(self.ctx.handle, self._h, perm.ctypes.data),
-               "tc_graph_relabel_aorder")
+        if model is None:
+            _check(_lib.tc_graph_relabel_aorder(self.ctx.handle, self._h, perm.ctypes.data),
+                   "tc_graph_relabel_aorder")
+        else:
+            _check(_lib.tc_graph_relabel_aorder_model(self.ctx.handle, self._h, C.byref(model),
+                                                      perm.ctypes.data),
+                   "tc_graph_relabel_aorder_model")
         return perm[:n]
 
     def orient(self, mode: int, prune_deg1: bool = False):

@@ -170,6 +170,35 @@ int tc_graph_simplify(tc_ctx* ctx, tc_graph* g);
  * the records are rewritten with new ids.  perm (optional, n entries) gets
  * old->new.  TC_ERR_UNSUPPORTED outside the reference's defined range. */
 int tc_graph_relabel_aorder(tc_ctx* ctx, tc_graph* g, uint32_t* perm);
+/* Bandwidth model behind A-order's bucket scores (A-order.cpp:20-36): the
+ * bandwidth of reading a list of len entries is table[len] for len < 10 and
+ * a * len^b (computed in double, rounded to float) from 10 on; the
+ * "equal bandwidth" term is lambda[len] * sqrt(bw) / sqrt(len) for len < 10
+ * and lambda_tail * sqrt(bw) / sqrt(len) from 10 on (float). */
+typedef struct tc_bw_model {
+  float table[10];
+  double a, b;
+  float lambda[10];
+  float lambda_tail;
+} tc_bw_model;
+/* The reference's constants (table {.., 271, 310, ..}, 384.1 len^0.1491,
+ * lambda {.., 145.51, ..}, 12), measured on the GPU the reference targets. */
+int tc_bw_model_reference(tc_bw_model* out);
+/* Achieved bandwidth (GB/s of list entries) of reading lists of each of the
+ * count lengths lens[i] at random 4-byte offsets of a 1 GiB array, one wave
+ * per list, the whole GPU busy (the access the counting kernels make to
+ * N+(v)).  No counterpart in the reference, which ships measured numbers. */
+int tc_measure_list_bandwidth(tc_ctx* ctx, const uint32_t* lens, uint32_t count, float* gbps);
+/* A model fitted to such measurements: table[len] = the measured value for
+ * 2 <= len <= 9 (all of them must be present), (a, b) the least-squares fit
+ * of log gbps on log len over the lengths >= 10 (at least two), lambda and
+ * lambda_tail the reference's (the reference gives no way to re-derive
+ * them). */
+int tc_bw_model_fit(const uint32_t* lens, const float* gbps, uint32_t count, tc_bw_model* out);
+/* A-order with another bandwidth model; tc_graph_relabel_aorder uses the
+ * reference's. */
+int tc_graph_relabel_aorder_model(tc_ctx* ctx, tc_graph* g, const tc_bw_model* model,
+                                  uint32_t* perm);
 /* Id relabellings of the reference's format tools, applied to the raw
  * records on the GPU (records keep their order).  perm (optional) gets
  * old -> new for ids 0 .. max id (max id + 1 entries, max over the records

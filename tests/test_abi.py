@@ -176,3 +176,27 @@ def test_tool_drivers_roundtrip(tmp_path):
     subprocess.run([os.path.join(exe, "bin2list"), str(tmp_path / "g.bin"), str(tmp_path / "h.txt")],
                    check=True, capture_output=True)
     assert (tmp_path / "h.txt").read_text().split() == txt.read_text().split()
+
+
+def test_bw_model_reference_constants():
+    # vertex_ordering/A-order/A-order.cpp:20-36
+    m = tcamd.bw_model_reference()
+    assert list(m.table) == pytest.approx([0, 0, 271, 310, 399, 419, 458, 478, 524, 539])
+    assert list(m.lambda_) == pytest.approx([0, 0, 145.51, 121.98, 53.26, 23.92, 16, 14, 12.94,
+                                             10.24], rel=1e-6)
+    assert (m.a, m.b, m.lambda_tail) == (384.1, 0.1491, 12.0)
+
+
+def test_bw_model_fit_recovers_a_power_law():
+    lens = list(range(2, 10)) + [10, 16, 32, 64, 128, 512, 1024]
+    table = [100 + 10 * l for l in range(2, 10)]
+    gbps = table + [500 * l ** 0.2 for l in lens[8:]]
+    m = tcamd.fit_bw_model(lens, gbps)
+    assert m.a == pytest.approx(500, rel=1e-4) and m.b == pytest.approx(0.2, rel=1e-4)
+    assert list(m.table)[2:] == pytest.approx(table)
+    ref = tcamd.bw_model_reference()
+    assert list(m.lambda_) == list(ref.lambda_) and m.lambda_tail == ref.lambda_tail
+    with pytest.raises(tcamd.TCError):
+        tcamd.fit_bw_model(lens[1:], gbps[1:])        # length 2 missing
+    with pytest.raises(tcamd.TCError):
+        tcamd.fit_bw_model(lens[:9], gbps[:9])        # one length >= 10 only

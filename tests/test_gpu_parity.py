@@ -379,3 +379,33 @@ def test_relabel_tools_match_oracle(ctx, case):
     g = ctx.graph(e, e)
     assert g.relabel(tcamd.RELABEL_DEGREE).tolist() == [0]
     g.close()
+
+
+def test_aorder_bandwidth_model(ctx):
+    # the reference model through the model entry point gives the same
+    # permutation; the measured MI355X model gives a bijection and the count
+    s, d = rmat(12, 16, 5)
+    g1 = ctx.graph(s, d)
+    g1.simplify()
+    p_default = g1.relabel_aorder()
+    g2 = ctx.graph(s, d)
+    g2.simplify()
+    p_ref = g2.relabel_aorder(tcamd.bw_model_reference())
+    np.testing.assert_array_equal(p_default, p_ref)
+    g3 = ctx.graph(s, d)
+    g3.simplify()
+    p_gpu = g3.relabel_aorder(tcamd.load_bw_model())
+    assert np.array_equal(np.sort(p_gpu), np.arange(len(p_gpu)))
+    expect = orc.pipeline_count(s, d, "degree")[0]
+    for g in (g1, g2, g3):
+        assert g.orient(tcamd.ORIENT_ADIR).count(tcamd.KERNEL_AUTO).count == expect
+        g.close()
+
+
+def test_list_bandwidth_measurement(ctx):
+    lens = [2, 9, 64, 512]
+    bw = tcamd.measure_list_bandwidth(ctx, lens)
+    assert np.all(bw > 0)
+    assert bw[2] > bw[0]  # a wave reading 64 entries moves more than one reading 2
+    with pytest.raises(tcamd.TCError):
+        tcamd.measure_list_bandwidth(ctx, [0])
