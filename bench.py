@@ -87,7 +87,7 @@ def cpu_baseline(rp, col, target_s=15.0):
     est_full = max(t1, 1e-4) * stride
     stride2 = max(1, int(round(est_full / target_s)))
     t0 = time.perf_counter()
-    orc.count_rows_strided(rp, col, 0, stride2)
+    c1 = orc.count_rows_strided(rp, col, 0, stride2)
     t = time.perf_counter() - t0
     edges = int(deg[0::stride2].sum())
     return {
@@ -97,7 +97,38 @@ def cpu_baseline(rp, col, target_s=15.0):
         "kind": "port",
         "sample": (f"serial forward merge counter (oracle) over every {stride2}-th row of the "
                    f"same oriented CSR: {edges} of {int(deg.sum())} edges, {t:.1f} s"),
-    }
+        "host": host_info(),
+    }, (stride2, edges, c1)
+
+
+def cpu_baseline_parallel(rp, col, sample):
+    """The same counter and sample with OpenMP over the host's cores (the
+    optional second CPU column; labelled parallel)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    stride, edges, c1 = sample
+    threads = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    c = orc.count_rows_strided_omp(rp, col, 0, stride, threads)
+    t = time.perf_counter() - t0
+    if c != c1:
+        raise SystemExit(f"parallel oracle sample {c} != serial {c1}")
+    return {"value": edges / t, "unit": "edges/s", "cores": threads, "kind": "port-parallel",
+            "sample": f"the serial sample above with OpenMP over {threads} threads, {t:.2f} s"}
+
+
+def host_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
 
 
 def cpu_baseline_batch(graphs, target_s=15.0):
@@ -421,11 +452,13 @@ def main():
         with open(prof) as f:
             traffic = json.load(f).get(head)
 
-    cpu = None
+    cpu = cpu_par = None
     if not args.no_cpu:
         rp, col = g.csr()
-        cpu = cpu_baseline(rp, col)
+        cpu, sample = cpu_baseline(rp, col)
         log(f"[bench] cpu baseline {cpu['value']:.3e} edges/s ({cpu['sample']})")
+        cpu_par = cpu_baseline_parallel(rp, col, sample)
+        log(f"[bench] cpu baseline, parallel: {cpu_par['value']:.3e} edges/s ({cpu_par['sample']})")
 
     line = {
         "metric": "TC edges/s + HBM GB/s % peak; learned/batched vs plain HIP, exact counts",
@@ -452,6 +485,7 @@ def main():
             "bytes_alg_per_count": info.bytes_alg,
         },
         "cpu_baseline": cpu,
+        "cpu_baseline_parallel": cpu_par,
         "kernels": {k: {kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in r.items()}
                     for k, r in results.items()},
         "end_to_end_s": {"generate": t_gen, "preprocess": t_prep},

@@ -553,6 +553,17 @@ uint64_t orc_count_rows_strided(uint32_t n, const uint32_t* rowptr, const uint32
   return total;
 }
 
+uint64_t orc_count_rows_strided_omp(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
+                                    uint32_t start, uint32_t stride, int threads) {
+  uint64_t total = 0;
+  if (stride == 0) stride = 1;
+  const long long rows = start < n ? ((long long)n - start + stride - 1) / stride : 0;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 64) reduction(+ : total)
+  for (long long i = 0; i < rows; ++i)
+    total += vertex_tri(rowptr, col, (uint32_t)(start + (uint64_t)i * stride));
+  return total;
+}
+
 void orc_count_per_vertex(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
                           uint64_t* tri) {
   for (uint32_t u = 0; u < n; ++u) tri[u] = vertex_tri(rowptr, col, u);

@@ -128,6 +128,11 @@ uint64_t orc_count_rows(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
 uint64_t orc_count_rows_strided(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
                                 uint32_t start, uint32_t stride);
 
+/* orc_count_rows_strided with OpenMP over the sampled rows, `threads`
+ * threads (labelled as parallel wherever timed). */
+uint64_t orc_count_rows_strided_omp(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
+                                    uint32_t start, uint32_t stride, int threads);
+
 /* Per-vertex triangle counts of the forward counter: tri[u] = sum over
  * v in N+(u) of |N+(u) & N+(v)|. */
 void orc_count_per_vertex(uint32_t n, const uint32_t* rowptr,

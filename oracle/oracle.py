@@ -55,6 +55,8 @@ _sig("orc_count", C.c_uint64, [C.c_uint32, _u32p, _u32p])
 _sig("orc_count_omp", C.c_uint64, [C.c_uint32, _u32p, _u32p])
 _sig("orc_count_rows", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32])
 _sig("orc_count_rows_strided", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32])
+_sig("orc_count_rows_strided_omp", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32,
+                                                C.c_int])
 _sig("orc_count_per_vertex", None, [C.c_uint32, _u32p, _u32p, _u64p])
 _sig("orc_kclique", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32])
 _sig("orc_kclique_rows_strided", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32,
@@ -197,6 +199,12 @@ def count_rows_strided(rp, col, start, stride) -> int:
     rp = _u32(rp)
     return int(_lib.orc_count_rows_strided(len(rp) - 1, rp, _nz(_u32(col), np.uint32), start,
                                            stride))
+
+
+def count_rows_strided_omp(rp, col, start, stride, threads) -> int:
+    rp = _u32(rp)
+    return int(_lib.orc_count_rows_strided_omp(len(rp) - 1, rp, _nz(_u32(col), np.uint32), start,
+                                               stride, threads))
 
 
 def count_per_vertex(rp, col):

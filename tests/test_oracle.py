@@ -494,3 +494,13 @@ def test_relabel_oracle_properties():
     deg = np.bincount(np.concatenate([s, d]), minlength=n)
     perm = orc.relabel(s, d, "degree")
     assert perm[np.argmax(deg)] == 0 or deg[perm == 0][0] == deg.max()
+
+
+def test_parallel_sample_counter_equals_serial():
+    s, d = rmat(11, 16, 9)
+    c, rp, col = orc.pipeline_count(s, d, "degree")
+    n = len(rp) - 1
+    assert orc.count_rows_strided_omp(rp, col, 0, 1, 4) == c
+    for start, stride in ((0, 3), (5, 7), (n - 1, 2), (n + 3, 1)):
+        assert (orc.count_rows_strided_omp(rp, col, start, stride, 3)
+                == orc.count_rows_strided(rp, col, start, stride))
