@@ -51,7 +51,7 @@ void parallel_for(uint64_t n, void (*fn)(uint64_t, uint64_t, void*), void* arg,
                   unsigned max_threads) {
   unsigned hw = std::thread::hardware_concurrency();
   if (hw == 0) hw = 1;
-  if (hw > 16) hw = 16;  // the GPU box grants 16 host cores per GPU
+  if (hw > 16) hw = 16;  // at most 16 host threads per GPU process
   if (max_threads && hw > max_threads) hw = max_threads;
   if (n < 4096 || hw == 1) {
     fn(0, n, arg);

@@ -6,7 +6,7 @@
 2. train(): the same 8-32-32-2 MLP as the triangle scheduler, trained by the
    engine's own forward / backward / Adam kernels (tc_model_train), predicts
    log2 of both costs;
-3. judge(): its error next to an analytic estimate (least-squares fit of the
+3. evaluate(): its error next to an analytic estimate (least-squares fit of the
    log2 cost on log2 d+ and on the log2 clique estimate of a random graph of
    the root's density), on held-out roots;
 4. Graph.kclique_learned(k, model) counts with the learned plan: a path per
@@ -106,7 +106,7 @@ def train(ctx, X, T, M, steps=3000, seed=0):
     return m, loss
 
 
-def judge(model, X, T, M):
+def evaluate(model, X, T, M):
     """Mean absolute error (log2 CU-us) of the model and of the analytic
     estimate (least-squares fit on [1, log2(1+d+), clique estimate]) on the
     same roots."""

@@ -5,7 +5,7 @@
    vertex (tc_graph_measure_costs, tc_graph_features);
 2. train(): the MLP is trained on the GPU by the engine's own forward /
    backward / Adam kernels (tc_model_train);
-3. judge(): the model's prediction error is compared with the analytic
+3. evaluate(): the model's prediction error is compared with the analytic
    A-direction work estimate log2(1 + sum_{v in N+(u)} d+(v)) fitted to the
    same data - the fallback the engine uses without a model;
 4. Graph.count_learned(model) counts with the learned schedule.
@@ -120,7 +120,7 @@ def train(ctx, X, T, M, steps=3000, seed=0):
     return m, loss
 
 
-def judge(model, X, T, M):
+def evaluate(model, X, T, M):
     """Mean absolute error (log2 us) of the model and of the analytic
     A-direction work estimate (least-squares fit of log2 cost on
     log2(1 + sum d+(v)), per variant) on the same samples."""

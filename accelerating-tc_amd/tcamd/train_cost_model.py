@@ -41,8 +41,8 @@ def main():
         model, loss = learned.train(ctx, X[tr], T[tr], M[tr], steps=args.steps)
         print(f"[train] {args.steps} Adam steps in {time.time() - t1:.1f}s, loss {loss:.4f}",
               flush=True)
-        j_tr = learned.judge(model, X[tr], T[tr], M[tr])
-        j_te = learned.judge(model, X[te], T[te], M[te])
+        j_tr = learned.evaluate(model, X[tr], T[tr], M[tr])
+        j_te = learned.evaluate(model, X[te], T[te], M[te])
         print("[train] held-out error (log2 us):", json.dumps(j_te), flush=True)
         learned.save(model, args.out, meta={"train": j_tr, "heldout": j_te, "samples": len(X),
                                             "steps": args.steps, "loss": loss})

@@ -39,8 +39,8 @@ def main():
         model, loss = kl.train(ctx, X[tr], T[tr], M[tr], steps=args.steps)
         print(f"[train] {args.steps} Adam steps in {time.time() - t1:.1f}s, loss {loss:.4f}",
               flush=True)
-        j_tr = kl.judge(model, X[tr], T[tr], M[tr])
-        j_te = kl.judge(model, X[te], T[te], M[te])
+        j_tr = kl.evaluate(model, X[tr], T[tr], M[tr])
+        j_te = kl.evaluate(model, X[te], T[te], M[te])
         print("[train] held-out error (log2 CU-us):", json.dumps(j_te), flush=True)
         kl.save(model, args.out, meta={"train": j_tr, "heldout": j_te, "roots": len(X),
                                        "steps": args.steps, "loss": loss})

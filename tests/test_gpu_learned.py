@@ -100,13 +100,13 @@ def test_learned_schedules_are_exact(ctx, make):
 
 
 def test_learned_pipeline_small(ctx):
-    """collect -> train -> judge -> count on a held-out graph (small sizes)."""
+    """collect -> train -> evaluate -> count on a held-out graph (small sizes)."""
     graphs = [("rmat", dict(scale=14, ef=16, abc=(0.57, 0.19, 0.19), seed=11)),
               ("ba", dict(n=1 << 14, k=8, seed=12))]
     X, T, M = learned.collect(ctx, graphs, per_graph=5000, log=lambda *a: None)
     model, loss = learned.train(ctx, X, T, M, steps=800)
     assert np.isfinite(loss)
-    j = learned.judge(model, X, T, M)
+    j = learned.evaluate(model, X, T, M)
     assert j["block"]["mae_model"] <= j["block"]["mae_analytic"] * 1.05
     s, d = rmat(15, 16, 99)
     expect = orc.pipeline_count(s, d, "degree")[0]
