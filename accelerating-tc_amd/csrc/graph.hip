@@ -675,6 +675,14 @@ int tc_graph_relabel_aorder(tc_ctx* c, tc_graph* g, uint32_t* perm_out) {
 int tc_graph_relabel_aorder_model(tc_ctx* c, tc_graph* g, const tc_bw_model* model,
                                   uint32_t* perm_out) {
   if (!model) return fail(TC_ERR_ARG, "NULL bandwidth model");
+  // the bucket heaps need a strict weak order: finite, positive bandwidths
+  for (int l = 2; l < 10; ++l)
+    if (!(std::isfinite(model->table[l]) && model->table[l] > 0 &&
+          std::isfinite(model->lambda[l]) && model->lambda[l] >= 0))
+      return fail(TC_ERR_ARG, "bandwidth model entry %d is not finite and positive", l);
+  if (!(std::isfinite(model->a) && model->a > 0 && std::isfinite(model->b) &&
+        std::isfinite(model->lambda_tail) && model->lambda_tail >= 0))
+    return fail(TC_ERR_ARG, "bandwidth model power law is not finite and positive");
   TCX_TRY(check_graph(g));
   if (c != g->ctx) return fail(TC_ERR_ARG, "graph belongs to another context");
   uint32_t n = g->n;

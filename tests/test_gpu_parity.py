@@ -400,6 +400,18 @@ def test_aorder_bandwidth_model(ctx):
     for g in (g1, g2, g3):
         assert g.orient(tcamd.ORIENT_ADIR).count(tcamd.KERNEL_AUTO).count == expect
         g.close()
+    # models the bucket heaps cannot order are rejected
+    g4 = ctx.graph(s, d)
+    for field, value in (("a", float("nan")), ("a", -1.0), ("b", float("inf"))):
+        bad = tcamd.bw_model_reference()
+        setattr(bad, field, value)
+        with pytest.raises(tcamd.TCError):
+            g4.relabel_aorder(bad)
+    bad = tcamd.bw_model_reference()
+    bad.table[3] = 0.0
+    with pytest.raises(tcamd.TCError):
+        g4.relabel_aorder(bad)
+    g4.close()
 
 
 def test_list_bandwidth_measurement(ctx):
