@@ -49,7 +49,7 @@ def test_method_drivers(graph_file):
     assert triangles(run("tc_fox", "-f", p, 1)) == t
     assert triangles(run("tric_gpu", "-f", p)) == t
     for k in ("auto", "hu", "tricore"):
-        for o in ("adir", "kcore", "degree"):
+        for o in ("adir", "kcore", "degree", "degeneracy", "id", "tricore"):
             assert triangles(run("tc", "-f", p, "--kernel", k, "--orient", o)) == t
     assert triangles(run("tc", "-f", p, "--aorder", "--simplify")) == t
 
