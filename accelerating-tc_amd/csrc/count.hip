@@ -859,9 +859,14 @@ int tc_count(tc_ctx* c, tc_graph* g, const tc_sched* sched_in, uint64_t* count,
       const uint32_t G = s.grid_size ? s.grid_size : (uint32_t)c->num_cus * 8u;
       if (sc.n_large || sc.n_small) {
         TCX_TRY(lm.mark());
-        hipLaunchKernelGGL(k_auto<2048>, dim3(G), dim3(TB), 0, c->stream, rp, col,
-                           sc.auto_large.as<uint32_t>(), (uint32_t)sc.n_large,
-                           sc.auto_small.as<uint32_t>(), (uint32_t)sc.n_small, cursor, acc);
+        if (sc.n_small <= (uint64_t)kStaticWaves * G * (TB / 64))
+          hipLaunchKernelGGL((k_auto<2048, true>), dim3(G), dim3(TB), 0, c->stream, rp, col,
+                             sc.auto_large.as<uint32_t>(), (uint32_t)sc.n_large,
+                             sc.auto_small.as<uint32_t>(), (uint32_t)sc.n_small, cursor, acc);
+        else
+          hipLaunchKernelGGL(k_auto<2048>, dim3(G), dim3(TB), 0, c->stream, rp, col,
+                             sc.auto_large.as<uint32_t>(), (uint32_t)sc.n_large,
+                             sc.auto_small.as<uint32_t>(), (uint32_t)sc.n_small, cursor, acc);
         tm.launches++;
       }
       if (sc.n_huge) {

@@ -26,6 +26,7 @@ constexpr int TB = 256;                 // threads per workgroup (4 waves)
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kWaveMax = 512;      // longest N+(u) the wave variant holds in LDS
 constexpr uint32_t kWaveChunk = 16;     // wave tasks taken per cursor bump
+constexpr uint32_t kStaticWaves = 16;   // wave tasks per wave up to which they are strided
 constexpr uint32_t kQueue = 320;        // per-wave survivor queue (< 64 + 4 * 64 held)
 constexpr uint32_t kFlatMin = 64;       // lists this long or longer are streamed whole
 constexpr uint32_t kFilterMin = 1;      // wave variant: bitmap for every d+(u) above this
@@ -415,7 +416,7 @@ __device__ inline uint32_t block_vertex(const uint32_t* __restrict__ rp,
 //  phase 2 - wave tasks (list `wav`, d+ <= kWaveMax), one wave each,
 //            kWaveChunk at a time via cursor[1]; they fill the tail left by
 //            phase 1.
-template <int CAP>
+template <int CAP, bool STATIC_WAVES = false>
 __global__ __launch_bounds__(TB) void k_auto(const uint32_t* __restrict__ rp,
                                              const uint32_t* __restrict__ col,
                                              const uint32_t* __restrict__ blk, uint32_t nblk,
@@ -433,6 +434,15 @@ __global__ __launch_bounds__(TB) void k_auto(const uint32_t* __restrict__ rp,
     cnt += block_vertex<CAP>(rp, col, blk[t], S);
   }
   const WaveSlot W = wave_slot(S, wv);
+  if constexpr (STATIC_WAVES) {
+    // few wave tasks: a fixed stride over them, no counter (claims on one
+    // atomic counter are throughput-bound, about 20 ns each)
+    const uint32_t nw = gridDim.x * (TB / 64);
+    for (uint32_t i = blockIdx.x * (TB / 64) + wv; i < nwav; i += nw)
+      cnt += wave_vertex(rp, col, wav[i], lane, W);
+    flush_count(cnt, total);
+    return;
+  }
   for (;;) {
     uint32_t i0 = 0;
     if (lane == 0) i0 = atomicAdd(cursor + 1, kWaveChunk);

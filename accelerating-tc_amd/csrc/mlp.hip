@@ -959,9 +959,15 @@ int tc_count_learned(tc_ctx* c, tc_graph* g, tc_model* md, uint64_t* count, tc_t
   const uint32_t* col = g->col.as<uint32_t>();
   TCX_HIP(hipEventRecord(c->ev0, c->stream));
   if (L.nl_blk || L.nl_wav) {
-    hipLaunchKernelGGL(k_auto<2048>, dim3((uint32_t)c->num_cus * 8u), dim3(TB), 0, c->stream, rp,
-                       col, L.l_blk.as<uint32_t>(), (uint32_t)L.nl_blk, L.l_wav.as<uint32_t>(),
-                       (uint32_t)L.nl_wav, cursor, acc);
+    const uint32_t G = (uint32_t)c->num_cus * 8u;
+    if (L.nl_wav <= (uint64_t)kStaticWaves * G * (TB / 64))
+      hipLaunchKernelGGL((k_auto<2048, true>), dim3(G), dim3(TB), 0, c->stream, rp, col,
+                         L.l_blk.as<uint32_t>(), (uint32_t)L.nl_blk, L.l_wav.as<uint32_t>(),
+                         (uint32_t)L.nl_wav, cursor, acc);
+    else
+      hipLaunchKernelGGL(k_auto<2048>, dim3(G), dim3(TB), 0, c->stream, rp, col,
+                         L.l_blk.as<uint32_t>(), (uint32_t)L.nl_blk, L.l_wav.as<uint32_t>(),
+                         (uint32_t)L.nl_wav, cursor, acc);
     tm.launches++;
   }
   if (L.nl_huge) {

@@ -421,3 +421,16 @@ def test_list_bandwidth_measurement(ctx):
     assert bw[2] > bw[0]  # a wave reading 64 entries moves more than one reading 2
     with pytest.raises(tcamd.TCError):
         tcamd.measure_list_bandwidth(ctx, [0])
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_auto_strided_and_claimed_wave_tasks(ctx, case):
+    # AUTO strides over the wave tasks when they are few (at most 16 per
+    # wave) and claims them from a counter otherwise; a small grid makes the
+    # same graph take the claimed path
+    s, d = CASES[case]()
+    expect = truth(case)
+    g = ctx.graph(s, d).orient(tcamd.ORIENT_ADIR)
+    for grid in (0, 1, 2, 7, 64):
+        assert g.count(tcamd.KERNEL_AUTO, grid_size=grid).count == expect, grid
+    g.close()
