@@ -573,6 +573,30 @@ static int kcore_peel(tc_ctx* c, uint32_t n, const uint32_t* rp, const uint32_t*
   return TC_OK;
 }
 
+int graph_clone_records(tc_ctx* c, const tc_graph* g, tc_graph** out) {
+  *out = nullptr;
+  const uint64_t m = g->m_raw;
+  tc_graph* h = new tc_graph();
+  h->ctx = c;
+  h->m_raw = m;
+  int st = h->src.reserve(4 * std::max<uint64_t>(m, 1));
+  if (st == TC_OK) st = h->dst.reserve(4 * std::max<uint64_t>(m, 1));
+  if (st == TC_OK && m &&
+      (hipMemcpyAsync(h->src.p, g->src.p, 4 * m, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+       hipMemcpyAsync(h->dst.p, g->dst.p, 4 * m, hipMemcpyDeviceToDevice, c->stream) != hipSuccess))
+    st = fail(TC_ERR_HIP, "record copy failed");
+  uint32_t ms = 0, md = 0;
+  if (st == TC_OK && m) st = max_u32(c, h->src.as<uint32_t>(), m, &ms);
+  if (st == TC_OK && m) st = max_u32(c, h->dst.as<uint32_t>(), m, &md);
+  if (st != TC_OK) {
+    delete h;
+    return st;
+  }
+  h->n = m ? std::max(ms, md) + 1 : 0;
+  *out = h;
+  return TC_OK;
+}
+
 }  // namespace tcx
 
 using namespace tcx;

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -90,6 +91,10 @@ struct SchedCache {
   uint64_t kcl_version = ~0ull, kcl_model = ~0ull;
   uint32_t kcl_k = 0;
   DevBuf kcl_path, kcl_key;
+  // degeneracy-oriented copy of the graph for k-clique counting when the
+  // caller's orientation leaves roots wider than the LDS bit matrix
+  uint64_t kcl_degen_version = ~0ull;
+  std::shared_ptr<tc_graph> kcl_degen;
 };
 }  // namespace tcx
 
@@ -143,6 +148,9 @@ int build_csr_from_keys(tc_ctx* c, tc_graph* g, uint64_t* keys, uint64_t nkeys,
                         int end_bit);
 int compute_csr_stats(tc_ctx* c, tc_graph* g);
 int build_sym(tc_ctx* c, tc_graph* g, DevBuf& rowptr, DevBuf& col, uint64_t* m2);
+// A new graph holding a device copy of g's raw records (not oriented); its
+// vertex count is the records' largest id + 1.
+int graph_clone_records(tc_ctx* c, const tc_graph* g, tc_graph** out);
 
 // ---- cost model and per-vertex counts (mlp.hip, count.hip) ---------------
 // Predicted log2 costs Y[n][2] for device features X[n][8]; fails when the

@@ -963,6 +963,33 @@ int kcl_count(tc_ctx* c, tc_graph* g, uint32_t k, int variant, uint64_t* count,
     s.kernel = TC_KERNEL_AUTO;
     return tc_count(c, g, &s, count, timing);
   }
+  if (variant == KCL_VARIANT_AUTO && g->max_out > kPivotMax &&
+      g->orient_mode != TC_ORIENT_DEGENERACY) {
+    // Roots wider than the LDS bit matrix would take the merge path.  Every
+    // acyclic orientation gives the same count, and the degeneracy order
+    // bounds each out-degree by the degeneracy: count on such a copy of the
+    // graph (built once, kept with the graph's schedules) when it is
+    // narrower.
+    SchedCache& sc = g->sc;
+    float prep_ms = 0;
+    if (sc.kcl_degen_version != g->sched_version || !sc.kcl_degen) {
+      const auto t0 = std::chrono::steady_clock::now();
+      tc_graph* h = nullptr;
+      TCX_TRY(graph_clone_records(c, g, &h));
+      std::shared_ptr<tc_graph> keep(h, [](tc_graph* x) { tc_graph_destroy(x); });
+      TCX_TRY(tc_graph_orient(c, h, TC_ORIENT_DEGENERACY, 0));
+      sc.kcl_degen = keep;
+      sc.kcl_degen_version = g->sched_version;
+      prep_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0)
+                    .count();
+    }
+    tc_graph* h = sc.kcl_degen.get();
+    if (h->max_out < g->max_out) {
+      TCX_TRY(kcl_run(c, h, k, variant, nullptr, nullptr, count, timing));
+      if (timing) timing->ms_prep += prep_ms;
+      return TC_OK;
+    }
+  }
   return kcl_run(c, g, k, variant, nullptr, nullptr, count, timing);
 }
 

@@ -365,6 +365,12 @@ int tc_batch_count_pergraph(tc_ctx* ctx, tc_batch* b, const tc_batch_sched* sche
                             uint64_t* counts, tc_timing* timing);
 
 /* ---- k-clique (k-clique/kcl.cu:16-215) -------------------------------- */
+/* Exact k-clique count over the graph's oriented CSR.  KCL_VARIANT_AUTO on
+ * an orientation other than TC_ORIENT_DEGENERACY that leaves roots with
+ * d+ > 1024 counts on a degeneracy-oriented copy of the same records when
+ * that copy is narrower (the count does not depend on the acyclic
+ * orientation); the copy is built once, kept with the graph, and its build
+ * time is added to timing->ms_prep. */
 int kcl_count(tc_ctx* ctx, tc_graph* g, uint32_t k, int variant, uint64_t* count,
               tc_timing* timing);
 /* File-level form of kcl_gpu_solver (k-clique/kcl.h:6-7): reads a Galois

@@ -434,3 +434,27 @@ def test_auto_strided_and_claimed_wave_tasks(ctx, case):
     for grid in (0, 1, 2, 7, 64):
         assert g.count(tcamd.KERNEL_AUTO, grid_size=grid).count == expect, grid
     g.close()
+
+
+def test_kclique_auto_recounts_wide_orientations(ctx):
+    # a hub joined to 1 500 vertices carrying a 40-clique: under the ID
+    # orientation the hub's out-degree exceeds the LDS bit matrix, so AUTO
+    # counts on a degeneracy-oriented copy; every variant and the oracle agree
+    from math import comb
+    a, b = clique(40)
+    hub = np.full(1500, 6000, np.uint32)
+    leaves = np.arange(1500, dtype=np.uint32)
+    rs, rd = er(1500, 6000, 11)
+    s = np.concatenate([a, hub, rs])
+    d = np.concatenate([b, leaves, rd])
+    rp_o, col_o = oracle_csr(s, d, "id")
+    g = ctx.graph(s, d).orient(tcamd.ORIENT_ID)
+    assert g.info().max_out_degree > 1024
+    for k in (3, 4, 5, 6):
+        expect = orc.kclique(rp_o, col_o, k)
+        r1 = g.kclique(k)
+        r2 = g.kclique(k)
+        assert r1.count == r2.count == expect, k
+        assert g.kclique(k, tcamd.KCL_MERGE).count == expect
+    assert expect >= comb(40, 6) + comb(40, 5)
+    g.close()
