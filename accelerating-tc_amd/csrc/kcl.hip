@@ -833,6 +833,32 @@ int root_features(tc_ctx* c, tc_graph* g, uint32_t k, DevBuf& X) {
   return TC_OK;
 }
 
+// Roots wider than the LDS bit matrix would take the merge path.  Every
+// acyclic orientation gives the same count, and the degeneracy order bounds
+// each out-degree by the degeneracy: *out is a degeneracy-oriented copy of g
+// (built once, kept with g's schedules) when g's orientation leaves such
+// roots and the copy is narrower, else g itself.  prep_ms receives the
+// copy's build time (0 when kept).
+int narrow_graph(tc_ctx* c, tc_graph* g, tc_graph** out, float* prep_ms) {
+  *out = g;
+  *prep_ms = 0;
+  if (g->max_out <= kPivotMax || g->orient_mode == TC_ORIENT_DEGENERACY) return TC_OK;
+  SchedCache& sc = g->sc;
+  if (sc.kcl_degen_version != g->sched_version || !sc.kcl_degen) {
+    const auto t0 = std::chrono::steady_clock::now();
+    tc_graph* h = nullptr;
+    TCX_TRY(graph_clone_records(c, g, &h));
+    std::shared_ptr<tc_graph> keep(h, [](tc_graph* x) { tc_graph_destroy(x); });
+    TCX_TRY(tc_graph_orient(c, h, TC_ORIENT_DEGENERACY, 0));
+    sc.kcl_degen = keep;
+    sc.kcl_degen_version = g->sched_version;
+    *prep_ms =
+        std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  if (sc.kcl_degen->max_out < g->max_out) *out = sc.kcl_degen.get();
+  return TC_OK;
+}
+
 // The learned plan of (graph, k, model), built once and kept with the
 // graph's schedules (rebuilt when the CSR, k or the model's parameters
 // change).  prep_ms (optional) receives the build time, 0 when cached.
@@ -932,11 +958,13 @@ int kcl_count_learned(tc_ctx* c, tc_graph* g, uint32_t k, tc_model* m, uint64_t*
     if (timing) *timing = tm;
     return TC_OK;
   }
-  float prep_ms = 0;
-  TCX_TRY(cached_plan(c, g, k, m, &prep_ms));
-  const KclPlan plan{g->sc.kcl_path.as<uint8_t>(), g->sc.kcl_key.as<uint32_t>()};
-  TCX_TRY(kcl_run(c, g, k, KCL_VARIANT_AUTO, &plan, nullptr, count, &tm));
-  tm.ms_prep = prep_ms;
+  float copy_ms = 0, prep_ms = 0;
+  tc_graph* h = nullptr;
+  TCX_TRY(narrow_graph(c, g, &h, &copy_ms));
+  TCX_TRY(cached_plan(c, h, k, m, &prep_ms));
+  const KclPlan plan{h->sc.kcl_path.as<uint8_t>(), h->sc.kcl_key.as<uint32_t>()};
+  TCX_TRY(kcl_run(c, h, k, KCL_VARIANT_AUTO, &plan, nullptr, count, &tm));
+  tm.ms_prep = copy_ms + prep_ms;
   if (timing) *timing = tm;
   return TC_OK;
 }
@@ -963,28 +991,11 @@ int kcl_count(tc_ctx* c, tc_graph* g, uint32_t k, int variant, uint64_t* count,
     s.kernel = TC_KERNEL_AUTO;
     return tc_count(c, g, &s, count, timing);
   }
-  if (variant == KCL_VARIANT_AUTO && g->max_out > kPivotMax &&
-      g->orient_mode != TC_ORIENT_DEGENERACY) {
-    // Roots wider than the LDS bit matrix would take the merge path.  Every
-    // acyclic orientation gives the same count, and the degeneracy order
-    // bounds each out-degree by the degeneracy: count on such a copy of the
-    // graph (built once, kept with the graph's schedules) when it is
-    // narrower.
-    SchedCache& sc = g->sc;
+  if (variant == KCL_VARIANT_AUTO) {
     float prep_ms = 0;
-    if (sc.kcl_degen_version != g->sched_version || !sc.kcl_degen) {
-      const auto t0 = std::chrono::steady_clock::now();
-      tc_graph* h = nullptr;
-      TCX_TRY(graph_clone_records(c, g, &h));
-      std::shared_ptr<tc_graph> keep(h, [](tc_graph* x) { tc_graph_destroy(x); });
-      TCX_TRY(tc_graph_orient(c, h, TC_ORIENT_DEGENERACY, 0));
-      sc.kcl_degen = keep;
-      sc.kcl_degen_version = g->sched_version;
-      prep_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0)
-                    .count();
-    }
-    tc_graph* h = sc.kcl_degen.get();
-    if (h->max_out < g->max_out) {
+    tc_graph* h = nullptr;
+    TCX_TRY(narrow_graph(c, g, &h, &prep_ms));
+    if (h != g) {
       TCX_TRY(kcl_run(c, h, k, variant, nullptr, nullptr, count, timing));
       if (timing) timing->ms_prep += prep_ms;
       return TC_OK;

@@ -407,8 +407,10 @@ int kcl_learned_plan(tc_ctx* ctx, tc_graph* g, uint32_t k, tc_model* m, uint8_t*
 /* Exact k-clique count (3 <= k <= 12) with the learned plan; each launch
  * takes its roots widest first, as kcl_count.  The plan is built once per
  * (graph, k, model parameters) and kept with the graph.  m NULL runs
- * kcl_count(KCL_VARIANT_AUTO).  timing->ms_prep holds the plan's build time
- * (0 when it was kept). */
+ * kcl_count(KCL_VARIANT_AUTO).  Like kcl_count, it counts on the kept
+ * degeneracy-oriented copy when the graph's orientation leaves roots with
+ * d+ > 1024.  timing->ms_prep holds the build time of the plan (and of the
+ * copy), 0 when both were kept. */
 int kcl_count_learned(tc_ctx* ctx, tc_graph* g, uint32_t k, tc_model* m, uint64_t* count,
                       tc_timing* timing);
 
