@@ -833,6 +833,29 @@ int root_features(tc_ctx* c, tc_graph* g, uint32_t k, DevBuf& X) {
   return TC_OK;
 }
 
+// Plan for an explicit root list: listed roots with d+ >= need take the
+// pivot path (d+ <= kPivotMax) or the merge path, widest first; path must
+// hold 255 everywhere before the call.
+__global__ void k_roots_plan(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ roots,
+                             uint64_t nroots, uint32_t n, uint32_t need, uint8_t* __restrict__ path,
+                             uint32_t* __restrict__ key) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nroots;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t u = roots[i];
+    if (u >= n) continue;
+    const uint32_t d = rp[u + 1] - rp[u];
+    if (d < need) continue;
+    path[u] = d <= kPivotMax ? 0 : 1;
+    key[u] = 0xFFFFFFFFu - d;
+  }
+}
+
+__global__ void k_fill_u8(uint8_t* __restrict__ p, uint64_t n, uint8_t v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
 // Roots wider than the LDS bit matrix would take the merge path.  Every
 // acyclic orientation gives the same count, and the degeneracy order bounds
 // each out-degree by the degeneracy: *out is a degeneracy-oriented copy of g
@@ -1002,6 +1025,36 @@ int kcl_count(tc_ctx* c, tc_graph* g, uint32_t k, int variant, uint64_t* count,
     }
   }
   return kcl_run(c, g, k, variant, nullptr, nullptr, count, timing);
+}
+
+int kcl_count_roots(tc_ctx* c, tc_graph* g, uint32_t k, const uint32_t* roots, uint64_t nroots,
+                    uint64_t* count, tc_timing* timing) {
+  TCX_TRY(kcl_check(c, g, k));
+  if (!count || (nroots && !roots)) return fail(TC_ERR_ARG, "NULL argument");
+  *count = 0;
+  tc_timing tm;
+  memset(&tm, 0, sizeof(tm));
+  const uint32_t n = g->n;
+  if (g->m_or == 0 || n == 0 || nroots == 0) {
+    if (timing) *timing = tm;
+    return TC_OK;
+  }
+  for (uint64_t i = 0; i < nroots; ++i)
+    if (roots[i] >= n) return fail(TC_ERR_RANGE, "root %u >= n = %u", roots[i], n);
+  DevBuf droots, path, key;
+  TCX_TRY(droots.reserve(4 * nroots));
+  TCX_TRY(path.reserve((size_t)n + 1));
+  TCX_TRY(key.reserve(4 * ((size_t)n + 1)));
+  TCX_HIP(hipMemcpyAsync(droots.p, roots, 4 * nroots, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n)), dim3(256), 0, c->stream, path.as<uint8_t>(),
+                     (uint64_t)n, (uint8_t)255);
+  hipLaunchKernelGGL(k_roots_plan, dim3(grid_for(nroots)), dim3(256), 0, c->stream,
+                     g->rowptr.as<uint32_t>(), droots.as<uint32_t>(), nroots, n, k - 1,
+                     path.as<uint8_t>(), key.as<uint32_t>());
+  TCX_HIP(hipGetLastError());
+  const KclPlan plan{path.as<uint8_t>(), key.as<uint32_t>()};
+  TCX_TRY(kcl_run(c, g, k, KCL_VARIANT_AUTO, &plan, nullptr, count, timing));
+  return TC_OK;
 }
 
 int kcl_gpu_solver(const char* path, uint32_t k, uint64_t* total, size_t n_chunk) {

@@ -176,6 +176,8 @@ _sig("kcl_count", C.c_int, [_vp, _vp, C.c_uint32, C.c_int, C.POINTER(C.c_uint64)
                             C.POINTER(Timing)])
 _sig("kcl_gpu_solver", C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_uint64), C.c_size_t])
 _sig("kcl_root_features", C.c_int, [_vp, _vp, C.c_uint32, _f32p])
+_sig("kcl_count_roots", C.c_int, [_vp, _vp, C.c_uint32, _vp, C.c_uint64, C.POINTER(C.c_uint64),
+                                  C.POINTER(Timing)])
 _sig("kcl_measure_costs", C.c_int, [_vp, _vp, C.c_uint32, C.c_int, _f32p, C.POINTER(C.c_uint64)])
 _sig("kcl_learned_plan", C.c_int, [_vp, _vp, C.c_uint32, _vp, _u8p])
 _sig("kcl_count_learned", C.c_int, [_vp, _vp, C.c_uint32, _vp, C.POINTER(C.c_uint64),
@@ -218,6 +220,7 @@ EXPORTED = [
     "tc_hu_workload", "tc_hu_block_starts", "tc_fox_workload", "tc_fox_bin_sizes", "tc_count",
     "tc_count_per_vertex", "tc_cost_functions", "tc_approx_ratio", "kcl_count", "kcl_gpu_solver",
     "kcl_root_features", "kcl_measure_costs", "kcl_learned_plan", "kcl_count_learned",
+    "kcl_count_roots",
     "tc_tricore_init_gpu",
     "tc_tricore_cal_degree", "tc_tricore_redirect_edges", "tc_tricore_cal_part_num",
     "tc_tricore_split", "tc_tricore_tricount_gpu", "tc_model_num_params", "tc_model_create",
@@ -642,6 +645,15 @@ class Graph:
         t = Timing()
         _check(_lib.kcl_count(self.ctx.handle, self._h, k, variant, C.byref(out), C.byref(t)),
                "kcl_count")
+        return CountResult(out.value, t.as_dict())
+
+    def kclique_roots(self, k: int, roots) -> CountResult:
+        """k-cliques whose DAG source is one of `roots` (each counted once)."""
+        r = np.ascontiguousarray(roots, np.uint32)
+        out = C.c_uint64()
+        t = Timing()
+        _check(_lib.kcl_count_roots(self.ctx.handle, self._h, k, r.ctypes.data if len(r) else None,
+                                    len(r), C.byref(out), C.byref(t)), "kcl_count_roots")
         return CountResult(out.value, t.as_dict())
 
     def kclique_features(self, k: int):

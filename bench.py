@@ -269,6 +269,27 @@ def main_kclique(args, w):
     if k3 != tri:
         raise SystemExit(f"3-cliques {k3} != triangles {tri}")
     ks = tuple(int(x) for x in args.ks.split(",") if x) or w["ks"]
+    chunks = max(1, args.kcl_chunks)
+    if chunks > 1:
+        # roots dealt round-robin in widest-first order: every chunk gets the
+        # same mix of wide and narrow roots, and each call prints when done
+        rp_h, _ = g.csr()
+        dp = np.diff(rp_h.astype(np.int64))
+        order = np.argsort(-dp, kind="stable")
+        order = order[dp[order] >= 2]
+        root_chunks = [np.sort(order[j::chunks]).astype(np.uint32) for j in range(chunks)]
+
+    def kcount(k):
+        if chunks == 1:
+            return g.kclique(k)
+        total, ms = 0, 0.0
+        for j, rc in enumerate(root_chunks):
+            r = g.kclique_roots(k, rc)
+            total += r.count
+            ms += r.timing["ms_count"]
+            log(f"[bench] k={k}: chunk {j + 1}/{chunks} ({len(rc)} roots) {r.count} cliques, "
+                f"{r.timing['ms_count'] / 1e3:.2f} s")
+        return tcamd.CountResult(total, {"ms_count": ms})
     kmodel = None
     from tcamd import kclique_learned as kl
     if os.path.exists(kl.DEFAULT_MODEL):
@@ -276,12 +297,12 @@ def main_kclique(args, w):
     per_k = {}
     for k in ks:
         t0 = time.perf_counter()
-        ref = g.kclique(k).count  # also the warm-up
+        ref = kcount(k).count  # also the warm-up
         log(f"[bench] k={k}: reference count {ref} in {time.perf_counter() - t0:.2f} s")
         ms = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            r = g.kclique(k)
+            r = kcount(k)
             if r.count != ref:
                 raise SystemExit(f"k={k}: count changed between steps")
             ms.append(r.timing["ms_count"])
@@ -339,7 +360,8 @@ def main_kclique(args, w):
         "config": {"workload": f"k-clique k={','.join(map(str, ks))} on {desc}, "
                                f"{KCL_ORIENTS[orient][1]} orientation", "n": info.n, "m": m,
                    "max_out_degree": info.max_out_degree,
-                   "triangles": tri, "engine": "k-clique pivot", "parallelism": "single GPU"},
+                   "triangles": tri, "engine": "k-clique pivot", "root_chunks": chunks,
+                   "parallelism": "single GPU"},
         "roofline": None,
         "cpu_baseline": cpu,
         "kclique": {str(k): v for k, v in per_k.items()},
@@ -359,6 +381,9 @@ def main():
     ap.add_argument("--kernels", default=",".join(KERNELS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--ks", default="", help="k-clique workloads: comma-separated k values")
+    ap.add_argument("--kcl-chunks", type=int, default=1,
+                    help="k-clique workloads: count each k over this many root chunks "
+                         "(kcl_count_roots), printing after each")
     ap.add_argument("--orient", default="", choices=["", "degeneracy", "adir", "degree", "kcore"],
                     help="k-clique workloads: orientation (default: degeneracy)")
     ap.add_argument("--copy-calibration", action="store_true",

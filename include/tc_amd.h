@@ -373,6 +373,13 @@ int tc_batch_count_pergraph(tc_ctx* ctx, tc_batch* b, const tc_batch_sched* sche
  * time is added to timing->ms_prep. */
 int kcl_count(tc_ctx* ctx, tc_graph* g, uint32_t k, int variant, uint64_t* count,
               tc_timing* timing);
+/* k-cliques whose source in the DAG (the lowest vertex in the orientation
+ * order) is one of roots[0 .. nroots), each listed root counted once, on
+ * the graph's own orientation (3 <= k <= 12).  Disjoint root sets add up to
+ * kcl_count's total: the unit of work to split one count over several calls
+ * or devices.  No counterpart in the reference. */
+int kcl_count_roots(tc_ctx* ctx, tc_graph* g, uint32_t k, const uint32_t* roots,
+                    uint64_t nroots, uint64_t* count, tc_timing* timing);
 /* File-level form of kcl_gpu_solver (k-clique/kcl.h:6-7): reads a Galois
  * .gr file (path ending in ".gr", the input of k-clique/kcl_gpu.cpp:22-27)
  * or a list2bin file, degeneracy orientation (TC_ORIENT_DEGENERACY),

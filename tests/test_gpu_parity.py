@@ -458,3 +458,24 @@ def test_kclique_auto_recounts_wide_orientations(ctx):
         assert g.kclique(k, tcamd.KCL_MERGE).count == expect
     assert expect >= comb(40, 6) + comb(40, 5)
     g.close()
+
+
+@pytest.mark.parametrize("case", ["rmat12", "rgg", "hub", "k6", "messy"])
+def test_kclique_root_subsets_add_up(ctx, case):
+    s, d = CASES[case]()
+    g = ctx.graph(s, d).orient(tcamd.ORIENT_DEGENERACY)
+    n = g.info().n
+    rng = np.random.default_rng(7)
+    part = rng.integers(0, 3, n)
+    for k in (3, 4, 5):
+        full = g.kclique(k).count
+        parts = [g.kclique_roots(k, np.nonzero(part == j)[0]).count for j in range(3)]
+        assert sum(parts) == full, (k, parts, full)
+        assert g.kclique_roots(k, np.arange(n)).count == full
+        # a listed root counts once however often it is listed
+        twice = np.concatenate([np.nonzero(part == 0)[0]] * 2)
+        assert g.kclique_roots(k, twice).count == parts[0]
+    assert g.kclique_roots(4, np.zeros(0, np.uint32)).count == 0
+    with pytest.raises(tcamd.TCError):
+        g.kclique_roots(4, np.array([n], np.uint32))
+    g.close()
