@@ -13,7 +13,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle.so")
+# ORACLE_LIB selects another build of the same source (the sanitizer build
+# of the host sanitizer test)
+LIB = os.environ.get("ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 REF_DIR = os.path.join(HERE, "_ref")
 
 
