@@ -152,6 +152,7 @@ def cpu_baseline_batch(graphs, target_s=15.0):
         "value": e / t, "unit": "edges/s", "cores": 1, "kind": "port",
         "sample": (f"oracle pipeline (degree orientation, CSR build, serial merge count) over "
                    f"every {step2}-th graph of the batch: {e} oriented edges, {t:.1f} s"),
+        "host": host_info(),
     }
 
 
@@ -346,7 +347,8 @@ def main_kclique(args, w):
         cpu = {"value": edges / t, "unit": "edges/s", "cores": 1, "kind": "port",
                "sample": (f"oracle depth-first {k}-clique counter over {roots} roots drawn at "
                           f"random (seed 0) from the same oriented CSR: {edges} of {m} edges, "
-                          f"{t:.1f} s")}
+                          f"{t:.1f} s"),
+               "host": host_info()}
         log(f"[bench] cpu baseline {cpu['value']:.3e} edges/s ({cpu['sample']})")
     total_wall = sum(v["ms_wall"] for v in per_k.values()) * 1e-3
     line = {
