@@ -548,19 +548,26 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
       int L = -1;
       uint32_t piv = 0;
       uint64_t todo = 0;
-      {
-        const bool br = node(1, 0, &piv, &todo);
-        if (br) {
-          L = 0;
-          if (lane < (uint32_t)W) {
-            stk[lane] = s_C[lane];
-            stk[W + lane] = todo;
+      // one call site of node(): the node in s_C with (nh, np) is evaluated
+      // when pending (first the root), then the walk picks the next branch
+      uint32_t nh = 1, np = 0;
+      bool pending = true;
+      for (;;) {
+        if (pending) {
+          const bool br = node(nh, np, &piv, &todo);
+          if (br) {
+            ++L;
+            uint64_t* nv = stk + (uint64_t)L * LW;
+            if (lane < (uint32_t)W) {
+              nv[lane] = s_C[lane];
+              nv[W + lane] = todo;
+            }
+            if (lane == 0) nv[2 * W] = (uint64_t)piv | ((uint64_t)nh << 16) | ((uint64_t)np << 32);
           }
-          if (lane == 0) stk[2 * W] = (uint64_t)piv | (1ull << 16);
+          wave_sync_kcl();
+          pending = false;
         }
-      }
-      wave_sync_kcl();
-      while (L >= 0) {
+        if (L < 0) break;
         uint64_t* lv = stk + (uint64_t)L * LW;
         const uint64_t todo_w = lane < (uint32_t)W ? lv[W + lane] : 0ull;
         const uint64_t nzl = __ballot(todo_w != 0);
@@ -606,17 +613,9 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
         }
         wave_sync_kcl();
         if (skip) continue;
-        const bool br = node(h2, p2, &piv, &todo);
-        if (br) {
-          ++L;
-          uint64_t* nv = stk + (uint64_t)L * LW;
-          if (lane < (uint32_t)W) {
-            nv[lane] = s_C[lane];
-            nv[W + lane] = todo;
-          }
-          if (lane == 0) nv[2 * W] = (uint64_t)piv | ((uint64_t)h2 << 16) | ((uint64_t)p2 << 32);
-        }
-        wave_sync_kcl();
+        nh = h2;
+        np = p2;
+        pending = true;
       }
       __syncthreads();
       if (cost && tid == 0) atomicAdd(cost + u, (float)(wall_clock64() - t_task) * cost_scale);
