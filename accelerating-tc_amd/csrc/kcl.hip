@@ -473,8 +473,9 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
         if (r == 3) {
           // binom(p,3) + binom(p,2)|C| + p edges(C) + triangles(C).  Lane x
           // holds A = N(x) & C above x: |A| edges start at x, and the pairs
-          // (y, z) of A with y ~ z, summed over y in A, count each triangle
-          // whose smallest vertex is x twice.
+          // y < z of A with y ~ z count each triangle whose smallest vertex
+          // is x once.  z runs over A above y only: the rest of y's word and
+          // the words after it.
           uint64_t esum = 0, tsum = 0;
           for (int t = 0; t < W; ++t) {
             if (cr[t] == 0 || (part && (uint32_t)t % Q != q)) continue;
@@ -492,10 +493,10 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
               uint64_t ys = ax[w];
               while (ys) {
                 const uint32_t y = 64u * w + (uint32_t)__builtin_ctzll(ys);
-                ys &= ys - 1;
-                uint32_t c3 = 0;
+                ys &= ys - 1;  // now the members of A in word w above y
+                uint32_t c3 = (uint32_t)__popcll(ys & s_M[y * W + w]);
 #pragma unroll
-                for (int w2 = 0; w2 < W; ++w2) c3 += (uint32_t)__popcll(ax[w2] & s_M[y * W + w2]);
+                for (int w2 = w + 1; w2 < W; ++w2) c3 += (uint32_t)__popcll(ax[w2] & s_M[y * W + w2]);
                 tsum += c3;
               }
             }
@@ -503,7 +504,7 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
           const unsigned long long e = wave_sum(esum), t2 = wave_sum(tsum);
           if (lane == 0)
             cnt += (lead ? binom_small(p, 3) + binom_small(p, 2) * cs : 0ull) +
-                   (unsigned long long)p * e + t2 / 2;
+                   (unsigned long long)p * e + t2;
           return false;
         }
         // pivot: the member of C with most neighbours in C, the smallest such
