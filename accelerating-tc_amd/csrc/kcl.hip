@@ -22,6 +22,7 @@ namespace {
 constexpr int kMaxK = 12;
 constexpr int kKclFeatures = TC_MODEL_FEATURES;
 constexpr uint32_t kPivotMax = 1024;  // widest bit matrix in LDS (16 words per row)
+constexpr int kCompactWords = 4;      // pivot matrices this wide pack C's members per pass
 
 __device__ inline uint64_t gtid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ inline uint64_t gstride() { return (uint64_t)gridDim.x * blockDim.x; }
@@ -367,6 +368,7 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
                                                        float* __restrict__ cost, float cost_scale) {
   constexpr uint32_t D = 64 * W;
   constexpr uint32_t LW = 2 * W + 1;  // stack words per level
+  static_assert(NW <= W, "each wave's 64-entry member slot lies in s_list[64 W]");
   __shared__ uint32_t s_list[D];
   __shared__ uint64_t s_M[D * W];
   __shared__ uint64_t s_Cs[NW][W];
@@ -422,7 +424,7 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
       // (lanes < W, word `lane`) are written to *piv_out / *todo_out.
       // Nodes with h = 1 are walked by every walker: their single terms are
       // counted by walker 0 (`lead`), and the sums of the r = 2, 3 closed
-      // forms are split over the walkers by 64-vertex word of C.  Those sums
+      // forms are split over the walkers by pass (64 members of C).  Those sums
       // count every edge of C once (at its smaller end) and every triangle
       // once (at its smallest vertex), so each walker's share is an integer.
       auto node = [&](uint32_t h, uint32_t p, uint32_t* piv_out, uint64_t* todo_out) -> bool {
@@ -449,22 +451,65 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
           if (lane == 0 && lead) cnt += p + cs;
           return false;
         }
-        if (r == 2) {
-          // binom(p,2) + p|C| + edges(C); lane x (word t) counts its
-          // neighbours in C above x
-          uint32_t esum = 0;
+        // The members of C are taken 64 at a time, one per lane.  With
+        // W >= kCompactWords words they are packed in index order through the
+        // wave's 64-entry slot of s_list (free once the bit matrix is built),
+        // so a sparse C takes ceil(|C| / 64) passes; narrower matrices take
+        // one pass per non-empty word.
+        constexpr uint32_t kNone = 0xFFFFFFFFu;
+        uint32_t pcw = 0, pex = 0;  // lane w < W: members in word w / before it
+        if constexpr (W >= kCompactWords) {
+          pcw = lane < (uint32_t)W ? (uint32_t)__popcll(s_C[lane]) : 0u;
+          uint32_t inc = pcw;
 #pragma unroll
-          for (int t = 0; t < W; ++t) {
-            if (cr[t] == 0 || (part && (uint32_t)t % Q != q)) continue;
-            const uint32_t x = 64u * t + lane;
-            if ((cr[t] >> lane) & 1ull) {
-#pragma unroll
-              for (int w = 0; w < W; ++w) {
-                const uint64_t above = w < t ? 0ull : (w > t ? ~0ull : ~((2ull << lane) - 1));
-                esum += (uint32_t)__popcll(s_M[x * W + w] & cr[w] & above);
+          for (uint32_t off = 1; off < (uint32_t)W; off <<= 1) {
+            const uint32_t v = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += v;
+          }
+          pex = inc - pcw;
+        }
+        auto for_members = [&](bool split, auto&& f) {
+          if constexpr (W >= kCompactWords) {
+            uint32_t* pl = s_list + 64u * wv;
+            for (uint32_t j = 0, lo = 0; lo < cs; ++j, lo += 64) {
+              if (split && j % Q != q) continue;
+              uint64_t words = __ballot(lane < (uint32_t)W && pex < lo + 64 && pex + pcw > lo);
+              while (words) {
+                const uint32_t w = (uint32_t)__builtin_ctzll(words);
+                words &= words - 1;
+                const uint64_t v = s_C[w];
+                const uint32_t rk = __shfl(pex, w, 64) +
+                                    __builtin_amdgcn_mbcnt_hi((uint32_t)(v >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)v, 0u));
+                if (((v >> lane) & 1ull) && rk >= lo && rk < lo + 64) pl[rk - lo] = 64u * w + lane;
               }
+              wave_sync_kcl();
+              const uint32_t x = lo + lane < cs ? pl[lane] : kNone;
+              wave_sync_kcl();
+              f(x);
+            }
+          } else {
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+              if (cr[t] == 0 || (split && (uint32_t)t % Q != q)) continue;
+              f(((cr[t] >> lane) & 1ull) ? 64u * t + lane : kNone);
             }
           }
+        };
+        if (r == 2) {
+          // binom(p,2) + p|C| + edges(C); lane x counts its neighbours in C
+          // above x
+          uint32_t esum = 0;
+          for_members(part, [&](uint32_t x) {
+            if (x == kNone) return;
+            const uint32_t t = x >> 6;
+            const uint64_t lm = ~((2ull << (x & 63)) - 1);
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+              const uint64_t above = (uint32_t)w < t ? 0ull : ((uint32_t)w > t ? ~0ull : lm);
+              esum += (uint32_t)__popcll(s_M[x * W + w] & cr[w] & above);
+            }
+          });
           const unsigned long long e = wave_sum_u32(esum);
           if (lane == 0)
             cnt += (lead ? binom_small(p, 2) + (unsigned long long)p * cs : 0ull) + e;
@@ -477,14 +522,14 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
           // is x once.  z runs over A above y only: the rest of y's word and
           // the words after it.
           uint64_t esum = 0, tsum = 0;
-          for (int t = 0; t < W; ++t) {
-            if (cr[t] == 0 || (part && (uint32_t)t % Q != q)) continue;
-            const uint32_t x = 64u * t + lane;
-            if (!((cr[t] >> lane) & 1ull)) continue;
+          for_members(part, [&](uint32_t x) {
+            if (x == kNone) return;
+            const uint32_t t = x >> 6;
+            const uint64_t lm = ~((2ull << (x & 63)) - 1);
             uint64_t ax[W];
 #pragma unroll
             for (int w = 0; w < W; ++w) {
-              const uint64_t above = w < t ? 0ull : (w > t ? ~0ull : ~((2ull << lane) - 1));
+              const uint64_t above = (uint32_t)w < t ? 0ull : ((uint32_t)w > t ? ~0ull : lm);
               ax[w] = s_M[x * W + w] & cr[w] & above;
               esum += (uint32_t)__popcll(ax[w]);
             }
@@ -500,7 +545,7 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
                 tsum += c3;
               }
             }
-          }
+          });
           const unsigned long long e = wave_sum(esum), t2 = wave_sum(tsum);
           if (lane == 0)
             cnt += (lead ? binom_small(p, 3) + binom_small(p, 2) * cs : 0ull) +
@@ -509,20 +554,17 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
         }
         // pivot: the member of C with most neighbours in C, the smallest such
         // index on ties (any member gives exact counts; the best one leaves
-        // the fewest branches)
+        // the fewest branches).  Every walker searches all of C.
         uint32_t best = 0;
+        auto search = [&](uint32_t x) {
+          if (x == kNone) return;
+          uint32_t sc = 0;
 #pragma unroll
-        for (int t = 0; t < W; ++t) {
-          if (cr[t] == 0) continue;
-          const uint32_t x = 64u * t + lane;
-          if ((cr[t] >> lane) & 1ull) {
-            uint32_t sc = 0;
-#pragma unroll
-            for (int w = 0; w < W; ++w) sc += (uint32_t)__popcll(s_M[x * W + w] & cr[w]);
-            const uint32_t key = (sc << 16) | (0xFFFFu - x);  // (score, smallest index)
-            best = key > best ? key : best;
-          }
-        }
+          for (int w = 0; w < W; ++w) sc += (uint32_t)__popcll(s_M[x * W + w] & cr[w]);
+          const uint32_t key = (sc << 16) | (0xFFFFu - x);  // (score, smallest index)
+          best = key > best ? key : best;
+        };
+        for_members(false, search);
         best = wave_max_u32(best);
         const uint32_t piv = 0xFFFFu - (best & 0xFFFFu);
         *piv_out = piv;
