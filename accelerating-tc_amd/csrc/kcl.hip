@@ -370,7 +370,11 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
   constexpr uint32_t LW = 2 * W + 1;  // stack words per level
   static_assert(NW <= W, "each wave's 64-entry member slot lies in s_list[64 W]");
   __shared__ uint32_t s_list[D];
-  __shared__ uint64_t s_M[D * W];
+  // bit-matrix row stride in words: rows of 8 and 16 words are padded by
+  // two words (16-byte aligned, so row reads stay 128-bit), which spreads
+  // the rows that the lanes read at once over the LDS banks
+  constexpr uint32_t RS = W >= 8 ? W + 2 : W;
+  __shared__ uint64_t s_M[D * RS];
   __shared__ uint64_t s_Cs[NW][W];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   uint64_t* s_C = s_Cs[wv];
@@ -395,7 +399,7 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
       const uint32_t q = (uint32_t)(ti % copies) * NW + wv;  // this wave's walker number
       const uint32_t base = rp[u], d = rp[u + 1] - base;
       for (uint32_t i = tid; i < d; i += 64 * NW) s_list[i] = col[base + i];
-      for (uint32_t i = tid; i < d * W; i += 64 * NW) s_M[i] = 0;
+      for (uint32_t i = tid; i < d * RS; i += 64 * NW) s_M[i] = 0;
       if (tid == 0) s_next = NW;
       __syncthreads();
       // symmetric adjacency of N+(u): bit (i, j) for every arc c_i -> c_j
@@ -411,8 +415,8 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
             len -= half;
           }
           if (s_list[lo] == x) {
-            atomicOr((unsigned long long*)&s_M[i * W + (lo >> 6)], 1ull << (lo & 63));
-            atomicOr((unsigned long long*)&s_M[lo * W + (i >> 6)], 1ull << (i & 63));
+            atomicOr((unsigned long long*)&s_M[i * RS + (lo >> 6)], 1ull << (lo & 63));
+            atomicOr((unsigned long long*)&s_M[lo * RS + (i >> 6)], 1ull << (i & 63));
           }
         }
       }
@@ -507,7 +511,7 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
 #pragma unroll
             for (int w = 0; w < W; ++w) {
               const uint64_t above = (uint32_t)w < t ? 0ull : ((uint32_t)w > t ? ~0ull : lm);
-              esum += (uint32_t)__popcll(s_M[x * W + w] & cr[w] & above);
+              esum += (uint32_t)__popcll(s_M[x * RS + w] & cr[w] & above);
             }
           });
           const unsigned long long e = wave_sum_u32(esum);
@@ -530,7 +534,7 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
 #pragma unroll
             for (int w = 0; w < W; ++w) {
               const uint64_t above = (uint32_t)w < t ? 0ull : ((uint32_t)w > t ? ~0ull : lm);
-              ax[w] = s_M[x * W + w] & cr[w] & above;
+              ax[w] = s_M[x * RS + w] & cr[w] & above;
               esum += (uint32_t)__popcll(ax[w]);
             }
 #pragma unroll
@@ -539,9 +543,9 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
               while (ys) {
                 const uint32_t y = 64u * w + (uint32_t)__builtin_ctzll(ys);
                 ys &= ys - 1;  // now the members of A in word w above y
-                uint32_t c3 = (uint32_t)__popcll(ys & s_M[y * W + w]);
+                uint32_t c3 = (uint32_t)__popcll(ys & s_M[y * RS + w]);
 #pragma unroll
-                for (int w2 = w + 1; w2 < W; ++w2) c3 += (uint32_t)__popcll(ax[w2] & s_M[y * W + w2]);
+                for (int w2 = w + 1; w2 < W; ++w2) c3 += (uint32_t)__popcll(ax[w2] & s_M[y * RS + w2]);
                 tsum += c3;
               }
             }
@@ -560,7 +564,7 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
           if (x == kNone) return;
           uint32_t sc = 0;
 #pragma unroll
-          for (int w = 0; w < W; ++w) sc += (uint32_t)__popcll(s_M[x * W + w] & cr[w]);
+          for (int w = 0; w < W; ++w) sc += (uint32_t)__popcll(s_M[x * RS + w] & cr[w]);
           const uint32_t key = (sc << 16) | (0xFFFFu - x);  // (score, smallest index)
           best = key > best ? key : best;
         };
@@ -569,7 +573,7 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
         const uint32_t piv = 0xFFFFu - (best & 0xFFFFu);
         *piv_out = piv;
         if (lane < (uint32_t)W) {
-          uint64_t todo = s_C[lane] & ~s_M[piv * W + lane];  // C outside N(pivot)
+          uint64_t todo = s_C[lane] & ~s_M[piv * RS + lane];  // C outside N(pivot)
           if (lane == (piv >> 6)) todo |= 1ull << (piv & 63);
           *todo_out = todo;
         }
@@ -650,7 +654,7 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
         if (lane < (uint32_t)W) {
           const uint64_t c = lv[lane];
           const uint64_t xb = lane == fwl ? (1ull << (x & 63)) : 0ull;
-          if (!skip) s_C[lane] = c & s_M[x * W + lane];
+          if (!skip) s_C[lane] = c & s_M[x * RS + lane];
           lv[lane] = c & ~xb;
           lv[W + lane] = todo_w & ~xb;
         }
