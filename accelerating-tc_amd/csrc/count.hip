@@ -434,25 +434,6 @@ __global__ __launch_bounds__(64) void k_tricore(const uint32_t* __restrict__ rp,
   flush_count(cnt, total);
 }
 
-// ---- per-vertex counts (wave per vertex, global binary search) ----------
-__global__ __launch_bounds__(TB) void k_per_vertex(const uint32_t* __restrict__ rp,
-                                                   const uint32_t* __restrict__ col, uint32_t n,
-                                                   unsigned long long* __restrict__ tri) {
-  const uint32_t lane = threadIdx.x & 63;
-  const uint64_t wave = gtid() >> 6, nwaves = gstride() >> 6;
-  for (uint64_t u = wave; u < n; u += nwaves) {
-    const uint32_t ab = rp[u], du = rp[u + 1] - ab;
-    unsigned long long cnt = 0;
-    for (uint32_t k = 0; k < du; ++k) {
-      uint32_t v = col[ab + k];
-      uint32_t vb = rp[v], ve = rp[v + 1];
-      for (uint32_t j = vb + lane; j < ve; j += 64) cnt += contains(col, ab, ab + du, col[j]);
-    }
-    cnt = wave_sum(cnt);
-    if (lane == 0) tri[u] = cnt;
-  }
-}
-
 }  // namespace
 
 // ---- schedule builders ------------------------------------------------------
@@ -477,12 +458,32 @@ static unsigned grid_for(uint64_t n, int tb = TB) {
   return (unsigned)b;
 }
 
+static int auto_schedule(tc_ctx* c, tc_graph* g, uint32_t wmax, uint32_t hmin);
+
+// tri[u] = the triangles whose first vertex is u: the AUTO kernel's block
+// and wave tasks (default thresholds) with one output per vertex.
 int per_vertex_counts(tc_ctx* c, tc_graph* g, unsigned long long* tri) {
   if (g->n == 0) return TC_OK;
-  hipLaunchKernelGGL(k_per_vertex, dim3(std::min<uint64_t>(grid_for(g->n * 64ull), 65536)),
-                     dim3(TB), 0, c->stream, g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(), g->n,
-                     tri);
+  TCX_TRY(auto_schedule(c, g, 64u, 2048u));
+  const SchedCache& sc = g->sc;
+  const uint32_t* rp = g->rowptr.as<uint32_t>();
+  const uint32_t* col = g->col.as<uint32_t>();
+  DevBuf cur;
+  TCX_TRY(cur.reserve(4 * sizeof(uint32_t)));
+  uint32_t* cursor = cur.as<uint32_t>();
+  TCX_HIP(hipMemsetAsync(cursor, 0, 4 * sizeof(uint32_t), c->stream));
+  TCX_HIP(hipMemsetAsync(tri, 0, 8 * (size_t)g->n, c->stream));
+  if (sc.n_large || sc.n_small)
+    hipLaunchKernelGGL(k_auto_per_vertex<2048>, dim3((uint32_t)c->num_cus * 8u), dim3(TB), 0,
+                       c->stream, rp, col, sc.auto_large.as<uint32_t>(), (uint32_t)sc.n_large,
+                       sc.auto_small.as<uint32_t>(), (uint32_t)sc.n_small, cursor, tri);
+  if (sc.n_huge)
+    hipLaunchKernelGGL(k_auto_per_vertex<8192>,
+                       dim3(std::min<uint64_t>((uint64_t)c->num_cus * 4, sc.n_huge)), dim3(TB), 0,
+                       c->stream, rp, col, sc.auto_huge.as<uint32_t>(), (uint32_t)sc.n_huge,
+                       (const uint32_t*)nullptr, 0u, cursor + 2, tri);
   TCX_HIP(hipGetLastError());
+  TCX_HIP(hipStreamSynchronize(c->stream));  // before `cur` is freed
   return TC_OK;
 }
 

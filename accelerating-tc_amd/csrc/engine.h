@@ -454,5 +454,43 @@ __global__ __launch_bounds__(TB) void k_auto(const uint32_t* __restrict__ rp,
   flush_count(cnt, total);
 }
 
+// Per-vertex form of k_auto: tri[u] = the triangles whose first vertex (the
+// tail of the two other arcs) is u, for every vertex of the two task lists.
+// A block task adds one partial per wave, so tri must start at zero.
+template <int CAP>
+__global__ __launch_bounds__(TB) void k_auto_per_vertex(const uint32_t* __restrict__ rp,
+                                                        const uint32_t* __restrict__ col,
+                                                        const uint32_t* __restrict__ blk,
+                                                        uint32_t nblk,
+                                                        const uint32_t* __restrict__ wav,
+                                                        uint32_t nwav, uint32_t* cursor,
+                                                        unsigned long long* __restrict__ tri) {
+  __shared__ BlockLds<CAP> S;
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  clear_bitmap(S);
+  for (;;) {
+    if (tid == 0) S.t = atomicAdd(cursor, 1u);
+    __syncthreads();
+    const uint32_t t = S.t;
+    if (t >= nblk) break;
+    const uint32_t u = blk[t];
+    const unsigned long long c = wave_sum(block_vertex<CAP>(rp, col, u, S));
+    if (lane == 0 && c) atomicAdd(tri + u, c);
+  }
+  const WaveSlot W = wave_slot(S, wv);
+  for (;;) {
+    uint32_t i0 = 0;
+    if (lane == 0) i0 = atomicAdd(cursor + 1, kWaveChunk);
+    i0 = __shfl(i0, 0, 64);
+    if (i0 >= nwav) break;
+    const uint32_t i1 = min(nwav, i0 + kWaveChunk);
+    for (uint32_t i = i0; i < i1; ++i) {
+      const uint32_t u = wav[i];
+      const unsigned long long c = wave_sum(wave_vertex(rp, col, u, lane, W));
+      if (lane == 0) tri[u] = c;
+    }
+  }
+}
+
 }  // namespace dev
 }  // namespace tcx

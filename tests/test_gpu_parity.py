@@ -173,12 +173,17 @@ def test_work_models_match_oracle(ctx, case):
     g.close()
 
 
-@pytest.mark.parametrize("case", ["er", "rmat12", "hub", "messy"])
-def test_per_vertex_counts(ctx, case):
+@pytest.mark.parametrize("orient", ["degree", "id"])
+@pytest.mark.parametrize("case", ["er", "rmat12", "hub", "messy", "k6", "ba"])
+def test_per_vertex_counts(ctx, case, orient):
+    # under ID orientation the hub's row (6 000 out-neighbours) is a huge task
     s, d = CASES[case]()
-    rp_o, col_o = oracle_csr(s, d, "degree")
-    g = ctx.graph(s, d).orient(tcamd.ORIENT_DEGREE)
-    np.testing.assert_array_equal(g.count_per_vertex(), orc.count_per_vertex(rp_o, col_o))
+    rp_o, col_o = oracle_csr(s, d, orient)
+    g = ctx.graph(s, d).orient(ORIENTS[orient])
+    per = g.count_per_vertex()
+    want = orc.count_per_vertex(rp_o, col_o)
+    np.testing.assert_array_equal(per, want)
+    assert int(per.sum()) == g.count("auto").count
     g.close()
 
 
