@@ -257,7 +257,9 @@ int tc_approx_ratio(tc_ctx* ctx, tc_graph* g, double* ratio);
  * NULL (AUTO with defaults).  timing may be NULL. */
 int tc_count(tc_ctx* ctx, tc_graph* g, const tc_sched* sched, uint64_t* count,
              tc_timing* timing);
-/* Per-vertex forward counts tri[u] (n entries), same kernels. */
+/* Per-vertex forward counts tri[u] (n entries): the triangles whose first
+ * vertex (the tail of the two other arcs) is u, from the AUTO kernel's block
+ * and wave tasks with default thresholds.  They sum to tc_count's total. */
 int tc_count_per_vertex(tc_ctx* ctx, tc_graph* g, uint64_t* tri);
 
 /* ---- learned scheduling of the AUTO engine ------------------------------
