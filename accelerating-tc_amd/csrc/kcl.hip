@@ -321,7 +321,8 @@ __global__ __launch_bounds__(64) void k_kcl_merge(const uint32_t* __restrict__ r
 //                               binom(p, 2) + p |C| + edges(C),
 // so the tree is only walked where at least three more vertices are needed.
 // The walk's stack (C and the branches still to take, per level) lives in
-// the wave's slice of `stk`: d + 1 levels of 2W + 1 words.
+// LDS for one- and two-word matrices and in the wave's slice of `stk`
+// otherwise: d + 1 levels of 2W + 1 words.
 __device__ inline unsigned long long binom_small(uint32_t n, uint32_t r) {
   if (r > n) return 0;
   unsigned __int128 c = 1;
@@ -378,7 +379,12 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
   __shared__ uint64_t s_Cs[NW][W];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   uint64_t* s_C = s_Cs[wv];
-  uint64_t* stk = stk_all + ((uint64_t)blockIdx.x * NW + wv) * (D + 1) * LW;
+  // one- and two-word matrices keep the walk's stack in LDS (one wave per
+  // workgroup), the wider ones in their slice of stk_all
+  constexpr bool kLdsStack = W <= 2;
+  static_assert(!kLdsStack || NW == 1, "an LDS stack holds one walker");
+  __shared__ uint64_t s_stk[kLdsStack ? (D + 1) * LW : 1];
+  uint64_t* stk = kLdsStack ? s_stk : stk_all + ((uint64_t)blockIdx.x * NW + wv) * (D + 1) * LW;
   unsigned long long cnt = 0;  // lane 0 of each wave accumulates
   const uint32_t Q = copies * NW;  // walkers per root
   // (root, copy) tasks are claimed `chunk` at a time from a counter, in the
@@ -801,7 +807,7 @@ int kcl_run(tc_ctx* c, tc_graph* g, uint32_t k, int variant, const KclPlan* plan
       // workgroups resident per CU (LDS- and register-bound for the wide
       // matrices); the wide ones run 8 or 16 walkers per root, and a root
       // is repeated `copies` times when few roots would leave CUs idle
-      const uint64_t per_cu = b.w <= 4 ? 16 : (b.w == 8 ? 2 : 1);
+      const uint64_t per_cu = b.w == 1 ? 32 : (b.w <= 4 ? 16 : (b.w == 8 ? 2 : 1));
       const uint64_t nw = b.w <= 4 ? 1 : (b.w == 8 ? 8 : 16);
       const uint64_t slots = (uint64_t)c->num_cus * per_cu;
       const uint64_t max_copies = b.w <= 4 ? 1 : (b.w == 8 ? 8 : 32);
