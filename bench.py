@@ -72,33 +72,38 @@ def describe(w):
 
 
 def cpu_baseline(rp, col, target_s=15.0):
-    """Serial oracle counter on a strided sample of the same oriented CSR."""
+    """Serial oracle counter on a systematic sample of the same oriented CSR:
+    every S-th row in out-degree order, so each degree class is represented
+    in proportion (a stride over ids can take or miss the few heavy rows)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as orc
 
     n = len(rp) - 1
     deg = np.diff(rp.astype(np.int64))
-    # calibrate on a sparse stride, then size the sample to ~target_s seconds
-    stride = max(1, n // 2000)
+    order = np.argsort(deg, kind="stable").astype(np.uint32)
+    # calibrate on a sparse sample, then size the sample to ~target_s seconds
+    s0 = max(1, n // 2000)
     t0 = time.perf_counter()
-    orc.count_rows_strided(rp, col, 0, stride)
+    orc.count_row_list(rp, col, order[s0 // 2::s0])
     t1 = time.perf_counter() - t0
-    est_full = max(t1, 1e-4) * stride
-    stride2 = max(1, int(round(est_full / target_s)))
+    est_full = max(t1, 1e-4) * s0
+    stride = max(1, int(round(est_full / target_s)))
+    rows = order[stride // 2::stride]
     t0 = time.perf_counter()
-    c1 = orc.count_rows_strided(rp, col, 0, stride2)
+    c1 = orc.count_row_list(rp, col, rows)
     t = time.perf_counter() - t0
-    edges = int(deg[0::stride2].sum())
+    edges = int(deg[rows].sum())
     return {
         "value": edges / t,
         "unit": "edges/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"serial forward merge counter (oracle) over every {stride2}-th row of the "
-                   f"same oriented CSR: {edges} of {int(deg.sum())} edges, {t:.1f} s"),
+        "sample": (f"serial forward merge counter (oracle) over every {stride}-th row in "
+                   f"out-degree order of the same oriented CSR: {edges} of {int(deg.sum())} "
+                   f"edges, {t:.1f} s"),
         "host": host_info(),
-    }, (stride2, edges, c1)
+    }, (rows, edges, c1)
 
 
 def cpu_baseline_parallel(rp, col, sample):
@@ -107,10 +112,10 @@ def cpu_baseline_parallel(rp, col, sample):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
-    stride, edges, c1 = sample
+    rows, edges, c1 = sample
     threads = max(1, min(16, os.cpu_count() or 1))
     t0 = time.perf_counter()
-    c = orc.count_rows_strided_omp(rp, col, 0, stride, threads)
+    c = orc.count_row_list(rp, col, rows, threads)
     t = time.perf_counter() - t0
     if c != c1:
         raise SystemExit(f"parallel oracle sample {c} != serial {c1}")

@@ -564,6 +564,22 @@ uint64_t orc_count_rows_strided_omp(uint32_t n, const uint32_t* rowptr, const ui
   return total;
 }
 
+// Triangles at the listed rows (ids >= n are skipped); OpenMP over the list
+// when threads > 1.  Used to time a sample of rows chosen by the caller.
+uint64_t orc_count_row_list(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
+                            const uint32_t* rows, uint64_t nrows, int threads) {
+  uint64_t total = 0;
+  if (threads <= 1) {
+    for (uint64_t i = 0; i < nrows; ++i)
+      if (rows[i] < n) total += vertex_tri(rowptr, col, rows[i]);
+    return total;
+  }
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 64) reduction(+ : total)
+  for (long long i = 0; i < (long long)nrows; ++i)
+    if (rows[i] < n) total += vertex_tri(rowptr, col, rows[i]);
+  return total;
+}
+
 void orc_count_per_vertex(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
                           uint64_t* tri) {
   for (uint32_t u = 0; u < n; ++u) tri[u] = vertex_tri(rowptr, col, u);

@@ -59,6 +59,7 @@ _sig("orc_count_rows", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_ui
 _sig("orc_count_rows_strided", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32])
 _sig("orc_count_rows_strided_omp", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32,
                                                 C.c_int])
+_sig("orc_count_row_list", C.c_uint64, [C.c_uint32, _u32p, _u32p, _u32p, C.c_uint64, C.c_int])
 _sig("orc_count_per_vertex", None, [C.c_uint32, _u32p, _u32p, _u64p])
 _sig("orc_kclique", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32])
 _sig("orc_kclique_rows_strided", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32,
@@ -207,6 +208,14 @@ def count_rows_strided_omp(rp, col, start, stride, threads) -> int:
     rp = _u32(rp)
     return int(_lib.orc_count_rows_strided_omp(len(rp) - 1, rp, _nz(_u32(col), np.uint32), start,
                                                stride, threads))
+
+
+def count_row_list(rp, col, rows, threads=1) -> int:
+    """Triangles at the listed rows (oracle/oracle.cpp orc_count_row_list)."""
+    rp = _u32(rp)
+    rows = _u32(rows)
+    return int(_lib.orc_count_row_list(len(rp) - 1, rp, _nz(_u32(col), np.uint32),
+                                       _nz(rows, np.uint32), len(rows), threads))
 
 
 def count_per_vertex(rp, col):

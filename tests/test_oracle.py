@@ -504,3 +504,21 @@ def test_parallel_sample_counter_equals_serial():
     for start, stride in ((0, 3), (5, 7), (n - 1, 2), (n + 3, 1)):
         assert (orc.count_rows_strided_omp(rp, col, start, stride, 3)
                 == orc.count_rows_strided(rp, col, start, stride))
+
+
+def test_row_list_counter_partitions_the_count():
+    # the CPU baseline times systematic samples of rows in out-degree order:
+    # the S samples of one stride partition the rows, so their counts add up
+    s, d = rmat(11, 16, 9)
+    c, rp, col = orc.pipeline_count(s, d, "degree")
+    n = len(rp) - 1
+    order = np.argsort(np.diff(rp.astype(np.int64)), kind="stable").astype(np.uint32)
+    for stride in (1, 3, 8):
+        parts = [orc.count_row_list(rp, col, order[o::stride]) for o in range(stride)]
+        assert sum(parts) == c
+        assert [orc.count_row_list(rp, col, order[o::stride], 4) for o in range(stride)] == parts
+    per = orc.count_per_vertex(rp, col)
+    rows = np.array([3, 3, n - 1, 0], np.uint32)
+    assert orc.count_row_list(rp, col, rows) == int(per[3] * 2 + per[n - 1] + per[0])
+    assert orc.count_row_list(rp, col, np.array([n, n + 7], np.uint32)) == 0  # ids >= n skipped
+    assert orc.count_row_list(rp, col, np.zeros(0, np.uint32)) == 0

@@ -43,6 +43,8 @@ for s, d in graphs():
         r[name] = orc.count(rp, col)
     r["omp"] = orc.count_omp(rp, col)
     r["strided"] = orc.count_rows_strided(rp, col, 1, 3) + orc.count_rows_strided_omp(rp, col, 0, 2, 4)
+    rows = np.arange(n + 2, dtype=np.uint32)[::-3]                # includes ids >= n
+    r["row_list"] = [orc.count_row_list(rp, col, rows), orc.count_row_list(rp, col, rows, 4)]
     r["per_vertex"] = int(orc.count_per_vertex(rp, col).sum())
     r["hu"] = int(orc.hu_workload(rp, col).sum())
     r["hu_blocks"] = orc.hu_block_starts(orc.hu_workload(rp, col), 7).tolist()
