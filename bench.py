@@ -123,6 +123,26 @@ def cpu_baseline_parallel(rp, col, sample):
             "sample": f"the serial sample above with OpenMP over {threads} threads, {t:.2f} s"}
 
 
+def vs_plain(results, head, traffic):
+    """The headline engine against the four reference methods (Hu, Bisson,
+    Fox, TriCore) and against AUTO's fixed rule, by edges/s; with the HBM
+    fraction from measured traffic when a traffic profile exists."""
+    plain = {k: results[k]["edges_per_s"] for k in ("hu", "bisson", "fox", "tricore")
+             if k in results}
+    if not plain:
+        return None
+    best = max(plain, key=plain.get)
+    out = {"fastest_plain": best,
+           "vs_fastest_plain": results[head]["edges_per_s"] / plain[best]}
+    if "auto" in results and head != "auto":
+        out["vs_auto_fixed_rule"] = results[head]["edges_per_s"] / results["auto"]["edges_per_s"]
+    if traffic:
+        gbps = traffic / (results[head]["ms_kernels"] * 1e-3) / 1e9
+        out["hbm_gbps_measured"] = gbps
+        out["hbm_frac_measured"] = gbps / HBM_PEAK_GBPS
+    return out
+
+
 def host_info():
     model = ""
     try:
@@ -518,6 +538,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "cpu_baseline_parallel": cpu_par,
+        "learned_vs_plain": vs_plain(results, head, traffic),
         "kernels": {k: {kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in r.items()}
                     for k, r in results.items()},
         "end_to_end_s": {"generate": t_gen, "preprocess": t_prep},
