@@ -408,21 +408,35 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
       for (uint32_t i = tid; i < d * RS; i += 64 * NW) s_M[i] = 0;
       if (tid == 0) s_next = NW;
       __syncthreads();
-      // symmetric adjacency of N+(u): bit (i, j) for every arc c_i -> c_j
-      for (uint32_t i = wv; i < d; i += NW) {
-        const uint32_t ci = s_list[i];
-        const uint32_t e1 = rp[ci + 1];
-        for (uint32_t e = rp[ci] + lane; e < e1; e += 64) {
-          const uint32_t x = col[e];
-          uint32_t lo = 0, len = d;
-          while (len > 1) {
-            const uint32_t half = len >> 1;
-            lo = s_list[lo + half] <= x ? lo + half : lo;
-            len -= half;
-          }
-          if (s_list[lo] == x) {
-            atomicOr((unsigned long long*)&s_M[i * RS + (lo >> 6)], 1ull << (lo & 63));
-            atomicOr((unsigned long long*)&s_M[lo * RS + (i >> 6)], 1ull << (i & 63));
+      // symmetric adjacency of N+(u): bit (i, j) for every arc c_i -> c_j.
+      // A wave takes four out-lists per step: when none is longer than 64,
+      // each gets 16 lanes; otherwise the whole wave streams them in turn.
+      auto add_arcs = [&](uint32_t i, uint32_t x) {
+        uint32_t lo = 0, len = d;
+        while (len > 1) {
+          const uint32_t half = len >> 1;
+          lo = s_list[lo + half] <= x ? lo + half : lo;
+          len -= half;
+        }
+        if (s_list[lo] == x) {
+          atomicOr((unsigned long long*)&s_M[i * RS + (lo >> 6)], 1ull << (lo & 63));
+          atomicOr((unsigned long long*)&s_M[lo * RS + (i >> 6)], 1ull << (i & 63));
+        }
+      };
+      for (uint32_t i0 = wv * 4u; i0 < d; i0 += NW * 4u) {
+        const uint32_t gi = i0 + (lane >> 4);
+        uint32_t eb = 0, ee = 0;
+        if (gi < d) {
+          const uint32_t ci = s_list[gi];
+          eb = rp[ci];
+          ee = rp[ci + 1];
+        }
+        if (__ballot(ee - eb > 64u) == 0) {
+          for (uint32_t e = eb + (lane & 15u); e < ee; e += 16u) add_arcs(gi, col[e]);
+        } else {
+          for (uint32_t g = 0; g < 4u && i0 + g < d; ++g) {
+            const uint32_t b = __shfl(eb, 16 * g, 64), e1 = __shfl(ee, 16 * g, 64);
+            for (uint32_t e = b + lane; e < e1; e += 64) add_arcs(i0 + g, col[e]);
           }
         }
       }
