@@ -432,12 +432,12 @@ static int sym_keys(tc_ctx* c, tc_graph* g, DevBuf& keys, DevBuf& tmp, uint64_t*
   TCX_TRY(keys.reserve(8 * std::max<uint64_t>(nk, 1)));
   TCX_TRY(tmp.reserve(8 * std::max<uint64_t>(nk, 1)));
   hipLaunchKernelGGL(k_sym_keys, dim3(grid_for(g->m_raw)), dim3(TB), 0, c->stream,
-                     g->src.as<uint32_t>(), g->dst.as<uint32_t>(), g->m_raw, g->n,
+                     g->src.as<uint32_t>(), g->dst.as<uint32_t>(), g->m_raw, g->n_raw,
                      keys.as<uint64_t>());
   TCX_HIP(hipGetLastError());
   uint64_t* sorted;
   TCX_TRY(sort_u64(c, keys.as<uint64_t>(), tmp.as<uint64_t>(), nk,
-                   32 + bit_width_u32(g->n), &sorted));
+                   32 + bit_width_u32(g->n_raw), &sorted));
   uint64_t* other = sorted == keys.as<uint64_t>() ? tmp.as<uint64_t>() : keys.as<uint64_t>();
   uint64_t nu = 0;
   TCX_TRY(unique_u64(c, sorted, other, nk, &nu));
@@ -445,7 +445,7 @@ static int sym_keys(tc_ctx* c, tc_graph* g, DevBuf& keys, DevBuf& tmp, uint64_t*
     uint64_t last = 0;
     TCX_HIP(hipMemcpyAsync(&last, other + nu - 1, 8, hipMemcpyDeviceToHost, c->stream));
     TCX_HIP(hipStreamSynchronize(c->stream));
-    if ((uint32_t)(last >> 32) == g->n) --nu;
+    if ((uint32_t)(last >> 32) == g->n_raw) --nu;
   }
   *out = other;
   *m2 = nu;
@@ -593,6 +593,7 @@ int graph_clone_records(tc_ctx* c, const tc_graph* g, tc_graph** out) {
     return st;
   }
   h->n = m ? std::max(ms, md) + 1 : 0;
+  h->n_raw = h->n;
   *out = h;
   return TC_OK;
 }
@@ -645,6 +646,7 @@ int tc_graph_create(tc_ctx* c, const uint32_t* src, const uint32_t* dst, uint64_
   } else {
     g->n = n_hint;
   }
+  g->n_raw = g->n;
   *out = g;
   return TC_OK;
 }
@@ -667,12 +669,12 @@ int tc_graph_simplify(tc_ctx* c, tc_graph* g) {
   TCX_TRY(keys.reserve(8 * g->m_raw));
   TCX_TRY(tmp.reserve(8 * g->m_raw));
   hipLaunchKernelGGL(k_simple_keys, dim3(grid_for(g->m_raw)), dim3(TB), 0, c->stream,
-                     g->src.as<uint32_t>(), g->dst.as<uint32_t>(), g->m_raw, g->n,
+                     g->src.as<uint32_t>(), g->dst.as<uint32_t>(), g->m_raw, g->n_raw,
                      keys.as<uint64_t>());
   TCX_HIP(hipGetLastError());
   uint64_t* sorted;
   TCX_TRY(sort_u64(c, keys.as<uint64_t>(), tmp.as<uint64_t>(), g->m_raw,
-                   32 + bit_width_u32(g->n), &sorted));
+                   32 + bit_width_u32(g->n_raw), &sorted));
   uint64_t* other = sorted == keys.as<uint64_t>() ? tmp.as<uint64_t>() : keys.as<uint64_t>();
   uint64_t nu = 0;
   TCX_TRY(unique_u64(c, sorted, other, g->m_raw, &nu));
@@ -680,7 +682,7 @@ int tc_graph_simplify(tc_ctx* c, tc_graph* g) {
     uint64_t last = 0;
     TCX_HIP(hipMemcpyAsync(&last, other + nu - 1, 8, hipMemcpyDeviceToHost, c->stream));
     TCX_HIP(hipStreamSynchronize(c->stream));
-    if ((uint32_t)(last >> 32) == g->n) --nu;
+    if ((uint32_t)(last >> 32) == g->n_raw) --nu;
   }
   hipLaunchKernelGGL(k_unzip, dim3(grid_for(nu)), dim3(TB), 0, c->stream, other, nu,
                      g->src.as<uint32_t>(), g->dst.as<uint32_t>());
@@ -709,7 +711,7 @@ int tc_graph_relabel_aorder_model(tc_ctx* c, tc_graph* g, const tc_bw_model* mod
     return fail(TC_ERR_ARG, "bandwidth model power law is not finite and positive");
   TCX_TRY(check_graph(g));
   if (c != g->ctx) return fail(TC_ERR_ARG, "graph belongs to another context");
-  uint32_t n = g->n;
+  uint32_t n = g->n_raw;
   if (n == 0) return fail(TC_ERR_UNSUPPORTED, "A-order on an empty graph");
   DevBuf deg, rtd, perm;
   TCX_TRY(deg.reserve(4 * (size_t)n));
@@ -734,8 +736,19 @@ int tc_graph_relabel_aorder_model(tc_ctx* c, tc_graph* g, const tc_bw_model* mod
                      g->src.as<uint32_t>(), g->dst.as<uint32_t>(), g->m_raw, deg.as<uint32_t>(),
                      perm.as<uint32_t>());
   TCX_HIP(hipGetLastError());
+  // The relabelled records are what A-order writes to its "reordered" file
+  // (A-order.cpp:206-228); a program reading them back takes the vertex count
+  // as the largest id + 1 (A-direction-seri.cpp:187-193), not the old n.
+  uint32_t new_n = 0;
+  if (g->m_raw) {
+    uint32_t ms = 0, md = 0;
+    TCX_TRY(max_u32(c, g->src.as<uint32_t>(), g->m_raw, &ms));
+    TCX_TRY(max_u32(c, g->dst.as<uint32_t>(), g->m_raw, &md));
+    new_n = std::max(ms, md) + 1;
+  }
   TCX_HIP(hipStreamSynchronize(c->stream));
   if (perm_out) std::copy(h_perm.begin(), h_perm.end(), perm_out);
+  g->n = g->n_raw = new_n;
   g->orient_mode = -1;  // any CSR built before is stale
   return TC_OK;
 }
@@ -809,7 +822,7 @@ int tc_graph_relabel(tc_ctx* c, tc_graph* g, int mode, uint32_t* perm_out) {
   if (perm_out)
     TCX_HIP(hipMemcpyAsync(perm_out, perm.p, 4ull * n, hipMemcpyDeviceToHost, c->stream));
   TCX_HIP(hipStreamSynchronize(c->stream));
-  g->n = n_new;
+  g->n = g->n_raw = n_new;
   g->orient_mode = -1;  // any CSR built before is stale
   return TC_OK;
 }
@@ -821,7 +834,9 @@ int tc_graph_orient(tc_ctx* c, tc_graph* g, int mode, uint32_t flags) {
     return fail(TC_ERR_ARG, "unknown orientation mode %d", mode);
   if ((flags & TC_FLAG_PRUNE_DEG1) && mode != TC_ORIENT_DEGREE)
     return fail(TC_ERR_ARG, "degree<=1 pruning applies to TC_ORIENT_DEGREE only");
-  const uint32_t n = g->n;
+  // Every pass over the raw records uses their id range; only a pruned CSR
+  // has fewer rows.
+  const uint32_t n = g->n_raw;
   const uint64_t m = g->m_raw;
   if (mode == TC_ORIENT_ID || mode == TC_ORIENT_DEGREE || mode == TC_ORIENT_DEGREE_TRICORE) {
     DevBuf deg, pruned, keys, tmp;

@@ -100,7 +100,8 @@ struct SchedCache {
 
 struct tc_graph {
   tc_ctx* ctx = nullptr;
-  uint32_t n = 0;
+  uint32_t n = 0;           // rows of the oriented CSR (after pruning, if any)
+  uint32_t n_raw = 0;       // every raw record id is below this
   uint64_t m_raw = 0;
   tcx::DevBuf src, dst;     // raw records (uint32 each)
   // oriented CSR

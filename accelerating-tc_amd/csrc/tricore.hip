@@ -218,7 +218,7 @@ uint64_t tc_tricore_tricount_gpu(uint64_t part_num, uint32_t* adj, const uint64_
   // An all-device graph built from the split CSR (rows already sorted/unique).
   tc_graph* g = new tc_graph();
   g->ctx = c;
-  g->n = n;
+  g->n = g->n_raw = n;
   g->m_raw = 0;
   int st = g->rowptr.reserve(4 * ((size_t)n + 1));
   if (!st) st = g->col.reserve(4 * std::max<uint64_t>(m, 1));

@@ -2,17 +2,25 @@
 """Benchmark of the triangle-counting hot path on one MI355X.
 
 One step = one exact triangle count (tc_count) over the oriented CSR that is
-already resident in HBM.  Default workload: R-MAT scale 22, edge factor 16,
-(a,b,c,d) = (0.57,0.19,0.19,0.05), seed 22, symmetrised and deduplicated,
+already resident in HBM.  Default workload: R-MAT scale 24, edge factor 16,
+(a,b,c,d) = (0.57,0.19,0.19,0.05), seed 24, symmetrised and deduplicated,
 A-order relabelling + A-direction orientation; every method (Hu, Bisson,
-Fox, TriCore) and the degree-binned AUTO engine are timed, counts must agree.
+Fox, TriCore), the degree-binned AUTO engine and the learned schedule are
+timed, and their counts must agree.  The default run adds two legs of the
+same configuration set on the same line: Barabasi-Albert n = 2^24, m = 8
+(A-direction; learned vs the fastest plain method) and the 4,096-graph batch
+(one hipGraph replay vs one launch per graph).
 
-    python bench.py [--steps K] [--warmup W]
-                    [--workload rmat22|rmat24|er|ba24|batched|kclique_rgg|kclique_rmat20]
+    python bench.py [--steps K] [--warmup W] [--legs ba24,batched]
+                    [--workload rmat24|rmat22|er|ba24|batched|kclique_rgg|kclique_rmat20]
 
 Prints ONE JSON line.  `value` = TC edges/s of the head engine (the learned
 schedule when a trained cost model is present, else AUTO): oriented edges /
-count time, wall clock over K steps.  The `batched` workload counts 4,096
+count time, wall clock over K steps (the plain methods, 5-20x slower, are
+timed over min(K, 5) steps after one warm-up).  `roofline.achieved` = the
+head kernel's own streamed bytes per count, 4(n+1) + 12m+ + 4 sum_v d-(v) d+(v)
+(row offsets, each out-list once, and for every edge u->v the bounds and the
+list of v), over the hipEvent time of the count launch.  The `batched` workload counts 4,096
 small graphs (vertex counts log-uniform in [64, 65536], average degree
 uniform in [4, 64], 1/3 each Erdos-Renyi / R-MAT / random geometric, seed 5)
 with one hipGraph replay per step and reports the one-launch-per-graph
@@ -30,6 +38,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "accelerating-tc_amd"))
 
 HBM_PEAK_GBPS = 8000.0
+METRIC = "TC edges/s + HBM GB/s % peak; learned/batched vs plain HIP, exact counts"
 KERNELS = ["hu", "bisson", "fox", "tricore", "auto", "learned"]
 KERNEL_SYMBOL = {
     "hu": "k_hu<2048, 1024>", "bisson": "k_bisson", "fox": "k_fox<L>",
@@ -71,10 +80,19 @@ def describe(w):
     return d + (", A-order + A-direction" if w["aorder"] else ", A-direction")
 
 
-def cpu_baseline(rp, col, target_s=15.0):
+def kernel_bytes(info):
+    """Bytes the AUTO kernel streams per count: the row offsets, every out-list
+    once (staged in LDS per task), and for every edge u->v the two bounds of
+    row v and the list N+(v): 4(n+1) + 4m+ + 8m+ + 4 sum_v d-(v) d+(v)."""
+    return 4.0 * (info.n + 1) + 12.0 * info.m_oriented + 4.0 * info.sum_in_out
+
+
+def cpu_baseline(rp, col, per_vertex=None, target_s=15.0):
     """Serial oracle counter on a systematic sample of the same oriented CSR:
     every S-th row in out-degree order, so each degree class is represented
-    in proportion (a stride over ids can take or miss the few heavy rows)."""
+    in proportion (a stride over ids can take or miss the few heavy rows).
+    With the GPU's per-vertex counts, the sample's count must equal their sum
+    over the same rows."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as orc
@@ -94,6 +112,9 @@ def cpu_baseline(rp, col, target_s=15.0):
     c1 = orc.count_row_list(rp, col, rows)
     t = time.perf_counter() - t0
     edges = int(deg[rows].sum())
+    if per_vertex is not None and c1 != int(per_vertex[rows].sum()):
+        raise SystemExit(f"oracle sample count {c1} != GPU per-vertex counts "
+                         f"{int(per_vertex[rows].sum())} over the same {len(rows)} rows")
     return {
         "value": edges / t,
         "unit": "edges/s",
@@ -156,17 +177,21 @@ def host_info():
     return {"nproc": os.cpu_count(), "cpu_model": model}
 
 
-def cpu_baseline_batch(graphs, target_s=15.0):
-    """Oracle pipeline (degree orientation + serial count) over every k-th graph."""
+def cpu_baseline_batch(graphs, gpu_counts, target_s=15.0):
+    """Oracle pipeline (degree orientation + serial count) over every k-th
+    graph; each graph's count must equal the GPU's."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
     def run(step):
         edges, t0 = 0, time.perf_counter()
-        for s, d, _ in graphs[::step]:
+        for i in range(0, len(graphs), step):
+            s, d, _ = graphs[i]
             if len(s):
-                _, rp, _ = orc.pipeline_count(s, d, "degree")
+                c, rp, _ = orc.pipeline_count(s, d, "degree")
                 edges += int(rp[-1])
+                if c != int(gpu_counts[i]):
+                    raise SystemExit(f"graph {i}: oracle {c} != GPU {int(gpu_counts[i])}")
         return edges, time.perf_counter() - t0
 
     step = max(1, len(graphs) // 32)
@@ -181,7 +206,9 @@ def cpu_baseline_batch(graphs, target_s=15.0):
     }
 
 
-def main_batched(args, w):
+def run_batched(args, w, steps, warmup, with_cpu):
+    """The batch counted by one hipGraph replay per step, and by one launch
+    per graph; both must give the same per-graph counts."""
     import numpy as np
     import tcamd
     from tcamd import batch as tb
@@ -204,61 +231,76 @@ def main_batched(args, w):
     t_tune = time.perf_counter() - t0
     log(f"[bench] autotune {sched.as_dict()} in {t_tune:.1f}s")
 
-    def timed(fn):
+    def timed(fn, nsteps, nwarm):
         ref, _ = fn(sched)
-        for _ in range(args.warmup):
+        for _ in range(nwarm):
             fn(sched)
         ev = []
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(nsteps):
             c, t = fn(sched)
             ev.append(t["ms_count"])
             if not np.array_equal(c, ref):
                 raise SystemExit("per-graph counts changed between steps")
-        wall = (time.perf_counter() - t0) / args.steps
+        wall = (time.perf_counter() - t0) / nsteps
         return ref, wall, float(np.mean(ev)), t["launches"]
 
-    cb, wall_b, ev_b, _ = timed(b.count)
-    cp, wall_p, ev_p, launches_p = timed(b.count_pergraph)
+    cb, wall_b, ev_b, _ = timed(b.count, steps, warmup)
+    cp, wall_p, ev_p, launches_p = timed(b.count_pergraph, min(steps, 3), min(warmup, 1))
     if not np.array_equal(cb, cp):
         raise SystemExit("batched and per-graph counts disagree")
     log(f"[bench] batched  wall {wall_b*1e3:.3f} ms graph {ev_b:.3f} ms; per-graph wall "
         f"{wall_p*1e3:.3f} ms kernels {ev_p:.3f} ms ({launches_p} launches); "
         f"speedup {wall_p / wall_b:.2f}x")
-    cpu = None if args.no_cpu else cpu_baseline_batch(graphs)
+    cpu = cpu_baseline_batch(graphs, cb) if with_cpu else None
     if cpu:
         log(f"[bench] cpu baseline {cpu['value']:.3e} edges/s ({cpu['sample']})")
-    achieved = info.bytes_alg / (ev_b * 1e-3) / 1e9
+    b.close()
+    ctx.close()
+    return dict(info=info, m=m, graphs=len(graphs), counts=cb, sched=sched.as_dict(),
+                wall_b=wall_b, ev_b=ev_b, wall_p=wall_p, ev_p=ev_p, launches_p=launches_p,
+                cpu=cpu, t_gen=t_gen, t_prep=t_prep, t_tune=t_tune)
+
+
+def batched_desc(w):
+    return (f"batched: {w['count']} graphs, n log-uniform [64, 65536], average degree "
+            f"U[4, 64], 1/3 ER / R-MAT (0.57,0.19,0.19,0.05) / random geometric, seed "
+            f"{w['seed']}, degree orientation per graph")
+
+
+def main_batched(args, w):
+    r = run_batched(args, w, args.steps, args.warmup, not args.no_cpu)
+    info, m = r["info"], r["m"]
+    achieved = info.bytes_alg / (r["ev_b"] * 1e-3) / 1e9
     line = {
-        "metric": "TC edges/s + HBM GB/s % peak; learned/batched vs plain HIP, exact counts",
-        "value": m / wall_b,
+        "metric": METRIC,
+        "value": m / r["wall_b"],
         "unit": "edges/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": wall_b * 1e3,
+        "ms_per_step": r["wall_b"] * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u32", "data": "synthetic",
         "config": {
-            "workload": (f"batched: {w['count']} graphs, n log-uniform [64, 65536], average "
-                         f"degree U[4, 64], 1/3 ER / R-MAT (0.57,0.19,0.19,0.05) / random "
-                         f"geometric, seed {w['seed']}, degree orientation per graph"),
-            "n": info.n, "m": m, "graphs": len(graphs), "total_count": int(cb.sum()),
-            "engine": "batched", "sched": sched.as_dict(), "parallelism": "single GPU",
+            "workload": batched_desc(w),
+            "n": info.n, "m": m, "graphs": r["graphs"], "total_count": int(r["counts"].sum()),
+            "engine": "batched", "sched": r["sched"], "parallelism": "single GPU",
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": "k_batch<2048>",
-            "bytes_alg_per_count": info.bytes_alg,
+            "bytes_per_count": info.bytes_alg,
+            "bytes_rule": "B_alg of the union CSR: 4(n+1) + 8m+ + 4 sum d+^2 + 4 sum d- d+",
         },
-        "cpu_baseline": cpu,
-        "batched_vs_pergraph": wall_p / wall_b,
-        "pergraph": {"ms_wall": wall_p * 1e3, "ms_kernels": ev_p, "launches": launches_p,
-                     "edges_per_s": m / wall_p},
-        "batched": {"ms_wall": wall_b * 1e3, "ms_graph": ev_b, "edges_per_s": m / wall_b},
-        "end_to_end_s": {"generate": t_gen, "preprocess": t_prep, "autotune": t_tune},
+        "cpu_baseline": r["cpu"],
+        "batched_vs_pergraph": r["wall_p"] / r["wall_b"],
+        "pergraph": {"ms_wall": r["wall_p"] * 1e3, "ms_kernels": r["ev_p"],
+                     "launches": r["launches_p"], "edges_per_s": m / r["wall_p"]},
+        "batched": {"ms_wall": r["wall_b"] * 1e3, "ms_graph": r["ev_b"],
+                    "edges_per_s": m / r["wall_b"]},
+        "end_to_end_s": {"generate": r["t_gen"], "preprocess": r["t_prep"],
+                         "autotune": r["t_tune"]},
     }
     print(json.dumps(line), flush=True)
-    b.close()
-    ctx.close()
 
 
 def main_kclique(args, w):
@@ -377,7 +419,7 @@ def main_kclique(args, w):
         log(f"[bench] cpu baseline {cpu['value']:.3e} edges/s ({cpu['sample']})")
     total_wall = sum(v["ms_wall"] for v in per_k.values()) * 1e-3
     line = {
-        "metric": "TC edges/s + HBM GB/s % peak; learned/batched vs plain HIP, exact counts",
+        "metric": METRIC,
         "value": m * len(per_k) / total_wall,
         "unit": "edges/s",
         "n_gpus": 1, "steps": args.steps, "warmup": 1,
@@ -399,13 +441,143 @@ def main_kclique(args, w):
     ctx.close()
 
 
+def run_triangles(args, w, kernels, with_cpu):
+    """One triangle workload: every method timed on the same resident CSR;
+    the counts must agree with each other (and the CPU sample with the
+    GPU's per-vertex counts)."""
+    import numpy as np
+    import tcamd
+
+    t_all = time.perf_counter()
+    src, dst = make_input(tcamd, w)
+    t_gen = time.perf_counter() - t_all
+    log(f"[bench] {describe(w)}: generated {len(src)} records in {t_gen:.2f}s")
+    ctx = tcamd.Context(0)
+    t0 = time.perf_counter()
+    g = ctx.graph(src, dst)
+    del src, dst
+    g.simplify()
+    t_simpl = time.perf_counter() - t0
+    if w["aorder"]:
+        g.relabel_aorder()
+    t_aorder = time.perf_counter() - t0 - t_simpl
+    g.orient({"adir": tcamd.ORIENT_ADIR, "degree": tcamd.ORIENT_DEGREE}[w["orient"]])
+    t_prep = time.perf_counter() - t0
+    info = g.info()
+    m = info.m_oriented
+    log(f"[bench] n={info.n} m={m} max_out={info.max_out_degree} B_alg={info.bytes_alg/1e9:.2f} GB "
+        f"B_kernel={kernel_bytes(info)/1e9:.2f} GB prep {t_prep:.2f}s (simplify {t_simpl:.2f}s, "
+        f"aorder {t_aorder:.2f}s)")
+
+    model = None
+    if "learned" in kernels:
+        from tcamd import learned
+        model = learned.load(ctx) if os.path.exists(learned.DEFAULT_MODEL) else None
+        if model is None:
+            log("[bench] no trained cost model; 'learned' uses the analytic fallback")
+
+    def run(k):
+        return g.count_learned(model) if k == "learned" else g.count(k)
+
+    head = "learned" if "learned" in kernels else ("auto" if "auto" in kernels else kernels[0])
+    results = {}
+    counts = set()
+    for k in kernels:
+        plain = k in ("hu", "bisson", "fox", "tricore")
+        nsteps = min(args.steps, 5) if plain and k != head else args.steps
+        nwarm = min(args.warmup, 1) if plain and k != head else args.warmup
+        r = run(k)  # builds and caches the schedule
+        counts.add(r.count)
+        for _ in range(nwarm):
+            run(k)
+        ev_ms, dom_ms = [], []
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            r = run(k)
+            ev_ms.append(r.timing["ms_count"])
+            dom_ms.append(r.timing["ms_dominant"])
+            counts.add(r.count)
+        wall = (time.perf_counter() - t0) / nsteps
+        ev = float(np.mean(ev_ms))
+        results[k] = dict(count=r.count, steps=nsteps, warmup=nwarm, ms_wall=wall * 1e3,
+                          ms_kernels=ev, ms_dominant=float(np.mean(dom_ms)),
+                          launches=r.timing["launches"], ms_prep=r.timing["ms_prep"],
+                          edges_per_s=m / wall, edges_per_s_kernel=m / (ev * 1e-3),
+                          gbps_alg=info.bytes_alg / (ev * 1e-3) / 1e9)
+        log(f"[bench] {k:8s} count={r.count} wall {wall*1e3:.3f} ms kernels {ev:.3f} ms "
+            f"({results[k]['gbps_alg']:.0f} GB/s alg, {nsteps} steps)")
+    if len(counts) != 1:
+        raise SystemExit(f"kernels disagree on the count: {sorted(counts)}")
+
+    cpu = cpu_par = None
+    if with_cpu:
+        rp, col = g.csr()
+        per = g.count_per_vertex()
+        if int(per.sum()) != results[head]["count"]:
+            raise SystemExit("per-vertex counts do not add up to the count")
+        cpu, sample = cpu_baseline(rp, col, per)
+        log(f"[bench] cpu baseline {cpu['value']:.3e} edges/s ({cpu['sample']}); the sample's "
+            f"count equals the GPU's per-vertex counts over the same rows")
+        cpu_par = cpu_baseline_parallel(rp, col, sample)
+        log(f"[bench] cpu baseline, parallel: {cpu_par['value']:.3e} edges/s ({cpu_par['sample']})")
+        del rp, col, per
+    g.close()
+    ctx.close()
+    return dict(info=info, m=m, head=head, results=results, cpu=cpu, cpu_par=cpu_par,
+                t_gen=t_gen, t_prep=t_prep)
+
+
+def triangle_roofline(info, hr, traffic):
+    kb = kernel_bytes(info)
+    achieved = kb / (hr["ms_kernels"] * 1e-3) / 1e9
+    return {
+        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic, "kernel": "k_auto<2048>",
+        "bytes_per_count": kb,
+        "bytes_rule": "4(n+1) + 12 m+ + 4 sum_v d-(v) d+(v): row offsets, each out-list once, "
+                      "and per edge u->v the bounds and list of v",
+        "bytes_alg_edge_model": info.bytes_alg,
+    }
+
+
+def leg_summary(name, w, r, traffic=None):
+    """The compact record of an extra leg on the headline line."""
+    if w["kind"] == "batched":
+        info, m = r["info"], r["m"]
+        return {"workload": batched_desc(w), "graphs": r["graphs"],
+                "total_count": int(r["counts"].sum()), "m": m,
+                "batched_edges_per_s": m / r["wall_b"], "pergraph_edges_per_s": m / r["wall_p"],
+                "batched_vs_pergraph": r["wall_p"] / r["wall_b"],
+                "ms_batched": r["wall_b"] * 1e3, "ms_pergraph": r["wall_p"] * 1e3,
+                "sched": r["sched"],
+                "gbps_alg": info.bytes_alg / (r["ev_b"] * 1e-3) / 1e9}
+    hr = r["results"][r["head"]]
+    return {"workload": describe(w), "n": r["info"].n, "m": r["m"], "count": hr["count"],
+            "engine": r["head"], "edges_per_s": hr["edges_per_s"], "ms_per_step": hr["ms_wall"],
+            "roofline": triangle_roofline(r["info"], hr, traffic),
+            "learned_vs_plain": vs_plain(r["results"], r["head"], traffic),
+            "kernels": {k: {kk: (round(v, 4) if isinstance(v, float) else v)
+                            for kk, v in x.items()} for k, x in r["results"].items()}}
+
+
+def load_traffic(workload, head):
+    prof = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            return json.load(f).get(head)
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="rmat22", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="rmat24", choices=sorted(WORKLOADS))
     ap.add_argument("--kernels", default=",".join(KERNELS))
+    ap.add_argument("--legs", default=None,
+                    help="extra legs on the same line (default for rmat24: ba24,batched; "
+                         "'' for none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--ks", default="", help="k-clique workloads: comma-separated k values")
     ap.add_argument("--kcl-chunks", type=int, default=1,
@@ -436,84 +608,27 @@ def main():
     if w["kind"] == "kclique":
         return main_kclique(args, w)
 
-    import numpy as np
-    import tcamd
-
     kernels = [k for k in args.kernels.split(",") if k]
-    t_all = time.perf_counter()
-    src, dst = make_input(tcamd, w)
-    t_gen = time.perf_counter() - t_all
-    log(f"[bench] generated {len(src)} records in {t_gen:.2f}s")
-    ctx = tcamd.Context(0)
-    t0 = time.perf_counter()
-    g = ctx.graph(src, dst)
-    g.simplify()
-    t_simpl = time.perf_counter() - t0
-    if w["aorder"]:
-        g.relabel_aorder()
-    t_aorder = time.perf_counter() - t0 - t_simpl
-    g.orient({"adir": tcamd.ORIENT_ADIR, "degree": tcamd.ORIENT_DEGREE}[w["orient"]])
-    t_prep = time.perf_counter() - t0
-    info = g.info()
-    m = info.m_oriented
-    log(f"[bench] n={info.n} m={m} max_out={info.max_out_degree} B_alg={info.bytes_alg/1e9:.2f} GB "
-        f"prep {t_prep:.2f}s (simplify {t_simpl:.2f}s, aorder {t_aorder:.2f}s)")
-
-    model = None
-    if "learned" in kernels:
-        from tcamd import learned
-        model = learned.load(ctx) if os.path.exists(learned.DEFAULT_MODEL) else None
-        if model is None:
-            log("[bench] no trained cost model; 'learned' uses the analytic fallback")
-
-    def run(k):
-        return g.count_learned(model) if k == "learned" else g.count(k)
-
-    results = {}
-    counts = set()
-    for k in kernels:
-        r = run(k)  # builds and caches the schedule
-        counts.add(r.count)
-        for _ in range(args.warmup):
-            run(k)
-        ev_ms, dom_ms = [], []
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            r = run(k)
-            ev_ms.append(r.timing["ms_count"])
-            dom_ms.append(r.timing["ms_dominant"])
-            counts.add(r.count)
-        wall = (time.perf_counter() - t0) / args.steps
-        ev = float(np.mean(ev_ms))
-        results[k] = dict(count=r.count, ms_wall=wall * 1e3, ms_kernels=ev,
-                          ms_dominant=float(np.mean(dom_ms)), launches=r.timing["launches"],
-                          ms_prep=r.timing["ms_prep"],
-                          edges_per_s=m / wall, edges_per_s_kernel=m / (ev * 1e-3),
-                          gbps_alg=info.bytes_alg / (ev * 1e-3) / 1e9)
-        log(f"[bench] {k:8s} count={r.count} wall {wall*1e3:.3f} ms kernels {ev:.3f} ms "
-            f"({results[k]['gbps_alg']:.0f} GB/s alg)")
-    if len(counts) != 1:
-        raise SystemExit(f"kernels disagree on the count: {sorted(counts)}")
-
-    head = "learned" if "learned" in results else ("auto" if "auto" in results else kernels[0])
-    hr = results[head]
-    achieved = info.bytes_alg / (hr["ms_kernels"] * 1e-3) / 1e9
-    traffic = None
-    prof = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
-    if os.path.exists(prof):
-        with open(prof) as f:
-            traffic = json.load(f).get(head)
-
-    cpu = cpu_par = None
-    if not args.no_cpu:
-        rp, col = g.csr()
-        cpu, sample = cpu_baseline(rp, col)
-        log(f"[bench] cpu baseline {cpu['value']:.3e} edges/s ({cpu['sample']})")
-        cpu_par = cpu_baseline_parallel(rp, col, sample)
-        log(f"[bench] cpu baseline, parallel: {cpu_par['value']:.3e} edges/s ({cpu_par['sample']})")
-
+    legs = args.legs if args.legs is not None else ("ba24,batched" if args.workload == "rmat24"
+                                                    else "")
+    legs = [x for x in legs.split(",") if x and x != args.workload]
+    r = run_triangles(args, w, kernels, not args.no_cpu)
+    info, m, head = r["info"], r["m"], r["head"]
+    hr = r["results"][head]
+    traffic = load_traffic(args.workload, head)
+    extra = {}
+    for name in legs:
+        lw = WORKLOADS[name]
+        if lw["kind"] == "batched":
+            lr = run_batched(args, lw, min(args.steps, 10), min(args.warmup, 2), False)
+        elif lw["kind"] in ("rmat", "er", "ba"):
+            lk = [k for k in ("tricore", "auto", "learned") if k in kernels]
+            lr = run_triangles(args, lw, lk, False)
+        else:
+            raise SystemExit(f"leg {name}: only triangle and batched workloads")
+        extra[name] = leg_summary(name, lw, lr, load_traffic(name, lr.get("head", "")))
     line = {
-        "metric": "TC edges/s + HBM GB/s % peak; learned/batched vs plain HIP, exact counts",
+        "metric": METRIC,
         "value": hr["edges_per_s"],
         "unit": "edges/s",
         "n_gpus": 1,
@@ -530,22 +645,16 @@ def main():
             "n": info.n, "m": m, "count": hr["count"], "engine": head,
             "parallelism": "single GPU",
         },
-        "roofline": {
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-            "kernel": KERNEL_SYMBOL[head],
-            "bytes_alg_per_count": info.bytes_alg,
-        },
-        "cpu_baseline": cpu,
-        "cpu_baseline_parallel": cpu_par,
-        "learned_vs_plain": vs_plain(results, head, traffic),
-        "kernels": {k: {kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in r.items()}
-                    for k, r in results.items()},
-        "end_to_end_s": {"generate": t_gen, "preprocess": t_prep},
+        "roofline": triangle_roofline(info, hr, traffic),
+        "cpu_baseline": r["cpu"],
+        "cpu_baseline_parallel": r["cpu_par"],
+        "learned_vs_plain": vs_plain(r["results"], head, traffic),
+        "kernels": {k: {kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in x.items()}
+                    for k, x in r["results"].items()},
+        "legs": extra,
+        "end_to_end_s": {"generate": r["t_gen"], "preprocess": r["t_prep"]},
     }
     print(json.dumps(line), flush=True)
-    g.close()
-    ctx.close()
 
 
 if __name__ == "__main__":

@@ -167,8 +167,12 @@ int tc_graph_destroy(tc_graph* g);
  * dropped, duplicates merged, each edge once as (min,max), sorted. */
 int tc_graph_simplify(tc_ctx* ctx, tc_graph* g);
 /* A-order relabelling of the raw records (vertex_ordering/A-order/A-order.cpp:62-230);
- * the records are rewritten with new ids.  perm (optional, n entries) gets
- * old->new.  TC_ERR_UNSUPPORTED outside the reference's defined range. */
+ * the records are rewritten with new ids.  perm (optional, n entries, n
+ * before the call) gets old->new.  The graph's n becomes the largest new id
+ * + 1 over the records, as reading A-order's "reordered" file back gives
+ * (edge_directing/A-direction-seri.cpp:187-193), so an A-direction peel after
+ * it uses the reference chain's threshold.  TC_ERR_UNSUPPORTED outside the
+ * reference's defined range. */
 int tc_graph_relabel_aorder(tc_ctx* ctx, tc_graph* g, uint32_t* perm);
 /* Bandwidth model behind A-order's bucket scores (A-order.cpp:20-36): the
  * bandwidth of reading a list of len entries is table[len] for len < 10 and

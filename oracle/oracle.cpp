@@ -612,6 +612,25 @@ uint64_t orc_kclique_rows_strided(uint32_t n, const uint32_t* rowptr, const uint
   return total;
 }
 
+// k-cliques whose lowest vertex is rows[i], one count per listed row (ids >= n
+// give 0); OpenMP over the list when threads > 1.  The same depth-first
+// intersection as orc_kclique, restricted to chosen roots.
+void orc_kclique_row_list(uint32_t n, const uint32_t* rowptr, const uint32_t* col, uint32_t k,
+                          const uint32_t* rows, uint64_t nrows, uint64_t* counts, int threads) {
+#pragma omp parallel for num_threads(threads > 1 ? threads : 1) schedule(dynamic, 1)
+  for (long long i = 0; i < (long long)nrows; ++i) {
+    const uint32_t u = rows[i];
+    uint64_t c = 0;
+    if (u < n && k >= 2) {
+      std::vector<uint32_t> S(col + rowptr[u], col + rowptr[u + 1]);
+      if (S.size() + 1 >= k) c = clique_rec(rowptr, col, S, k - 1);
+    } else if (u < n && k == 1) {
+      c = 1;
+    }
+    counts[i] = c;
+  }
+}
+
 void orc_hu_workload(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
                      int64_t* w) {
   for (uint32_t u = 0; u < n; ++u) {  // Hu/tc.cu:22-35

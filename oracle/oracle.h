@@ -147,6 +147,10 @@ uint64_t orc_kclique(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
  * bounded sample for timing; the rows partition the cliques by root). */
 uint64_t orc_kclique_rows_strided(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
                                   uint32_t k, uint32_t start, uint32_t stride);
+/* k-cliques rooted at each listed row (lowest vertex = rows[i]); counts[i]
+ * per row, ids >= n give 0.  OpenMP over the list when threads > 1. */
+void orc_kclique_row_list(uint32_t n, const uint32_t* rowptr, const uint32_t* col, uint32_t k,
+                          const uint32_t* rows, uint64_t nrows, uint64_t* counts, int threads);
 
 /* Hu work model, Hu/tc.cu:22-35: W(u) = (sum_{v in N+(u)} d+(v)) *
  * (unsigned)log2(d+(u)); 0 when d+(u) <= 1.  pinned: closed form. */

@@ -64,6 +64,8 @@ _sig("orc_count_per_vertex", None, [C.c_uint32, _u32p, _u32p, _u64p])
 _sig("orc_kclique", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32])
 _sig("orc_kclique_rows_strided", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32,
                                                C.c_uint32])
+_sig("orc_kclique_row_list", None, [C.c_uint32, _u32p, _u32p, C.c_uint32, _u32p, C.c_uint64,
+                                    _u64p, C.c_int])
 _sig("orc_hu_workload", None, [C.c_uint32, _u32p, _u32p, _i64p])
 for _name in ("orc_relabel_compact", "orc_relabel_first_src", "orc_relabel_degree"):
     _sig(_name, None, [_u32p, _u32p, C.c_uint64, _u32p])
@@ -235,6 +237,17 @@ def kclique_rows_strided(rp, col, k, start, stride) -> int:
     rp = _u32(rp)
     return int(_lib.orc_kclique_rows_strided(len(rp) - 1, rp, _nz(_u32(col), np.uint32), k,
                                              start, stride))
+
+
+def kclique_row_list(rp, col, k, rows, threads=1):
+    """Per-row counts of the k-cliques rooted at each listed row (oracle/oracle.cpp
+    orc_kclique_row_list)."""
+    rp = _u32(rp)
+    rows = _u32(rows)
+    out = np.zeros(max(len(rows), 1), np.uint64)
+    _lib.orc_kclique_row_list(len(rp) - 1, rp, _nz(_u32(col), np.uint32), k,
+                              _nz(rows, np.uint32), len(rows), out, threads)
+    return out[:len(rows)]
 
 
 def hu_workload(rp, col):

@@ -58,3 +58,35 @@ def test_bench_refuses_several_gpus():
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_cpu_baseline_checks_the_gpu_per_vertex_counts(bench):
+    s, d = rmat(10, 16, 4)
+    _, rp, col = orc.pipeline_count(s, d, "degree")
+    per = orc.count_per_vertex(rp, col)
+    cpu, (rows, _, c1) = bench.cpu_baseline(rp, col, per, target_s=0.01)
+    assert c1 == int(per[rows].sum())
+    wrong = per.copy()
+    wrong[rows[len(rows) // 2]] += 1
+    with pytest.raises(SystemExit):
+        bench.cpu_baseline(rp, col, wrong, target_s=0.01)
+
+
+def test_kernel_bytes_rule(bench):
+    # the bytes the AUTO kernel streams: row offsets, each out-list once, and
+    # per edge u->v the two bounds of row v and N+(v)
+    s, d = rmat(9, 8, 2)
+    _, rp, col = orc.pipeline_count(s, d, "degree")
+    n, m = len(rp) - 1, len(col)
+    dp = np.diff(rp.astype(np.int64))
+    want = 4 * (n + 1) + 4 * m
+    for u in range(n):
+        for v in col[rp[u]:rp[u + 1]]:
+            want += 8 + 4 * int(dp[v])
+
+    class Info:
+        pass
+    info = Info()
+    info.n, info.m_oriented = n, m
+    info.sum_in_out = int((dp * np.bincount(col, minlength=n)).sum())
+    assert bench.kernel_bytes(info) == want

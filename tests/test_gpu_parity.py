@@ -145,9 +145,49 @@ def test_aorder_matches_oracle(ctx, case):
     rs, rd = g.edges()
     np.testing.assert_array_equal(rs, perm_o[os_])
     np.testing.assert_array_equal(rd, perm_o[od])
+    # the reference chain: A-direction-seri reads A-order's "reordered"
+    # records and takes nodeNum = largest id + 1 (A-direction-seri.cpp:187-193)
+    n_file = orc.num_nodes(rs, rd)
+    assert g.info().n == n_file
+    srp, scol = orc.build_sym(rs, rd, n_file)
+    prio_o = orc.adir_priority(n_file, srp, scol)
+    os2, od2 = orc.adir_orient(n_file, srp, scol, prio_o)
+    rp_o, col_o = orc.build_csr(os2, od2, n_file)
     g.orient(tcamd.ORIENT_ADIR)
+    np.testing.assert_array_equal(g.vertex_key(), prio_o)
+    rp, col = g.csr()
+    np.testing.assert_array_equal(rp, rp_o)
+    np.testing.assert_array_equal(col, col_o)
     for k in KERNELS:
         assert g.count(k).count == truth(case)
+    g.close()
+
+
+@pytest.mark.parametrize("case", ["er", "rmat12", "ba", "messy", "hub"])
+def test_reorient_after_prune(ctx, case):
+    """Pruning shrinks the CSR's rows, not the raw records' id range: every
+    later orientation of the same graph (and simplify, A-order) sees all ids."""
+    s, d = CASES[case]()
+    expect = truth(case)
+    g = ctx.graph(s, d).orient(tcamd.ORIENT_DEGREE, prune_deg1=True)
+    assert g.count("auto").count == expect
+    for mode in ("id", "adir", "kcore", "degeneracy", "tricore", "degree"):
+        g.orient(ORIENTS[mode])
+        rp_o, col_o = oracle_csr(s, d, mode)
+        rp, col = g.csr()
+        np.testing.assert_array_equal(rp, rp_o)
+        np.testing.assert_array_equal(col, col_o)
+        for k in KERNELS:
+            assert g.count(k).count == expect, (mode, k)
+        g.orient(tcamd.ORIENT_DEGREE, prune_deg1=True)
+    g.simplify()
+    g.orient(tcamd.ORIENT_ADIR)
+    assert g.count("auto").count == expect
+    if case in ("er", "rmat12", "ba"):  # inside A-order's defined range
+        g.orient(tcamd.ORIENT_DEGREE, prune_deg1=True)
+        g.relabel_aorder()
+        g.orient(tcamd.ORIENT_ADIR)
+        assert g.count("auto").count == expect
     g.close()
 
 
