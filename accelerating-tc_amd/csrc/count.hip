@@ -85,6 +85,95 @@ __global__ void k_auto_work(const uint32_t* __restrict__ rp, const uint32_t* __r
   }
 }
 
+// Work-list key of every CSR edge u->v: the task that streams the shorter
+// out-list takes it ((u << 32 | v) when d+(v) <= d+(u), else (v << 32 | u)).
+// Edges that close no triangle become the sentinel (n << 32): v has no
+// out-edge, or the owner u has only v as out-neighbour.
+__global__ void k_wl_keys(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+                          const uint32_t* __restrict__ esrc, uint64_t m, uint32_t n,
+                          uint64_t* __restrict__ key) {
+  for (uint64_t e = gtid(); e < m; e += gstride()) {
+    const uint32_t u = esrc[e], v = col[e];
+    const uint32_t du = rp[u + 1] - rp[u], dv = rp[v + 1] - rp[v];
+    uint64_t k;
+    if (dv == 0 || (dv <= du && du < 2)) k = (uint64_t)n << 32;
+    else if (dv <= du) k = ((uint64_t)u << 32) | v;
+    else k = ((uint64_t)v << 32) | u;
+    key[e] = k;
+  }
+}
+
+// Per task x of the work-list schedule: work = d+(x) + sum of d+(y) over
+// W(x) (as ~work, for a descending sort), its class (0 none: W(x) empty or
+// d+(x) < 2; 1 wave task; 2 block task; 3 block task of the 8192-entry
+// kernel; 4 lane task) and the bytes it streams (task id, row bounds of x in
+// both CSRs, N+(x), and per y: its id, its row bounds and N+(y)).
+__global__ void k_wl_work(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ wrp,
+                          const uint32_t* __restrict__ wcol, uint32_t n, uint32_t wmax,
+                          uint32_t hmin, uint32_t lmax, uint64_t wave_work,
+                          uint64_t* __restrict__ key, uint8_t* __restrict__ cls,
+                          unsigned long long* bytes) {
+  unsigned long long b = 0;
+  for (uint64_t x = gtid(); x < n; x += gstride()) {
+    const uint32_t dx = rp[x + 1] - rp[x];
+    const uint32_t wb = wrp[x], we = wrp[x + 1];
+    uint64_t work = dx, streamed = 0;
+    for (uint32_t i = wb; i < we; ++i) {
+      const uint32_t y = wcol[i];
+      streamed += rp[y + 1] - rp[y];
+    }
+    work += streamed;
+    uint8_t k = 0;
+    if (we > wb && dx >= 2) {
+      if (dx <= lmax && we - wb <= dev::kLaneMaxList) k = 4;
+      else if (dx > hmin) k = 3;
+      else if (dx > wmax || work > wave_work) k = 2;
+      else k = 1;
+      b += 20ull + 4ull * dx + 12ull * (we - wb) + 4ull * streamed;
+    }
+    key[x] = ~work;
+    cls[x] = k;
+  }
+  b = dev::wave_sum(b);
+  if ((threadIdx.x & 63) == 0 && b) atomicAdd(bytes, b);
+}
+
+// rearrangeEdge key (Fox/tc.cu:133-139): the longer of the two lists.
+__global__ void k_fox_rkey(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+                           const uint32_t* __restrict__ esrc, const uint32_t* __restrict__ eids,
+                           uint64_t S, uint32_t* __restrict__ key) {
+  for (uint64_t i = gtid(); i < S; i += gstride()) {
+    const uint32_t e = eids[i], u = esrc[e], v = col[e];
+    const uint32_t a = rp[u + 1] - rp[u], b = rp[v + 1] - rp[v];
+    key[i] = a > b ? a : b;
+  }
+}
+
+// Deals the S sorted edges in groups of G round-robin over B workgroup
+// slots: group g goes to slot g % B as its (g / B)-th group, and slot b's
+// groups are laid out contiguously, slots in order (Fox/tc.cu:164-174 meant
+// this interleave; here it is a permutation for any S).  Only the last group
+// can be short, and it is the last of its slot.
+__global__ void k_fox_deal(const uint32_t* __restrict__ sorted, uint64_t S, uint64_t G,
+                           uint64_t B, uint32_t* __restrict__ out) {
+  const uint64_t ng = (S + G - 1) / G, last = ng - 1, short_by = ng * G - S;
+  for (uint64_t p = gtid(); p < S; p += gstride()) {
+    const uint64_t g = p / G, r = p % G, b = g % B, t = g / B;
+    uint64_t before = G * (b * (ng / B) + (b < ng % B ? b : ng % B));  // groups of slots < b
+    if (last % B < b) before -= short_by;
+    out[before + t * G + r] = sorted[p];
+  }
+}
+
+__global__ void k_low32(const uint64_t* __restrict__ keys, uint64_t m, uint32_t* __restrict__ out) {
+  for (uint64_t i = gtid(); i < m; i += gstride()) out[i] = (uint32_t)keys[i];
+}
+
+__global__ void k_eq_flags(const uint8_t* __restrict__ in, uint64_t n, uint8_t v,
+                           uint8_t* __restrict__ out) {
+  for (uint64_t i = gtid(); i < n; i += gstride()) out[i] = in[i] == v ? 1 : 0;
+}
+
 __global__ void k_gather_u8(const uint8_t* __restrict__ in, const uint32_t* __restrict__ idx,
                             uint32_t n, uint8_t* __restrict__ out) {
   for (uint64_t i = gtid(); i < n; i += gstride()) out[i] = in[idx[i]];
@@ -444,6 +533,7 @@ static int ensure_version(tc_graph* g) {
   if (sc.version != g->sched_version) {
     sc.hu_nblocks = 0;
     sc.auto_ok = false;
+    sc.wl_ok = false;
     sc.rows_ok = false;
     sc.fox_lanes = 0;
     sc.version = g->sched_version;
@@ -549,9 +639,9 @@ static void fox_thresholds(int max_work, uint32_t max_lanes, std::vector<int32_t
   }
 }
 
-static int fox_schedule(tc_ctx* c, tc_graph* g, uint32_t max_lanes) {
+static int fox_schedule(tc_ctx* c, tc_graph* g, uint32_t max_lanes, bool rearrange) {
   SchedCache& sc = g->sc;
-  if (sc.fox_lanes == max_lanes) return TC_OK;
+  if (sc.fox_lanes == max_lanes && sc.fox_rearr == rearrange) return TC_OK;
   const uint64_t m = g->m_or;
   DevBuf w, bin, btmp, ids, itmp;
   TCX_TRY(w.reserve(4 * std::max<uint64_t>(m, 1)));
@@ -589,9 +679,35 @@ static int fox_schedule(tc_ctx* c, tc_graph* g, uint32_t max_lanes) {
   TCX_TRY(sort_pairs_u32_u32(c, bin.as<uint32_t>(), btmp.as<uint32_t>(), ids.as<uint32_t>(),
                              itmp.as<uint32_t>(), m, 3, &sk, &sv));
   TCX_TRY(sc.fox_eids.reserve(4 * std::max<uint64_t>(m, 1)));
-  TCX_HIP(hipMemcpyAsync(sc.fox_eids.p, sv, 4 * m, hipMemcpyDeviceToDevice, c->stream));
+  if (!rearrange) {
+    TCX_HIP(hipMemcpyAsync(sc.fox_eids.p, sv, 4 * m, hipMemcpyDeviceToDevice, c->stream));
+  } else {
+    // rearrangeEdge per bin: sort by max(d+(u), d+(v)), then deal groups of
+    // up to 32 edges round-robin over the bin launch's workgroups
+    uint32_t* ids_in = sv == ids.as<uint32_t>() ? ids.as<uint32_t>() : itmp.as<uint32_t>();
+    uint32_t* spare = ids_in == ids.as<uint32_t>() ? itmp.as<uint32_t>() : ids.as<uint32_t>();
+    for (int j = 0; j < nb; ++j) {
+      const uint64_t lo = sc.fox_off[j], S = sc.fox_off[j + 1] - lo;
+      if (!S) continue;
+      hipLaunchKernelGGL(k_fox_rkey, dim3(grid_for(S)), dim3(TB), 0, c->stream,
+                         g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(), g->esrc.as<uint32_t>(),
+                         ids_in + lo, S, bin.as<uint32_t>());
+      TCX_HIP(hipGetLastError());
+      uint32_t *rk, *rv;
+      TCX_TRY(sort_pairs_u32_u32(c, bin.as<uint32_t>(), btmp.as<uint32_t>(), ids_in + lo,
+                                 spare + lo, S, 32, &rk, &rv));
+      const uint32_t lanes = 1u << j;
+      const uint64_t E = TB / lanes;  // edges per workgroup of k_fox<lanes>
+      const uint64_t G = std::min<uint64_t>(32, E);
+      const uint64_t B = std::max<uint64_t>(1, (S + E - 1) / E);
+      hipLaunchKernelGGL(k_fox_deal, dim3(grid_for(S)), dim3(TB), 0, c->stream, rv, S, G, B,
+                         sc.fox_eids.as<uint32_t>() + lo);
+      TCX_HIP(hipGetLastError());
+    }
+  }
   TCX_HIP(hipStreamSynchronize(c->stream));
   sc.fox_lanes = max_lanes;
+  sc.fox_rearr = rearrange;
   return TC_OK;
 }
 
@@ -648,6 +764,94 @@ static int auto_schedule(tc_ctx* c, tc_graph* g, uint32_t wmax, uint32_t hmin) {
   sc.auto_ok = true;
   sc.auto_wmax = wmax;
   sc.auto_hmin = hmin;
+  return TC_OK;
+}
+
+// Wave tasks whose work exceeds this many list entries run as block tasks
+// (a hub reached by many in-edges can carry millions).
+constexpr uint64_t kWlWaveWork = 1ull << 16;
+
+static int wl_schedule(tc_ctx* c, tc_graph* g, uint32_t wmax, uint32_t hmin, uint32_t lmax) {
+  SchedCache& sc = g->sc;
+  if (sc.wl_ok && sc.wl_wmax == wmax && sc.wl_hmin == hmin && sc.wl_lmax == lmax) return TC_OK;
+  const uint32_t n = g->n;
+  const uint64_t m = g->m_or;
+  const uint32_t* rp = g->rowptr.as<uint32_t>();
+  {
+    DevBuf keys, tmp;
+    TCX_TRY(keys.reserve(8 * std::max<uint64_t>(m, 1)));
+    TCX_TRY(tmp.reserve(8 * std::max<uint64_t>(m, 1)));
+    hipLaunchKernelGGL(k_wl_keys, dim3(grid_for(m)), dim3(TB), 0, c->stream, rp,
+                       g->col.as<uint32_t>(), g->esrc.as<uint32_t>(), m, n, keys.as<uint64_t>());
+    TCX_HIP(hipGetLastError());
+    uint64_t* sorted;
+    TCX_TRY(sort_u64(c, keys.as<uint64_t>(), tmp.as<uint64_t>(), m, 32 + bit_width_u32(n),
+                     &sorted));
+    // the sentinels sort last; the kept entries are those below row n
+    TCX_TRY(sc.wl_rp.reserve(4 * ((size_t)n + 1)));
+    hipLaunchKernelGGL(dev::k_rowptr_search<uint64_t>, dim3(grid_for((uint64_t)n + 1)), dim3(TB), 0,
+                       c->stream, sorted, m, n, sc.wl_rp.as<uint32_t>());
+    TCX_HIP(hipGetLastError());
+    uint32_t kept = 0;
+    TCX_HIP(hipMemcpyAsync(&kept, sc.wl_rp.as<uint32_t>() + n, 4, hipMemcpyDeviceToHost, c->stream));
+    TCX_HIP(hipStreamSynchronize(c->stream));
+    sc.wl_entries = kept;
+    TCX_TRY(sc.wl_col.reserve(4 * std::max<uint64_t>(kept, 1)));
+    if (kept)
+      hipLaunchKernelGGL(k_low32, dim3(grid_for(kept)), dim3(TB), 0, c->stream, sorted,
+                         (uint64_t)kept, sc.wl_col.as<uint32_t>());
+    TCX_HIP(hipGetLastError());
+  }
+  DevBuf key, ktmp, ids, itmp, cls, flags, dflag;
+  TCX_TRY(key.reserve(8 * ((size_t)n + 1)));
+  TCX_TRY(ktmp.reserve(8 * ((size_t)n + 1)));
+  TCX_TRY(ids.reserve(4 * ((size_t)n + 1)));
+  TCX_TRY(itmp.reserve(4 * ((size_t)n + 1)));
+  TCX_TRY(cls.reserve((size_t)n + 1));
+  TCX_TRY(flags.reserve((size_t)n + 1));
+  TCX_TRY(dflag.reserve((size_t)n + 1));
+  TCX_TRY(c->accum.reserve(64));
+  unsigned long long* acc = c->accum.as<unsigned long long>() + 4;
+  TCX_HIP(hipMemsetAsync(acc, 0, 8, c->stream));
+  hipLaunchKernelGGL(k_wl_work, dim3(grid_for(n)), dim3(TB), 0, c->stream, rp,
+                     sc.wl_rp.as<uint32_t>(), sc.wl_col.as<uint32_t>(), n, wmax, hmin, lmax,
+                     kWlWaveWork, key.as<uint64_t>(), cls.as<uint8_t>(), acc);
+  hipLaunchKernelGGL(k_iota, dim3(grid_for(n)), dim3(TB), 0, c->stream, ids.as<uint32_t>(), n);
+  hipLaunchKernelGGL(k_eq_flags, dim3(grid_for(n)), dim3(TB), 0, c->stream, cls.as<uint8_t>(),
+                     (uint64_t)n, (uint8_t)1, flags.as<uint8_t>());
+  TCX_HIP(hipGetLastError());
+  TCX_TRY(sc.wl_small.reserve(4 * ((size_t)n + 1)));
+  TCX_TRY(select_indices(c, flags.as<uint8_t>(), n, sc.wl_small.as<uint32_t>(), &sc.nw_small));
+  hipLaunchKernelGGL(k_eq_flags, dim3(grid_for(n)), dim3(TB), 0, c->stream, cls.as<uint8_t>(),
+                     (uint64_t)n, (uint8_t)4, flags.as<uint8_t>());
+  TCX_HIP(hipGetLastError());
+  TCX_TRY(sc.wl_lane.reserve(4 * ((size_t)n + 1)));
+  TCX_TRY(select_indices(c, flags.as<uint8_t>(), n, sc.wl_lane.as<uint32_t>(), &sc.nw_lane));
+  uint64_t* sk;
+  uint32_t* sv;
+  TCX_TRY(sort_pairs_u64_u32(c, key.as<uint64_t>(), ktmp.as<uint64_t>(), ids.as<uint32_t>(),
+                             itmp.as<uint32_t>(), n, 64, &sk, &sv));
+  TCX_TRY(gather_flags(c, cls.as<uint8_t>(), sv, n, dflag.as<uint8_t>()));
+  TCX_TRY(sc.wl_large.reserve(4 * ((size_t)n + 1)));
+  TCX_TRY(sc.wl_huge.reserve(4 * ((size_t)n + 1)));
+  hipLaunchKernelGGL(k_eq_flags, dim3(grid_for(n)), dim3(TB), 0, c->stream, dflag.as<uint8_t>(),
+                     (uint64_t)n, (uint8_t)2, flags.as<uint8_t>());
+  TCX_HIP(hipGetLastError());
+  TCX_TRY(select_flagged_u32(c, sv, flags.as<uint8_t>(), sc.wl_large.as<uint32_t>(), n,
+                             &sc.nw_large));
+  hipLaunchKernelGGL(k_eq_flags, dim3(grid_for(n)), dim3(TB), 0, c->stream, dflag.as<uint8_t>(),
+                     (uint64_t)n, (uint8_t)3, flags.as<uint8_t>());
+  TCX_HIP(hipGetLastError());
+  TCX_TRY(select_flagged_u32(c, sv, flags.as<uint8_t>(), sc.wl_huge.as<uint32_t>(), n,
+                             &sc.nw_huge));
+  unsigned long long bytes = 0;
+  TCX_HIP(hipMemcpyAsync(&bytes, acc, 8, hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  sc.wl_bytes = (double)bytes;
+  sc.wl_ok = true;
+  sc.wl_wmax = wmax;
+  sc.wl_hmin = hmin;
+  sc.wl_lmax = lmax;
   return TC_OK;
 }
 
@@ -754,7 +958,8 @@ int tc_count(tc_ctx* c, tc_graph* g, const tc_sched* sched_in, uint64_t* count,
       uint32_t L = s.fox_max_lanes ? s.fox_max_lanes : 64;
       if (L != 1 && L != 2 && L != 4 && L != 8 && L != 16 && L != 32 && L != 64)
         return fail(TC_ERR_ARG, "fox_max_lanes must be a power of two <= 64");
-      TCX_TRY(fox_schedule(c, g, L));
+      if (s.fox_rearrange > 1) return fail(TC_ERR_ARG, "fox_rearrange must be 0 or 1");
+      TCX_TRY(fox_schedule(c, g, L, s.fox_rearrange == 1));
       break;
     }
     case TC_KERNEL_AUTO: {
@@ -762,7 +967,13 @@ int tc_count(tc_ctx* c, tc_graph* g, const tc_sched* sched_in, uint64_t* count,
       const uint32_t hmin = s.auto_thresholds[1] ? s.auto_thresholds[1] : 2048u;
       if (wmax > kWaveMax) return fail(TC_ERR_ARG, "auto_thresholds[0] must be <= %u", kWaveMax);
       if (hmin < wmax) return fail(TC_ERR_ARG, "auto_thresholds[1] must be >= auto_thresholds[0]");
-      TCX_TRY(auto_schedule(c, g, wmax, hmin));
+      if (s.auto_thresholds[2] > TC_AUTO_PUSH)
+        return fail(TC_ERR_ARG, "auto_thresholds[2] must be TC_AUTO_WORKLIST or TC_AUTO_PUSH");
+      if (s.auto_thresholds[3] > kLaneMaxDeg)
+        return fail(TC_ERR_ARG, "auto_thresholds[3] must be <= %u", kLaneMaxDeg);
+      const uint32_t lmax = s.auto_thresholds[3] ? s.auto_thresholds[3] : kLaneMaxDeg;
+      if (s.auto_thresholds[2] == TC_AUTO_PUSH) TCX_TRY(auto_schedule(c, g, wmax, hmin));
+      else TCX_TRY(wl_schedule(c, g, wmax, hmin, lmax));
       break;
     }
     default:
@@ -856,8 +1067,37 @@ int tc_count(tc_ctx* c, tc_graph* g, const tc_sched* sched_in, uint64_t* count,
       const SchedCache& sc = g->sc;
       TCX_TRY(c->scratch.reserve(64));
       uint32_t* cursor = c->scratch.as<uint32_t>();
-      TCX_HIP(hipMemsetAsync(cursor, 0, 4 * sizeof(uint32_t), c->stream));
+      TCX_HIP(hipMemsetAsync(cursor, 0, 8 * sizeof(uint32_t), c->stream));
       const uint32_t G = s.grid_size ? s.grid_size : (uint32_t)c->num_cus * 8u;
+      if (s.auto_thresholds[2] != TC_AUTO_PUSH) {
+        // cursors: [0] block, [1] wave, [2] lane tasks; [4..6] the 8192-entry launch
+        const uint32_t* wrp = sc.wl_rp.as<uint32_t>();
+        const uint32_t* wcol = sc.wl_col.as<uint32_t>();
+        if (sc.nw_large || sc.nw_small || sc.nw_lane) {
+          TCX_TRY(lm.mark());
+          if (sc.nw_small <= (uint64_t)kStaticWaves * G * (TB / 64))
+            hipLaunchKernelGGL((k_auto_wl<2048, true>), dim3(G), dim3(TB), 0, c->stream, rp, col,
+                               wrp, wcol, sc.wl_large.as<uint32_t>(), (uint32_t)sc.nw_large,
+                               sc.wl_small.as<uint32_t>(), (uint32_t)sc.nw_small,
+                               sc.wl_lane.as<uint32_t>(), (uint32_t)sc.nw_lane, cursor, acc);
+          else
+            hipLaunchKernelGGL((k_auto_wl<2048>), dim3(G), dim3(TB), 0, c->stream, rp, col, wrp,
+                               wcol, sc.wl_large.as<uint32_t>(), (uint32_t)sc.nw_large,
+                               sc.wl_small.as<uint32_t>(), (uint32_t)sc.nw_small,
+                               sc.wl_lane.as<uint32_t>(), (uint32_t)sc.nw_lane, cursor, acc);
+          tm.launches++;
+        }
+        if (sc.nw_huge) {
+          TCX_TRY(lm.mark());
+          hipLaunchKernelGGL((k_auto_wl<8192>),
+                             dim3(std::min<uint64_t>((uint64_t)c->num_cus * 4, sc.nw_huge)),
+                             dim3(TB), 0, c->stream, rp, col, wrp, wcol, sc.wl_huge.as<uint32_t>(),
+                             (uint32_t)sc.nw_huge, (const uint32_t*)nullptr, 0u,
+                             (const uint32_t*)nullptr, 0u, cursor + 4, acc);
+          tm.launches++;
+        }
+        break;
+      }
       if (sc.n_large || sc.n_small) {
         TCX_TRY(lm.mark());
         if (sc.n_small <= (uint64_t)kStaticWaves * G * (TB / 64))
@@ -888,6 +1128,32 @@ int tc_count(tc_ctx* c, tc_graph* g, const tc_sched* sched_in, uint64_t* count,
   *count = h;
   tm.ms_prep = prep.ms;
   if (timing) *timing = tm;
+  return TC_OK;
+}
+
+int tc_auto_stream_bytes(tc_ctx* c, tc_graph* g, const tc_sched* sched, double* bytes) {
+  TCX_TRY(check_ready(c, g));
+  if (!bytes) return fail(TC_ERR_ARG, "NULL output");
+  tc_sched s;
+  memset(&s, 0, sizeof(s));
+  if (sched) s = *sched;
+  const uint32_t wmax = s.auto_thresholds[0] ? s.auto_thresholds[0] : 64u;
+  const uint32_t hmin = s.auto_thresholds[1] ? s.auto_thresholds[1] : 2048u;
+  if (wmax > kWaveMax || hmin < wmax || s.auto_thresholds[2] > TC_AUTO_PUSH ||
+      s.auto_thresholds[3] > kLaneMaxDeg)
+    return fail(TC_ERR_ARG, "bad AUTO thresholds");
+  const uint32_t lmax = s.auto_thresholds[3] ? s.auto_thresholds[3] : kLaneMaxDeg;
+  if (s.auto_thresholds[2] == TC_AUTO_PUSH) {
+    *bytes = 4.0 * ((double)g->n + 1) + 12.0 * (double)g->m_or + 4.0 * (double)g->sum_in_out;
+    return TC_OK;
+  }
+  ensure_version(g);
+  if (g->m_or == 0 || g->n == 0) {
+    *bytes = 0;
+    return TC_OK;
+  }
+  TCX_TRY(wl_schedule(c, g, wmax, hmin, lmax));
+  *bytes = g->sc.wl_bytes;
   return TC_OK;
 }
 
@@ -955,7 +1221,7 @@ int tc_fox_bin_sizes(tc_ctx* c, tc_graph* g, uint32_t max_lanes, uint64_t* count
     for (uint32_t j = 0; j < nb; ++j) count[j] = 0;
     return TC_OK;
   }
-  TCX_TRY(fox_schedule(c, g, max_lanes));
+  TCX_TRY(fox_schedule(c, g, max_lanes, g->sc.fox_lanes == max_lanes && g->sc.fox_rearr));
   for (uint32_t j = 0; j < nb; ++j) count[j] = g->sc.fox_off[j + 1] - g->sc.fox_off[j];
   return TC_OK;
 }

@@ -30,6 +30,9 @@ constexpr uint32_t kStaticWaves = 16;   // wave tasks per wave up to which they 
 constexpr uint32_t kQueue = 320;        // per-wave survivor queue (< 64 + 4 * 64 held)
 constexpr uint32_t kFlatMin = 64;       // lists this long or longer are streamed whole
 constexpr uint32_t kFilterMin = 1;      // wave variant: bitmap for every d+(u) above this
+constexpr uint32_t kLaneMaxDeg = 8;     // lane tasks: d+(x) up to this ...
+constexpr uint32_t kLaneMaxList = 16;   // ... and at most this many work-list entries
+constexpr uint32_t kLaneClaim = 256;    // lane tasks per wave claim (4 per lane)
 
 __device__ inline uint64_t gtid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ inline uint64_t gstride() { return (uint64_t)gridDim.x * blockDim.x; }
@@ -314,10 +317,14 @@ struct WaveSlot {
 // takes the j-th out-neighbour of each 64-neighbour group.  The bitmap is
 // built for d+(u) > FMIN and its words are cleared again at the end (below
 // that, every entry goes straight to the search).
-template <uint32_t FMIN = kFilterMin, int U = 2>
+// With WL, the lists streamed against N+(u) are N+(y) for y in the work list
+// wl[0, wn) (global memory) instead of N+(v) for v in N+(u).
+template <uint32_t FMIN = kFilterMin, int U = 2, bool WL = false>
 __device__ inline uint32_t wave_vertex(const uint32_t* __restrict__ rp,
                                        const uint32_t* __restrict__ col, uint32_t u,
-                                       uint32_t lane, const WaveSlot& W) {
+                                       uint32_t lane, const WaveSlot& W,
+                                       const uint32_t* __restrict__ wl = nullptr,
+                                       uint32_t wn = 0) {
   const uint32_t ub = rp[u], du = rp[u + 1] - ub;
   const bool filt = du > FMIN;
   for (uint32_t i = lane; i < du; i += 64) {
@@ -328,10 +335,11 @@ __device__ inline uint32_t wave_vertex(const uint32_t* __restrict__ rp,
   wave_sync();
   Probe P{W.L, du, W.bm, W.shift, W.Q, 0u, filt, false};
   uint32_t c = 0;
-  for (uint32_t j0 = 0; j0 < du; j0 += 64) {
+  const uint32_t nl = WL ? wn : du;
+  for (uint32_t j0 = 0; j0 < nl; j0 += 64) {
     uint32_t mvb = 0, mve = 0;
-    if (j0 + lane < du) {
-      const uint32_t v = W.L[j0 + lane];
+    if (j0 + lane < nl) {
+      const uint32_t v = WL ? wl[j0 + lane] : W.L[j0 + lane];
       mvb = rp[v];
       mve = rp[v + 1];
     }
@@ -375,11 +383,13 @@ __device__ inline void clear_bitmap(BlockLds<CAP>& S) {
 // into LDS (when it fits; longer lists are searched in global memory) and
 // hashed into the bitmap; wave w takes the out-neighbours w, w+4, w+8, ...
 // 64 at a time.  Every thread must call it; it ends with the bitmap clear
-// and a barrier.
-template <int CAP, int U = 2>
+// and a barrier.  With WL, the lists streamed are N+(y) for y in the work
+// list wl[0, wn) (global memory), wave w taking entries w, w+4, w+8, ...
+template <int CAP, int U = 2, bool WL = false>
 __device__ inline uint32_t block_vertex(const uint32_t* __restrict__ rp,
                                         const uint32_t* __restrict__ col, uint32_t u,
-                                        BlockLds<CAP>& S) {
+                                        BlockLds<CAP>& S, const uint32_t* __restrict__ wl = nullptr,
+                                        uint32_t wn = 0) {
   constexpr uint32_t shift = BlockLds<CAP>::kBlockShift;
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const uint32_t ub = rp[u], du = rp[u + 1] - ub;
@@ -393,11 +403,12 @@ __device__ inline uint32_t block_vertex(const uint32_t* __restrict__ rp,
   const uint32_t* a = in_lds ? S.u : col + ub;
   Probe P{a, du, S.bm, shift, S.q[wv], 0u, true, !in_lds};
   uint32_t c = 0;
-  const uint32_t mine = du > wv ? (du - wv + 3) / 4 : 0;
+  const uint32_t nl = WL ? wn : du;
+  const uint32_t mine = nl > wv ? (nl - wv + 3) / 4 : 0;
   for (uint32_t g0 = 0; g0 < mine; g0 += 64) {
     uint32_t vb = 0, ve = 0;
     if (g0 + lane < mine) {
-      const uint32_t v = a[4 * (g0 + lane) + wv];
+      const uint32_t v = WL ? wl[4 * (g0 + lane) + wv] : a[4 * (g0 + lane) + wv];
       vb = rp[v];
       ve = rp[v + 1];
     }
@@ -450,6 +461,98 @@ __global__ __launch_bounds__(TB) void k_auto(const uint32_t* __restrict__ rp,
     if (i0 >= nwav) break;
     const uint32_t i1 = min(nwav, i0 + kWaveChunk);
     for (uint32_t i = i0; i < i1; ++i) cnt += wave_vertex(rp, col, wav[i], lane, W);
+  }
+  flush_count(cnt, total);
+}
+
+// Lane task: one lane counts task x of the work-list schedule when
+// d+(x) <= kLaneMaxDeg.  N+(x) is held in registers; every list N+(y), y in
+// W(x), has at most d+(x) entries (the work-list rule streams the shorter
+// list), and each of its entries is compared with all of N+(x).  Unused
+// register slots hold kEmpty and unused list slots kEmpty - 1, which match
+// nothing (vertex ids are below 2^31 - 1).
+__device__ inline uint32_t lane_task(const uint32_t* __restrict__ rp,
+                                     const uint32_t* __restrict__ col,
+                                     const uint32_t* __restrict__ wrp,
+                                     const uint32_t* __restrict__ wcol, uint32_t x) {
+  const uint32_t b = rp[x], dx = rp[x + 1] - b;
+  uint32_t s[kLaneMaxDeg];
+#pragma unroll
+  for (uint32_t k = 0; k < kLaneMaxDeg; ++k) s[k] = k < dx ? col[b + k] : kEmpty;
+  const uint32_t wb = wrp[x], we = wrp[x + 1];
+  uint32_t c = 0;
+  for (uint32_t i = wb; i < we; ++i) {
+    const uint32_t y = wcol[i];
+    const uint32_t yb = rp[y], dy = rp[y + 1] - yb;
+#pragma unroll
+    for (uint32_t k = 0; k < kLaneMaxDeg; ++k) {
+      const uint32_t e = k < dy ? col[yb + k] : kEmpty - 1;
+#pragma unroll
+      for (uint32_t j = 0; j < kLaneMaxDeg; ++j) c += e == s[j] ? 1u : 0u;
+    }
+  }
+  return c;
+}
+
+// The AUTO count kernel over work lists (the default schedule): task x
+// stages N+(x) as k_auto does and streams N+(y) for y in W(x) =
+// wcol[wrp[x], wrp[x+1]).  Each oriented edge u->v sits in exactly one work
+// list, at the endpoint whose task streams the shorter out-list (u when
+// d+(v) <= d+(u), else v), so the sum over tasks of |N+(x) & N+(y)| is the
+// triangle count; edges into a vertex with no out-edge are left out.
+// Phase 3 counts the lane tasks (`lan`), kLaneClaim per wave claim from
+// cursor[2].
+template <int CAP, bool STATIC_WAVES = false>
+__global__ __launch_bounds__(TB) void k_auto_wl(const uint32_t* __restrict__ rp,
+                                                const uint32_t* __restrict__ col,
+                                                const uint32_t* __restrict__ wrp,
+                                                const uint32_t* __restrict__ wcol,
+                                                const uint32_t* __restrict__ blk, uint32_t nblk,
+                                                const uint32_t* __restrict__ wav, uint32_t nwav,
+                                                const uint32_t* __restrict__ lan, uint32_t nlan,
+                                                uint32_t* cursor, unsigned long long* total) {
+  __shared__ BlockLds<CAP> S;
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  clear_bitmap(S);
+  unsigned long long cnt = 0;
+  for (;;) {
+    if (tid == 0) S.t = atomicAdd(cursor, 1u);
+    __syncthreads();
+    const uint32_t t = S.t;
+    if (t >= nblk) break;
+    const uint32_t x = blk[t];
+    const uint32_t wb = wrp[x];
+    cnt += block_vertex<CAP, 2, true>(rp, col, x, S, wcol + wb, wrp[x + 1] - wb);
+  }
+  const WaveSlot W = wave_slot(S, wv);
+  if constexpr (STATIC_WAVES) {
+    const uint32_t nw = gridDim.x * (TB / 64);
+    for (uint32_t i = blockIdx.x * (TB / 64) + wv; i < nwav; i += nw) {
+      const uint32_t x = wav[i];
+      const uint32_t wb = wrp[x];
+      cnt += wave_vertex<kFilterMin, 2, true>(rp, col, x, lane, W, wcol + wb, wrp[x + 1] - wb);
+    }
+  } else {
+    for (;;) {
+      uint32_t i0 = 0;
+      if (lane == 0) i0 = atomicAdd(cursor + 1, kWaveChunk);
+      i0 = __shfl(i0, 0, 64);
+      if (i0 >= nwav) break;
+      const uint32_t i1 = min(nwav, i0 + kWaveChunk);
+      for (uint32_t i = i0; i < i1; ++i) {
+        const uint32_t x = wav[i];
+        const uint32_t wb = wrp[x];
+        cnt += wave_vertex<kFilterMin, 2, true>(rp, col, x, lane, W, wcol + wb, wrp[x + 1] - wb);
+      }
+    }
+  }
+  for (;;) {
+    uint32_t i0 = 0;
+    if (lane == 0) i0 = atomicAdd(cursor + 2, kLaneClaim);
+    i0 = __shfl(i0, 0, 64);
+    if (i0 >= nlan) break;
+    for (uint32_t i = i0 + lane; i < min(nlan, i0 + kLaneClaim); i += 64)
+      cnt += lane_task(rp, col, wrp, wcol, lan[i]);
   }
   flush_count(cnt, total);
 }

@@ -9,6 +9,8 @@
 //   vkey       : int32[n]        A-direction priority or core number
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstring>
 
 #include "engine.h"
 #include "internal.h"
@@ -575,15 +577,18 @@ static int kcore_peel(tc_ctx* c, uint32_t n, const uint32_t* rp, const uint32_t*
 
 int graph_clone_records(tc_ctx* c, const tc_graph* g, tc_graph** out) {
   *out = nullptr;
-  const uint64_t m = g->m_raw;
+  const bool from_csr = g->csr_only;
+  const uint64_t m = from_csr ? (g->orient_mode >= 0 ? g->m_or : 0) : g->m_raw;
+  const void* ps = from_csr ? g->esrc.p : g->src.p;
+  const void* pd = from_csr ? g->col.p : g->dst.p;
   tc_graph* h = new tc_graph();
   h->ctx = c;
   h->m_raw = m;
   int st = h->src.reserve(4 * std::max<uint64_t>(m, 1));
   if (st == TC_OK) st = h->dst.reserve(4 * std::max<uint64_t>(m, 1));
   if (st == TC_OK && m &&
-      (hipMemcpyAsync(h->src.p, g->src.p, 4 * m, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
-       hipMemcpyAsync(h->dst.p, g->dst.p, 4 * m, hipMemcpyDeviceToDevice, c->stream) != hipSuccess))
+      (hipMemcpyAsync(h->src.p, ps, 4 * m, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+       hipMemcpyAsync(h->dst.p, pd, 4 * m, hipMemcpyDeviceToDevice, c->stream) != hipSuccess))
     st = fail(TC_ERR_HIP, "record copy failed");
   uint32_t ms = 0, md = 0;
   if (st == TC_OK && m) st = max_u32(c, h->src.as<uint32_t>(), m, &ms);
@@ -605,6 +610,14 @@ using namespace tcx;
 static int check_graph(tc_graph* g) {
   if (!g || !g->ctx) return fail(TC_ERR_ARG, "NULL graph");
   TCX_HIP(hipSetDevice(g->ctx->device));
+  return TC_OK;
+}
+
+// Passes over the raw records need them: a graph loaded from a CSR has none.
+static int check_records(tc_graph* g) {
+  TCX_TRY(check_graph(g));
+  if (g->csr_only)
+    return fail(TC_ERR_STATE, "the graph was built from an oriented CSR and holds no raw records");
   return TC_OK;
 }
 
@@ -662,7 +675,7 @@ int tc_graph_destroy(tc_graph* g) {
 }
 
 int tc_graph_simplify(tc_ctx* c, tc_graph* g) {
-  TCX_TRY(check_graph(g));
+  TCX_TRY(check_records(g));
   if (c != g->ctx) return fail(TC_ERR_ARG, "graph belongs to another context");
   if (g->m_raw == 0) return TC_OK;
   DevBuf keys, tmp;
@@ -709,7 +722,7 @@ int tc_graph_relabel_aorder_model(tc_ctx* c, tc_graph* g, const tc_bw_model* mod
   if (!(std::isfinite(model->a) && model->a > 0 && std::isfinite(model->b) &&
         std::isfinite(model->lambda_tail) && model->lambda_tail >= 0))
     return fail(TC_ERR_ARG, "bandwidth model power law is not finite and positive");
-  TCX_TRY(check_graph(g));
+  TCX_TRY(check_records(g));
   if (c != g->ctx) return fail(TC_ERR_ARG, "graph belongs to another context");
   uint32_t n = g->n_raw;
   if (n == 0) return fail(TC_ERR_UNSUPPORTED, "A-order on an empty graph");
@@ -754,7 +767,7 @@ int tc_graph_relabel_aorder_model(tc_ctx* c, tc_graph* g, const tc_bw_model* mod
 }
 
 int tc_graph_relabel(tc_ctx* c, tc_graph* g, int mode, uint32_t* perm_out) {
-  TCX_TRY(check_graph(g));
+  TCX_TRY(check_records(g));
   if (c != g->ctx) return fail(TC_ERR_ARG, "graph belongs to another context");
   if (mode < TC_RELABEL_COMPACT || mode > TC_RELABEL_DEGREE)
     return fail(TC_ERR_ARG, "unknown relabel mode %d", mode);
@@ -828,7 +841,7 @@ int tc_graph_relabel(tc_ctx* c, tc_graph* g, int mode, uint32_t* perm_out) {
 }
 
 int tc_graph_orient(tc_ctx* c, tc_graph* g, int mode, uint32_t flags) {
-  TCX_TRY(check_graph(g));
+  TCX_TRY(check_records(g));
   if (c != g->ctx) return fail(TC_ERR_ARG, "graph belongs to another context");
   if (mode < TC_ORIENT_ID || mode > TC_ORIENT_DEGENERACY)
     return fail(TC_ERR_ARG, "unknown orientation mode %d", mode);
@@ -909,6 +922,127 @@ int tc_graph_orient(tc_ctx* c, tc_graph* g, int mode, uint32_t flags) {
   return TC_OK;
 }
 
+// Stage file of an oriented CSR: "TCAMDCSR", uint32 version 1, uint32 n,
+// uint64 m, int32 orientation, uint32 has_key, rowptr[n+1], col[m] (uint32),
+// then key[n] (int32) when has_key; little endian.
+namespace {
+struct CsrFileHeader {
+  char magic[8];
+  uint32_t version, n;
+  uint64_t m;
+  int32_t orient_mode;
+  uint32_t has_key;
+};
+static_assert(sizeof(CsrFileHeader) == 32, "stage file header is 32 bytes");
+constexpr char kCsrMagic[8] = {'T', 'C', 'A', 'M', 'D', 'C', 'S', 'R'};
+
+bool has_vertex_key(int mode) {
+  return mode == TC_ORIENT_ADIR || mode == TC_ORIENT_KCORE || mode == TC_ORIENT_DEGENERACY;
+}
+}  // namespace
+
+int tc_graph_save_csr(const tc_graph* g, const char* path) {
+  if (!g || !g->ctx || !path) return fail(TC_ERR_ARG, "NULL argument");
+  if (g->orient_mode < 0) return fail(TC_ERR_STATE, "graph not oriented");
+  const uint32_t n = g->n;
+  const uint64_t m = g->m_or;
+  std::vector<uint32_t> rp((size_t)n + 1), col(std::max<uint64_t>(m, 1));
+  std::vector<int32_t> key;
+  const bool wk = has_vertex_key(g->orient_mode);
+  TCX_TRY(tc_graph_get_csr(g, rp.data(), col.data()));
+  if (wk && n) {
+    key.resize(n);
+    TCX_TRY(tc_graph_get_vertex_key(g, key.data()));
+  }
+  CsrFileHeader h;
+  memcpy(h.magic, kCsrMagic, 8);
+  h.version = 1;
+  h.n = n;
+  h.m = m;
+  h.orient_mode = g->orient_mode;
+  h.has_key = wk ? 1u : 0u;
+  FILE* f = fopen(path, "wb");
+  if (!f) return fail(TC_ERR_IO, "cannot open %s for writing", path);
+  bool ok = fwrite(&h, sizeof(h), 1, f) == 1 &&
+            fwrite(rp.data(), 4, (size_t)n + 1, f) == (size_t)n + 1 &&
+            (m == 0 || fwrite(col.data(), 4, m, f) == m) &&
+            (!wk || n == 0 || fwrite(key.data(), 4, n, f) == n);
+  ok = (fclose(f) == 0) && ok;
+  if (!ok) return fail(TC_ERR_IO, "write to %s failed", path);
+  return TC_OK;
+}
+
+int tc_graph_load_csr(tc_ctx* c, const char* path, tc_graph** out) {
+  if (!c || !path || !out) return fail(TC_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(TC_ERR_IO, "cannot open %s", path);
+  CsrFileHeader h;
+  if (fread(&h, sizeof(h), 1, f) != 1 || memcmp(h.magic, kCsrMagic, 8) != 0 || h.version != 1 ||
+      h.orient_mode < TC_ORIENT_ID || h.orient_mode > TC_ORIENT_DEGENERACY ||
+      h.has_key != (has_vertex_key(h.orient_mode) ? 1u : 0u) || h.n > 0x7FFFFFFEu ||
+      h.m >= 0xFFFFFFFFull) {
+    fclose(f);
+    return fail(TC_ERR_FORMAT, "%s is not an oriented-CSR stage file", path);
+  }
+  const uint32_t n = h.n;
+  const uint64_t m = h.m;
+  std::vector<uint32_t> rp((size_t)n + 1), col(std::max<uint64_t>(m, 1));
+  std::vector<int32_t> key(h.has_key ? std::max<uint32_t>(n, 1) : 0);
+  bool ok = fread(rp.data(), 4, (size_t)n + 1, f) == (size_t)n + 1 &&
+            (m == 0 || fread(col.data(), 4, m, f) == m) &&
+            (!h.has_key || n == 0 || fread(key.data(), 4, n, f) == n);
+  char extra;
+  ok = ok && fread(&extra, 1, 1, f) == 0;  // nothing after the arrays
+  fclose(f);
+  if (!ok) return fail(TC_ERR_FORMAT, "%s is truncated or has trailing bytes", path);
+  // the kernels index rp[col[e] + 1]: every structural property is checked
+  if (rp[0] != 0 || rp[n] != m) return fail(TC_ERR_FORMAT, "%s: row offsets do not span m", path);
+  std::vector<uint32_t> esrc(std::max<uint64_t>(m, 1));
+  for (uint32_t v = 0; v < n; ++v) {
+    if (rp[v + 1] < rp[v]) return fail(TC_ERR_FORMAT, "%s: row offsets decrease at %u", path, v);
+    for (uint32_t e = rp[v]; e < rp[v + 1]; ++e) {
+      if (col[e] >= n || col[e] == v || (e > rp[v] && col[e] <= col[e - 1]))
+        return fail(TC_ERR_FORMAT, "%s: row %u is not ascending, loop-free and in range", path, v);
+      esrc[e] = v;
+    }
+  }
+  TCX_HIP(hipSetDevice(c->device));
+  tc_graph* g = new tc_graph();
+  g->ctx = c;
+  g->n = g->n_raw = n;
+  g->csr_only = true;
+  g->m_or = m;
+  int st = g->rowptr.reserve(4 * ((size_t)n + 1));
+  if (st == TC_OK) st = g->col.reserve(4 * std::max<uint64_t>(m, 1));
+  if (st == TC_OK) st = g->esrc.reserve(4 * std::max<uint64_t>(m, 1));
+  if (st == TC_OK && h.has_key) st = g->vkey.reserve(4 * ((size_t)n + 1));
+  if (st == TC_OK &&
+      (hipMemcpyAsync(g->rowptr.p, rp.data(), 4 * ((size_t)n + 1), hipMemcpyHostToDevice,
+                      c->stream) != hipSuccess ||
+       (m && hipMemcpyAsync(g->col.p, col.data(), 4 * m, hipMemcpyHostToDevice, c->stream) !=
+                 hipSuccess) ||
+       (m && hipMemcpyAsync(g->esrc.p, esrc.data(), 4 * m, hipMemcpyHostToDevice, c->stream) !=
+                 hipSuccess) ||
+       (h.has_key && n &&
+        hipMemcpyAsync(g->vkey.p, key.data(), 4 * (size_t)n, hipMemcpyHostToDevice, c->stream) !=
+            hipSuccess)))
+    st = fail(TC_ERR_HIP, "stage file upload failed");
+  if (st == TC_OK && hipStreamSynchronize(c->stream) != hipSuccess)
+    st = fail(TC_ERR_HIP, "stage file upload failed");
+  if (st == TC_OK) {
+    g->orient_mode = h.orient_mode;
+    g->sched_version++;
+    st = compute_csr_stats(c, g);
+  }
+  if (st != TC_OK) {
+    delete g;
+    return st;
+  }
+  *out = g;
+  return TC_OK;
+}
+
 int tc_graph_info_get(const tc_graph* g, tc_graph_info* info) {
   if (!g || !info) return fail(TC_ERR_ARG, "NULL argument");
   info->n = g->n;
@@ -916,7 +1050,7 @@ int tc_graph_info_get(const tc_graph* g, tc_graph_info* info) {
   info->m_raw = g->m_raw;
   info->m_oriented = g->orient_mode >= 0 ? g->m_or : 0;
   info->orient_mode = g->orient_mode;
-  info->pad0 = 0;
+  info->n_raw = g->n_raw;
   info->sum_out_sq = g->orient_mode >= 0 ? g->sum_out_sq : 0;
   info->sum_in_out = g->orient_mode >= 0 ? g->sum_in_out : 0;
   double n1 = (double)g->n + 1, mo = (double)info->m_oriented;

@@ -76,10 +76,19 @@ struct SchedCache {
   DevBuf auto_large;               // d+ in (64, 2048], by descending work
   DevBuf auto_huge;                // d+ > 2048, by descending work
   uint64_t n_small = 0, n_large = 0, n_huge = 0;
+  // AUTO over work lists: W(x) CSR (wl_rp, wl_col) and the task lists
+  bool wl_ok = false;
+  uint32_t wl_wmax = 0, wl_hmin = 0, wl_lmax = 0;
+  DevBuf wl_rp, wl_col;            // uint32[n+1], uint32[entries]
+  DevBuf wl_small, wl_large, wl_huge, wl_lane;
+  uint64_t nw_small = 0, nw_large = 0, nw_huge = 0, nw_lane = 0;
+  uint64_t wl_entries = 0;         // edges kept in the work lists
+  double wl_bytes = 0;             // bytes one count streams (see count.hip)
   bool rows_ok = false;
   DevBuf rows;                     // rows with d+ >= 2
   uint64_t nrows = 0;
   uint32_t fox_lanes = 0;
+  bool fox_rearr = false;
   DevBuf fox_eids;                 // edge ids grouped by lanes-per-edge bin
   std::vector<uint64_t> fox_off;   // bin offsets (bins+1)
   float ms_prep = 0;
@@ -103,6 +112,7 @@ struct tc_graph {
   uint32_t n = 0;           // rows of the oriented CSR (after pruning, if any)
   uint32_t n_raw = 0;       // every raw record id is below this
   uint64_t m_raw = 0;
+  bool csr_only = false;    // built from a CSR (stage file, TriCore parts): no raw records
   tcx::DevBuf src, dst;     // raw records (uint32 each)
   // oriented CSR
   int orient_mode = -1;
@@ -150,7 +160,8 @@ int build_csr_from_keys(tc_ctx* c, tc_graph* g, uint64_t* keys, uint64_t nkeys,
 int compute_csr_stats(tc_ctx* c, tc_graph* g);
 int build_sym(tc_ctx* c, tc_graph* g, DevBuf& rowptr, DevBuf& col, uint64_t* m2);
 // A new graph holding a device copy of g's raw records (not oriented); its
-// vertex count is the records' largest id + 1.
+// vertex count is the records' largest id + 1.  A graph without raw records
+// (csr_only) gives its oriented CSR's edges instead.
 int graph_clone_records(tc_ctx* c, const tc_graph* g, tc_graph** out);
 
 // ---- cost model and per-vertex counts (mlp.hip, count.hip) ---------------

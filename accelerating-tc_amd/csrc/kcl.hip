@@ -1124,8 +1124,8 @@ int kcl_count_roots(tc_ctx* c, tc_graph* g, uint32_t k, const uint32_t* roots, u
 }
 
 int kcl_gpu_solver(const char* path, uint32_t k, uint64_t* total, size_t n_chunk) {
-  (void)n_chunk;
   if (!path || !total) return fail(TC_ERR_ARG, "NULL argument");
+  if (n_chunk == 0) return fail(TC_ERR_ARG, "n_chunk must be at least 1");
   uint64_t m = 0;
   uint32_t n = 0;
   const size_t len = strlen(path);
@@ -1149,7 +1149,31 @@ int kcl_gpu_solver(const char* path, uint32_t k, uint64_t* total, size_t n_chunk
   // degeneracy order: every out-degree is at most the degeneracy, which
   // keeps the roots on the LDS pivot path
   if (st == TC_OK) st = tc_graph_orient(c, g, TC_ORIENT_DEGENERACY, 0);
-  if (st == TC_OK) st = kcl_count(c, g, k, KCL_VARIANT_AUTO, total, nullptr);
+  if (st == TC_OK && (n_chunk == 1 || k < 3)) {
+    st = kcl_count(c, g, k, KCL_VARIANT_AUTO, total, nullptr);
+  } else if (st == TC_OK) {
+    // k-clique/kcl.cu:163-209 splits the DAG's edges into N_CHUNK contiguous
+    // ranges and counts the cliques whose first edge lies in each range in
+    // turn.  Here a root's out-edges stay together: root u belongs to chunk
+    // rowptr[u] / ceil(m / N_CHUNK), and the chunks are counted one after the
+    // other through kcl_count_roots.
+    const uint32_t nv = g->n;
+    std::vector<uint32_t> rp((size_t)nv + 1);
+    st = tc_graph_get_csr(g, rp.data(), nullptr);
+    const uint64_t mo = g->m_or;
+    const uint64_t len = mo ? (mo - 1) / n_chunk + 1 : 1;
+    std::vector<std::vector<uint32_t>> chunks(n_chunk);
+    for (uint32_t u = 0; st == TC_OK && u < nv; ++u)
+      if (rp[u + 1] > rp[u]) chunks[std::min<uint64_t>(rp[u] / len, n_chunk - 1)].push_back(u);
+    uint64_t sum = 0;
+    for (size_t j = 0; st == TC_OK && j < n_chunk; ++j) {
+      uint64_t part = 0;
+      if (!chunks[j].empty())
+        st = kcl_count_roots(c, g, k, chunks[j].data(), chunks[j].size(), &part, nullptr);
+      sum += part;
+    }
+    if (st == TC_OK) *total = sum;
+  }
   tc_graph_destroy(g);
   tc_ctx_destroy(c);
   return st;

@@ -220,6 +220,7 @@ uint64_t tc_tricore_tricount_gpu(uint64_t part_num, uint32_t* adj, const uint64_
   g->ctx = c;
   g->n = g->n_raw = n;
   g->m_raw = 0;
+  g->csr_only = true;
   int st = g->rowptr.reserve(4 * ((size_t)n + 1));
   if (!st) st = g->col.reserve(4 * std::max<uint64_t>(m, 1));
   if (!st) st = g->esrc.reserve(4 * std::max<uint64_t>(m, 1));

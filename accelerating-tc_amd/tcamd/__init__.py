@@ -43,6 +43,7 @@ FLAG_PRUNE_DEG1 = 1
 KERNEL_HU, KERNEL_BISSON, KERNEL_FOX, KERNEL_TRICORE, KERNEL_AUTO = 0, 1, 2, 3, 4
 KERNEL_NAMES = {"hu": KERNEL_HU, "bisson": KERNEL_BISSON, "fox": KERNEL_FOX,
                 "tricore": KERNEL_TRICORE, "auto": KERNEL_AUTO}
+AUTO_WORKLIST, AUTO_PUSH = 0, 1
 KCL_AUTO, KCL_BITMAP, KCL_MERGE, KCL_PIVOT = 0, 1, 2, 3
 RELABEL_COMPACT, RELABEL_FIRST_SRC, RELABEL_DEGREE = 0, 1, 2
 
@@ -55,7 +56,7 @@ _vp = C.c_void_p
 
 class GraphInfo(C.Structure):
     _fields_ = [("n", C.c_uint32), ("max_out_degree", C.c_uint32), ("m_raw", C.c_uint64),
-                ("m_oriented", C.c_uint64), ("orient_mode", C.c_int32), ("pad0", C.c_int32),
+                ("m_oriented", C.c_uint64), ("orient_mode", C.c_int32), ("n_raw", C.c_uint32),
                 ("sum_out_sq", C.c_uint64), ("sum_in_out", C.c_uint64),
                 ("bytes_alg", C.c_double), ("bytes_min", C.c_double)]
 
@@ -65,7 +66,7 @@ class Sched(C.Structure):
                 ("lds_slice", C.c_uint32), ("bitmap_min_deg", C.c_uint32),
                 ("fox_max_lanes", C.c_uint32), ("tricore_parts", C.c_uint32),
                 ("auto_bins", C.c_uint32), ("auto_thresholds", C.c_uint32 * 4),
-                ("reserved", C.c_uint32 * 4)]
+                ("fox_rearrange", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 class Timing(C.Structure):
@@ -145,6 +146,9 @@ _sig("tc_fox_bin_sizes", C.c_int, [_vp, _vp, C.c_uint32, _u64p])
 _sig("tc_count", C.c_int, [_vp, _vp, C.POINTER(Sched), C.POINTER(C.c_uint64),
                            C.POINTER(Timing)])
 _sig("tc_count_per_vertex", C.c_int, [_vp, _vp, _u64p])
+_sig("tc_graph_save_csr", C.c_int, [_vp, C.c_char_p])
+_sig("tc_graph_load_csr", C.c_int, [_vp, C.c_char_p, C.POINTER(_vp)])
+_sig("tc_auto_stream_bytes", C.c_int, [_vp, _vp, C.c_void_p, C.POINTER(C.c_double)])
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 
@@ -218,7 +222,7 @@ EXPORTED = [
     "tc_bw_model_fit",
     "tc_graph_get_csr", "tc_graph_get_edges", "tc_graph_get_vertex_key", "tc_graph_device_csr",
     "tc_hu_workload", "tc_hu_block_starts", "tc_fox_workload", "tc_fox_bin_sizes", "tc_count",
-    "tc_count_per_vertex", "tc_cost_functions", "tc_approx_ratio", "kcl_count", "kcl_gpu_solver",
+    "tc_count_per_vertex", "tc_auto_stream_bytes", "tc_cost_functions", "tc_approx_ratio", "kcl_count", "kcl_gpu_solver",
     "kcl_root_features", "kcl_measure_costs", "kcl_learned_plan", "kcl_count_learned",
     "kcl_count_roots",
     "tc_tricore_init_gpu",
@@ -228,7 +232,7 @@ EXPORTED = [
     "tc_model_get_norm", "tc_model_forward", "tc_model_loss_grad", "tc_model_train",
     "tc_graph_features", "tc_graph_measure_costs", "tc_learned_assignment", "tc_count_learned",
     "tc_batch_create", "tc_batch_destroy", "tc_batch_info_get", "tc_batch_autotune", "tc_batch_count",
-    "tc_batch_count_pergraph",
+    "tc_batch_count_pergraph", "tc_graph_save_csr", "tc_graph_load_csr",
 ]
 VARIANT_BLOCK, VARIANT_WAVE = 0, 1
 N_FEATURES, N_OUTPUTS = 8, 2
@@ -456,6 +460,16 @@ class Context:
     def graph(self, src, dst, n: int = 0) -> "Graph":
         return Graph(self, src, dst, n)
 
+    def load_csr(self, path: str) -> "Graph":
+        """A graph from an oriented-CSR stage file (tc_graph_load_csr)."""
+        h = _vp()
+        _check(_lib.tc_graph_load_csr(self.handle, path.encode(), C.byref(h)), "tc_graph_load_csr")
+        g = Graph.__new__(Graph)
+        g.ctx = self
+        g._h = h
+        self._children.add(g)
+        return g
+
     def batch(self, graphs) -> "Batch":
         return Batch(self, graphs)
 
@@ -492,6 +506,11 @@ class Graph:
         self.close()
 
     # -- preprocessing --
+    def save_csr(self, path: str):
+        """Writes the oriented CSR stage file (tc_graph_save_csr)."""
+        _check(_lib.tc_graph_save_csr(self._h, path.encode()), "tc_graph_save_csr")
+        return self
+
     def simplify(self):
         _check(_lib.tc_graph_simplify(self.ctx.handle, self._h), "tc_graph_simplify")
         return self
@@ -508,8 +527,10 @@ class Graph:
         return perm
 
     def relabel_aorder(self, model: "BwModel | None" = None):
-        """A-order; model (optional) replaces the reference's bandwidth model."""
-        n = self.info().n
+        """A-order; model (optional) replaces the reference's bandwidth model.
+        Returns old -> new for every raw id (n_raw entries; the CSR's n is
+        smaller after a pruned orientation)."""
+        n = self.info().n_raw
         perm = np.empty(max(n, 1), np.uint32)
         if model is None:
             _check(_lib.tc_graph_relabel_aorder(self.ctx.handle, self._h, perm.ctypes.data),
@@ -575,7 +596,8 @@ class Graph:
         return out
 
     # -- counting --
-    def count(self, kernel=KERNEL_AUTO, **sched) -> CountResult:
+    @staticmethod
+    def _sched(kernel, sched):
         if isinstance(kernel, str):
             kernel = KERNEL_NAMES[kernel]
         s = Sched()
@@ -584,8 +606,22 @@ class Graph:
             if k == "auto_thresholds":
                 for i, x in enumerate(v):
                     s.auto_thresholds[i] = x
+            elif k == "auto_schedule":
+                s.auto_thresholds[2] = {"worklist": AUTO_WORKLIST, "push": AUTO_PUSH}.get(v, v)
             else:
                 setattr(s, k, v)
+        return s
+
+    def auto_stream_bytes(self, **sched) -> float:
+        """Bytes one AUTO count streams under the schedule (tc_auto_stream_bytes)."""
+        s = self._sched(KERNEL_AUTO, sched)
+        out = C.c_double()
+        _check(_lib.tc_auto_stream_bytes(self.ctx.handle, self._h, C.byref(s), C.byref(out)),
+               "tc_auto_stream_bytes")
+        return out.value
+
+    def count(self, kernel=KERNEL_AUTO, **sched) -> CountResult:
+        s = self._sched(kernel, sched)
         out = C.c_uint64()
         t = Timing()
         _check(_lib.tc_count(self.ctx.handle, self._h, C.byref(s), C.byref(out), C.byref(t)),

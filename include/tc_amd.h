@@ -63,6 +63,17 @@ extern "C" {
 #define TC_KERNEL_TRICORE 3     /* tricore/src/tricount_gpu.cu:215-290 */
 #define TC_KERNEL_AUTO 4        /* degree-binned engine (fixed or learned bins) */
 
+/* AUTO schedule (tc_sched.auto_thresholds[2]).  Both count every oriented
+ * edge u->v as |N+(u) & N+(v)| by a task that holds one of the two lists in
+ * LDS and streams the other from HBM.
+ *   WORKLIST: the edge goes to the task of whichever endpoint makes the
+ *             streamed list the shorter one (u when d+(v) <= d+(u), else v),
+ *             from a per-vertex work-list CSR built once per graph: HBM
+ *             streams 4 sum_e min(d+(u), d+(v)) bytes of lists.
+ *   PUSH:     task u streams N+(v) for every v in N+(u): 4 sum_v d-(v) d+(v). */
+#define TC_AUTO_WORKLIST 0
+#define TC_AUTO_PUSH 1
+
 /* k-clique variant. */
 #define KCL_VARIANT_AUTO 0      /* k = 3: triangle engine; else pivot (d+ <= 1024) or merge */
 #define KCL_VARIANT_BITMAP 1    /* LDS bitmap depth-first path (d+ <= 512, k <= 9)  */
@@ -79,7 +90,8 @@ typedef struct tc_graph_info {
   uint64_t m_raw;           /* records currently held as raw edges              */
   uint64_t m_oriented;      /* oriented unique edges (= undirected simple edges) */
   int32_t orient_mode;      /* TC_ORIENT_*, -1 before tc_graph_orient           */
-  int32_t pad0;
+  uint32_t n_raw;           /* every raw record id is below this (n differs
+                               after pruning); A-order's perm has n_raw entries */
   uint64_t sum_out_sq;      /* sum_u d+(u)^2                                    */
   uint64_t sum_in_out;      /* sum_v d-(v) d+(v)  (= wedge count)               */
   double bytes_alg;         /* B_alg = 4(n+1) + 8m+ + 4 sum d+^2 + 4 sum d- d+  */
@@ -100,8 +112,17 @@ typedef struct tc_sched {
   uint32_t auto_thresholds[4]; /* AUTO: [0] largest d+ run as a wave task (64,
                                at most 512); [1] largest d+ run by the 2048-entry
                                block kernel, above it the 8192-entry one (2048);
-                               [2], [3] reserved                                 */
-  uint32_t reserved[4];
+                               [2] schedule: TC_AUTO_WORKLIST (0, default) or
+                               TC_AUTO_PUSH; [3] WORKLIST: largest d+ of a task
+                               run by one lane (N+(x) in registers) when its
+                               work list has at most 16 entries (8; at most 8;
+                               1 turns lane tasks off)                         */
+  uint32_t fox_rearrange;   /* Fox: 1 = rearrangeEdge (Fox/tc.cu:127-183,345):
+                               each bin's edges sorted by max(d+(u), d+(v)) and
+                               dealt in groups of up to 32 round-robin over the
+                               launch's workgroups, so every workgroup gets
+                               light and heavy edges; counts are unchanged     */
+  uint32_t reserved[3];
 } tc_sched;
 
 typedef struct tc_timing {
@@ -167,8 +188,8 @@ int tc_graph_destroy(tc_graph* g);
  * dropped, duplicates merged, each edge once as (min,max), sorted. */
 int tc_graph_simplify(tc_ctx* ctx, tc_graph* g);
 /* A-order relabelling of the raw records (vertex_ordering/A-order/A-order.cpp:62-230);
- * the records are rewritten with new ids.  perm (optional, n entries, n
- * before the call) gets old->new.  The graph's n becomes the largest new id
+ * the records are rewritten with new ids.  perm (optional, n_raw entries as
+ * tc_graph_info reports before the call) gets old->new.  The graph's n becomes the largest new id
  * + 1 over the records, as reading A-order's "reordered" file back gives
  * (edge_directing/A-direction-seri.cpp:187-193), so an A-direction peel after
  * it uses the reference chain's threshold.  TC_ERR_UNSUPPORTED outside the
@@ -229,6 +250,16 @@ int tc_graph_get_edges(const tc_graph* g, uint32_t* src, uint32_t* dst /*m_raw*/
 /* ADIR: peel priorities (A-direction-seri.cpp numbering); KCORE: core numbers;
  * DEGENERACY: removal steps. */
 int tc_graph_get_vertex_key(const tc_graph* g, int32_t* key /*n*/);
+/* Stage file of the oriented CSR: the preprocessing result (n, m+, the
+ * orientation, rowptr, col and, after ADIR / KCORE / DEGENERACY, the vertex
+ * keys), so a later run loads it instead of repeating the upload, sort and
+ * peel that every reference program re-runs (Hu/preTC.cu:186-291).  A loaded
+ * graph holds no raw records: every count works on it; simplify, relabel and
+ * orient return TC_ERR_STATE.  load checks the file's structure (rows
+ * ascending, loop-free, ids below n) and returns TC_ERR_FORMAT otherwise.
+ * No counterpart in the reference. */
+int tc_graph_save_csr(const tc_graph* g, const char* path);
+int tc_graph_load_csr(tc_ctx* ctx, const char* path, tc_graph** out);
 /* Device addresses of the CSR (for callers sharing HBM buffers). */
 int tc_graph_device_csr(const tc_graph* g, const uint32_t** rowptr,
                         const uint32_t** col);
@@ -261,6 +292,12 @@ int tc_approx_ratio(tc_ctx* ctx, tc_graph* g, double* ratio);
  * NULL (AUTO with defaults).  timing may be NULL. */
 int tc_count(tc_ctx* ctx, tc_graph* g, const tc_sched* sched, uint64_t* count,
              tc_timing* timing);
+/* Bytes one AUTO count streams under sched (NULL: defaults): for
+ * TC_AUTO_WORKLIST the sum over tasks x of 20 + 4 d+(x) (task id, row bounds
+ * in both CSRs, N+(x)) and over the work list of x of 12 + 4 d+(y) (id, row
+ * bounds, N+(y)); for TC_AUTO_PUSH 4(n+1) + 12 m+ + 4 sum_v d-(v) d+(v).
+ * Builds the schedule when needed.  No counterpart in the reference. */
+int tc_auto_stream_bytes(tc_ctx* ctx, tc_graph* g, const tc_sched* sched, double* bytes);
 /* Per-vertex forward counts tri[u] (n entries): the triangles whose first
  * vertex (the tail of the two other arcs) is u, from the AUTO kernel's block
  * and wave tasks with default thresholds.  They sum to tc_count's total. */
@@ -389,8 +426,10 @@ int kcl_count_roots(tc_ctx* ctx, tc_graph* g, uint32_t k, const uint32_t* roots,
 /* File-level form of kcl_gpu_solver (k-clique/kcl.h:6-7): reads a Galois
  * .gr file (path ending in ".gr", the input of k-clique/kcl_gpu.cpp:22-27)
  * or a list2bin file, degeneracy orientation (TC_ORIENT_DEGENERACY),
- * counts k-cliques.  n_chunk is accepted for signature parity (HBM holds
- * every level; chunks are not needed). */
+ * counts k-cliques.  n_chunk > 1 counts the DAG in n_chunk parts one after
+ * the other, as N_CHUNK splits its edges (k-clique/kcl.cu:163-209): the roots
+ * whose out-edges start in each of n_chunk equal ranges of the CSR's edges
+ * form one part (a root's edges are never split); 0 is TC_ERR_ARG. */
 int kcl_gpu_solver(const char* path, uint32_t k, uint64_t* total, size_t n_chunk);
 
 /* ---- learned k-clique plan --------------------------------------------

@@ -185,7 +185,11 @@ def test_reorient_after_prune(ctx, case):
     assert g.count("auto").count == expect
     if case in ("er", "rmat12", "ba"):  # inside A-order's defined range
         g.orient(tcamd.ORIENT_DEGREE, prune_deg1=True)
-        g.relabel_aorder()
+        n_raw = g.info().n_raw
+        assert n_raw == orc.num_nodes(s, d) and g.info().n <= n_raw
+        perm = g.relabel_aorder()  # old -> new for every raw id, not just the CSR's rows
+        assert len(perm) == n_raw
+        np.testing.assert_array_equal(np.sort(perm), np.arange(n_raw))
         g.orient(tcamd.ORIENT_ADIR)
         assert g.count("auto").count == expect
     g.close()
@@ -240,6 +244,13 @@ def test_per_vertex_counts(ctx, case, orient):
     dict(kernel="auto", auto_thresholds=[2, 2]),
     dict(kernel="auto", auto_thresholds=[512, 512], grid_size=5),
     dict(kernel="auto", auto_thresholds=[16, 40]),
+    dict(kernel="auto", auto_schedule="push"),
+    dict(kernel="auto", auto_schedule="push", auto_thresholds=[2, 2]),
+    dict(kernel="auto", auto_schedule="push", auto_thresholds=[512, 512], grid_size=5),
+    dict(kernel="auto", auto_schedule="worklist", auto_thresholds=[16, 40], grid_size=3),
+    dict(kernel="auto", auto_thresholds=[64, 2048, 0, 1]),          # no lane tasks
+    dict(kernel="auto", auto_thresholds=[2, 2, 0, 4], grid_size=2),  # lane tasks up to d+ 4
+    dict(kernel="auto", auto_thresholds=[8, 8, 0, 8], grid_size=1),
 ])
 @pytest.mark.parametrize("case", ["rmat12", "hub", "k6", "messy"])
 def test_schedule_variants(ctx, case, sched):
@@ -369,7 +380,10 @@ def test_kcl_gpu_solver_file(ctx, tmp_path):
     p = str(tmp_path / "g.bin")
     tcamd.write_edges(p, s, d)
     rp_o, col_o = oracle_csr(s, d, "kcore")
-    assert tcamd.kcl_gpu_solver(p, 4) == orc.kclique(rp_o, col_o, 4)
+    for k in (3, 4, 5):
+        want = orc.kclique(rp_o, col_o, k)
+        for n_chunk in (1, 2, 7, 64):
+            assert tcamd.kcl_gpu_solver(p, k, n_chunk) == want, (k, n_chunk)
 
 
 def test_larger_rmat_all_kernels(ctx):
@@ -477,7 +491,41 @@ def test_auto_strided_and_claimed_wave_tasks(ctx, case):
     expect = truth(case)
     g = ctx.graph(s, d).orient(tcamd.ORIENT_ADIR)
     for grid in (0, 1, 2, 7, 64):
-        assert g.count(tcamd.KERNEL_AUTO, grid_size=grid).count == expect, grid
+        for sched in ("worklist", "push"):
+            assert g.count(tcamd.KERNEL_AUTO, grid_size=grid, auto_schedule=sched).count == expect, (
+                grid, sched)
+    g.close()
+
+
+def worklist_bytes(rp, col):
+    """Bytes of the work-list schedule, recounted from a CSR: each edge u->v
+    goes to the task of u when d+(v) <= d+(u), else of v; it is dropped when
+    v has no out-edge or the owner u has d+(u) < 2.  A task x (d+(x) >= 2,
+    non-empty list) reads 20 + 4 d+(x) bytes, each list entry y 12 + 4 d+(y)."""
+    n = len(rp) - 1
+    dp = np.diff(rp.astype(np.int64))
+    src = np.repeat(np.arange(n), dp)
+    du, dv = dp[src], dp[col]
+    keep = ~((dv == 0) | ((dv <= du) & (du < 2)))
+    push = dv <= du
+    owner = np.where(push, src, col)[keep]
+    streamed = np.where(push, col, src)[keep]
+    nw = np.bincount(owner, minlength=n)
+    tasks = (nw > 0) & (dp >= 2)
+    assert nw[~tasks].sum() == 0  # every kept entry belongs to a runnable task
+    return int((20 + 4 * dp[tasks]).sum() + (12 + 4 * dp[streamed]).sum())
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("orient", ["degree", "adir", "id"])
+def test_auto_stream_bytes(ctx, case, orient):
+    s, d = CASES[case]()
+    rp_o, col_o = oracle_csr(s, d, orient)
+    g = ctx.graph(s, d).orient(ORIENTS[orient])
+    assert g.auto_stream_bytes() == worklist_bytes(rp_o, col_o)
+    info = g.info()
+    assert g.auto_stream_bytes(auto_schedule="push") == (
+        4 * (info.n + 1) + 12 * info.m_oriented + 4 * info.sum_in_out)
     g.close()
 
 
@@ -524,3 +572,61 @@ def test_kclique_root_subsets_add_up(ctx, case):
     with pytest.raises(tcamd.TCError):
         g.kclique_roots(4, np.array([n], np.uint32))
     g.close()
+
+
+@pytest.mark.parametrize("case", ["er", "rmat12", "hub", "messy"])
+def test_csr_stage_file_round_trip(ctx, case, tmp_path):
+    """An oriented CSR saved and loaded back counts like the original under
+    every method (k-cliques too, through the degeneracy copy that wide
+    orientations take), and refuses the passes that need raw records."""
+    s, d = CASES[case]()
+    expect = truth(case)
+    for orient in ("id", "adir", "degeneracy"):
+        g = ctx.graph(s, d).orient(ORIENTS[orient])
+        p = str(tmp_path / f"{orient}.csr")
+        g.save_csr(p)
+        h = ctx.load_csr(p)
+        a, b = g.info(), h.info()
+        assert (a.n, a.m_oriented, a.orient_mode, a.sum_out_sq, a.sum_in_out, a.max_out_degree) == (
+            b.n, b.m_oriented, b.orient_mode, b.sum_out_sq, b.sum_in_out, b.max_out_degree)
+        for x, y in zip(g.csr(), h.csr()):
+            np.testing.assert_array_equal(x, y)
+        if orient != "id":
+            np.testing.assert_array_equal(g.vertex_key(), h.vertex_key())
+        for k in KERNELS:
+            assert h.count(k).count == expect, (orient, k)
+        for k in (4, 5):
+            assert h.kclique(k).count == g.kclique(k).count, (orient, k)
+        with pytest.raises(tcamd.TCError) as e:
+            h.orient(tcamd.ORIENT_DEGREE)
+        assert e.value.status == tcamd.ERR_STATE
+        g.close()
+        h.close()
+
+
+def test_csr_stage_file_rejects_bad_files(ctx, tmp_path):
+    s, d = CASES["k6"]()
+    g = ctx.graph(s, d).orient(tcamd.ORIENT_DEGREE)
+    p = tmp_path / "g.csr"
+    g.save_csr(str(p))
+    rp, _ = g.csr()
+    g.close()
+    raw = bytearray(p.read_bytes())
+    n = len(rp) - 1
+    c0 = 32 + 4 * (n + 1)  # 32-byte header, then rowptr[n + 1], then col
+    row = int(np.nonzero(np.diff(rp.astype(np.int64)) >= 2)[0][0])  # a row of 2+ entries
+    e = c0 + 4 * int(rp[row])
+    swapped = raw[:e] + raw[e + 4:e + 8] + raw[e:e + 4] + raw[e + 8:]
+    bad = {
+        "magic": b"XXXXXXXX" + raw[8:],
+        "truncated": raw[:-4],
+        "trailing": raw + b"\0",
+        "col_range": raw[:c0] + (n + 93).to_bytes(4, "little") + raw[c0 + 4:],
+        "col_order": swapped,
+    }
+    for name, data in bad.items():
+        q = tmp_path / f"{name}.csr"
+        q.write_bytes(bytes(data))
+        with pytest.raises(tcamd.TCError) as e:
+            ctx.load_csr(str(q))
+        assert e.value.status == tcamd.ERR_FORMAT, name
