@@ -43,7 +43,8 @@ inline Records load(const char* path) {
 
 // Loads the file, orients it and counts with one method; prints the
 // reference's lines (node/edge numbers, the triangle line, kernel time).
-inline int count_file(const char* path, int orient, uint32_t flags, int kernel) {
+inline int count_file(const char* path, int orient, uint32_t flags, int kernel,
+                      uint32_t fox_rearrange = 0) {
   const double t0 = now_s();
   Records r = load(path);
   std::printf("the size of input file is %llu Byte. \n", (unsigned long long)(r.m * 8));
@@ -60,6 +61,7 @@ inline int count_file(const char* path, int orient, uint32_t flags, int kernel) 
               (unsigned long long)info.m_oriented);
   tc_sched sched = {};
   sched.kernel = kernel;
+  sched.fox_rearrange = fox_rearrange;
   uint64_t count = 0;
   tc_timing tm;
   check(tc_count(ctx, g, &sched, &count, &tm), "tc_count");

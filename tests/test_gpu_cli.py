@@ -46,7 +46,8 @@ def test_method_drivers(graph_file):
     assert triangles(run("tc_hu", "-f", p)) == t
     assert triangles(run("tc_bisson", "-f", p, 0)) == t
     assert triangles(run("tc_bisson", "-f", p, 1)) == t
-    assert triangles(run("tc_fox", "-f", p, 1)) == t
+    assert triangles(run("tc_fox", "-f", p, 0)) == t
+    assert triangles(run("tc_fox", "-f", p, 1)) == t  # rearrangeEdge
     assert triangles(run("tric_gpu", "-f", p)) == t
     for k in ("auto", "hu", "tricore"):
         for o in ("adir", "kcore", "degree", "degeneracy", "id", "tricore"):

@@ -236,6 +236,8 @@ def test_per_vertex_counts(ctx, case, orient):
     dict(kernel="bisson", bitmap_min_deg=16, grid_size=7),
     dict(kernel="fox", fox_max_lanes=32),
     dict(kernel="fox", fox_max_lanes=1),
+    dict(kernel="fox", fox_rearrange=1),
+    dict(kernel="fox", fox_max_lanes=4, fox_rearrange=1),
     dict(kernel="tricore", tricore_parts=3),
     dict(kernel="tricore", tricore_parts=8, grid_size=5),
     dict(kernel="hu", grid_size=1),
@@ -630,3 +632,21 @@ def test_csr_stage_file_rejects_bad_files(ctx, tmp_path):
         with pytest.raises(tcamd.TCError) as e:
             ctx.load_csr(str(q))
         assert e.value.status == tcamd.ERR_FORMAT, name
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_fox_rearrange_keeps_bins_and_count(ctx, case):
+    """rearrangeEdge permutes each bin's edges: the bin sizes and the count
+    stay those of the plain Fox schedule, and switching back and forth
+    rebuilds the schedule each time."""
+    s, d = CASES[case]()
+    g = ctx.graph(s, d).orient(tcamd.ORIENT_ID)
+    expect = truth(case)
+    for flag in (0, 1, 0, 1):
+        assert g.count("fox", fox_rearrange=flag).count == expect, flag
+    plain = g.fox_bin_sizes(64)
+    g.count("fox", fox_rearrange=1)
+    np.testing.assert_array_equal(g.fox_bin_sizes(64), plain)
+    with pytest.raises(tcamd.TCError):
+        g.count("fox", fox_rearrange=2)
+    g.close()
