@@ -102,6 +102,7 @@ int tc_ctx_create(int device, tc_ctx** out) {
   if (device < 0 || device >= ndev)
     return fail(TC_ERR_ARG, "device %d out of range (have %d)", device, ndev);
   TCX_HIP(hipSetDevice(device));
+  TCX_TRY(tcx::noalloc_setup(device));
   tc_ctx* c = new tc_ctx();
   c->device = device;
   hipDeviceProp_t prop;

@@ -174,6 +174,11 @@ uint64_t model_id(const tc_model* m);
 // (uint64 per vertex, device array of n entries).
 int per_vertex_counts(tc_ctx* c, tc_graph* g, unsigned long long* tri);
 
+// ---- allocation-free counting (noalloc.hip) --------------------------------
+// Looks up the device address of the static scratch array on `device` (the
+// device must be current); called once per device by tc_ctx_create.
+int noalloc_setup(int device);
+
 // ---- A-order (aorder.cpp, host heap) -------------------------------------
 int aorder_perm_from_outdeg(const uint32_t* rtd, uint32_t n, uint32_t* perm,
                             const tc_bw_model& model);

@@ -149,6 +149,8 @@ _sig("tc_count_per_vertex", C.c_int, [_vp, _vp, _u64p])
 _sig("tc_graph_save_csr", C.c_int, [_vp, C.c_char_p])
 _sig("tc_graph_load_csr", C.c_int, [_vp, C.c_char_p, C.POINTER(_vp)])
 _sig("tc_auto_stream_bytes", C.c_int, [_vp, _vp, C.c_void_p, C.POINTER(C.c_double)])
+_sig("tc_count_csr_noalloc", C.c_int, [_vp, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                                       C.c_void_p])
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 
@@ -232,7 +234,7 @@ EXPORTED = [
     "tc_model_get_norm", "tc_model_forward", "tc_model_loss_grad", "tc_model_train",
     "tc_graph_features", "tc_graph_measure_costs", "tc_learned_assignment", "tc_count_learned",
     "tc_batch_create", "tc_batch_destroy", "tc_batch_info_get", "tc_batch_autotune", "tc_batch_count",
-    "tc_batch_count_pergraph", "tc_graph_save_csr", "tc_graph_load_csr",
+    "tc_batch_count_pergraph", "tc_count_csr_noalloc", "tc_graph_save_csr", "tc_graph_load_csr",
 ]
 VARIANT_BLOCK, VARIANT_WAVE = 0, 1
 N_FEATURES, N_OUTPUTS = 8, 2

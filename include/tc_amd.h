@@ -303,6 +303,27 @@ int tc_auto_stream_bytes(tc_ctx* ctx, tc_graph* g, const tc_sched* sched, double
  * and wave tasks with default thresholds.  They sum to tc_count's total. */
 int tc_count_per_vertex(tc_ctx* ctx, tc_graph* g, uint64_t* tri);
 
+/* ---- counting without device allocation --------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the exact triangle count of an oriented CSR the caller
+ * already holds in device memory (rowptr[n+1], col[rowptr[n]], every row
+ * ascending and without repeats, as tc_graph_device_csr gives), written to
+ * the caller's device uint64 *count.  The work is enqueued on ctx's stream
+ * and not waited for.  The kernels are tc_count's AUTO push form without
+ * its task lists.  Their scratch memory (two claim cursors and one partial
+ * count per workgroup) is a static device array of the library,
+ * TC_NOALLOC_SCRATCH_BYTES on each device, held until the process ends; its
+ * address is looked up when a context is created on the device.  grid =
+ * workgroups (0: 8 per CU, at most TC_NOALLOC_MAX_GRID); a larger grid is
+ * TC_ERR_RANGE.  Every call on one device uses that device's array, and the
+ * cursors are cleared at the start of each call: calls that overlap on one
+ * device (other streams, other contexts) race and must be serialised. */
+#define TC_NOALLOC_SCRATCH_BYTES 65536
+#define TC_NOALLOC_MAX_GRID 4096
+int tc_count_csr_noalloc(tc_ctx* ctx, const uint32_t* rowptr, const uint32_t* col, uint32_t n,
+                         uint32_t grid, unsigned long long* count);
+
 /* ---- learned scheduling of the AUTO engine ------------------------------
  * No counterpart in the reference.  The reference's schedules come from
  * fixed work formulas (Hu/tc.cu:22-35, Fox/tc.cu:18-33,316-331) and its
