@@ -81,9 +81,10 @@ def describe(w):
 
 
 def kernel_bytes(info):
-    """Bytes the AUTO kernel streams per count: the row offsets, every out-list
-    once (staged in LDS per task), and for every edge u->v the two bounds of
-    row v and the list N+(v): 4(n+1) + 4m+ + 8m+ + 4 sum_v d-(v) d+(v)."""
+    """Bytes the push form of the AUTO kernel streams per count: the row
+    offsets, every out-list once (staged in LDS per task), and for every edge
+    u->v the two bounds of row v and the list N+(v):
+    4(n+1) + 4m+ + 8m+ + 4 sum_v d-(v) d+(v)."""
     return 4.0 * (info.n + 1) + 12.0 * info.m_oriented + 4.0 * info.sum_in_out
 
 
@@ -271,7 +272,8 @@ def batched_desc(w):
 def main_batched(args, w):
     r = run_batched(args, w, args.steps, args.warmup, not args.no_cpu)
     info, m = r["info"], r["m"]
-    achieved = info.bytes_alg / (r["ev_b"] * 1e-3) / 1e9
+    kb = kernel_bytes(info)  # k_batch streams the push form over the union CSR
+    achieved = kb / (r["ev_b"] * 1e-3) / 1e9
     line = {
         "metric": METRIC,
         "value": m / r["wall_b"],
@@ -287,9 +289,10 @@ def main_batched(args, w):
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": "k_batch<2048>",
-            "bytes_per_count": info.bytes_alg,
-            "bytes_rule": "B_alg of the union CSR: 4(n+1) + 8m+ + 4 sum d+^2 + 4 sum d- d+",
+            "frac": achieved / HBM_PEAK_GBPS, "traffic": load_traffic("batched", "batched"),
+            "kernel": "k_batch<2048>", "bytes_per_count": kb,
+            "bytes_rule": "4(n+1) + 12 m+ + 4 sum_v d-(v) d+(v) of the union CSR",
+            "bytes_alg_edge_model": info.bytes_alg,
         },
         "cpu_baseline": r["cpu"],
         "batched_vs_pergraph": r["wall_p"] / r["wall_b"],
@@ -465,9 +468,10 @@ def run_triangles(args, w, kernels, with_cpu):
     t_prep = time.perf_counter() - t0
     info = g.info()
     m = info.m_oriented
+    stream = g.auto_stream_bytes()  # the work-list schedule's bytes per count
     log(f"[bench] n={info.n} m={m} max_out={info.max_out_degree} B_alg={info.bytes_alg/1e9:.2f} GB "
-        f"B_kernel={kernel_bytes(info)/1e9:.2f} GB prep {t_prep:.2f}s (simplify {t_simpl:.2f}s, "
-        f"aorder {t_aorder:.2f}s)")
+        f"work-list stream {stream/1e9:.2f} GB (push form {kernel_bytes(info)/1e9:.2f} GB) "
+        f"prep {t_prep:.2f}s (simplify {t_simpl:.2f}s, aorder {t_aorder:.2f}s)")
 
     model = None
     if "learned" in kernels:
@@ -524,18 +528,21 @@ def run_triangles(args, w, kernels, with_cpu):
     g.close()
     ctx.close()
     return dict(info=info, m=m, head=head, results=results, cpu=cpu, cpu_par=cpu_par,
-                t_gen=t_gen, t_prep=t_prep)
+                t_gen=t_gen, t_prep=t_prep, stream=stream)
 
 
-def triangle_roofline(info, hr, traffic):
-    kb = kernel_bytes(info)
-    achieved = kb / (hr["ms_kernels"] * 1e-3) / 1e9
+def triangle_roofline(info, hr, traffic, stream):
+    """The head kernel (k_auto_wl, AUTO and the learned schedule) streams the
+    work-list bytes of tc_auto_stream_bytes whatever variant runs each task."""
+    achieved = stream / (hr["ms_kernels"] * 1e-3) / 1e9
     return {
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic, "kernel": "k_auto<2048>",
-        "bytes_per_count": kb,
-        "bytes_rule": "4(n+1) + 12 m+ + 4 sum_v d-(v) d+(v): row offsets, each out-list once, "
-                      "and per edge u->v the bounds and list of v",
+        "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic, "kernel": "k_auto_wl<2048>",
+        "bytes_per_count": stream,
+        "bytes_rule": "tc_auto_stream_bytes: per task x, 20 + 4 d+(x) (task id, row bounds in "
+                      "both CSRs, N+(x)); per work-list entry y, 12 + 4 d+(y) (id, row bounds, "
+                      "N+(y))",
+        "bytes_push_form": kernel_bytes(info),
         "bytes_alg_edge_model": info.bytes_alg,
     }
 
@@ -550,11 +557,11 @@ def leg_summary(name, w, r, traffic=None):
                 "batched_vs_pergraph": r["wall_p"] / r["wall_b"],
                 "ms_batched": r["wall_b"] * 1e3, "ms_pergraph": r["wall_p"] * 1e3,
                 "sched": r["sched"],
-                "gbps_alg": info.bytes_alg / (r["ev_b"] * 1e-3) / 1e9}
+                "gbps_stream": kernel_bytes(info) / (r["ev_b"] * 1e-3) / 1e9}
     hr = r["results"][r["head"]]
     return {"workload": describe(w), "n": r["info"].n, "m": r["m"], "count": hr["count"],
             "engine": r["head"], "edges_per_s": hr["edges_per_s"], "ms_per_step": hr["ms_wall"],
-            "roofline": triangle_roofline(r["info"], hr, traffic),
+            "roofline": triangle_roofline(r["info"], hr, traffic, r["stream"]),
             "learned_vs_plain": vs_plain(r["results"], r["head"], traffic),
             "kernels": {k: {kk: (round(v, 4) if isinstance(v, float) else v)
                             for kk, v in x.items()} for k, x in r["results"].items()}}
@@ -645,7 +652,7 @@ def main():
             "n": info.n, "m": m, "count": hr["count"], "engine": head,
             "parallelism": "single GPU",
         },
-        "roofline": triangle_roofline(info, hr, traffic),
+        "roofline": triangle_roofline(info, hr, traffic, r["stream"]),
         "cpu_baseline": r["cpu"],
         "cpu_baseline_parallel": r["cpu_par"],
         "learned_vs_plain": vs_plain(r["results"], head, traffic),
