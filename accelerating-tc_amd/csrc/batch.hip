@@ -2,8 +2,9 @@
 //
 // The batch is the disjoint union of its graphs, degree-oriented in one CSR
 // (each graph keeps its own relative order, so the union's orientation is
-// every graph's own).  The AUTO engine's block and wave variants count the
-// vertices; per-vertex counts are folded into per-graph uint64 accumulators.
+// every graph's own).  The AUTO engine's work-list tasks count it (an edge
+// never leaves its graph, so neither does a work list); block and wave
+// tasks' counts are folded into per-graph uint64 accumulators.
 // The reset + count sequence is captured once in a hipGraph and replayed.
 #include <algorithm>
 #include <atomic>
@@ -36,6 +37,8 @@ __device__ inline void flush_graph(unsigned long long& acc, uint32_t g,
 template <int CAP, uint32_t FMIN>
 __global__ __launch_bounds__(TB) void k_batch(const uint32_t* __restrict__ rp,
                                               const uint32_t* __restrict__ col,
+                                              const uint32_t* __restrict__ wrp,
+                                              const uint32_t* __restrict__ wcol,
                                               const uint32_t* __restrict__ gid,
                                               const uint32_t* __restrict__ blk, uint32_t nblk,
                                               const uint32_t* __restrict__ wav, uint32_t nwav,
@@ -44,21 +47,21 @@ __global__ __launch_bounds__(TB) void k_batch(const uint32_t* __restrict__ rp,
   __shared__ BlockLds<CAP> S;
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   clear_bitmap(S);
-  for (;;) {
-    if (tid == 0) S.t = atomicAdd(cursor, 1u);
+  for (bool first = true;; first = false) {
+    if (tid == 0) S.t = claim(cursor, nblk, 1u, first);
     __syncthreads();
     const uint32_t t = S.t;
     if (t >= nblk) break;
-    const uint32_t u = blk[t];
-    unsigned long long c = block_vertex<CAP, 1>(rp, col, u, S);
+    const uint32_t u = blk[t], wb = wrp[u];
+    unsigned long long c = block_vertex<CAP, 1, true>(rp, col, u, S, wcol + wb, wrp[u + 1] - wb);
     flush_graph(c, gid[u], counts);
   }
   const WaveSlot W = wave_slot(S, wv);
   unsigned long long acc = 0;
   uint32_t cur = 0xFFFFFFFFu;
-  for (;;) {
+  for (bool first = true;; first = false) {
     uint32_t i0 = 0;
-    if (lane == 0) i0 = atomicAdd(cursor + 1, wave_chunk);
+    if (lane == 0) i0 = claim(cursor + 1, nwav, wave_chunk, first);
     i0 = __shfl(i0, 0, 64);
     if (i0 >= nwav) break;
     const uint32_t i1 = min(nwav, i0 + wave_chunk);
@@ -69,27 +72,30 @@ __global__ __launch_bounds__(TB) void k_batch(const uint32_t* __restrict__ rp,
         if (cur != 0xFFFFFFFFu) flush_graph(acc, cur, counts);
         cur = g;
       }
-      acc += wave_vertex<FMIN, 1>(rp, col, u, lane, W);
+      const uint32_t wb = wrp[u];
+      acc += wave_vertex<FMIN, 1, true>(rp, col, u, lane, W, wcol + wb, wrp[u + 1] - wb);
     }
   }
   if (cur != 0xFFFFFFFFu) flush_graph(acc, cur, counts);
 }
 
-__global__ void k_bflags(const uint32_t* __restrict__ rp, uint32_t n, uint32_t lo, uint32_t hi,
-                         uint8_t* __restrict__ f) {
+// tasks (non-empty work list, d+ >= 2) with d+ in [lo, hi]
+__global__ void k_bflags(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ wrp,
+                         uint32_t n, uint32_t lo, uint32_t hi, uint8_t* __restrict__ f) {
   for (uint64_t u = gtid(); u < n; u += gstride()) {
     const uint32_t d = rp[u + 1] - rp[u];
-    f[u] = (d >= lo && d <= hi) ? 1 : 0;
+    f[u] = (wrp[u + 1] > wrp[u] && d >= 2 && d >= lo && d <= hi) ? 1 : 0;
   }
 }
 
-__global__ void k_bwork(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
-                        const uint32_t* __restrict__ verts, uint32_t nv, uint64_t* __restrict__ key,
-                        uint32_t* __restrict__ val) {
+// descending work (d+(x) + sum of d+(y) over W(x)) as the sort key
+__global__ void k_bwork(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ wrp,
+                        const uint32_t* __restrict__ wcol, const uint32_t* __restrict__ verts,
+                        uint32_t nv, uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
   for (uint64_t i = gtid(); i < nv; i += gstride()) {
     const uint32_t u = verts[i];
     uint64_t w = rp[u + 1] - rp[u];
-    for (uint32_t k = rp[u]; k < rp[u + 1]; ++k) w += rp[col[k] + 1] - rp[col[k]];
+    for (uint32_t k = wrp[u]; k < wrp[u + 1]; ++k) w += rp[wcol[k] + 1] - rp[wcol[k]];
     key[i] = ~w;
     val[i] = u;
   }
@@ -203,17 +209,28 @@ int schedule(tc_ctx* c, tc_batch* b, uint32_t wave_max_deg) {
   if (b->sched_deg == wave_max_deg) return TC_OK;
   tc_graph* g = b->g;
   const uint32_t n = g->n;
+  if (g->m_or == 0 || n == 0) {  // no edges: no work lists and no tasks
+    b->nwav = b->nblk = 0;
+    b->h_blk_off.assign((size_t)b->ngraphs + 1, 0);
+    b->h_wav_off.assign((size_t)b->ngraphs + 1, 0);
+    b->sched_deg = wave_max_deg;
+    release_capture(b);
+    return TC_OK;
+  }
   const uint32_t* rp = g->rowptr.as<uint32_t>();
+  TCX_TRY(worklists(c, g));
+  const uint32_t* wrp = g->sc.wl_rp.as<uint32_t>();
+  const uint32_t* wcol = g->sc.wl_col.as<uint32_t>();
   DevBuf flags, key, ktmp, vtmp;
   TCX_TRY(flags.reserve((size_t)n + 1));
   TCX_TRY(b->wav.reserve(4 * ((size_t)n + 1)));
   TCX_TRY(b->blk_id.reserve(4 * ((size_t)n + 1)));
   TCX_TRY(b->blk_lpt.reserve(4 * ((size_t)n + 1)));
-  hipLaunchKernelGGL(k_bflags, dim3(grid_for(n)), dim3(256), 0, c->stream, rp, n, 2u, wave_max_deg,
-                     flags.as<uint8_t>());
+  hipLaunchKernelGGL(k_bflags, dim3(grid_for(n)), dim3(256), 0, c->stream, rp, wrp, n, 2u,
+                     wave_max_deg, flags.as<uint8_t>());
   TCX_TRY(select_indices(c, flags.as<uint8_t>(), n, b->wav.as<uint32_t>(), &b->nwav));
-  hipLaunchKernelGGL(k_bflags, dim3(grid_for(n)), dim3(256), 0, c->stream, rp, n, wave_max_deg + 1,
-                     0xFFFFFFFFu, flags.as<uint8_t>());
+  hipLaunchKernelGGL(k_bflags, dim3(grid_for(n)), dim3(256), 0, c->stream, rp, wrp, n,
+                     wave_max_deg + 1, 0xFFFFFFFFu, flags.as<uint8_t>());
   TCX_TRY(select_indices(c, flags.as<uint8_t>(), n, b->blk_id.as<uint32_t>(), &b->nblk));
   // block tasks for the batched launch: descending work (longest first)
   const uint64_t nb = b->nblk;
@@ -223,9 +240,9 @@ int schedule(tc_ctx* c, tc_batch* b, uint32_t wave_max_deg) {
   DevBuf vals;
   TCX_TRY(vals.reserve(4 * (nb + 1)));
   if (nb) {
-    hipLaunchKernelGGL(k_bwork, dim3(grid_for(nb)), dim3(256), 0, c->stream, rp,
-                       g->col.as<uint32_t>(), b->blk_id.as<uint32_t>(), (uint32_t)nb,
-                       key.as<uint64_t>(), vals.as<uint32_t>());
+    hipLaunchKernelGGL(k_bwork, dim3(grid_for(nb)), dim3(256), 0, c->stream, rp, wrp, wcol,
+                       b->blk_id.as<uint32_t>(), (uint32_t)nb, key.as<uint64_t>(),
+                       vals.as<uint32_t>());
     TCX_HIP(hipGetLastError());
     uint64_t* sk;
     uint32_t* sv;
@@ -261,6 +278,7 @@ int enqueue_batched(tc_ctx* c, tc_batch* b, const tc_batch_sched& s) {
   hipLaunchKernelGGL(k_batch_reset, dim3(grid_for(2ull + b->ngraphs)), dim3(256), 0, c->stream,
                      b->cursors.as<uint32_t>(), 2u, b->counts.as<unsigned long long>(), b->ngraphs);
   launch_batch(s, dim3(s.grid), c->stream, g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(),
+               g->sc.wl_rp.as<uint32_t>(), g->sc.wl_col.as<uint32_t>(),
                b->gid.as<uint32_t>(), b->blk_lpt.as<uint32_t>(), (uint32_t)b->nblk,
                b->wav.as<uint32_t>(), (uint32_t)b->nwav, s.wave_chunk, b->cursors.as<uint32_t>(),
                b->counts.as<unsigned long long>());
@@ -340,6 +358,12 @@ int tc_batch_info_get(const tc_batch* b, tc_graph_info* info) {
   return tc_graph_info_get(b->g, info);
 }
 
+int tc_batch_stream_bytes(tc_ctx* c, tc_batch* b, double* bytes) {
+  TCX_TRY(check_batch(c, b));
+  if (!bytes) return fail(TC_ERR_ARG, "NULL output");
+  return tc_auto_stream_bytes(c, b->g, nullptr, bytes);
+}
+
 int tc_batch_count(tc_ctx* c, tc_batch* b, const tc_batch_sched* sched, uint64_t* counts,
                    tc_timing* timing) {
   TCX_TRY(check_batch(c, b));
@@ -408,6 +432,7 @@ int tc_batch_count_pergraph(tc_ctx* c, tc_batch* b, const tc_batch_sched* sched,
     const uint64_t need = (b1 - b0) + ((w1 - w0) + 3) / 4;
     const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(s.grid, need));
     launch_batch(s, dim3(grid), c->stream, g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(),
+                 g->sc.wl_rp.as<uint32_t>(), g->sc.wl_col.as<uint32_t>(),
                  b->gid.as<uint32_t>(), b->blk_id.as<uint32_t>() + b0, b1 - b0,
                  b->wav.as<uint32_t>() + w0, w1 - w0, s.wave_chunk, cur + 2 * q,
                  b->counts.as<unsigned long long>());

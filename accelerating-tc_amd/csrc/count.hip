@@ -767,9 +767,6 @@ static int auto_schedule(tc_ctx* c, tc_graph* g, uint32_t wmax, uint32_t hmin) {
   return TC_OK;
 }
 
-// Wave tasks whose work exceeds this many list entries run as block tasks
-// (a hub reached by many in-edges can carry millions).
-constexpr uint64_t kWlWaveWork = 1ull << 16;
 
 static int wl_schedule(tc_ctx* c, tc_graph* g, uint32_t wmax, uint32_t hmin, uint32_t lmax) {
   SchedCache& sc = g->sc;
@@ -853,6 +850,12 @@ static int wl_schedule(tc_ctx* c, tc_graph* g, uint32_t wmax, uint32_t hmin, uin
   sc.wl_hmin = hmin;
   sc.wl_lmax = lmax;
   return TC_OK;
+}
+
+int worklists(tc_ctx* c, tc_graph* g) {
+  ensure_version(g);
+  if (g->m_or == 0 || g->n == 0) return TC_OK;
+  return wl_schedule(c, g, 64u, 2048u, kLaneMaxDeg);
 }
 
 }  // namespace tcx

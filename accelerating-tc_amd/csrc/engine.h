@@ -33,6 +33,10 @@ constexpr uint32_t kFilterMin = 1;      // wave variant: bitmap for every d+(u) 
 constexpr uint32_t kLaneMaxDeg = 8;     // lane tasks: d+(x) up to this ...
 constexpr uint32_t kLaneMaxList = 16;   // ... and at most this many work-list entries
 constexpr uint32_t kLaneClaim = 256;    // lane tasks per wave claim (4 per lane)
+// Wave tasks whose work (d+(x) + sum of d+(y) over W(x)) exceeds this many
+// list entries run as block tasks (a hub reached by many in-edges can carry
+// millions).
+constexpr uint64_t kWlWaveWork = 1ull << 16;
 
 __device__ inline uint64_t gtid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ inline uint64_t gstride() { return (uint64_t)gridDim.x * blockDim.x; }
@@ -45,6 +49,35 @@ __device__ inline unsigned long long wave_sum(unsigned long long v) {
 __device__ inline void flush_count(unsigned long long cnt, unsigned long long* total) {
   cnt = wave_sum(cnt);
   if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(total, cnt);
+}
+
+// The workgroup's count in one atomic; every thread of the workgroup calls
+// it (red: TB / 64 words of LDS).
+__device__ inline void flush_block(unsigned long long cnt, unsigned long long* total,
+                                   unsigned long long* red) {
+  cnt = wave_sum(cnt);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long s = 0;
+    for (uint32_t w = 0; w < blockDim.x / 64; ++w) s += red[w];
+    if (s) atomicAdd(total, s);
+  }
+}
+
+// Claims `amount` entries of a task list of n from its cursor and returns
+// the first (n or more when the list is exhausted).  A worker's first claim
+// in a phase (first = true) reads the cursor before it adds: atomics on one
+// address are serialised (about 10-20 ns each), and without the read every
+// wave of a persistent grid would pay one per phase whose list ran out
+// before it got there (about 0.1 ms a launch).  Later claims add at once:
+// the read costs a memory round trip that short tasks would feel.  An
+// empty list costs nothing.
+__device__ inline uint32_t claim(uint32_t* cursor, uint32_t n, uint32_t amount, bool first) {
+  if (n == 0) return 0;
+  if (first && __hip_atomic_load(cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n)
+    return n;
+  return atomicAdd(cursor, amount);
 }
 
 // Histogram of ids: h[x] += 1 for every x of a[0, m) (and of b[0, m) when b
@@ -434,11 +467,12 @@ __global__ __launch_bounds__(TB) void k_auto(const uint32_t* __restrict__ rp,
                                              const uint32_t* __restrict__ wav, uint32_t nwav,
                                              uint32_t* cursor, unsigned long long* total) {
   __shared__ BlockLds<CAP> S;
+  __shared__ unsigned long long red[TB / 64];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   clear_bitmap(S);
   unsigned long long cnt = 0;
-  for (;;) {
-    if (tid == 0) S.t = atomicAdd(cursor, 1u);
+  for (bool first = true;; first = false) {
+    if (tid == 0) S.t = claim(cursor, nblk, 1u, first);
     __syncthreads();
     const uint32_t t = S.t;
     if (t >= nblk) break;
@@ -451,18 +485,18 @@ __global__ __launch_bounds__(TB) void k_auto(const uint32_t* __restrict__ rp,
     const uint32_t nw = gridDim.x * (TB / 64);
     for (uint32_t i = blockIdx.x * (TB / 64) + wv; i < nwav; i += nw)
       cnt += wave_vertex(rp, col, wav[i], lane, W);
-    flush_count(cnt, total);
+    flush_block(cnt, total, red);
     return;
   }
-  for (;;) {
+  for (bool first = true;; first = false) {
     uint32_t i0 = 0;
-    if (lane == 0) i0 = atomicAdd(cursor + 1, kWaveChunk);
+    if (lane == 0) i0 = claim(cursor + 1, nwav, kWaveChunk, first);
     i0 = __shfl(i0, 0, 64);
     if (i0 >= nwav) break;
     const uint32_t i1 = min(nwav, i0 + kWaveChunk);
     for (uint32_t i = i0; i < i1; ++i) cnt += wave_vertex(rp, col, wav[i], lane, W);
   }
-  flush_count(cnt, total);
+  flush_block(cnt, total, red);
 }
 
 // Lane task: one lane counts task x of the work-list schedule when
@@ -512,11 +546,12 @@ __global__ __launch_bounds__(TB) void k_auto_wl(const uint32_t* __restrict__ rp,
                                                 const uint32_t* __restrict__ lan, uint32_t nlan,
                                                 uint32_t* cursor, unsigned long long* total) {
   __shared__ BlockLds<CAP> S;
+  __shared__ unsigned long long red[TB / 64];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   clear_bitmap(S);
   unsigned long long cnt = 0;
-  for (;;) {
-    if (tid == 0) S.t = atomicAdd(cursor, 1u);
+  for (bool first = true;; first = false) {
+    if (tid == 0) S.t = claim(cursor, nblk, 1u, first);
     __syncthreads();
     const uint32_t t = S.t;
     if (t >= nblk) break;
@@ -533,9 +568,9 @@ __global__ __launch_bounds__(TB) void k_auto_wl(const uint32_t* __restrict__ rp,
       cnt += wave_vertex<kFilterMin, 2, true>(rp, col, x, lane, W, wcol + wb, wrp[x + 1] - wb);
     }
   } else {
-    for (;;) {
+    for (bool first = true;; first = false) {
       uint32_t i0 = 0;
-      if (lane == 0) i0 = atomicAdd(cursor + 1, kWaveChunk);
+      if (lane == 0) i0 = claim(cursor + 1, nwav, kWaveChunk, first);
       i0 = __shfl(i0, 0, 64);
       if (i0 >= nwav) break;
       const uint32_t i1 = min(nwav, i0 + kWaveChunk);
@@ -546,15 +581,15 @@ __global__ __launch_bounds__(TB) void k_auto_wl(const uint32_t* __restrict__ rp,
       }
     }
   }
-  for (;;) {
+  for (bool first = true;; first = false) {
     uint32_t i0 = 0;
-    if (lane == 0) i0 = atomicAdd(cursor + 2, kLaneClaim);
+    if (lane == 0) i0 = claim(cursor + 2, nlan, kLaneClaim, first);
     i0 = __shfl(i0, 0, 64);
     if (i0 >= nlan) break;
     for (uint32_t i = i0 + lane; i < min(nlan, i0 + kLaneClaim); i += 64)
       cnt += lane_task(rp, col, wrp, wcol, lan[i]);
   }
-  flush_count(cnt, total);
+  flush_block(cnt, total, red);
 }
 
 // Per-vertex form of k_auto: tri[u] = the triangles whose first vertex (the
@@ -571,8 +606,8 @@ __global__ __launch_bounds__(TB) void k_auto_per_vertex(const uint32_t* __restri
   __shared__ BlockLds<CAP> S;
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   clear_bitmap(S);
-  for (;;) {
-    if (tid == 0) S.t = atomicAdd(cursor, 1u);
+  for (bool first = true;; first = false) {
+    if (tid == 0) S.t = claim(cursor, nblk, 1u, first);
     __syncthreads();
     const uint32_t t = S.t;
     if (t >= nblk) break;
@@ -581,9 +616,9 @@ __global__ __launch_bounds__(TB) void k_auto_per_vertex(const uint32_t* __restri
     if (lane == 0 && c) atomicAdd(tri + u, c);
   }
   const WaveSlot W = wave_slot(S, wv);
-  for (;;) {
+  for (bool first = true;; first = false) {
     uint32_t i0 = 0;
-    if (lane == 0) i0 = atomicAdd(cursor + 1, kWaveChunk);
+    if (lane == 0) i0 = claim(cursor + 1, nwav, kWaveChunk, first);
     i0 = __shfl(i0, 0, 64);
     if (i0 >= nwav) break;
     const uint32_t i1 = min(nwav, i0 + kWaveChunk);

@@ -179,6 +179,9 @@ __global__ void k_adir_scan(uint32_t n, const int32_t* __restrict__ rtd, int32_t
   }
 }
 
+// A peeled neighbour takes 2 from the runtime degree (it is listed twice in
+// allEdges); a vertex is peeled when its degree first reaches the threshold,
+// which makes the labels independent of the order of the frontier.
 __global__ void k_adir_expand(const uint32_t* __restrict__ frontier, uint32_t fsize,
                               const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                               int32_t* rtd, int32_t* prio, float t, int32_t label,
@@ -196,6 +199,110 @@ __global__ void k_adir_expand(const uint32_t* __restrict__ frontier, uint32_t fs
         next[atomicAdd(ncount, 1u)] = y;
       }
     }
+  }
+}
+
+// ---- Bisson/A-direction.cpp degree record ----------------------------------
+// Bisson's peel (Bisson/A-direction.cpp:43-163) records degreeRecord[v] =
+// runTimeDegree[v] when the scan peels v (:96), but a cascade push writes the
+// slot of the pushing frontier element's INDEX, not of the pushed vertex
+// (:118 writes degreeRecord[i]).  The record therefore depends on the serial
+// frontier order, which these kernels rebuild: a round's first frontier is
+// ascending ids (:91-98); the next frontier lists pushed vertices in the order
+// of their crossing decrement, i.e. by (pusher index p, position e of the
+// vertex in the pusher's allEdges list).  allEdges[x] (:75-80 over the (i,
+// nbr) pairs of :253-258) is [N(x) below x] + [N(x)] + [N(x) above x], so a
+// neighbour y of rank r in N(x), with L neighbours below x, sits at
+// e1 = r, e2 = L + r when y < x and at e1 = L + r, e2 = deg(x) + r when y > x.
+
+__global__ void k_adir_scan_flags(uint32_t n, const int32_t* __restrict__ rtd,
+                                  const int32_t* __restrict__ prio, float t,
+                                  uint8_t* __restrict__ flags) {
+  for (uint64_t v = gtid(); v < n; v += gstride())
+    flags[v] = (prio[v] == -1 && (float)rtd[v] <= t) ? 1 : 0;
+}
+
+__global__ void k_adir_mark_scan(const uint32_t* __restrict__ F, uint32_t fs,
+                                 const int32_t* __restrict__ rtd, int32_t* __restrict__ prio,
+                                 int32_t label, int32_t* __restrict__ rec) {
+  for (uint64_t i = gtid(); i < fs; i += gstride()) {
+    const uint32_t v = F[i];
+    prio[v] = label;
+    rec[v] = rtd[v];
+  }
+}
+
+// One event (y << 32 | p) per frontier element p and unpeeled neighbour y
+// (each stands for y's two entries in allEdges[F[p]]); rtd[y] -= 2.
+__global__ void k_adir_events(const uint32_t* __restrict__ F, uint32_t fs,
+                              const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+                              int32_t* rtd, const int32_t* __restrict__ prio,
+                              uint64_t* __restrict__ ev, unsigned long long* ecount) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t wave = gtid() >> 6, nwaves = gstride() >> 6;
+  for (uint64_t p = wave; p < fs; p += nwaves) {
+    const uint32_t x = F[p], b = rp[x], e = rp[x + 1];
+    for (uint32_t base = b; base < e; base += 64) {
+      const uint32_t i = base + lane;
+      const uint32_t y = i < e ? col[i] : 0;
+      const bool live = i < e && prio[y] == -1;
+      const uint64_t mask = __ballot(live);
+      unsigned long long at = 0;
+      if (lane == 0 && mask) at = atomicAdd(ecount, (unsigned long long)__popcll(mask));
+      at = __shfl(at, 0, 64);
+      if (live) {
+        atomicSub(rtd + y, 2);
+        ev[at + __popcll(mask & ((1ull << lane) - 1))] = ((uint64_t)y << 32) | p;
+      }
+    }
+  }
+}
+
+__device__ inline uint32_t lower_bound_u32(const uint32_t* a, uint32_t lo, uint32_t hi,
+                                           uint32_t v) {
+  while (lo < hi) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    if (a[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Over the events sorted by (y, p): y's k-th decrement (k = its runtime
+// degree at the level's start - floor(t)) peels it, if it has that many.
+// The j = ceil(k/2)-th pusher p* made it, at e1 (k odd) or e2; slot p* of
+// the record takes floor(t) (the runtime degree right after the crossing).
+// Pushed vertices go out with the key (p* << 32 | e*) of the next order.
+__global__ void k_adir_cross(const uint64_t* __restrict__ ev, uint64_t E,
+                             const uint32_t* __restrict__ F, const uint32_t* __restrict__ rp,
+                             const uint32_t* __restrict__ col, const int32_t* __restrict__ rtd,
+                             int32_t* __restrict__ prio, int32_t tfloor, int32_t label,
+                             int32_t* __restrict__ rec, uint64_t* __restrict__ nkey,
+                             uint32_t* __restrict__ nid, uint32_t* ncount) {
+  for (uint64_t i = gtid(); i < E; i += gstride()) {
+    const uint32_t y = (uint32_t)(ev[i] >> 32);
+    if (i > 0 && (uint32_t)(ev[i - 1] >> 32) == y) continue;
+    uint64_t lo = i, hi = E;  // first event of a larger y
+    const uint64_t bound = ((uint64_t)y + 1) << 32;
+    while (lo < hi) {
+      const uint64_t mid = lo + ((hi - lo) >> 1);
+      if (ev[mid] < bound) lo = mid + 1;
+      else hi = mid;
+    }
+    const int64_t cy = (int64_t)(lo - i);
+    const int64_t k = (int64_t)rtd[y] + 2 * cy - tfloor;
+    if (k < 1 || k > 2 * cy) continue;
+    const uint32_t p = (uint32_t)ev[i + (uint64_t)((k + 1) / 2) - 1];
+    const uint32_t x = F[p], b = rp[x], e = rp[x + 1];
+    const uint32_t r = lower_bound_u32(col, b, e, y) - b;
+    const uint32_t L = lower_bound_u32(col, b, e, x) - b;
+    const uint32_t e1 = y < x ? r : L + r;
+    const uint32_t e2 = y < x ? L + r : (e - b) + r;
+    prio[y] = label;
+    rec[p] = tfloor;
+    const uint32_t at = atomicAdd(ncount, 1u);
+    nkey[at] = ((uint64_t)p << 32) | ((k & 1) ? e1 : e2);
+    nid[at] = y;
   }
 }
 
@@ -265,7 +372,8 @@ __global__ void k_sym_keep(const uint64_t* __restrict__ keys, uint64_t m2,
     uint32_t i = (uint32_t)(kk >> 32), j = (uint32_t)kk;
     int32_t pi = key[i], pj = key[j];
     bool k;
-    if (mode == TC_ORIENT_ADIR) k = pj > pi || (pj == pi && i < j);  // A-direction-seri.cpp:259-261
+    if (mode == TC_ORIENT_ADIR || mode == TC_ORIENT_ADIR_OMP)
+      k = pj > pi || (pj == pi && i < j);  // A-direction-seri.cpp:259-261, A-direction.cpp:266-268
     else k = pi < pj || (pi == pj && i < j);                         // kcore_direction.cpp:137-138
     keep[e] = k ? 1 : 0;
   }
@@ -312,6 +420,17 @@ __global__ void k_degree_keys(const uint32_t* __restrict__ deg, uint32_t n,
                               uint64_t* __restrict__ key, uint32_t* __restrict__ ids) {
   for (uint64_t v = gtid(); v < n; v += gstride()) {
     key[v] = ((uint64_t)deg[v] << 32) | v;
+    ids[v] = (uint32_t)v;
+  }
+}
+
+// Bisson/A-direction.cpp:287-293 order: (priority, runtime degree at peel)
+// ascending; ties keep ascending ids (stable sort of ids in order)
+__global__ void k_adir_relabel_keys(const int32_t* __restrict__ prio,
+                                    const int32_t* __restrict__ rec, uint32_t n,
+                                    uint64_t* __restrict__ key, uint32_t* __restrict__ ids) {
+  for (uint64_t v = gtid(); v < n; v += gstride()) {
+    key[v] = ((uint64_t)(uint32_t)prio[v] << 32) | (uint32_t)rec[v];
     ids[v] = (uint32_t)v;
   }
 }
@@ -462,51 +581,142 @@ static int sym_csr(tc_ctx* c, const uint64_t* keys, uint64_t m2, uint32_t n, Dev
   return TC_OK;
 }
 
+// One round-level of Bisson's peel with the serial frontier order (see the
+// degree-record kernels): frontier F (fs, in serial order) -> next frontier
+// N in serial order; *ns its size.
+struct AdirOrdered {
+  DevBuf ev, evtmp, nkey, nktmp, nid, nitmp, flags;
+};
+
+static int adir_level_ordered(tc_ctx* c, AdirOrdered& o, const uint32_t* F, uint32_t fs,
+                              const uint32_t* rp, const uint32_t* col, int32_t* rtd,
+                              int32_t* prio, float t, int32_t label, int32_t* rec, uint32_t* N,
+                              uint32_t* ns) {
+  unsigned long long* ecount = c->accum.as<unsigned long long>();
+  uint32_t* cnt = c->scratch.as<uint32_t>();
+  TCX_HIP(hipMemsetAsync(ecount, 0, 8, c->stream));
+  hipLaunchKernelGGL(k_adir_events, dim3(std::min<unsigned>(4096, grid_for(fs * 64ull))),
+                     dim3(TB), 0, c->stream, F, fs, rp, col, rtd, prio, o.ev.as<uint64_t>(),
+                     ecount);
+  TCX_HIP(hipGetLastError());
+  unsigned long long E = 0;
+  TCX_HIP(hipMemcpyAsync(&E, ecount, 8, hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  *ns = 0;
+  if (E == 0) return TC_OK;
+  uint64_t* sorted;
+  TCX_TRY(sort_u64(c, o.ev.as<uint64_t>(), o.evtmp.as<uint64_t>(), E, 64, &sorted));
+  TCX_HIP(hipMemsetAsync(cnt, 0, 4, c->stream));
+  hipLaunchKernelGGL(k_adir_cross, dim3(grid_for(E)), dim3(TB), 0, c->stream, sorted,
+                     (uint64_t)E, F, rp, col, rtd, prio, (int32_t)floorf(t), label, rec,
+                     o.nkey.as<uint64_t>(), o.nid.as<uint32_t>(), cnt);
+  TCX_HIP(hipGetLastError());
+  uint32_t pushed = 0;
+  TCX_TRY(read_u32(c, cnt, &pushed));
+  if (pushed) {
+    uint64_t* sk;
+    uint32_t* order;
+    TCX_TRY(sort_pairs_u64_u32(c, o.nkey.as<uint64_t>(), o.nktmp.as<uint64_t>(),
+                               o.nid.as<uint32_t>(), o.nitmp.as<uint32_t>(), pushed, 64, &sk,
+                               &order));
+    TCX_HIP(hipMemcpyAsync(N, order, 4ull * pushed, hipMemcpyDeviceToDevice, c->stream));
+  }
+  *ns = pushed;
+  return TC_OK;
+}
+
+// A-direction peel.  With rec non-NULL it also fills Bisson's degree record
+// (zero where nothing is written, as the reference's memset leaves it), which
+// needs the serial frontier order; without it the frontiers are unordered.
+// thre_param > 0 selects the OpenMP program's schedule
+// (edge_directing/A-direction.cpp:36-161): threshold0 = 2 totalEdge / n, a
+// round's cascade runs only while its frontier holds more than n / 5000 and
+// more than 200 vertices, for at most six levels, and the threshold grows by
+// (int)threshold0 * thre_param / 5 per round.
 static int adir_peel(tc_ctx* c, uint32_t n, uint64_t m2, const uint32_t* rp, const uint32_t* col,
-                     int32_t* prio) {
+                     int32_t* prio, int32_t* rec = nullptr, double thre_param = 0) {
+  const bool omp = thre_param > 0;
   DevBuf rtd, fa, fb;
   TCX_TRY(rtd.reserve(sizeof(int32_t) * ((size_t)n + 1)));
   TCX_TRY(fa.reserve(sizeof(uint32_t) * ((size_t)n + 1)));
   TCX_TRY(fb.reserve(sizeof(uint32_t) * ((size_t)n + 1)));
   TCX_TRY(c->scratch.reserve(64));
+  TCX_TRY(c->accum.reserve(64));
   uint32_t* cnt = c->scratch.as<uint32_t>();
+  AdirOrdered ord;
+  if (rec) {
+    TCX_TRY(ord.ev.reserve(8 * std::max<uint64_t>(m2, 1)));
+    TCX_TRY(ord.evtmp.reserve(8 * std::max<uint64_t>(m2, 1)));
+    TCX_TRY(ord.nkey.reserve(8 * ((size_t)n + 1)));
+    TCX_TRY(ord.nktmp.reserve(8 * ((size_t)n + 1)));
+    TCX_TRY(ord.nid.reserve(4 * ((size_t)n + 1)));
+    TCX_TRY(ord.nitmp.reserve(4 * ((size_t)n + 1)));
+    TCX_TRY(ord.flags.reserve((size_t)n + 1));
+    TCX_HIP(hipMemsetAsync(rec, 0, 4ull * n, c->stream));
+  }
   hipLaunchKernelGGL(k_adir_init, dim3(grid_for(n)), dim3(TB), 0, c->stream, rp, n,
                      rtd.as<int32_t>(), prio);
   TCX_HIP(hipGetLastError());
-  // A-direction-seri.cpp:37 and :77, with edgeNum = 2m (totalEdge, :228,:252)
+  // A-direction-seri.cpp:37 and :77, with edgeNum = 2m (totalEdge, :228,:252);
+  // A-direction.cpp:38,78
   unsigned int edgeNum = (unsigned int)m2;
   int nodeNum = (int)n;
-  float threshold = ((float)edgeNum) / nodeNum + 1;
+  float threshold = omp ? 2 * ((float)edgeNum) / nodeNum : ((float)edgeNum) / nodeNum + 1;
   int threshold_old = threshold;
+  auto step = [&]() {
+    if (omp) threshold += (double)threshold_old * thre_param / 5;  // A-direction.cpp:139
+    else threshold += threshold_old;                               // A-direction-seri.cpp:127
+  };
   int32_t label = 0;
   uint64_t peeled = 0;
   uint32_t* F = fa.as<uint32_t>();
   uint32_t* N = fb.as<uint32_t>();
   const unsigned expand_grid = 4096;
   for (;;) {
-    TCX_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), c->stream));
-    hipLaunchKernelGGL(k_adir_scan, dim3(grid_for(n)), dim3(TB), 0, c->stream, n,
-                       rtd.as<int32_t>(), prio, threshold, label, F, cnt);
-    TCX_HIP(hipGetLastError());
     uint32_t fs = 0;
-    TCX_TRY(read_u32(c, cnt, &fs));
+    if (rec) {  // ascending ids, as the serial scan appends them
+      hipLaunchKernelGGL(k_adir_scan_flags, dim3(grid_for(n)), dim3(TB), 0, c->stream, n,
+                         rtd.as<int32_t>(), prio, threshold, ord.flags.as<uint8_t>());
+      TCX_HIP(hipGetLastError());
+      uint64_t f64 = 0;
+      TCX_TRY(select_indices(c, ord.flags.as<uint8_t>(), n, F, &f64));
+      fs = (uint32_t)f64;
+      if (fs)
+        hipLaunchKernelGGL(k_adir_mark_scan, dim3(grid_for(fs)), dim3(TB), 0, c->stream, F, fs,
+                           rtd.as<int32_t>(), prio, label, rec);
+      TCX_HIP(hipGetLastError());
+    } else {
+      TCX_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), c->stream));
+      hipLaunchKernelGGL(k_adir_scan, dim3(grid_for(n)), dim3(TB), 0, c->stream, n,
+                         rtd.as<int32_t>(), prio, threshold, label, F, cnt);
+      TCX_HIP(hipGetLastError());
+      TCX_TRY(read_u32(c, cnt, &fs));
+    }
     peeled += fs;
     label++;
-    while (fs > 0) {
-      TCX_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), c->stream));
-      hipLaunchKernelGGL(k_adir_expand, dim3(std::min<unsigned>(expand_grid, grid_for(fs * 64ull))),
-                         dim3(TB), 0, c->stream, F, fs, rp, col, rtd.as<int32_t>(), prio,
-                         threshold, label, N, cnt);
-      TCX_HIP(hipGetLastError());
+    int inner = 0;
+    while (fs > 0 && (!omp || (fs > (uint32_t)(nodeNum / 5000) && fs > 200 && inner <= 5))) {
+      inner++;
       uint32_t ns = 0;
-      TCX_TRY(read_u32(c, cnt, &ns));
+      if (rec) {
+        TCX_TRY(adir_level_ordered(c, ord, F, fs, rp, col, rtd.as<int32_t>(), prio, threshold,
+                                   label, rec, N, &ns));
+      } else {
+        TCX_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), c->stream));
+        hipLaunchKernelGGL(k_adir_expand,
+                           dim3(std::min<unsigned>(expand_grid, grid_for(fs * 64ull))), dim3(TB),
+                           0, c->stream, F, fs, rp, col, rtd.as<int32_t>(), prio, threshold,
+                           label, N, cnt);
+        TCX_HIP(hipGetLastError());
+        TCX_TRY(read_u32(c, cnt, &ns));
+      }
       peeled += ns;
       std::swap(F, N);
       fs = ns;
       label++;
     }
     if (peeled >= n) break;
-    threshold += threshold_old;  // :127
+    step();
     // Rounds whose scan finds nothing only advance the label and threshold.
     TCX_HIP(hipMemsetAsync(cnt, 0x7f, sizeof(int32_t), c->stream));
     hipLaunchKernelGGL(k_min_unpeeled, dim3(grid_for(n)), dim3(TB), 0, c->stream, n,
@@ -516,7 +726,7 @@ static int adir_peel(tc_ctx* c, uint32_t n, uint64_t m2, const uint32_t* rp, con
     TCX_TRY(read_u32(c, cnt, (uint32_t*)&minr));
     while (!((float)minr <= threshold)) {
       label++;
-      threshold += threshold_old;
+      step();
     }
   }
   return TC_OK;
@@ -769,7 +979,7 @@ int tc_graph_relabel_aorder_model(tc_ctx* c, tc_graph* g, const tc_bw_model* mod
 int tc_graph_relabel(tc_ctx* c, tc_graph* g, int mode, uint32_t* perm_out) {
   TCX_TRY(check_records(g));
   if (c != g->ctx) return fail(TC_ERR_ARG, "graph belongs to another context");
-  if (mode < TC_RELABEL_COMPACT || mode > TC_RELABEL_DEGREE)
+  if (mode < TC_RELABEL_COMPACT || mode > TC_RELABEL_ADIR)
     return fail(TC_ERR_ARG, "unknown relabel mode %d", mode);
   const uint64_t m = g->m_raw;
   uint32_t* s = g->src.as<uint32_t>();
@@ -806,7 +1016,21 @@ int tc_graph_relabel(tc_ctx* c, tc_graph* g, int mode, uint32_t* perm_out) {
     TCX_TRY(ktmp.reserve(8ull * n));
     TCX_TRY(ids.reserve(4ull * n));
     TCX_TRY(itmp.reserve(4ull * n));
-    if (mode == TC_RELABEL_FIRST_SRC) {
+    if (mode == TC_RELABEL_ADIR) {
+      // peel the simple symmetric graph over the tools' n = max id + 1
+      DevBuf skeys_buf, stmp, srp, scol, prio, rec;
+      uint64_t* skeys = nullptr;
+      uint64_t m2 = 0;
+      TCX_TRY(sym_keys(c, g, skeys_buf, stmp, &skeys, &m2));
+      TCX_TRY(sym_csr(c, skeys, m2, n, srp, scol));
+      TCX_TRY(prio.reserve(4ull * n));
+      TCX_TRY(rec.reserve(4ull * n));
+      TCX_TRY(adir_peel(c, n, m2, srp.as<uint32_t>(), scol.as<uint32_t>(), prio.as<int32_t>(),
+                        rec.as<int32_t>()));
+      hipLaunchKernelGGL(k_adir_relabel_keys, dim3(grid_for(n)), dim3(TB), 0, c->stream,
+                         prio.as<int32_t>(), rec.as<int32_t>(), n, key.as<uint64_t>(),
+                         ids.as<uint32_t>());
+    } else if (mode == TC_RELABEL_FIRST_SRC) {
       DevBuf first;
       TCX_TRY(first.reserve(8ull * n));
       hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n)), dim3(TB), 0, c->stream,
@@ -826,7 +1050,7 @@ int tc_graph_relabel(tc_ctx* c, tc_graph* g, int mode, uint32_t* perm_out) {
     TCX_TRY(sort_pairs_u64_u32(c, key.as<uint64_t>(), ktmp.as<uint64_t>(), ids.as<uint32_t>(),
                                itmp.as<uint32_t>(), n, 64, &sk, &order));
     hipLaunchKernelGGL(k_rank_perm, dim3(grid_for(n)), dim3(TB), 0, c->stream, order, n,
-                       mode == TC_RELABEL_DEGREE, perm.as<uint32_t>());
+                       mode == TC_RELABEL_DEGREE || mode == TC_RELABEL_ADIR, perm.as<uint32_t>());
   }
   if (m)
     hipLaunchKernelGGL(k_apply_perm, dim3(grid_for(m)), dim3(TB), 0, c->stream, s, d, m,
@@ -839,6 +1063,8 @@ int tc_graph_relabel(tc_ctx* c, tc_graph* g, int mode, uint32_t* perm_out) {
   g->orient_mode = -1;  // any CSR built before is stale
   return TC_OK;
 }
+
+static int orient_peeled(tc_ctx* c, tc_graph* g, int mode, double thre_param);
 
 int tc_graph_orient(tc_ctx* c, tc_graph* g, int mode, uint32_t flags) {
   TCX_TRY(check_records(g));
@@ -888,8 +1114,13 @@ int tc_graph_orient(tc_ctx* c, tc_graph* g, int mode, uint32_t flags) {
     g->orient_mode = mode;
     return TC_OK;
   }
-  // A-direction / k-core: peel the simple symmetric graph, then keep the
-  // symmetric entries that point up the peel order.
+  return orient_peeled(c, g, mode, 0);
+}
+
+// A-direction / k-core: peel the simple symmetric graph, then keep the
+// symmetric entries that point up the peel order.
+static int orient_peeled(tc_ctx* c, tc_graph* g, int mode, double thre_param) {
+  const uint32_t n = g->n_raw;
   DevBuf keys, tmp, srp, scol, keep;
   uint64_t* skeys = nullptr;
   uint64_t m2 = 0;
@@ -897,8 +1128,9 @@ int tc_graph_orient(tc_ctx* c, tc_graph* g, int mode, uint32_t flags) {
   TCX_TRY(sym_csr(c, skeys, m2, n, srp, scol));
   TCX_TRY(g->vkey.reserve(4 * ((size_t)n + 1)));
   if (n > 0) {
-    if (mode == TC_ORIENT_ADIR) {
-      TCX_TRY(adir_peel(c, n, m2, srp.as<uint32_t>(), scol.as<uint32_t>(), g->vkey.as<int32_t>()));
+    if (mode == TC_ORIENT_ADIR || mode == TC_ORIENT_ADIR_OMP) {
+      TCX_TRY(adir_peel(c, n, m2, srp.as<uint32_t>(), scol.as<uint32_t>(), g->vkey.as<int32_t>(),
+                        nullptr, mode == TC_ORIENT_ADIR_OMP ? thre_param : 0));
     } else if (mode == TC_ORIENT_KCORE) {
       TCX_TRY(kcore_peel(c, n, srp.as<uint32_t>(), scol.as<uint32_t>(), g->vkey.as<int32_t>()));
     } else {
@@ -922,6 +1154,15 @@ int tc_graph_orient(tc_ctx* c, tc_graph* g, int mode, uint32_t flags) {
   return TC_OK;
 }
 
+int tc_graph_orient_adir_omp(tc_ctx* c, tc_graph* g, double thre_param) {
+  TCX_TRY(check_records(g));
+  if (c != g->ctx) return fail(TC_ERR_ARG, "graph belongs to another context");
+  if (!(thre_param > 0) || !std::isfinite(thre_param))
+    return fail(TC_ERR_ARG, "thre_param must be finite and positive (the threshold never grows "
+                            "otherwise)");
+  return orient_peeled(c, g, TC_ORIENT_ADIR_OMP, thre_param);
+}
+
 // Stage file of an oriented CSR: "TCAMDCSR", uint32 version 1, uint32 n,
 // uint64 m, int32 orientation, uint32 has_key, rowptr[n+1], col[m] (uint32),
 // then key[n] (int32) when has_key; little endian.
@@ -937,7 +1178,8 @@ static_assert(sizeof(CsrFileHeader) == 32, "stage file header is 32 bytes");
 constexpr char kCsrMagic[8] = {'T', 'C', 'A', 'M', 'D', 'C', 'S', 'R'};
 
 bool has_vertex_key(int mode) {
-  return mode == TC_ORIENT_ADIR || mode == TC_ORIENT_KCORE || mode == TC_ORIENT_DEGENERACY;
+  return mode == TC_ORIENT_ADIR || mode == TC_ORIENT_KCORE || mode == TC_ORIENT_DEGENERACY ||
+         mode == TC_ORIENT_ADIR_OMP;
 }
 }  // namespace
 
@@ -979,7 +1221,7 @@ int tc_graph_load_csr(tc_ctx* c, const char* path, tc_graph** out) {
   if (!f) return fail(TC_ERR_IO, "cannot open %s", path);
   CsrFileHeader h;
   if (fread(&h, sizeof(h), 1, f) != 1 || memcmp(h.magic, kCsrMagic, 8) != 0 || h.version != 1 ||
-      h.orient_mode < TC_ORIENT_ID || h.orient_mode > TC_ORIENT_DEGENERACY ||
+      h.orient_mode < TC_ORIENT_ID || h.orient_mode > TC_ORIENT_ADIR_OMP ||
       h.has_key != (has_vertex_key(h.orient_mode) ? 1u : 0u) || h.n > 0x7FFFFFFEu ||
       h.m >= 0xFFFFFFFFull) {
     fclose(f);
@@ -1087,8 +1329,9 @@ int tc_graph_get_edges(const tc_graph* g, uint32_t* src, uint32_t* dst) {
 int tc_graph_get_vertex_key(const tc_graph* g, int32_t* key) {
   if (!g || !g->ctx || !key) return fail(TC_ERR_ARG, "NULL argument");
   if (g->orient_mode != TC_ORIENT_ADIR && g->orient_mode != TC_ORIENT_KCORE &&
-      g->orient_mode != TC_ORIENT_DEGENERACY)
-    return fail(TC_ERR_STATE, "vertex keys exist after ADIR, KCORE or DEGENERACY orientation only");
+      g->orient_mode != TC_ORIENT_DEGENERACY && g->orient_mode != TC_ORIENT_ADIR_OMP)
+    return fail(TC_ERR_STATE,
+                "vertex keys exist after ADIR, ADIR_OMP, KCORE or DEGENERACY orientation only");
   TCX_HIP(hipSetDevice(g->ctx->device));
   if (g->n)
     TCX_HIP(hipMemcpyAsync(key, g->vkey.p, 4 * (size_t)g->n, hipMemcpyDeviceToHost, g->ctx->stream));

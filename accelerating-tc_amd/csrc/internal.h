@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/tc_amd.h"
@@ -45,6 +46,11 @@ struct DevBuf {
   void release();
   template <class T> T* as() const { return static_cast<T*>(p); }
 };
+
+inline void swap(DevBuf& a, DevBuf& b) {
+  std::swap(a.p, b.p);
+  std::swap(a.bytes, b.bytes);
+}
 
 }  // namespace tcx
 
@@ -94,8 +100,9 @@ struct SchedCache {
   float ms_prep = 0;
   // learned schedule (per model id)
   uint64_t learned_version = ~0ull, learned_model = ~0ull;
-  DevBuf l_blk, l_wav, l_huge;
-  uint64_t nl_blk = 0, nl_wav = 0, nl_huge = 0;
+  DevBuf l_blk, l_wav, l_huge, l_lan;
+  uint64_t nl_blk = 0, nl_wav = 0, nl_huge = 0, nl_lan = 0;
+  uint32_t learned_mask = 0;  // cells whose tasks run narrow
   // learned k-clique plan (per k and model id)
   uint64_t kcl_version = ~0ull, kcl_model = ~0ull;
   uint32_t kcl_k = 0;
@@ -163,6 +170,11 @@ int build_sym(tc_ctx* c, tc_graph* g, DevBuf& rowptr, DevBuf& col, uint64_t* m2)
 // vertex count is the records' largest id + 1.  A graph without raw records
 // (csr_only) gives its oriented CSR's edges instead.
 int graph_clone_records(tc_ctx* c, const tc_graph* g, tc_graph** out);
+
+// ---- work lists of the AUTO engine (count.hip) ---------------------------
+// Builds (or keeps) the default work-list schedule of g: sc.wl_rp / wl_col
+// and the fixed rule's task lists.
+int worklists(tc_ctx* c, tc_graph* g);
 
 // ---- cost model and per-vertex counts (mlp.hip, count.hip) ---------------
 // Predicted log2 costs Y[n][2] for device features X[n][8]; fails when the

@@ -1,12 +1,14 @@
 // Learned scheduling for the AUTO engine.
 //
-//  * per-vertex features of the oriented CSR (k_features);
-//  * measured per-vertex cost of the block and wave variants (k_time_*);
 //  * an 8-32-32-2 MLP with BF16 weights and FP32 accumulation, trained by
 //    this file's forward / backward / Adam kernels (deterministic gradient
 //    reduction: per-tile partial sums, then a fixed-order tile sum);
-//  * the learned schedule: each vertex goes to the variant with the lower
-//    predicted cost, both task lists ordered by descending predicted cost.
+//  * cell features of the work-list schedule (k_cell_accum) and timed count
+//    launches of one cell at a time under either variant (the training data);
+//  * the learned schedule: per cell, the variant the model predicts faster,
+//    with the fixed rule as the prior;
+//  * per-task features and per-task timed variants (k_wl_features,
+//    k_time_wl_*), kept as diagnostics.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -211,48 +213,75 @@ __global__ void k_to_bf16(const float* __restrict__ w, uint16_t* __restrict__ wb
   for (uint64_t p = gtid(); p < (uint64_t)P; p += gstride()) wbf[p] = to_bf16(w[p]);
 }
 
-// ---- features ------------------------------------------------------------
-__global__ void k_features(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
-                           const uint32_t* __restrict__ indeg, uint32_t n, float* __restrict__ X) {
-  for (uint64_t u = gtid(); u < n; u += gstride()) {
-    const uint32_t b = rp[u], e = rp[u + 1], du = e - b;
-    double w2 = 0, smin = 0, chunks = 0;
+// ---- features of the work-list tasks ---------------------------------------
+// The bandwidth model of tc_bw_model on the device: the rate (GB/s) at which
+// a list of `len` entries is read.  Lists shorter than two entries take the
+// two-entry rate (the model starts there).
+struct BwTable {
+  float table[10];
+  double a, b;
+};
+
+__device__ inline float bw_rate(const BwTable& m, uint32_t len) {
+  if (len < 10) return m.table[len < 2 ? 2 : len];
+  return (float)(m.a * pow((double)len, m.b));
+}
+
+// Task x of the work-list schedule: N+(x) staged, N+(y) streamed for y in
+// W(x).  Its eight features (0 for x without a task):
+//   log2(1+d+(x)), log2(1+|W(x)|), log2(1+sum d+(y)), log2(1+max d+(y)),
+//   log2(1+t) with t the ns its lists take to read under the bandwidth model
+//   (sum over y of 4 d+(y) bytes / rate(d+(y))), floor(log2(1+d+(x))), the
+//   fraction of y with d+(y) < 64, log2(1+sum ceil(d+(y)/64)).
+__global__ void k_wl_features(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ wrp,
+                              const uint32_t* __restrict__ wcol, uint32_t n, BwTable bw,
+                              float* __restrict__ X) {
+  for (uint64_t x = gtid(); x < n; x += gstride()) {
+    const uint32_t dx = rp[x + 1] - rp[x];
+    const uint32_t wb = wrp[x], we = wrp[x + 1], nw = we - wb;
+    double sum = 0, chunks = 0, tns = 0;
     uint32_t mx = 0, shortc = 0;
-    for (uint32_t k = b; k < e; ++k) {
-      const uint32_t v = col[k];
-      const uint32_t dv = rp[v + 1] - rp[v];
-      w2 += dv;
-      smin += dv < du ? dv : du;
-      chunks += (dv + 63) / 64;
-      mx = dv > mx ? dv : mx;
-      shortc += dv < 64;
+    for (uint32_t i = wb; i < we; ++i) {
+      const uint32_t y = wcol[i];
+      const uint32_t dy = rp[y + 1] - rp[y];
+      sum += dy;
+      chunks += (dy + 63) / 64;
+      mx = dy > mx ? dy : mx;
+      shortc += dy < 64;
+      tns += 4.0 * dy / bw_rate(bw, dy);
     }
-    float* x = X + u * F;
-    x[0] = log2f(1.f + du);
-    x[1] = log2f(1.f + indeg[u]);
-    x[2] = (float)log2(1.0 + w2);
-    x[3] = log2f(1.f + mx);
-    x[4] = (float)log2(1.0 + smin);
-    x[5] = floorf(log2f(1.f + du));
-    x[6] = du ? (float)shortc / du : 0.f;
-    x[7] = (float)log2(1.0 + chunks);
+    float* f = X + x * F;
+    const bool task = nw > 0 && dx >= 2;
+    f[0] = task ? log2f(1.f + dx) : 0.f;
+    f[1] = task ? log2f(1.f + nw) : 0.f;
+    f[2] = task ? (float)log2(1.0 + sum) : 0.f;
+    f[3] = task ? log2f(1.f + mx) : 0.f;
+    f[4] = task ? (float)log2(1.0 + tns) : 0.f;
+    f[5] = task ? floorf(log2f(1.f + dx)) : 0.f;
+    f[6] = task ? (float)shortc / nw : 0.f;
+    f[7] = task ? (float)log2(1.0 + chunks) : 0.f;
   }
 }
 
 // ---- timed variants (cost measurement) -------------------------------------
+// Costs are wave-microseconds (in clock ticks here): the time a task holds
+// one wave.  A block task holds four waves for its duration, a lane task one
+// 64th of its wave's round.
 template <int CAP>
-__global__ __launch_bounds__(TB) void k_time_block(const uint32_t* __restrict__ rp,
-                                                   const uint32_t* __restrict__ col,
-                                                   const uint32_t* __restrict__ verts, uint32_t n,
-                                                   uint32_t* cursor, float* __restrict__ cost,
-                                                   unsigned long long* total) {
+__global__ __launch_bounds__(TB) void k_time_wl_block(const uint32_t* __restrict__ rp,
+                                                      const uint32_t* __restrict__ col,
+                                                      const uint32_t* __restrict__ wrp,
+                                                      const uint32_t* __restrict__ wcol,
+                                                      const uint32_t* __restrict__ verts,
+                                                      uint32_t n, uint32_t* cursor,
+                                                      float* __restrict__ cost,
+                                                      unsigned long long* total) {
   __shared__ BlockLds<CAP> S;
   const uint32_t tid = threadIdx.x;
   clear_bitmap(S);
   unsigned long long cnt = 0;
   for (;;) {
-    // the task's cost includes fetching it (cursor atomic + barrier): for
-    // very small vertices that overhead is most of what the variant costs
+    // the task's cost includes fetching it (cursor atomic + barrier)
     uint64_t t0 = 0;
     if (tid == 0) {
       t0 = wall_clock64();
@@ -261,24 +290,27 @@ __global__ __launch_bounds__(TB) void k_time_block(const uint32_t* __restrict__ 
     __syncthreads();
     const uint32_t t = S.t;
     if (t >= n) break;
-    cnt += block_vertex<CAP>(rp, col, verts[t], S);  // ends with a barrier
-    if (tid == 0) cost[verts[t]] = (float)(wall_clock64() - t0);
+    const uint32_t x = verts[t], wb = wrp[x];
+    cnt += block_vertex<CAP, 2, true>(rp, col, x, S, wcol + wb, wrp[x + 1] - wb);  // ends with a barrier
+    if (tid == 0) cost[x] = 4.f * (float)(wall_clock64() - t0);
   }
   flush_count(cnt, total);
 }
 
-__global__ __launch_bounds__(TB) void k_time_wave(const uint32_t* __restrict__ rp,
-                                                  const uint32_t* __restrict__ col,
-                                                  const uint32_t* __restrict__ verts, uint32_t n,
-                                                  uint32_t* cursor, float* __restrict__ cost,
-                                                  unsigned long long* total) {
+__global__ __launch_bounds__(TB) void k_time_wl_wave(const uint32_t* __restrict__ rp,
+                                                     const uint32_t* __restrict__ col,
+                                                     const uint32_t* __restrict__ wrp,
+                                                     const uint32_t* __restrict__ wcol,
+                                                     const uint32_t* __restrict__ verts, uint32_t n,
+                                                     uint32_t* cursor, float* __restrict__ cost,
+                                                     unsigned long long* total) {
   __shared__ BlockLds<2048> S;
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   clear_bitmap(S);
   const WaveSlot W = wave_slot(S, wv);
   unsigned long long cnt = 0;
-  // tasks are fetched kWaveChunk at a time, as the count kernel does; each
-  // task's cost carries its share of the fetch
+  // fetched kWaveChunk at a time, as the count kernel does; each task's cost
+  // carries its share of the fetch
   for (;;) {
     uint64_t t0 = wall_clock64();
     uint32_t i0 = 0;
@@ -287,136 +319,175 @@ __global__ __launch_bounds__(TB) void k_time_wave(const uint32_t* __restrict__ r
     if (i0 >= n) break;
     const uint32_t i1 = min(n, i0 + kWaveChunk);
     for (uint32_t i = i0; i < i1; ++i) {
-      cnt += wave_vertex(rp, col, verts[i], lane, W);
+      const uint32_t x = verts[i], wb = wrp[x];
+      cnt += wave_vertex<kFilterMin, 2, true>(rp, col, x, lane, W, wcol + wb, wrp[x + 1] - wb);
       const uint64_t t1 = wall_clock64();
-      if (lane == 0) cost[verts[i]] = (float)(t1 - t0);
+      if (lane == 0) cost[x] = (float)(t1 - t0);
       t0 = t1;
     }
   }
   flush_count(cnt, total);
 }
 
-__global__ void k_deg_range_flags(const uint32_t* __restrict__ rp, uint32_t n, uint32_t lo,
-                                  uint32_t hi, uint8_t* __restrict__ f) {
-  for (uint64_t u = gtid(); u < n; u += gstride()) {
-    const uint32_t d = rp[u + 1] - rp[u];
-    f[u] = (d >= lo && d <= hi) ? 1 : 0;
+// One lane task per lane; each task is charged 1/64 of its round (the wave
+// is held until its longest lane finishes).
+__global__ __launch_bounds__(TB) void k_time_wl_lane(const uint32_t* __restrict__ rp,
+                                                     const uint32_t* __restrict__ col,
+                                                     const uint32_t* __restrict__ wrp,
+                                                     const uint32_t* __restrict__ wcol,
+                                                     const uint32_t* __restrict__ verts, uint32_t n,
+                                                     uint32_t* cursor, float* __restrict__ cost,
+                                                     unsigned long long* total) {
+  const uint32_t lane = threadIdx.x & 63;
+  unsigned long long cnt = 0;
+  for (;;) {
+    const uint64_t t0 = wall_clock64();
+    uint32_t i0 = 0;
+    if (lane == 0) i0 = atomicAdd(cursor, 64u);
+    i0 = __shfl(i0, 0, 64);
+    if (i0 >= n) break;
+    const uint32_t i = i0 + lane;
+    if (i < n) cnt += lane_task(rp, col, wrp, wcol, verts[i]);
+    const uint64_t t1 = wall_clock64();
+    if (i < n) cost[verts[i]] = (float)(t1 - t0) / 64.f;
+  }
+  flush_count(cnt, total);
+}
+
+// Tasks of the work-list schedule (non-empty W(x), d+(x) >= 2) with d+ in
+// [lo, hi].
+__global__ void k_task_flags(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ wrp,
+                             uint32_t n, uint32_t lo, uint32_t hi, uint8_t* __restrict__ f) {
+  for (uint64_t x = gtid(); x < n; x += gstride()) {
+    const uint32_t d = rp[x + 1] - rp[x];
+    f[x] = (wrp[x + 1] > wrp[x] && d >= 2 && d >= lo && d <= hi) ? 1 : 0;
   }
 }
 
-// variant per vertex + sort key (descending predicted cost of that variant)
-// predicted total wave-time (us) of the cheaper variant per vertex
-__global__ void k_predicted_work(const uint32_t* __restrict__ rp, uint32_t n,
-                                 const float* __restrict__ Y, float* __restrict__ total) {
-  float acc = 0.f;
-  for (uint64_t u = gtid(); u < n; u += gstride()) {
-    const uint32_t d = rp[u + 1] - rp[u];
-    if (d < 2) continue;
-    const float cb = Y[u * O + TC_VARIANT_BLOCK] + 2.f, cw = Y[u * O + TC_VARIANT_WAVE];
-    acc += exp2f(fminf(d <= kWaveMax ? fminf(cb, cw) : cb, 40.f));
+// ---- learned schedule -----------------------------------------------------
+// Decision cells: d+ itself for 2..8 (cells 2..8), then (2^(b-1), 2^b] for
+// b = 4..9, i.e. d+ 9..16, 17..32, ..., 257..512 (cells 9..14).  Each cell's
+// tasks run their narrow variant (a lane for d+ <= 8, a wave above) or their
+// wide one (a wave for d+ <= 8, a workgroup above); tasks above d+ 512 are
+// block tasks.  Whatever its cell runs, a task keeps the fixed rule's limits:
+// a lane holds at most kLaneMaxList streamed lists and a wave at most
+// kWlWaveWork entries of work, else the task runs one size up.
+constexpr int kCells = TC_CELLS;
+constexpr int kAcc = 8;  // per-cell accumulators of k_cell_accum
+// The fixed rule (AUTO's work-list thresholds 64 / 8): narrow up to d+ 64.
+constexpr uint32_t kFixedNarrow = 0xFFCu;  // cells 2..11
+static_assert(kCells == 15, "cell layout");
+
+__device__ __host__ inline int task_cell(uint32_t d) {
+  if (d < 2 || d > kWaveMax) return -1;
+  if (d <= kLaneMaxDeg) return (int)d;
+  const int b = 32 - __builtin_clz(d - 1);  // ceil(log2 d): 4 .. 9
+  return (int)kLaneMaxDeg + 1 + (b - 4);
+}
+
+// TC_VARIANT_* of a task (d >= 2, nw >= 1 lists, work = d + sum d+(y)).
+__device__ inline uint8_t task_variant(uint32_t d, uint32_t nw, uint64_t work,
+                                       uint32_t narrow_cells) {
+  const int cell = task_cell(d);
+  const bool narrow = cell >= 0 && ((narrow_cells >> cell) & 1u);
+  if (d <= kLaneMaxDeg) {
+    if (narrow && nw <= kLaneMaxList) return TC_VARIANT_LANE;
+    return work > kWlWaveWork ? TC_VARIANT_BLOCK : TC_VARIANT_WAVE;
   }
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  if ((threadIdx.x & 63) == 0) atomicAdd(total, acc);
+  return (narrow && work <= kWlWaveWork) ? TC_VARIANT_WAVE : TC_VARIANT_BLOCK;
 }
 
-// A degree bin leaves the fixed rule's variant only for a predicted saving
-// above this fraction of its wave-time.
-constexpr float kSwitchMargin = 0.25f;
-
-__global__ void k_fill_f32(float* __restrict__ p, uint32_t n) {
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.f;
+// Tasks of sampling level s: all for s = 0, else those whose hashed id has
+// s low zero bits (about 2^-s of each cell).
+__device__ __host__ inline bool in_level(uint32_t x, uint32_t level) {
+  if (level == 0) return true;
+  uint32_t h = x * 0x9E3779B1u;
+  h ^= h >> 16;
+  return (h & ((1u << level) - 1u)) == 0;
 }
 
-// Per degree bin b = floor(log2 d+) (2 <= d+ <= kWaveMax): predicted total
-// wave-time (us) of the bin's vertices under each variant.  A block task
-// holds four waves for its duration, a wave task one.
-__global__ void k_bin_costs(const uint32_t* __restrict__ rp, uint32_t n,
-                            const float* __restrict__ Y, float* __restrict__ sums /*32 x 2*/) {
-  __shared__ float acc[32][2];
-  if (threadIdx.x < 64) acc[threadIdx.x >> 1][threadIdx.x & 1] = 0.f;
+// Per cell, over the tasks of one level: [0] tasks, [1] sum |W(x)|, [2] sum
+// of streamed entries (d+(y) over W(x)), [3] sum d+(x), [4] the lists' read
+// time under the bandwidth model in picoseconds (per task rounded, then
+// summed), [5] streamed lists with d+(y) < 64, [6] the largest task work
+// d+(x) + sum d+(y), [7] tasks within their narrow variant's limit.
+__global__ void k_cell_accum(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ wrp,
+                             const uint32_t* __restrict__ wcol, uint32_t n, uint32_t level,
+                             BwTable bw, unsigned long long* __restrict__ acc) {
+  __shared__ unsigned long long sa[kCells * kAcc];
+  for (int i = threadIdx.x; i < kCells * kAcc; i += blockDim.x) sa[i] = 0;
   __syncthreads();
-  for (uint64_t u = gtid(); u < n; u += gstride()) {
-    const uint32_t d = rp[u + 1] - rp[u];
-    if (d < 2 || d > kWaveMax) continue;
-    const uint32_t b = 31u - (uint32_t)__builtin_clz(d);
-    atomicAdd(&acc[b][TC_VARIANT_BLOCK], exp2f(fminf(Y[u * O + TC_VARIANT_BLOCK] + 2.f, 40.f)));
-    atomicAdd(&acc[b][TC_VARIANT_WAVE], exp2f(fminf(Y[u * O + TC_VARIANT_WAVE], 40.f)));
-  }
-  __syncthreads();
-  if (threadIdx.x < 64) {
-    const float v = acc[threadIdx.x >> 1][threadIdx.x & 1];
-    if (v != 0.f) atomicAdd(sums + threadIdx.x, v);
-  }
-}
-
-// Variant per vertex: block unless its degree bin is in `wave_bins` (bins
-// whose vertices cost less wave-time as wave tasks) and its own predicted
-// wave time stays under `guard` (log2 us): a longer wave task would outlast
-// the other waves and leave the machine idle.  key: block tasks by
-// descending predicted cost.  wkey: wave tasks predicted above `big` first
-// (descending cost), then the rest in id order (neighbouring rows share
-// cache lines); other vertices sort last.
-__global__ void k_assign(const uint32_t* __restrict__ rp, uint32_t n, const float* __restrict__ Y,
-                         uint32_t wave_bins, float guard, float big, uint8_t* __restrict__ var,
-                         uint64_t* __restrict__ key, uint64_t* __restrict__ wkey, uint32_t* ids,
-                         uint32_t* ids2) {
-  for (uint64_t u = gtid(); u < n; u += gstride()) {
-    const uint32_t d = rp[u + 1] - rp[u];
-    uint8_t v = 255;
-    float c = 0.f;
-    if (d >= 2) {
-      const float cb = Y[u * O + TC_VARIANT_BLOCK] + 2.f, cw = Y[u * O + TC_VARIANT_WAVE];
-      const uint32_t b = 31u - (uint32_t)__builtin_clz(d);
-      const bool wave = d <= kWaveMax && ((wave_bins >> b) & 1u) && cw <= guard;
-      v = wave ? TC_VARIANT_WAVE : TC_VARIANT_BLOCK;
-      c = wave ? cw : cb;
+  for (uint64_t x = gtid(); x < n; x += gstride()) {
+    const uint32_t d = rp[x + 1] - rp[x];
+    const uint32_t wb = wrp[x], we = wrp[x + 1];
+    const int cell = task_cell(d);
+    if (we == wb || cell < 0 || !in_level((uint32_t)x, level)) continue;
+    uint64_t ent = 0;
+    uint32_t shortc = 0;
+    double tns = 0;
+    for (uint32_t i = wb; i < we; ++i) {
+      const uint32_t y = wcol[i];
+      const uint32_t dy = rp[y + 1] - rp[y];
+      ent += dy;
+      shortc += dy < 64;
+      tns += 4.0 * dy / bw_rate(bw, dy);
     }
-    var[u] = v;
-    // predicted log2 cost -> order-preserving integer, descending after sort
-    const float clamped = fminf(fmaxf(c, -30.f), 60.f);
-    const uint64_t desc = ~(uint64_t)(uint32_t)((clamped + 32.f) * 16777216.f);
-    key[u] = desc;
-    wkey[u] = v != TC_VARIANT_WAVE ? ~0ull
-              : c > big ? (desc & ~(1ull << 63)) : ((1ull << 63) | (uint64_t)u);
-    ids[u] = (uint32_t)u;
-    ids2[u] = (uint32_t)u;
+    const uint32_t nw = we - wb;
+    const uint64_t work = d + ent;
+    const bool ok = d <= kLaneMaxDeg ? nw <= kLaneMaxList : work <= kWlWaveWork;
+    unsigned long long* a = sa + cell * kAcc;
+    atomicAdd(a + 0, 1ull);
+    atomicAdd(a + 1, (unsigned long long)nw);
+    atomicAdd(a + 2, (unsigned long long)ent);
+    atomicAdd(a + 3, (unsigned long long)d);
+    atomicAdd(a + 4, (unsigned long long)llround(tns * 1000.0));
+    atomicAdd(a + 5, (unsigned long long)shortc);
+    atomicMax(a + 6, (unsigned long long)work);
+    atomicAdd(a + 7, ok ? 1ull : 0ull);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kCells * kAcc; i += blockDim.x) {
+    if (!sa[i]) continue;
+    if (i % kAcc == 6) atomicMax(acc + i, sa[i]);
+    else atomicAdd(acc + i, sa[i]);
   }
 }
 
-// fixed rule as costs: the chosen variant gets the analytic work estimate,
-// the other one a larger value
-__global__ void k_fixed_costs(const uint32_t* __restrict__ rp, const float* __restrict__ X,
-                              uint32_t n, float* __restrict__ Y) {
-  for (uint64_t u = gtid(); u < n; u += gstride()) {
-    const uint32_t d = rp[u + 1] - rp[u];
-    const float w = X[u * F + 2];
-    const bool wave = d <= 64;
-    Y[u * O + TC_VARIANT_BLOCK] = wave ? w + 1.f : w - 2.f;
-    Y[u * O + TC_VARIANT_WAVE] = wave ? w : w + 1.f;
+// Variant per task (TC_VARIANT_*, 255 none) under narrow_cells, restricted
+// to cell `only` (all cells and the tasks above d+ 512 when only < 0) and to
+// the tasks of `level`.  key: block tasks by descending work (the launch
+// order, as the fixed rule's), other tasks sort last.
+__global__ void k_task_var(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ wrp,
+                           const uint32_t* __restrict__ wcol, uint32_t n, uint32_t narrow_cells,
+                           int only, uint32_t level, uint8_t* __restrict__ var,
+                           uint64_t* __restrict__ key, uint32_t* __restrict__ ids) {
+  for (uint64_t x = gtid(); x < n; x += gstride()) {
+    const uint32_t d = rp[x + 1] - rp[x];
+    const uint32_t wb = wrp[x], we = wrp[x + 1];
+    uint8_t v = 255;
+    uint64_t work = 0;
+    if (we > wb && d >= 2 && (only < 0 || task_cell(d) == only) &&
+        in_level((uint32_t)x, level)) {
+      work = d;
+      for (uint32_t i = wb; i < we; ++i) {
+        const uint32_t y = wcol[i];
+        work += rp[y + 1] - rp[y];
+      }
+      v = task_variant(d, we - wb, work, narrow_cells);
+    }
+    var[x] = v;
+    key[x] = v == TC_VARIANT_BLOCK ? ~work : ~0ull;
+    ids[x] = (uint32_t)x;
   }
 }
 
-__global__ void k_pick(const uint8_t* __restrict__ var, const uint32_t* __restrict__ order,
-                       const uint32_t* __restrict__ rp, uint32_t n, uint8_t want, uint32_t dmax,
-                       uint8_t* __restrict__ flag) {
+__global__ void k_var_flags(const uint8_t* __restrict__ var, const uint32_t* __restrict__ order,
+                            const uint32_t* __restrict__ rp, uint64_t n, uint8_t want,
+                            uint32_t dlo, uint32_t dhi, uint8_t* __restrict__ flag) {
   for (uint64_t i = gtid(); i < n; i += gstride()) {
-    const uint32_t u = order[i];
-    flag[i] = (var[u] == want && rp[u + 1] - rp[u] <= dmax) ? 1 : 0;
-  }
-}
-
-__global__ void k_pick_by_id(const uint8_t* __restrict__ var, const uint32_t* __restrict__ rp,
-                             uint32_t n, uint8_t want, uint32_t dmax, uint8_t* __restrict__ flag) {
-  for (uint64_t u = gtid(); u < n; u += gstride())
-    flag[u] = (var[u] == want && rp[u + 1] - rp[u] <= dmax) ? 1 : 0;
-}
-
-__global__ void k_pick_huge(const uint8_t* __restrict__ var, const uint32_t* __restrict__ order,
-                            const uint32_t* __restrict__ rp, uint32_t n, uint32_t dmin,
-                            uint8_t* __restrict__ flag) {
-  for (uint64_t i = gtid(); i < n; i += gstride()) {
-    const uint32_t u = order[i];
-    flag[i] = (var[u] != 255 && rp[u + 1] - rp[u] > dmin) ? 1 : 0;
+    const uint32_t x = order ? order[i] : (uint32_t)i;
+    const uint32_t d = rp[x + 1] - rp[x];
+    flag[i] = (var[x] == want && d >= dlo && d <= dhi) ? 1 : 0;
   }
 }
 
@@ -441,6 +512,7 @@ struct tc_model {
   tc_ctx* ctx = nullptr;
   DevBuf w, wbf, m, v, grad, partial, lossp, loss;
   Norm nm;
+  tc_bw_model bw;  // list-read bandwidth model behind feature 4
   uint64_t step = 0;
   uint64_t id = 0;
 };
@@ -496,138 +568,170 @@ int check_model(tc_ctx* c, tc_model* md) {
 
 namespace {
 
-int graph_features(tc_ctx* c, tc_graph* g, DevBuf& X) {
+BwTable bw_table(const tc_bw_model& m) {
+  BwTable t;
+  for (int i = 0; i < 10; ++i) t.table[i] = m.table[i];
+  t.a = m.a;
+  t.b = m.b;
+  return t;
+}
+
+// Work-list task features (k_wl_features) with the bandwidth model `bw`.
+int graph_features(tc_ctx* c, tc_graph* g, const tc_bw_model& bw, DevBuf& X) {
   const uint32_t n = g->n;
-  DevBuf indeg;
-  TCX_TRY(indeg.reserve(4 * ((size_t)n + 1)));
+  TCX_TRY(worklists(c, g));
   TCX_TRY(X.reserve(4ull * F * ((size_t)n + 1)));
-  TCX_HIP(hipMemsetAsync(indeg.p, 0, 4 * ((size_t)n + 1), c->stream));
-  if (g->m_or)
-    hipLaunchKernelGGL(k_count_ids<8192>, dim3(std::min<uint64_t>(grid_for(g->m_or), (uint64_t)c->num_cus * 4u)),
-                       dim3(256), 0, c->stream, g->col.as<uint32_t>(), (const uint32_t*)nullptr,
-                       g->m_or, n, indeg.as<uint32_t>());
-  hipLaunchKernelGGL(k_features, dim3(grid_for(n)), dim3(256), 0, c->stream,
-                     g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(), indeg.as<uint32_t>(), n,
-                     X.as<float>());
+  hipLaunchKernelGGL(k_wl_features, dim3(grid_for(n)), dim3(256), 0, c->stream,
+                     g->rowptr.as<uint32_t>(), g->sc.wl_rp.as<uint32_t>(),
+                     g->sc.wl_col.as<uint32_t>(), n, bw_table(bw), X.as<float>());
   TCX_HIP(hipGetLastError());
   TCX_HIP(hipStreamSynchronize(c->stream));
   return TC_OK;
 }
 
-// per-vertex predicted [block, wave] log2 cost, or the fixed rule
-int assignment(tc_ctx* c, tc_graph* g, tc_model* md, DevBuf& var, DevBuf& key, DevBuf& wkey,
-               DevBuf& ids, DevBuf& ids2) {
+// Per-cell accumulators (kCells x kAcc) of the tasks of `level`.
+int cell_accum(tc_ctx* c, tc_graph* g, const tc_bw_model& bw, uint32_t level,
+               unsigned long long* out) {
   const uint32_t n = g->n;
-  TCX_TRY(var.reserve((size_t)n + 1));
-  TCX_TRY(key.reserve(8 * ((size_t)n + 1)));
-  TCX_TRY(wkey.reserve(8 * ((size_t)n + 1)));
-  TCX_TRY(ids2.reserve(4 * ((size_t)n + 1)));
-  TCX_TRY(ids.reserve(4 * ((size_t)n + 1)));
-  DevBuf Y;
-  TCX_TRY(Y.reserve(4ull * O * ((size_t)n + 1)));
-  if (md) {
-    DevBuf X;
-    TCX_TRY(graph_features(c, g, X));
-    hipLaunchKernelGGL(k_mlp_forward, dim3(std::min<unsigned>(grid_for(n), 8192)), dim3(256), 0,
-                       c->stream, X.as<float>(), n, md->wbf.as<uint16_t>(), md->nm, Y.as<float>());
-  } else {
-    // fallback: the fixed degree-bin rule, ordered by the analytic work
-    // log2(1 + sum_v d+(v)) (the AUTO schedule of tc_count)
-    DevBuf X;
-    TCX_TRY(graph_features(c, g, X));
-    hipLaunchKernelGGL(k_fixed_costs, dim3(grid_for(n)), dim3(256), 0, c->stream,
-                       g->rowptr.as<uint32_t>(), X.as<float>(), n, Y.as<float>());
-  }
-  // per-bin variant: the one with the smaller predicted total wave-time
-  TCX_TRY(c->prim_out.reserve(64 * 4 + 64));
-  float* dsum = c->prim_out.as<float>();
-  hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, c->stream, dsum, 65u);
-  hipLaunchKernelGGL(k_bin_costs, dim3(std::min<unsigned>(grid_for(n), 2048)), dim3(256), 0,
-                     c->stream, g->rowptr.as<uint32_t>(), n, Y.as<float>(), dsum);
-  uint32_t wave_bins = 0;
-  {
-    float sums[64];
-    TCX_HIP(hipMemcpyAsync(sums, dsum, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
-    TCX_HIP(hipStreamSynchronize(c->stream));
-    // the fixed rule (wave iff d+ <= 64, bins 1..5 and d+ = 64 of bin 6) is
-    // the prior: a bin changes variant only when the prediction says the
-    // other one saves more than kSwitchMargin of the bin's wave-time
-    for (int b = 0; b < 32; ++b) {
-      const float w = sums[2 * b + TC_VARIANT_WAVE], bl = sums[2 * b + TC_VARIANT_BLOCK];
-      if (!(w > 0.f)) continue;
-      const bool prior_wave = b <= 5;
-      const bool wave = prior_wave ? !(bl < (1.f - kSwitchMargin) * w)
-                                   : (w < (1.f - kSwitchMargin) * bl);
-      if (wave) wave_bins |= 1u << b;
+  TCX_TRY(worklists(c, g));
+  TCX_TRY(c->prim_out.reserve(8 * kCells * kAcc));
+  unsigned long long* d = c->prim_out.as<unsigned long long>();
+  TCX_HIP(hipMemsetAsync(d, 0, 8 * kCells * kAcc, c->stream));
+  hipLaunchKernelGGL(k_cell_accum, dim3(std::min<unsigned>(grid_for(n), 4096)), dim3(256), 0,
+                     c->stream, g->rowptr.as<uint32_t>(), g->sc.wl_rp.as<uint32_t>(),
+                     g->sc.wl_col.as<uint32_t>(), n, level, bw_table(bw), d);
+  TCX_HIP(hipGetLastError());
+  TCX_HIP(hipMemcpyAsync(out, d, 8 * kCells * kAcc, hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  return TC_OK;
+}
+
+// The eight cell features from the accumulators (0 for an empty cell):
+// log2(1+tasks), log2(1+lists), log2(1+streamed entries), log2(1+held
+// entries), log2(1+list-read ns under the bandwidth model), the fraction of
+// lists shorter than 64, log2(1+largest task work), the fraction of tasks
+// within the narrow variant's limit.
+void cell_features(const unsigned long long* acc, float* X) {
+  for (int c = 0; c < kCells; ++c) {
+    const unsigned long long* a = acc + c * kAcc;
+    float* f = X + c * F;
+    if (a[0] == 0) {
+      for (int j = 0; j < F; ++j) f[j] = 0.f;
+      continue;
     }
+    f[0] = (float)std::log2(1.0 + (double)a[0]);
+    f[1] = (float)std::log2(1.0 + (double)a[1]);
+    f[2] = (float)std::log2(1.0 + (double)a[2]);
+    f[3] = (float)std::log2(1.0 + (double)a[3]);
+    f[4] = (float)std::log2(1.0 + (double)a[4] / 1000.0);
+    f[5] = a[1] ? (float)((double)a[5] / (double)a[1]) : 0.f;
+    f[6] = (float)std::log2(1.0 + (double)a[6]);
+    f[7] = (float)((double)a[7] / (double)a[0]);
   }
-  // per-wave share of the predicted wave-time (log2 us): a wave task may take
-  // at most half of it; tasks above 1/16 of it (a fetch holds kWaveChunk
-  // tasks) go first so that they cannot end up in the tail
-  float guard = 1e30f, big = 1e30f;
-  if (md) {
-    float* dtot = c->prim_out.as<float>() + 64;
-    hipLaunchKernelGGL(k_predicted_work, dim3(std::min<unsigned>(grid_for(n), 2048)), dim3(256), 0,
-                       c->stream, g->rowptr.as<uint32_t>(), n, Y.as<float>(), dtot);
-    float tot = 0;
-    TCX_HIP(hipMemcpyAsync(&tot, dtot, 4, hipMemcpyDeviceToHost, c->stream));
-    TCX_HIP(hipStreamSynchronize(c->stream));
-    const double waves = (double)c->num_cus * 32.0;
-    const float share = (float)std::log2(std::max(1e-3, tot / waves));
-    guard = share - 1.f;
-    big = share - std::log2((float)kWaveChunk);
-  }
-  hipLaunchKernelGGL(k_assign, dim3(grid_for(n)), dim3(256), 0, c->stream,
-                     g->rowptr.as<uint32_t>(), n, Y.as<float>(), wave_bins, guard, big,
-                     var.as<uint8_t>(), key.as<uint64_t>(), wkey.as<uint64_t>(), ids.as<uint32_t>(),
-                     ids2.as<uint32_t>());
-  TCX_HIP(hipGetLastError());
-  TCX_HIP(hipStreamSynchronize(c->stream));
-  return TC_OK;
 }
 
-int learned_schedule(tc_ctx* c, tc_graph* g, tc_model* md) {
-  SchedCache& L = g->sc;
-  const uint64_t mid = md ? md->id : 0;
-  if (L.learned_version == g->sched_version && L.learned_model == mid) return TC_OK;
+// Task lists of one schedule: block tasks by descending work (split at
+// d+ 2048 between the two block kernels), wave and lane tasks in id order.
+struct TaskLists {
+  DevBuf blk, wav, lan, huge;
+  uint64_t nb = 0, nw = 0, nl = 0, nh = 0;
+};
+
+int lists_from_var(tc_ctx* c, tc_graph* g, DevBuf& var, DevBuf& key, DevBuf& ids, TaskLists& L) {
   const uint32_t n = g->n;
-  DevBuf var, key, wkey, ids, ids2, ktmp, itmp, flag;
-  TCX_TRY(assignment(c, g, md, var, key, wkey, ids, ids2));
+  const uint32_t* rp = g->rowptr.as<uint32_t>();
+  DevBuf ktmp, itmp, flag;
   TCX_TRY(ktmp.reserve(8 * ((size_t)n + 1)));
   TCX_TRY(itmp.reserve(4 * ((size_t)n + 1)));
   TCX_TRY(flag.reserve((size_t)n + 1));
+  TCX_TRY(L.wav.reserve(4 * ((size_t)n + 1)));
+  TCX_TRY(L.lan.reserve(4 * ((size_t)n + 1)));
+  TCX_TRY(L.blk.reserve(4 * ((size_t)n + 1)));
+  TCX_TRY(L.huge.reserve(4 * ((size_t)n + 1)));
+  hipLaunchKernelGGL(k_var_flags, dim3(grid_for(n)), dim3(256), 0, c->stream, var.as<uint8_t>(),
+                     (const uint32_t*)nullptr, rp, (uint64_t)n, (uint8_t)TC_VARIANT_WAVE, 0u,
+                     0xFFFFFFFFu, flag.as<uint8_t>());
+  TCX_TRY(select_indices(c, flag.as<uint8_t>(), n, L.wav.as<uint32_t>(), &L.nw));
+  hipLaunchKernelGGL(k_var_flags, dim3(grid_for(n)), dim3(256), 0, c->stream, var.as<uint8_t>(),
+                     (const uint32_t*)nullptr, rp, (uint64_t)n, (uint8_t)TC_VARIANT_LANE, 0u,
+                     0xFFFFFFFFu, flag.as<uint8_t>());
+  TCX_TRY(select_indices(c, flag.as<uint8_t>(), n, L.lan.as<uint32_t>(), &L.nl));
   uint64_t* sk;
   uint32_t* order;
   TCX_TRY(sort_pairs_u64_u32(c, key.as<uint64_t>(), ktmp.as<uint64_t>(), ids.as<uint32_t>(),
                              itmp.as<uint32_t>(), n, 64, &sk, &order));
-  TCX_TRY(L.l_blk.reserve(4 * ((size_t)n + 1)));
-  TCX_TRY(L.l_wav.reserve(4 * ((size_t)n + 1)));
-  TCX_TRY(L.l_huge.reserve(4 * ((size_t)n + 1)));
-  const uint32_t* rp = g->rowptr.as<uint32_t>();
-  hipLaunchKernelGGL(k_pick, dim3(grid_for(n)), dim3(256), 0, c->stream, var.as<uint8_t>(), order,
-                     rp, n, (uint8_t)TC_VARIANT_BLOCK, 2048u, flag.as<uint8_t>());
-  TCX_TRY(select_flagged_u32(c, order, flag.as<uint8_t>(), L.l_blk.as<uint32_t>(), n, &L.nl_blk));
-  // wave tasks: the predicted-long ones first, the rest in id order (many
-  // short tasks gain more from the locality of neighbouring rows than from a
-  // cost order); the count comes from the flags, the order from wkey
-  hipLaunchKernelGGL(k_pick_by_id, dim3(grid_for(n)), dim3(256), 0, c->stream, var.as<uint8_t>(),
-                     rp, n, (uint8_t)TC_VARIANT_WAVE, kWaveMax, flag.as<uint8_t>());
-  TCX_TRY(select_indices(c, flag.as<uint8_t>(), n, L.l_wav.as<uint32_t>(), &L.nl_wav));
-  hipLaunchKernelGGL(k_pick_huge, dim3(grid_for(n)), dim3(256), 0, c->stream, var.as<uint8_t>(),
-                     order, rp, n, 2048u, flag.as<uint8_t>());
-  TCX_TRY(select_flagged_u32(c, order, flag.as<uint8_t>(), L.l_huge.as<uint32_t>(), n, &L.nl_huge));
-  // (after the last use of `order`: this sort reuses its temporaries)
-  uint64_t* swk;
-  uint32_t* worder;
-  TCX_TRY(sort_pairs_u64_u32(c, wkey.as<uint64_t>(), ktmp.as<uint64_t>(), ids2.as<uint32_t>(),
-                             itmp.as<uint32_t>(), n, 64, &swk, &worder));
-  if (L.nl_wav)
-    TCX_HIP(hipMemcpyAsync(L.l_wav.p, worder, 4 * L.nl_wav, hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_var_flags, dim3(grid_for(n)), dim3(256), 0, c->stream, var.as<uint8_t>(),
+                     order, rp, (uint64_t)n, (uint8_t)TC_VARIANT_BLOCK, 0u, 2048u,
+                     flag.as<uint8_t>());
+  TCX_TRY(select_flagged_u32(c, order, flag.as<uint8_t>(), L.blk.as<uint32_t>(), n, &L.nb));
+  hipLaunchKernelGGL(k_var_flags, dim3(grid_for(n)), dim3(256), 0, c->stream, var.as<uint8_t>(),
+                     order, rp, (uint64_t)n, (uint8_t)TC_VARIANT_BLOCK, 2049u, 0xFFFFFFFFu,
+                     flag.as<uint8_t>());
+  TCX_TRY(select_flagged_u32(c, order, flag.as<uint8_t>(), L.huge.as<uint32_t>(), n, &L.nh));
   TCX_HIP(hipGetLastError());
-  L.learned_version = g->sched_version;
-  L.learned_model = mid;
+  TCX_HIP(hipStreamSynchronize(c->stream));
   return TC_OK;
 }
+
+// The count launches of a schedule (k_auto_wl as tc_count's AUTO launches
+// it, plus the 8192-entry block kernel for d+ > 2048); cursors and the
+// accumulator must be cleared first.
+int launch_lists(tc_ctx* c, tc_graph* g, const uint32_t* blk, uint64_t nb, const uint32_t* wav,
+                 uint64_t nw, const uint32_t* lan, uint64_t nl, const uint32_t* huge, uint64_t nh,
+                 uint32_t* cursor, unsigned long long* acc, uint32_t* launches) {
+  const uint32_t* rp = g->rowptr.as<uint32_t>();
+  const uint32_t* col = g->col.as<uint32_t>();
+  const uint32_t* wrp = g->sc.wl_rp.as<uint32_t>();
+  const uint32_t* wcol = g->sc.wl_col.as<uint32_t>();
+  if (nb || nw || nl) {
+    const uint32_t G = (uint32_t)c->num_cus * 8u;
+    if (nw <= (uint64_t)kStaticWaves * G * (TB / 64))
+      hipLaunchKernelGGL((k_auto_wl<2048, true>), dim3(G), dim3(TB), 0, c->stream, rp, col, wrp,
+                         wcol, blk, (uint32_t)nb, wav, (uint32_t)nw, lan, (uint32_t)nl, cursor,
+                         acc);
+    else
+      hipLaunchKernelGGL((k_auto_wl<2048>), dim3(G), dim3(TB), 0, c->stream, rp, col, wrp, wcol,
+                         blk, (uint32_t)nb, wav, (uint32_t)nw, lan, (uint32_t)nl, cursor, acc);
+    if (launches) ++*launches;
+  }
+  if (nh) {
+    hipLaunchKernelGGL((k_auto_wl<8192>), dim3(std::min<uint64_t>((uint64_t)c->num_cus * 4, nh)),
+                       dim3(TB), 0, c->stream, rp, col, wrp, wcol, huge, (uint32_t)nh,
+                       (const uint32_t*)nullptr, 0u, (const uint32_t*)nullptr, 0u, cursor + 4,
+                       acc);
+    if (launches) ++*launches;
+  }
+  TCX_HIP(hipGetLastError());
+  return TC_OK;
+}
+
+int launch_lists(tc_ctx* c, tc_graph* g, const TaskLists& L, uint32_t* cursor,
+                 unsigned long long* acc, uint32_t* launches) {
+  return launch_lists(c, g, L.blk.as<uint32_t>(), L.nb, L.wav.as<uint32_t>(), L.nw,
+                      L.lan.as<uint32_t>(), L.nl, L.huge.as<uint32_t>(), L.nh, cursor, acc,
+                      launches);
+}
+
+// Variant per task (TC_VARIANT_*, 255 none) for narrow_cells.
+int task_var(tc_ctx* c, tc_graph* g, uint32_t narrow_cells, int only, uint32_t level,
+             DevBuf& var, DevBuf& key, DevBuf& ids) {
+  const uint32_t n = g->n;
+  TCX_TRY(var.reserve((size_t)n + 1));
+  TCX_TRY(key.reserve(8 * ((size_t)n + 1)));
+  TCX_TRY(ids.reserve(4 * ((size_t)n + 1)));
+  hipLaunchKernelGGL(k_task_var, dim3(grid_for(n)), dim3(256), 0, c->stream,
+                     g->rowptr.as<uint32_t>(), g->sc.wl_rp.as<uint32_t>(),
+                     g->sc.wl_col.as<uint32_t>(), n, narrow_cells, only, level, var.as<uint8_t>(),
+                     key.as<uint64_t>(), ids.as<uint32_t>());
+  TCX_HIP(hipGetLastError());
+  return TC_OK;
+}
+
+// The model's cells: predictions Y (kCells x 2, log2 us of the cell's
+// launch, wide / narrow) and the narrow-cell mask.  The fixed rule is the
+// prior: a cell leaves it only for a predicted saving above kSwitchMargin.
+constexpr float kSwitchMargin = 0.10f;
+int learned_cells(tc_ctx* c, tc_graph* g, tc_model* md, uint32_t* mask, float* Y);
 
 }  // namespace
 
@@ -644,6 +748,60 @@ int model_predict(tc_ctx* c, tc_model* md, const float* dX, uint32_t n, float* d
 }
 }  // namespace tcx
 
+namespace {
+
+int learned_cells(tc_ctx* c, tc_graph* g, tc_model* md, uint32_t* mask, float* Y) {
+  unsigned long long acc[kCells * kAcc];
+  TCX_TRY(cell_accum(c, g, md->bw, 0, acc));
+  float X[kCells * F];
+  cell_features(acc, X);
+  DevBuf dX, dY;
+  TCX_TRY(upload(c, dX, X, sizeof(X)));
+  TCX_TRY(dY.reserve(4 * kCells * O));
+  TCX_TRY(model_predict(c, md, dX.as<float>(), kCells, dY.as<float>()));
+  TCX_HIP(hipMemcpyAsync(Y, dY.p, 4 * kCells * O, hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  uint32_t m = kFixedNarrow;
+  for (int cell = 2; cell < kCells; ++cell) {
+    if (acc[cell * kAcc] == 0) continue;
+    const bool prior = (kFixedNarrow >> cell) & 1u;
+    const double wide = std::exp2((double)Y[cell * O + 0]);
+    const double narrow = std::exp2((double)Y[cell * O + 1]);
+    const double keep = prior ? narrow : wide, other = prior ? wide : narrow;
+    if (other < (1.0 - kSwitchMargin) * keep) m ^= 1u << cell;
+  }
+  *mask = m;
+  return TC_OK;
+}
+
+// The learned task lists, kept with the graph per model parameters.
+int learned_schedule(tc_ctx* c, tc_graph* g, tc_model* md) {
+  SchedCache& S = g->sc;
+  TCX_TRY(worklists(c, g));
+  if (S.learned_version == g->sched_version && S.learned_model == md->id) return TC_OK;
+  uint32_t mask = 0;
+  float Y[kCells * O];
+  TCX_TRY(learned_cells(c, g, md, &mask, Y));
+  DevBuf var, key, ids;
+  TCX_TRY(task_var(c, g, mask, -1, 0, var, key, ids));
+  TaskLists L;
+  TCX_TRY(lists_from_var(c, g, var, key, ids, L));
+  swap(S.l_blk, L.blk);
+  swap(S.l_wav, L.wav);
+  swap(S.l_lan, L.lan);
+  swap(S.l_huge, L.huge);
+  S.nl_blk = L.nb;
+  S.nl_wav = L.nw;
+  S.nl_lan = L.nl;
+  S.nl_huge = L.nh;
+  S.learned_mask = mask;
+  S.learned_version = g->sched_version;
+  S.learned_model = md->id;
+  return TC_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 uint32_t tc_model_num_params(void) { return P; }
@@ -657,6 +815,7 @@ int tc_model_create(tc_ctx* c, uint64_t seed, tc_model** out) {
   md->id = g_model_ids++;
   for (int i = 0; i < F; ++i) { md->nm.xm[i] = 0.f; md->nm.xs[i] = 1.f; }
   for (int j = 0; j < O; ++j) { md->nm.ym[j] = 0.f; md->nm.ys[j] = 1.f; }
+  bw_model_reference(&md->bw);
   int st = md->w.reserve(4 * P);
   if (!st) st = md->wbf.reserve(2 * P);
   if (!st) st = md->m.reserve(4 * P);
@@ -848,13 +1007,33 @@ int tc_model_train(tc_ctx* c, tc_model* md, const float* X, const float* T, cons
   return TC_OK;
 }
 
-int tc_graph_features(tc_ctx* c, tc_graph* g, float* X) {
+int tc_model_set_bw_model(tc_model* md, const tc_bw_model* bw) {
+  if (!md || !bw) return fail(TC_ERR_ARG, "NULL argument");
+  for (int l = 2; l < 10; ++l)
+    if (!(std::isfinite(bw->table[l]) && bw->table[l] > 0))
+      return fail(TC_ERR_ARG, "bandwidth model entry %d is not finite and positive", l);
+  if (!(std::isfinite(bw->a) && bw->a > 0 && std::isfinite(bw->b)))
+    return fail(TC_ERR_ARG, "bandwidth model power law is not finite and positive");
+  md->bw = *bw;
+  md->id = g_model_ids++;  // schedules built with the old features are stale
+  return TC_OK;
+}
+
+int tc_graph_features(tc_ctx* c, tc_graph* g, const tc_model* md, float* X) {
   if (!c || !g || !X) return fail(TC_ERR_ARG, "NULL argument");
+  if (g->ctx != c) return fail(TC_ERR_ARG, "graph belongs to another context");
   if (g->orient_mode < 0) return fail(TC_ERR_STATE, "graph must be oriented");
   TCX_HIP(hipSetDevice(c->device));
   if (g->n == 0) return TC_OK;
+  tc_bw_model bw;
+  if (md) bw = md->bw;
+  else bw_model_reference(&bw);
+  if (g->m_or == 0) {
+    memset(X, 0, 4ull * F * g->n);
+    return TC_OK;
+  }
   DevBuf dX;
-  TCX_TRY(graph_features(c, g, dX));
+  TCX_TRY(graph_features(c, g, bw, dX));
   TCX_HIP(hipMemcpyAsync(X, dX.p, 4ull * F * g->n, hipMemcpyDeviceToHost, c->stream));
   TCX_HIP(hipStreamSynchronize(c->stream));
   return TC_OK;
@@ -862,12 +1041,19 @@ int tc_graph_features(tc_ctx* c, tc_graph* g, float* X) {
 
 int tc_graph_measure_costs(tc_ctx* c, tc_graph* g, int variant, float* cost_us, uint64_t* count) {
   if (!c || !g || !cost_us) return fail(TC_ERR_ARG, "NULL argument");
+  if (g->ctx != c) return fail(TC_ERR_ARG, "graph belongs to another context");
   if (g->orient_mode < 0) return fail(TC_ERR_STATE, "graph must be oriented");
-  if (variant != TC_VARIANT_BLOCK && variant != TC_VARIANT_WAVE)
+  if (variant != TC_VARIANT_BLOCK && variant != TC_VARIANT_WAVE && variant != TC_VARIANT_LANE)
     return fail(TC_ERR_ARG, "unknown variant %d", variant);
   TCX_HIP(hipSetDevice(c->device));
   const uint32_t n = g->n;
   if (n == 0) return TC_OK;
+  if (count) *count = 0;
+  if (g->m_or == 0) {
+    memset(cost_us, 0, 4ull * n);
+    return TC_OK;
+  }
+  TCX_TRY(worklists(c, g));
   DevBuf flags, verts, cost;
   TCX_TRY(flags.reserve((size_t)n + 1));
   TCX_TRY(verts.reserve(4 * ((size_t)n + 1)));
@@ -881,28 +1067,36 @@ int tc_graph_measure_costs(tc_ctx* c, tc_graph* g, int variant, float* cost_us, 
   TCX_HIP(hipMemsetAsync(acc, 0, 8, c->stream));
   const uint32_t* rp = g->rowptr.as<uint32_t>();
   const uint32_t* col = g->col.as<uint32_t>();
+  const uint32_t* wrp = g->sc.wl_rp.as<uint32_t>();
+  const uint32_t* wcol = g->sc.wl_col.as<uint32_t>();
   uint64_t nv = 0;
   const uint32_t G = (uint32_t)c->num_cus * 8u;
+  auto select = [&](uint32_t lo, uint32_t hi) -> int {
+    hipLaunchKernelGGL(k_task_flags, dim3(grid_for(n)), dim3(256), 0, c->stream, rp, wrp, n, lo,
+                       hi, flags.as<uint8_t>());
+    TCX_HIP(hipGetLastError());
+    return select_indices(c, flags.as<uint8_t>(), n, verts.as<uint32_t>(), &nv);
+  };
   if (variant == TC_VARIANT_BLOCK) {
-    hipLaunchKernelGGL(k_deg_range_flags, dim3(grid_for(n)), dim3(256), 0, c->stream, rp, n, 2u,
-                       2048u, flags.as<uint8_t>());
-    TCX_TRY(select_indices(c, flags.as<uint8_t>(), n, verts.as<uint32_t>(), &nv));
+    TCX_TRY(select(2u, 2048u));
     if (nv)
-      hipLaunchKernelGGL(k_time_block<2048>, dim3(G), dim3(TB), 0, c->stream, rp, col,
+      hipLaunchKernelGGL(k_time_wl_block<2048>, dim3(G), dim3(TB), 0, c->stream, rp, col, wrp, wcol,
                          verts.as<uint32_t>(), (uint32_t)nv, cursor, cost.as<float>(), acc);
-    hipLaunchKernelGGL(k_deg_range_flags, dim3(grid_for(n)), dim3(256), 0, c->stream, rp, n, 2049u,
-                       0xFFFFFFFFu, flags.as<uint8_t>());
-    TCX_TRY(select_indices(c, flags.as<uint8_t>(), n, verts.as<uint32_t>(), &nv));
+    TCX_TRY(select(2049u, 0xFFFFFFFFu));
     if (nv)
-      hipLaunchKernelGGL(k_time_block<8192>, dim3(std::min<uint64_t>(nv, (uint64_t)c->num_cus * 4)),
-                         dim3(TB), 0, c->stream, rp, col, verts.as<uint32_t>(), (uint32_t)nv,
+      hipLaunchKernelGGL(k_time_wl_block<8192>,
+                         dim3(std::min<uint64_t>(nv, (uint64_t)c->num_cus * 4)), dim3(TB), 0,
+                         c->stream, rp, col, wrp, wcol, verts.as<uint32_t>(), (uint32_t)nv,
                          cursor + 1, cost.as<float>(), acc);
-  } else {
-    hipLaunchKernelGGL(k_deg_range_flags, dim3(grid_for(n)), dim3(256), 0, c->stream, rp, n, 2u,
-                       kWaveMax, flags.as<uint8_t>());
-    TCX_TRY(select_indices(c, flags.as<uint8_t>(), n, verts.as<uint32_t>(), &nv));
+  } else if (variant == TC_VARIANT_WAVE) {
+    TCX_TRY(select(2u, kWaveMax));
     if (nv)
-      hipLaunchKernelGGL(k_time_wave, dim3(G), dim3(TB), 0, c->stream, rp, col,
+      hipLaunchKernelGGL(k_time_wl_wave, dim3(G), dim3(TB), 0, c->stream, rp, col, wrp, wcol,
+                         verts.as<uint32_t>(), (uint32_t)nv, cursor, cost.as<float>(), acc);
+  } else {
+    TCX_TRY(select(2u, kLaneMaxDeg));
+    if (nv)
+      hipLaunchKernelGGL(k_time_wl_lane, dim3(G), dim3(TB), 0, c->stream, rp, col, wrp, wcol,
                          verts.as<uint32_t>(), (uint32_t)nv, cursor, cost.as<float>(), acc);
   }
   TCX_HIP(hipGetLastError());
@@ -917,25 +1111,137 @@ int tc_graph_measure_costs(tc_ctx* c, tc_graph* g, int variant, float* cost_us, 
   return TC_OK;
 }
 
-int tc_learned_assignment(tc_ctx* c, tc_graph* g, tc_model* md, uint8_t* variant) {
-  if (!c || !g || !variant) return fail(TC_ERR_ARG, "NULL argument");
+namespace {
+int check_graph(tc_ctx* c, tc_graph* g, tc_model* md) {
+  if (!c || !g) return fail(TC_ERR_ARG, "NULL argument");
+  if (g->ctx != c) return fail(TC_ERR_ARG, "graph belongs to another context");
   if (md && md->ctx != c) return fail(TC_ERR_ARG, "model belongs to another context");
   if (g->orient_mode < 0) return fail(TC_ERR_STATE, "graph must be oriented");
   TCX_HIP(hipSetDevice(c->device));
-  if (g->n == 0) return TC_OK;
-  DevBuf var, key, wkey, ids, ids2;
-  TCX_TRY(assignment(c, g, md, var, key, wkey, ids, ids2));
-  TCX_HIP(hipMemcpyAsync(variant, var.p, g->n, hipMemcpyDeviceToHost, c->stream));
+  return TC_OK;
+}
+}  // namespace
+
+int tc_graph_cell_features(tc_ctx* c, tc_graph* g, const tc_model* md, uint32_t level,
+                           float* X) {
+  TCX_TRY(check_graph(c, g, nullptr));
+  if (!X) return fail(TC_ERR_ARG, "NULL argument");
+  if (md && md->ctx != c) return fail(TC_ERR_ARG, "model belongs to another context");
+  if (level > TC_CELL_MAX_LEVEL) return fail(TC_ERR_ARG, "level %u > %d", level, TC_CELL_MAX_LEVEL);
+  unsigned long long acc[kCells * kAcc] = {};
+  if (g->n && g->m_or) {
+    tc_bw_model bw;
+    if (md) bw = md->bw;
+    else bw_model_reference(&bw);
+    TCX_TRY(cell_accum(c, g, bw, level, acc));
+  }
+  cell_features(acc, X);
+  return TC_OK;
+}
+
+int tc_graph_measure_cells(tc_ctx* c, tc_graph* g, int narrow, uint32_t level, uint32_t reps,
+                           float* us, uint64_t* counts) {
+  TCX_TRY(check_graph(c, g, nullptr));
+  if (!us) return fail(TC_ERR_ARG, "NULL argument");
+  if (level > TC_CELL_MAX_LEVEL) return fail(TC_ERR_ARG, "level %u > %d", level, TC_CELL_MAX_LEVEL);
+  if (reps == 0 || reps > 64) return fail(TC_ERR_ARG, "reps must be 1..64");
+  for (int i = 0; i < kCells; ++i) {
+    us[i] = 0.f;
+    if (counts) counts[i] = 0;
+  }
+  if (g->n == 0 || g->m_or == 0) return TC_OK;
+  unsigned long long acc[kCells * kAcc];
+  tc_bw_model bw;
+  bw_model_reference(&bw);
+  TCX_TRY(cell_accum(c, g, bw, level, acc));
+  TCX_TRY(c->scratch.reserve(64));
+  TCX_TRY(c->accum.reserve(64));
+  uint32_t* cursor = c->scratch.as<uint32_t>();
+  unsigned long long* dacc = c->accum.as<unsigned long long>();
+  DevBuf var, key, ids;
+  for (int cell = 2; cell < kCells; ++cell) {
+    if (acc[cell * kAcc] == 0) continue;
+    TCX_TRY(task_var(c, g, narrow ? (1u << cell) : 0u, cell, level, var, key, ids));
+    TaskLists L;
+    TCX_TRY(lists_from_var(c, g, var, key, ids, L));
+    float best = 0.f;
+    unsigned long long first = 0;
+    for (uint32_t r = 0; r < reps; ++r) {
+      TCX_HIP(hipMemsetAsync(cursor, 0, 8 * sizeof(uint32_t), c->stream));
+      TCX_HIP(hipMemsetAsync(dacc, 0, 8, c->stream));
+      TCX_HIP(hipEventRecord(c->ev0, c->stream));
+      TCX_TRY(launch_lists(c, g, L, cursor, dacc, nullptr));
+      TCX_HIP(hipEventRecord(c->ev1, c->stream));
+      TCX_HIP(hipEventSynchronize(c->ev1));
+      float ms = 0.f;
+      TCX_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      unsigned long long h = 0;
+      TCX_HIP(hipMemcpyAsync(&h, dacc, 8, hipMemcpyDeviceToHost, c->stream));
+      TCX_HIP(hipStreamSynchronize(c->stream));
+      if (r == 0) first = h;
+      else if (h != first) return fail(TC_ERR_STATE, "cell %d counted %llu then %llu", cell, first, h);
+      if (r == 0 || ms < best) best = ms;
+    }
+    us[cell] = best * 1000.f;
+    if (counts) counts[cell] = first;
+  }
+  return TC_OK;
+}
+
+int tc_learned_cells(tc_ctx* c, tc_graph* g, tc_model* md, uint32_t* narrow_cells, float* Y) {
+  TCX_TRY(check_graph(c, g, md));
+  if (!narrow_cells) return fail(TC_ERR_ARG, "NULL argument");
+  float y[kCells * O] = {};
+  *narrow_cells = kFixedNarrow;
+  if (md && g->n && g->m_or) TCX_TRY(learned_cells(c, g, md, narrow_cells, y));
+  if (Y) memcpy(Y, y, sizeof(y));
+  return TC_OK;
+}
+
+int tc_learned_assignment(tc_ctx* c, tc_graph* g, tc_model* md, uint8_t* variant) {
+  TCX_TRY(check_graph(c, g, md));
+  if (!variant) return fail(TC_ERR_ARG, "NULL argument");
+  const uint32_t n = g->n;
+  if (n == 0) return TC_OK;
+  memset(variant, 255, n);
+  if (g->m_or == 0) return TC_OK;
+  TCX_TRY(worklists(c, g));
+  if (!md) {  // the fixed rule: the work-list schedule's own lists
+    std::vector<uint32_t> ids;
+    const SchedCache& L = g->sc;
+    auto mark = [&](const DevBuf& list, uint64_t cnt, uint8_t v) -> int {
+      ids.resize(std::max<uint64_t>(cnt, 1));
+      if (cnt)
+        TCX_HIP(hipMemcpyAsync(ids.data(), list.p, 4 * cnt, hipMemcpyDeviceToHost, c->stream));
+      TCX_HIP(hipStreamSynchronize(c->stream));
+      for (uint64_t i = 0; i < cnt; ++i) variant[ids[i]] = v;
+      return TC_OK;
+    };
+    TCX_TRY(mark(L.wl_large, L.nw_large, TC_VARIANT_BLOCK));
+    TCX_TRY(mark(L.wl_huge, L.nw_huge, TC_VARIANT_BLOCK));
+    TCX_TRY(mark(L.wl_small, L.nw_small, TC_VARIANT_WAVE));
+    TCX_TRY(mark(L.wl_lane, L.nw_lane, TC_VARIANT_LANE));
+    return TC_OK;
+  }
+  uint32_t mask = 0;
+  float Y[kCells * O];
+  TCX_TRY(learned_cells(c, g, md, &mask, Y));
+  DevBuf var, key, ids;
+  TCX_TRY(task_var(c, g, mask, -1, 0, var, key, ids));
+  TCX_HIP(hipMemcpyAsync(variant, var.p, n, hipMemcpyDeviceToHost, c->stream));
   TCX_HIP(hipStreamSynchronize(c->stream));
   return TC_OK;
 }
 
 int tc_count_learned(tc_ctx* c, tc_graph* g, tc_model* md, uint64_t* count, tc_timing* timing) {
-  if (!c || !g || !count) return fail(TC_ERR_ARG, "NULL argument");
-  if (g->ctx != c) return fail(TC_ERR_ARG, "graph belongs to another context");
-  if (md && md->ctx != c) return fail(TC_ERR_ARG, "model belongs to another context");
-  if (g->orient_mode < 0) return fail(TC_ERR_STATE, "graph must be oriented before counting");
-  TCX_HIP(hipSetDevice(c->device));
+  TCX_TRY(check_graph(c, g, md));
+  if (!count) return fail(TC_ERR_ARG, "NULL argument");
+  if (!md) {  // the fixed rule: AUTO's work-list schedule
+    tc_sched s;
+    memset(&s, 0, sizeof(s));
+    s.kernel = TC_KERNEL_AUTO;
+    return tc_count(c, g, &s, count, timing);
+  }
   tc_timing tm;
   memset(&tm, 0, sizeof(tm));
   *count = 0;
@@ -943,7 +1249,7 @@ int tc_count_learned(tc_ctx* c, tc_graph* g, tc_model* md, uint64_t* count, tc_t
     if (timing) *timing = tm;
     return TC_OK;
   }
-  const SchedCache& L = g->sc;
+  const SchedCache& S = g->sc;
   TCX_HIP(hipEventRecord(c->ev0, c->stream));
   TCX_TRY(learned_schedule(c, g, md));
   TCX_HIP(hipEventRecord(c->ev1, c->stream));
@@ -953,30 +1259,12 @@ int tc_count_learned(tc_ctx* c, tc_graph* g, tc_model* md, uint64_t* count, tc_t
   TCX_TRY(c->accum.reserve(64));
   uint32_t* cursor = c->scratch.as<uint32_t>();
   unsigned long long* acc = c->accum.as<unsigned long long>();
-  TCX_HIP(hipMemsetAsync(cursor, 0, 16, c->stream));
+  TCX_HIP(hipMemsetAsync(cursor, 0, 8 * sizeof(uint32_t), c->stream));
   TCX_HIP(hipMemsetAsync(acc, 0, 8, c->stream));
-  const uint32_t* rp = g->rowptr.as<uint32_t>();
-  const uint32_t* col = g->col.as<uint32_t>();
   TCX_HIP(hipEventRecord(c->ev0, c->stream));
-  if (L.nl_blk || L.nl_wav) {
-    const uint32_t G = (uint32_t)c->num_cus * 8u;
-    if (L.nl_wav <= (uint64_t)kStaticWaves * G * (TB / 64))
-      hipLaunchKernelGGL((k_auto<2048, true>), dim3(G), dim3(TB), 0, c->stream, rp, col,
-                         L.l_blk.as<uint32_t>(), (uint32_t)L.nl_blk, L.l_wav.as<uint32_t>(),
-                         (uint32_t)L.nl_wav, cursor, acc);
-    else
-      hipLaunchKernelGGL(k_auto<2048>, dim3(G), dim3(TB), 0, c->stream, rp, col,
-                         L.l_blk.as<uint32_t>(), (uint32_t)L.nl_blk, L.l_wav.as<uint32_t>(),
-                         (uint32_t)L.nl_wav, cursor, acc);
-    tm.launches++;
-  }
-  if (L.nl_huge) {
-    hipLaunchKernelGGL(k_auto<8192>, dim3(std::min<uint64_t>((uint64_t)c->num_cus * 4, L.nl_huge)),
-                       dim3(TB), 0, c->stream, rp, col, L.l_huge.as<uint32_t>(), (uint32_t)L.nl_huge,
-                       (const uint32_t*)nullptr, 0u, cursor + 2, acc);
-    tm.launches++;
-  }
-  TCX_HIP(hipGetLastError());
+  TCX_TRY(launch_lists(c, g, S.l_blk.as<uint32_t>(), S.nl_blk, S.l_wav.as<uint32_t>(), S.nl_wav,
+                       S.l_lan.as<uint32_t>(), S.nl_lan, S.l_huge.as<uint32_t>(), S.nl_huge,
+                       cursor, acc, &tm.launches));
   TCX_HIP(hipEventRecord(c->ev1, c->stream));
   TCX_HIP(hipEventSynchronize(c->ev1));
   TCX_HIP(hipEventElapsedTime(&tm.ms_count, c->ev0, c->ev1));

@@ -39,13 +39,14 @@ ERR_ARG, ERR_IO, ERR_FORMAT, ERR_RANGE, ERR_NOMEM, ERR_HIP, ERR_UNSUPPORTED, ERR
 
 ORIENT_ID, ORIENT_DEGREE, ORIENT_ADIR, ORIENT_KCORE, ORIENT_DEGREE_TRICORE = 0, 1, 2, 3, 4
 ORIENT_DEGENERACY = 5
+ORIENT_ADIR_OMP = 6  # set by Graph.orient_adir_omp
 FLAG_PRUNE_DEG1 = 1
 KERNEL_HU, KERNEL_BISSON, KERNEL_FOX, KERNEL_TRICORE, KERNEL_AUTO = 0, 1, 2, 3, 4
 KERNEL_NAMES = {"hu": KERNEL_HU, "bisson": KERNEL_BISSON, "fox": KERNEL_FOX,
                 "tricore": KERNEL_TRICORE, "auto": KERNEL_AUTO}
 AUTO_WORKLIST, AUTO_PUSH = 0, 1
 KCL_AUTO, KCL_BITMAP, KCL_MERGE, KCL_PIVOT = 0, 1, 2, 3
-RELABEL_COMPACT, RELABEL_FIRST_SRC, RELABEL_DEGREE = 0, 1, 2
+RELABEL_COMPACT, RELABEL_FIRST_SRC, RELABEL_DEGREE, RELABEL_ADIR = 0, 1, 2, 3
 
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -134,6 +135,7 @@ _sig("tc_bw_model_reference", C.c_int, [C.POINTER(BwModel)])
 _sig("tc_measure_list_bandwidth", C.c_int, [_vp, _vp, C.c_uint32, _vp])
 _sig("tc_bw_model_fit", C.c_int, [_vp, _vp, C.c_uint32, C.POINTER(BwModel)])
 _sig("tc_graph_orient", C.c_int, [_vp, _vp, C.c_int, C.c_uint32])
+_sig("tc_graph_orient_adir_omp", C.c_int, [_vp, _vp, C.c_double])
 _sig("tc_graph_relabel", C.c_int, [_vp, _vp, C.c_int, _vp])
 _sig("tc_graph_info_get", C.c_int, [_vp, C.POINTER(GraphInfo)])
 _sig("tc_graph_get_csr", C.c_int, [_vp, _vp, _vp])
@@ -146,6 +148,7 @@ _sig("tc_fox_bin_sizes", C.c_int, [_vp, _vp, C.c_uint32, _u64p])
 _sig("tc_count", C.c_int, [_vp, _vp, C.POINTER(Sched), C.POINTER(C.c_uint64),
                            C.POINTER(Timing)])
 _sig("tc_count_per_vertex", C.c_int, [_vp, _vp, _u64p])
+_sig("tc_batch_stream_bytes", C.c_int, [_vp, _vp, C.POINTER(C.c_double)])
 _sig("tc_graph_save_csr", C.c_int, [_vp, C.c_char_p])
 _sig("tc_graph_load_csr", C.c_int, [_vp, C.c_char_p, C.POINTER(_vp)])
 _sig("tc_auto_stream_bytes", C.c_int, [_vp, _vp, C.c_void_p, C.POINTER(C.c_double)])
@@ -174,8 +177,13 @@ _sig("tc_model_loss_grad", C.c_int, [_vp, _vp, _f32p, _f32p, _f32p, C.c_uint32,
                                      C.POINTER(C.c_float), _f32p])
 _sig("tc_model_train", C.c_int, [_vp, _vp, _f32p, _f32p, _f32p, C.c_uint32,
                                  C.POINTER(TrainOpts), C.POINTER(C.c_float)])
-_sig("tc_graph_features", C.c_int, [_vp, _vp, _f32p])
+_sig("tc_graph_features", C.c_int, [_vp, _vp, _vp, _f32p])
+_sig("tc_model_set_bw_model", C.c_int, [_vp, C.c_void_p])
 _sig("tc_graph_measure_costs", C.c_int, [_vp, _vp, C.c_int, _f32p, C.POINTER(C.c_uint64)])
+_sig("tc_graph_cell_features", C.c_int, [_vp, _vp, _vp, C.c_uint32, _f32p])
+_sig("tc_graph_measure_cells", C.c_int, [_vp, _vp, C.c_int, C.c_uint32, C.c_uint32, _f32p,
+                                         C.c_void_p])
+_sig("tc_learned_cells", C.c_int, [_vp, _vp, _vp, C.POINTER(C.c_uint32), C.c_void_p])
 _sig("tc_learned_assignment", C.c_int, [_vp, _vp, _vp, _u8p])
 _sig("tc_count_learned", C.c_int, [_vp, _vp, _vp, C.POINTER(C.c_uint64), C.POINTER(Timing)])
 _sig("kcl_count", C.c_int, [_vp, _vp, C.c_uint32, C.c_int, C.POINTER(C.c_uint64),
@@ -232,11 +240,16 @@ EXPORTED = [
     "tc_tricore_split", "tc_tricore_tricount_gpu", "tc_model_num_params", "tc_model_create",
     "tc_model_destroy", "tc_model_set_params", "tc_model_get_params", "tc_model_set_norm",
     "tc_model_get_norm", "tc_model_forward", "tc_model_loss_grad", "tc_model_train",
-    "tc_graph_features", "tc_graph_measure_costs", "tc_learned_assignment", "tc_count_learned",
+    "tc_graph_orient_adir_omp",
+    "tc_graph_features", "tc_model_set_bw_model", "tc_graph_measure_costs",
+    "tc_graph_cell_features", "tc_graph_measure_cells", "tc_learned_cells",
+    "tc_learned_assignment", "tc_count_learned",
     "tc_batch_create", "tc_batch_destroy", "tc_batch_info_get", "tc_batch_autotune", "tc_batch_count",
-    "tc_batch_count_pergraph", "tc_count_csr_noalloc", "tc_graph_save_csr", "tc_graph_load_csr",
+    "tc_batch_count_pergraph", "tc_batch_stream_bytes", "tc_count_csr_noalloc", "tc_graph_save_csr", "tc_graph_load_csr",
 ]
-VARIANT_BLOCK, VARIANT_WAVE = 0, 1
+VARIANT_BLOCK, VARIANT_WAVE, VARIANT_LANE = 0, 1, 2
+CELLS, CELL_MAX_LEVEL = 15, 16
+FIXED_NARROW = 0xFFC  # the fixed rule runs cells 2..11 (d+ <= 64) narrow
 N_FEATURES, N_OUTPUTS = 8, 2
 
 
@@ -519,8 +532,9 @@ class Graph:
 
     def relabel(self, mode: int):
         """Id relabelling of tools/serialize.cpp (RELABEL_COMPACT),
-        tools/adjust2tc.cpp (RELABEL_FIRST_SRC) or
-        tools/directionByReorder.cpp (RELABEL_DEGREE); returns old -> new."""
+        tools/adjust2tc.cpp (RELABEL_FIRST_SRC),
+        tools/directionByReorder.cpp (RELABEL_DEGREE) or
+        Bisson/A-direction.cpp (RELABEL_ADIR); returns old -> new."""
         s, d = self.edges()
         nids = int(max(s.max(), d.max())) + 1 if len(s) else 1
         perm = np.empty(nids, np.uint32)
@@ -546,6 +560,13 @@ class Graph:
     def orient(self, mode: int, prune_deg1: bool = False):
         _check(_lib.tc_graph_orient(self.ctx.handle, self._h, mode,
                                     FLAG_PRUNE_DEG1 if prune_deg1 else 0), "tc_graph_orient")
+        return self
+
+    def orient_adir_omp(self, thre_param: float):
+        """The OpenMP A-direction program's schedule
+        (edge_directing/A-direction.cpp, argv thre_param)."""
+        _check(_lib.tc_graph_orient_adir_omp(self.ctx.handle, self._h, float(thre_param)),
+               "tc_graph_orient_adir_omp")
         return self
 
     def info(self) -> GraphInfo:
@@ -648,20 +669,51 @@ class Graph:
         _check(_lib.tc_count_per_vertex(self.ctx.handle, self._h, tri), "tc_count_per_vertex")
         return tri[:n]
 
-    def features(self):
+    def features(self, model=None):
+        """Per-task features of the work-list schedule (tc_graph_features);
+        feature 4 uses the model's bandwidth model (the reference's without
+        one)."""
         n = self.info().n
         X = np.zeros((max(n, 1), N_FEATURES), np.float32)
-        _check(_lib.tc_graph_features(self.ctx.handle, self._h, X.reshape(-1)), "tc_graph_features")
+        _check(_lib.tc_graph_features(self.ctx.handle, self._h, model.handle if model else None,
+                                      X.reshape(-1)), "tc_graph_features")
         return X[:n]
 
     def measure_costs(self, variant: int):
-        """(cost_us per vertex, exact count of the timed run)."""
+        """(wave-microseconds per task under the variant, exact count of the
+        timed run)."""
         n = self.info().n
         c = np.zeros(max(n, 1), np.float32)
         cnt = C.c_uint64()
         _check(_lib.tc_graph_measure_costs(self.ctx.handle, self._h, variant, c, C.byref(cnt)),
                "tc_graph_measure_costs")
         return c[:n], cnt.value
+
+    def cell_features(self, model=None, level: int = 0):
+        """Features (CELLS x 8) of the decision cells over the tasks of a
+        sampling level (tc_graph_cell_features)."""
+        X = np.zeros((CELLS, N_FEATURES), np.float32)
+        _check(_lib.tc_graph_cell_features(self.ctx.handle, self._h,
+                                           model.handle if model else None, level, X.reshape(-1)),
+               "tc_graph_cell_features")
+        return X
+
+    def measure_cells(self, narrow: bool, level: int = 0, reps: int = 3):
+        """(microseconds of each cell's count launch alone, its exact count),
+        the cell run narrow or wide (tc_graph_measure_cells)."""
+        us = np.zeros(CELLS, np.float32)
+        cnt = np.zeros(CELLS, np.uint64)
+        _check(_lib.tc_graph_measure_cells(self.ctx.handle, self._h, int(bool(narrow)), level,
+                                           reps, us, cnt.ctypes.data), "tc_graph_measure_cells")
+        return us, cnt
+
+    def learned_cells(self, model=None):
+        """(narrow-cell mask, predictions CELLS x 2) of the model's choice."""
+        mask = C.c_uint32()
+        Y = np.zeros((CELLS, 2), np.float32)
+        _check(_lib.tc_learned_cells(self.ctx.handle, self._h, model.handle if model else None,
+                                     C.byref(mask), Y.ctypes.data), "tc_learned_cells")
+        return mask.value, Y
 
     def learned_assignment(self, model=None):
         n = self.info().n
@@ -785,6 +837,13 @@ class Batch:
         _check(_lib.tc_batch_info_get(self._h, C.byref(i)), "tc_batch_info_get")
         return i
 
+    def stream_bytes(self) -> float:
+        """Bytes one batched count streams (tc_batch_stream_bytes)."""
+        out = C.c_double()
+        _check(_lib.tc_batch_stream_bytes(self.ctx.handle, self._h, C.byref(out)),
+               "tc_batch_stream_bytes")
+        return out.value
+
     def autotune(self) -> BatchSched:
         s = BatchSched()
         _check(_lib.tc_batch_autotune(self.ctx.handle, self._h, C.byref(s)), "tc_batch_autotune")
@@ -808,7 +867,7 @@ class Batch:
 
 
 class Model:
-    """The AUTO engine's per-vertex cost model (8 -> 32 -> 32 -> 2 MLP)."""
+    """The AUTO engine's per-task cost model (8 -> 32 -> 32 -> 2 MLP)."""
     n_params = int(_lib.tc_model_num_params())
 
     def __init__(self, ctx: Context, seed: int = 0):
@@ -855,6 +914,11 @@ class Model:
     def norm(self, v):
         xm, xs, ym, ys = (np.ascontiguousarray(a, np.float32) for a in v)
         _check(_lib.tc_model_set_norm(self._h, xm, xs, ym, ys), "tc_model_set_norm")
+
+    def set_bw_model(self, bw: "BwModel"):
+        """The list-read bandwidth model behind feature 4 (tc_model_set_bw_model)."""
+        _check(_lib.tc_model_set_bw_model(self._h, C.byref(bw)), "tc_model_set_bw_model")
+        return self
 
     def forward(self, X):
         X = np.ascontiguousarray(X, np.float32)

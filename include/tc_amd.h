@@ -52,6 +52,10 @@ extern "C" {
  * by the removal step bounds every out-degree by the vertex's core number,
  * hence by the degeneracy (the k-clique path's best orientation). */
 #define TC_ORIENT_DEGENERACY 5
+/* The OpenMP program's A-direction schedule (edge_directing/A-direction.cpp:
+ * 36-161, argv `A-direction <bin> <thre_param>`): set by
+ * tc_graph_orient_adir_omp, not by tc_graph_orient. */
+#define TC_ORIENT_ADIR_OMP 6
 
 /* Orientation flags. */
 #define TC_FLAG_PRUNE_DEG1 1u   /* Hu/preTC.cu:135-169: drop degree<=1 vertices, compact ids */
@@ -238,11 +242,33 @@ int tc_graph_relabel_aorder_model(tc_ctx* ctx, tc_graph* g, const tc_bw_model* m
 #define TC_RELABEL_COMPACT 0
 #define TC_RELABEL_FIRST_SRC 1
 #define TC_RELABEL_DEGREE 2
+/*   TC_RELABEL_ADIR      Bisson/A-direction.cpp:287-303: the A-direction peel
+ *                        of A-direction-seri.cpp (threshold 2m/n + 1 over
+ *                        n = max id + 1), then ids sorted by (priority,
+ *                        runtime degree when peeled, id), the last gets id 0,
+ *                        so that the ID orientation (src = larger id, Bisson's
+ *                        chooseIndex 0) points every edge up the peel order.
+ *                        The degree recorded is the peeled vertex's own
+ *                        (the reference stores it at the frontier index
+ *                        instead, :118, which leaves cascade-peeled entries
+ *                        unset). */
+#define TC_RELABEL_ADIR 3
 int tc_graph_relabel(tc_ctx* ctx, tc_graph* g, int mode, uint32_t* perm);
 /* Orients the raw records and builds the oriented CSR in HBM
  * (Hu/preTC.cu:116-291, Fox/preTC.cu:88-203, tricore/src/tricount_gpu.cu:37-203,
  * edge_directing/A-direction-seri.cpp:35-273, edge_directing/kcore_direction.cpp:25-146). */
 int tc_graph_orient(tc_ctx* ctx, tc_graph* g, int mode, uint32_t flags);
+/* The OpenMP A-direction program (edge_directing/A-direction.cpp:36-161,
+ * 197-198) restated deterministically: threshold0 = 2 totalEdge / n (float);
+ * each round peels every unpeeled vertex whose runtime degree is within the
+ * threshold, and cascades (as the serial peel does) only while the frontier
+ * holds more than n / 5000 and more than 200 vertices, for at most six
+ * levels; the threshold then grows by (int)threshold0 * thre_param / 5.
+ * Edges point up (priority, id) (:259-268).  The reference runs this with
+ * unsynchronised OpenMP threads, so its labels vary from run to run; this
+ * is the one-thread order of its loops with a frontier of exactly the
+ * vertices peeled at that step.  thre_param must be positive. */
+int tc_graph_orient_adir_omp(tc_ctx* ctx, tc_graph* g, double thre_param);
 int tc_graph_info_get(const tc_graph* g, tc_graph_info* info);
 /* Copies of device state (host arrays of the documented sizes). */
 int tc_graph_get_csr(const tc_graph* g, uint32_t* rowptr /*n+1*/, uint32_t* col /*m+*/);
@@ -329,15 +355,27 @@ int tc_count_csr_noalloc(tc_ctx* ctx, const uint32_t* rowptr, const uint32_t* co
  * fixed work formulas (Hu/tc.cu:22-35, Fox/tc.cu:18-33,316-331) and its
  * orientation quality from fixed cost functions (tools/costFuncCal.cpp,
  * tools/verification.cpp).  Here a small MLP (8 features -> 32 -> 32 -> 2,
- * ReLU, BF16 weights, FP32 accumulate) predicts, per vertex, the log2 cost in
- * microseconds of the two AUTO variants; it is trained on costs the engine
- * measures on itself.  The schedule only decides WHICH variant counts a
- * vertex and in what order; every vertex with d+ >= 2 is counted exactly
+ * ReLU, BF16 weights, FP32 accumulate) chooses the per-bin variant of AUTO's
+ * work-list schedule (task x holds N+(x) and streams N+(y) for y in W(x)).
+ * The bins ("cells") are d+(x) = 2..8 (cells 2..8) and d+ in (2^(b-1), 2^b]
+ * for b = 4..9 (cells 9..14, up to d+ 512).  A cell's tasks run either their
+ * narrow variant (a lane for d+ <= 8, a wave above) or their wide one (a wave
+ * for d+ <= 8, a workgroup above); the fixed rule runs cells 2..11
+ * (d+ <= 64) narrow, which sets AUTO's bin thresholds.  Whatever the cell
+ * runs, a lane task keeps at most 16 streamed lists and a wave task at most
+ * 2^16 entries of work (else it runs one size up), and tasks above d+ 512
+ * run as workgroups.  The model predicts, from 8 features of a cell's tasks,
+ * the log2 microseconds of the cell's count launch under each variant; it is
+ * trained on launches the engine times on itself (tc_graph_measure_cells).
+ * The schedule only decides HOW each task runs; every task runs exactly
  * once, so the count is exact whatever the model predicts. */
 #define TC_MODEL_FEATURES 8
 #define TC_MODEL_OUTPUTS 2
-#define TC_VARIANT_BLOCK 0   /* a workgroup per vertex                        */
-#define TC_VARIANT_WAVE 1    /* a wave per vertex, d+ <= 512                   */
+#define TC_VARIANT_BLOCK 0   /* a workgroup per task                          */
+#define TC_VARIANT_WAVE 1    /* a wave per task, d+ <= 512                     */
+#define TC_VARIANT_LANE 2    /* a lane per task, d+ <= 8 (N+(x) in registers)  */
+#define TC_CELLS 15          /* cell arrays are indexed by cell, 0 and 1 unused  */
+#define TC_CELL_MAX_LEVEL 16 /* sampling levels of the cell calls               */
 
 typedef struct tc_model tc_model;
 
@@ -371,25 +409,57 @@ int tc_model_loss_grad(tc_ctx* ctx, tc_model* m, const float* X, const float* T,
 /* Sets the standardisation from the data, then trains with Adam on the GPU. */
 int tc_model_train(tc_ctx* ctx, tc_model* m, const float* X, const float* T, const float* M,
                    uint32_t N, const tc_train_opts* opts, float* final_loss);
-/* Per-vertex features X[n][8] of the oriented CSR:
- * log2(1+d+), log2(1+d-), log2(1+sum_v d+(v)), log2(1+max_v d+(v)),
- * log2(1+sum_v min(d+(u),d+(v))), floor(log2(1+d+)) (degree bin id),
- * fraction of out-neighbours with d+ < 64, log2(1+sum_v ceil(d+(v)/64)). */
-int tc_graph_features(tc_ctx* ctx, tc_graph* g, float* X);
-/* Measured per-vertex cost (microseconds) of one variant, from a timed run
- * of that variant over every vertex with d+ >= 2 (wave: d+ <= 512);
- * 0 where not measured.  count (optional) receives the run's exact count. */
+/* The list-read bandwidth model behind feature 4 (default: the reference's,
+ * tc_bw_model_reference; the MI355X one is measured by
+ * tc_measure_list_bandwidth / tc_bw_model_fit). */
+int tc_model_set_bw_model(tc_model* m, const tc_bw_model* bw);
+/* Per-task figures (diagnostics; the schedule uses the cell calls below).
+ * Per-task features X[n][8] of the work-list schedule (0 for vertices with
+ * no task: d+ < 2 or an empty work list): log2(1+d+(x)), log2(1+|W(x)|),
+ * log2(1+sum d+(y)), log2(1+max d+(y)), log2(1+t) with t the nanoseconds
+ * the lists N+(y) take to read under the model's bandwidth model (the
+ * reference's when m is NULL), floor(log2(1+d+(x))), the fraction of y with
+ * d+(y) < 64, log2(1+sum ceil(d+(y)/64)). */
+int tc_graph_features(tc_ctx* ctx, tc_graph* g, const tc_model* m, float* X);
+/* Measured cost of every task under one variant (TC_VARIANT_*), in
+ * wave-microseconds: a block task's time x 4 (it holds four waves), a wave
+ * task's time, a lane task's round / 64; from a timed run of that variant
+ * over every task it can run (wave: d+ <= 512, lane: d+ <= 8); 0 where not
+ * measured.  count (optional) receives the run's exact count. */
 int tc_graph_measure_costs(tc_ctx* ctx, tc_graph* g, int variant, float* cost_us,
                            uint64_t* count);
-/* Variant chosen for each vertex (255 = not scheduled, d+ < 2).  Per degree
- * bin floor(log2 d+), the model's summed wave-time of the two variants picks
- * the bin's variant, with the fixed rule (wave iff d+ <= 64) as the prior: a
- * bin switches only for a predicted saving above 25 %.  A wave task
- * predicted to take more than half of a wave's share goes to the block
- * variant.  model NULL gives the fixed rule. */
+/* Cell features X[TC_CELLS][8] over the tasks of sampling level `level`
+ * (level 0: all tasks; level s: those whose id hash h(x) = (x * 0x9E3779B1)
+ * xor'ed with its upper 16 bits has s low zero bits): log2(1+tasks),
+ * log2(1+sum |W(x)|), log2(1+sum of streamed entries d+(y)), log2(1+sum
+ * d+(x)), log2(1+t) with t the nanoseconds the streamed lists take under
+ * the model's bandwidth model (the reference's when m is NULL; per task
+ * rounded to picoseconds, then summed), the fraction of streamed lists with
+ * d+(y) < 64, log2(1+largest task work d+(x) + sum d+(y)), the fraction of
+ * tasks within their narrow variant's limit.  Empty cells are all zero. */
+int tc_graph_cell_features(tc_ctx* ctx, tc_graph* g, const tc_model* m, uint32_t level,
+                           float* X);
+/* Times each non-empty cell's count launch alone, over the tasks of
+ * `level`, with the cell run narrow (narrow != 0) or wide: us[cell] = the
+ * fastest of `reps` (1..64) timed launches; counts[cell] (optional) = the
+ * triangles those tasks count (the same in every repetition, else
+ * TC_ERR_STATE).  Entries of empty cells are 0. */
+int tc_graph_measure_cells(tc_ctx* ctx, tc_graph* g, int narrow, uint32_t level, uint32_t reps,
+                           float* us, uint64_t* counts);
+/* The model's choice: *narrow_cells = bit mask of the cells run narrow, Y
+ * (optional, TC_CELLS x 2) = its predictions [wide, narrow] from the cells'
+ * level-0 features (rows of empty cells are not used).  The
+ * fixed rule (cells 2..11) is the prior: a non-empty cell switches only
+ * when the other variant is predicted more than 10 % faster.  With m NULL,
+ * the fixed rule's mask and Y = 0. */
+int tc_learned_cells(tc_ctx* ctx, tc_graph* g, tc_model* m, uint32_t* narrow_cells, float* Y);
+/* Variant chosen for each vertex's task (TC_VARIANT_*, 255 = no task): the
+ * cell's choice within the task's limits (above).  model NULL gives the
+ * fixed rule's variants (tc_count's AUTO work-list schedule). */
 int tc_learned_assignment(tc_ctx* ctx, tc_graph* g, tc_model* m, uint8_t* variant);
-/* Exact count with the learned schedule; model NULL falls back to the fixed
- * degree-bin schedule of TC_KERNEL_AUTO. */
+/* Exact count with the learned schedule (block tasks by descending work, as
+ * the fixed rule orders them; wave and lane tasks in id order); model NULL
+ * counts with the fixed rule (tc_count, TC_KERNEL_AUTO). */
 int tc_count_learned(tc_ctx* ctx, tc_graph* g, tc_model* m, uint64_t* count,
                      tc_timing* timing);
 
@@ -397,8 +467,10 @@ int tc_count_learned(tc_ctx* ctx, tc_graph* g, tc_model* m, uint64_t* count,
  * No counterpart in the reference (one graph per process run).  A batch is
  * the disjoint union of its graphs in one degree-oriented CSR (orientation
  * rules are local, so this equals orienting each graph on its own); one
- * persistent kernel counts every graph and accumulates per-graph uint64
- * counts.  The count sequence is captured once in a hipGraph and replayed. */
+ * persistent kernel counts every graph over AUTO's work lists (TC_AUTO_WORKLIST:
+ * an edge never leaves its graph, so neither does a work list) and
+ * accumulates per-graph uint64 counts.  The count sequence is captured once
+ * in a hipGraph and replayed. */
 typedef struct tc_batch_sched {
   uint32_t wave_max_deg;  /* vertices with 2 <= d+ <= this run as wave tasks (64) */
   uint32_t wave_chunk;    /* wave tasks fetched per cursor bump (16)              */
@@ -417,6 +489,9 @@ int tc_batch_create(tc_ctx* ctx, uint32_t ngraphs, const uint64_t* edge_off,
 int tc_batch_destroy(tc_batch* b);
 /* Sizes of the batch's union CSR (n, m+, algorithmic bytes of one count). */
 int tc_batch_info_get(const tc_batch* b, tc_graph_info* info);
+/* Bytes one batched count streams: the work-list bytes of the union CSR
+ * (tc_auto_stream_bytes), whatever the schedule's task split. */
+int tc_batch_stream_bytes(tc_ctx* ctx, tc_batch* b, double* bytes);
 /* Tries wave_max_deg, then filter_min, then wave_chunk candidates on the
  * batch and returns the fastest (timed with hipEvents). */
 int tc_batch_autotune(tc_ctx* ctx, tc_batch* b, tc_batch_sched* best);

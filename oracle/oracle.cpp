@@ -255,13 +255,25 @@ uint64_t orc_build_sym(const uint32_t* src, const uint32_t* dst, uint64_t m,
 
 // Literal restatement of edge_directing/A-direction-seri.cpp:35-153 with the
 // originEdges of :242-252 (pairs (i, nbr) for i ascending, nbr ascending).
-// Parity unpinned beyond invariants: the reference needs Boost to build.
-void orc_adir_priority(uint32_t n, const uint32_t* sym_rowptr,
-                       const uint32_t* sym_col, int32_t* prio) {
+// The reference needs Boost to build; the restatement is pinned by
+// hand-traced graphs (tests/test_oracle.py, tests/test_oracle_pins.py).
+// rec (optional): Bisson/A-direction.cpp's degreeRecord: the scan writes the
+// peeled vertex's runtime degree (:96); a cascade push writes the pushed
+// vertex's runtime degree into the slot of the pushing element's frontier
+// index f (:118 writes degreeRecord[i], i the frontier loop index).
+// thre_param > 0: edge_directing/A-direction.cpp:36-161 (the OpenMP program)
+// in one-thread loop order, with each frontier holding exactly the vertices
+// peeled at its step: threshold0 = 2 edgeNum / nodeNum (:38), a cascade
+// level runs only while the frontier exceeds nodeNum / 5000 and 200 and for
+// at most six levels (:96), the threshold grows by threshold_old *
+// thre_param / 5 (:139).
+static void adir_peel_impl(uint32_t n, const uint32_t* sym_rowptr, const uint32_t* sym_col,
+                           int32_t* prio, int32_t* rec, double thre_param = 0) {
   if (n == 0) return;
+  const bool omp = thre_param > 0;
   unsigned int edgeNum = (unsigned int)sym_rowptr[n];  // totalEdge = 2m
   int nodeNum = (int)n;
-  float threshold = ((float)edgeNum) / nodeNum + 1;
+  float threshold = omp ? 2 * ((float)edgeNum) / nodeNum : ((float)edgeNum) / nodeNum + 1;
   for (uint32_t i = 0; i < n; ++i) prio[i] = -1;
   std::vector<int> degree(n, 0);
   // every (i,nbr) pair bumps both ends: degree = 2*|N(v)|  (:44-47)
@@ -297,10 +309,15 @@ void orc_adir_priority(uint32_t n, const uint32_t* sym_rowptr,
         frontier.push_back(i);
         peeled++;
         prio[i] = priorityNum;
+        if (rec) rec[i] = rtd[i];
       }
     }
     priorityNum++;
-    while (!frontier.empty()) {
+    int innerLoop = 0;
+    while (!frontier.empty() &&
+           (!omp || ((int)frontier.size() > nodeNum / 5000 && (int)frontier.size() > 200 &&
+                     innerLoop <= 5))) {
+      innerLoop++;
       for (size_t f = 0; f < frontier.size(); ++f) {
         int cur = frontier[f];
         const int* lst = allEdges.data() + off[cur];
@@ -316,6 +333,7 @@ void orc_adir_priority(uint32_t n, const uint32_t* sym_rowptr,
               next.push_back(node);
               peeled++;
               prio[node] = priorityNum;
+              if (rec) rec[f] = rtd[node];
             }
           }
         }
@@ -325,8 +343,44 @@ void orc_adir_priority(uint32_t n, const uint32_t* sym_rowptr,
       priorityNum++;
     }
     if (peeled >= nodeNum) break;
-    threshold += threshold_old;
+    if (omp) threshold += (double)threshold_old * thre_param / 5;
+    else threshold += threshold_old;
   }
+}
+
+void orc_adir_omp_priority(uint32_t n, const uint32_t* sym_rowptr, const uint32_t* sym_col,
+                           double thre_param, int32_t* prio) {
+  adir_peel_impl(n, sym_rowptr, sym_col, prio, nullptr, thre_param);
+}
+
+void orc_adir_priority(uint32_t n, const uint32_t* sym_rowptr, const uint32_t* sym_col,
+                       int32_t* prio) {
+  adir_peel_impl(n, sym_rowptr, sym_col, prio, nullptr);
+}
+
+// Bisson/A-direction.cpp:180-304: the peel over the records' simple
+// symmetric graph (nodeNum = max id + 1, :196-202; self-loops dropped and
+// lists deduplicated, :223-238), then units sorted by (priority, degree
+// record, id) ascending (:34-42,287-293) and lookUpTable[id] = nodeNum - 1 -
+// rank (:294-297).  perm has max id + 1 entries.
+void orc_relabel_adir(const uint32_t* src, const uint32_t* dst, uint64_t m, uint32_t* perm) {
+  const uint32_t n = orc_num_nodes(src, dst, m);
+  std::vector<uint32_t> rp((size_t)n + 1), col(std::max<uint64_t>(2 * m, 1));
+  const uint64_t m2 = orc_build_sym(src, dst, m, n, rp.data(), col.data());
+  (void)m2;
+  std::vector<int32_t> prio(n), rec(n, 0);
+  adir_peel_impl(n, rp.data(), col.data(), prio.data(), rec.data());
+  struct Unit {
+    int priority, degree, id;
+    bool operator<(const Unit& b) const {
+      return priority < b.priority || (priority == b.priority && degree < b.degree) ||
+             (priority == b.priority && degree == b.degree && id < b.id);
+    }
+  };
+  std::vector<Unit> units(n);
+  for (uint32_t i = 0; i < n; ++i) units[i] = Unit{prio[i], rec[i], (int)i};
+  std::sort(units.begin(), units.end());
+  for (uint32_t i = 0; i < n; ++i) perm[units[i].id] = n - 1 - i;
 }
 
 uint64_t orc_adir_orient(uint32_t n, const uint32_t* sym_rowptr,

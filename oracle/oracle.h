@@ -67,6 +67,13 @@ uint64_t orc_build_sym(const uint32_t* src, const uint32_t* dst, uint64_t m,
  * invariants only (the reference needs Boost and cannot be built here). */
 void orc_adir_priority(uint32_t n, const uint32_t* sym_rowptr,
                        const uint32_t* sym_col, int32_t* prio);
+/* Bisson/A-direction.cpp:180-304: A-direction peel, then ids ranked by
+ * (priority, runtime degree at peel, id), the last rank gets id 0. */
+/* edge_directing/A-direction.cpp:36-161 (OpenMP program, one-thread order,
+ * each frontier exactly the vertices peeled at its step); thre_param > 0. */
+void orc_adir_omp_priority(uint32_t n, const uint32_t* sym_rowptr, const uint32_t* sym_col,
+                           double thre_param, int32_t* prio);
+void orc_relabel_adir(const uint32_t* src, const uint32_t* dst, uint64_t m, uint32_t* perm);
 
 /* A-direction emit rule, edge_directing/A-direction-seri.cpp:254-273:
  * edge i->j iff prio[j] > prio[i] || (prio[j]==prio[i] && i<j).

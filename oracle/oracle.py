@@ -48,6 +48,7 @@ _sig("orc_prune_deg1", C.c_uint32, [_u32p, _u32p, C.c_uint64, C.c_uint32, _u32p]
 _sig("orc_build_csr", C.c_uint64, [_u32p, _u32p, C.c_uint64, C.c_uint32, _u32p, _u32p])
 _sig("orc_build_sym", C.c_uint64, [_u32p, _u32p, C.c_uint64, C.c_uint32, _u32p, _u32p])
 _sig("orc_adir_priority", None, [C.c_uint32, _u32p, _u32p, _i32p])
+_sig("orc_adir_omp_priority", None, [C.c_uint32, _u32p, _u32p, C.c_double, _i32p])
 _sig("orc_adir_orient", C.c_uint64, [C.c_uint32, _u32p, _u32p, _i32p, _u32p, _u32p])
 _sig("orc_kcore", None, [C.c_uint32, _u32p, _u32p, _u32p])
 _sig("orc_kcore_orient", C.c_uint64, [C.c_uint32, _u32p, _u32p, _u32p, _u32p, _u32p])
@@ -67,7 +68,8 @@ _sig("orc_kclique_rows_strided", C.c_uint64, [C.c_uint32, _u32p, _u32p, C.c_uint
 _sig("orc_kclique_row_list", None, [C.c_uint32, _u32p, _u32p, C.c_uint32, _u32p, C.c_uint64,
                                     _u64p, C.c_int])
 _sig("orc_hu_workload", None, [C.c_uint32, _u32p, _u32p, _i64p])
-for _name in ("orc_relabel_compact", "orc_relabel_first_src", "orc_relabel_degree"):
+for _name in ("orc_relabel_compact", "orc_relabel_first_src", "orc_relabel_degree",
+              "orc_relabel_adir"):
     _sig(_name, None, [_u32p, _u32p, C.c_uint64, _u32p])
 _sig("orc_hu_block_starts", None, [C.c_uint32, _i64p, C.c_uint32, _i32p])
 _sig("orc_fox_workload", None, [C.c_uint64, _u32p, _u32p, _u32p, _i32p])
@@ -139,6 +141,13 @@ def build_sym(src, dst, n):
 def adir_priority(n, srp, scol):
     prio = np.zeros(max(n, 1), np.int32)
     _lib.orc_adir_priority(n, _u32(srp), _nz(_u32(scol), np.uint32), prio)
+    return prio[:n]
+
+
+def adir_omp_priority(n, srp, scol, thre_param):
+    """edge_directing/A-direction.cpp (OpenMP program) in one-thread order."""
+    prio = np.zeros(max(n, 1), np.int32)
+    _lib.orc_adir_omp_priority(n, _u32(srp), _nz(_u32(scol), np.uint32), float(thre_param), prio)
     return prio[:n]
 
 
@@ -294,12 +303,13 @@ def verification(src, dst):
 
 def relabel(src, dst, mode):
     """old -> new id map of tools/serialize.cpp ("compact"),
-    tools/adjust2tc.cpp ("first_src") or tools/directionByReorder.cpp ("degree")."""
+    tools/adjust2tc.cpp ("first_src"), tools/directionByReorder.cpp ("degree")
+    or Bisson/A-direction.cpp ("adir")."""
     src, dst = _u32(src), _u32(dst)
     n = num_nodes(src, dst)
     perm = np.zeros(max(n, 1), np.uint32)
     fn = {"compact": _lib.orc_relabel_compact, "first_src": _lib.orc_relabel_first_src,
-          "degree": _lib.orc_relabel_degree}[mode]
+          "degree": _lib.orc_relabel_degree, "adir": _lib.orc_relabel_adir}[mode]
     fn(_nz(src, np.uint32), _nz(dst, np.uint32), len(src), perm)
     return perm[:n]
 

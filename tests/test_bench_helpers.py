@@ -26,12 +26,13 @@ def test_vs_plain_ratios(bench):
            "fox": {"edges_per_s": 6.0e8, "ms_kernels": 70.0},
            "tricore": {"edges_per_s": 8.0e8, "ms_kernels": 50.0},
            "auto": {"edges_per_s": 3.0e9, "ms_kernels": 20.0},
-           "learned": {"edges_per_s": 3.3e9, "ms_kernels": 20.0}}
-    r = bench.vs_plain(res, "learned", 80e9)
+           "learned": {"edges_per_s": 3.3e9, "ms_kernels": 16.0}}
+    r = bench.vs_plain(res, "learned", 64e9)
     assert r["fastest_plain"] == "tricore"
     assert r["vs_fastest_plain"] == pytest.approx(3.3e9 / 8.0e8)
     assert r["vs_auto_fixed_rule"] == pytest.approx(1.1)
-    assert r["hbm_gbps_measured"] == pytest.approx(4000.0)       # 80 GB in 20 ms
+    assert r["vs_auto_fixed_rule_kernel"] == pytest.approx(20.0 / 16.0)
+    assert r["hbm_gbps_measured"] == pytest.approx(4000.0)       # 64 GB in 16 ms
     assert r["hbm_frac_measured"] == pytest.approx(0.5)
     r = bench.vs_plain(res, "auto", None)
     assert "vs_auto_fixed_rule" not in r and "hbm_frac_measured" not in r

@@ -74,6 +74,34 @@ def test_a_direction_writes_oriented_file(graph_file, tmp_path):
     assert orc.count(rp, col) == t
 
 
+def test_openmp_a_direction_writes_oriented_file(graph_file, tmp_path):
+    """edge_directing/A-direction.cpp's argv form <bin> <thre_param>."""
+    p, s, d, t = graph_file
+    run("a_direction", p, 3, cwd=tmp_path)
+    os_, od = tcamd.read_edges(str(tmp_path / "lpBasedGraph.bin"))
+    n = orc.num_nodes(s, d)
+    srp, scol = orc.build_sym(s, d, n)
+    es, ed = orc.adir_orient(n, srp, scol, orc.adir_omp_priority(n, srp, scol, 3))
+    np.testing.assert_array_equal(os_, es)
+    np.testing.assert_array_equal(od, ed)
+    assert "Usage: ./A-direction inputFileName thre_param" in run("a_direction")
+
+
+def test_bisson_a_direction_relabels(graph_file, tmp_path):
+    """Bisson/A-direction.cpp: both directions of every simple edge, old id
+    ascending then neighbour ascending, through the A-direction lookUpTable."""
+    p, s, d, t = graph_file
+    run("bisson_a_direction", p, cwd=tmp_path)
+    rs, rd = tcamd.read_edges(str(tmp_path / "lpBasedGraph.bin"))
+    keep = s != d
+    pairs = np.unique(np.concatenate([s[keep].astype(np.uint64) << np.uint64(32) | d[keep],
+                                      d[keep].astype(np.uint64) << np.uint64(32) | s[keep]]))
+    perm = orc.relabel(s, d, "adir")
+    np.testing.assert_array_equal(rs, perm[(pairs >> np.uint64(32)).astype(np.int64)])
+    np.testing.assert_array_equal(rd, perm[(pairs & np.uint64(0xFFFFFFFF)).astype(np.int64)])
+    assert triangles(run("tc", "-f", str(tmp_path / "lpBasedGraph.bin"))) == t
+
+
 def test_peel_orients_every_record(graph_file, tmp_path):
     p, s, d, _ = graph_file
     out = str(tmp_path / "peeled.bin")

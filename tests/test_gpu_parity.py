@@ -91,6 +91,53 @@ def test_adir_priorities_exact(ctx, case):
     g.close()
 
 
+@pytest.mark.parametrize("case", ["er", "rmat12", "ba", "rgg", "messy", "hub"])
+@pytest.mark.parametrize("thre_param", [1, 5, 20])
+def test_adir_omp_priorities_exact(ctx, case, thre_param):
+    """edge_directing/A-direction.cpp's schedule: priorities and oriented CSR
+    equal the oracle's; the count is unchanged."""
+    s, d = CASES[case]()
+    n = orc.num_nodes(s, d)
+    srp, scol = orc.build_sym(s, d, n)
+    expect = orc.adir_omp_priority(n, srp, scol, thre_param)
+    g = ctx.graph(s, d).orient_adir_omp(thre_param)
+    np.testing.assert_array_equal(g.vertex_key(), expect)
+    es, ed = orc.adir_orient(n, srp, scol, expect)
+    rp, col = g.csr()
+    np.testing.assert_array_equal(np.repeat(np.arange(len(rp) - 1), np.diff(rp)), es)
+    np.testing.assert_array_equal(col, ed)
+    assert g.info().orient_mode == tcamd.ORIENT_ADIR_OMP
+    assert g.count("auto").count == truth(case)
+    g.close()
+
+
+def test_adir_omp_known_answers(ctx):
+    """The oracle's hand-traced graphs: a 150-leaf star (no cascade below 201
+    vertices), a 300-leaf star (cascade) and 16 layers of 210 vertices (the
+    six-level cascade cap)."""
+    g = ctx.graph(np.zeros(150, np.uint32), np.arange(1, 151, dtype=np.uint32))
+    p = g.orient_adir_omp(10).vertex_key()
+    assert p[0] == 50 and set(p[1:].tolist()) == {0}
+    g.close()
+    g = ctx.graph(np.zeros(300, np.uint32), np.arange(1, 301, dtype=np.uint32))
+    p = g.orient_adir_omp(5).vertex_key()
+    assert p[0] == 1 and set(p[1:].tolist()) == {0}
+    g.close()
+    src, dst = [], []
+    for i in range(15):
+        for j in range(210):
+            for t in range(3 + (i == 7)):
+                src.append(i * 210 + j)
+                dst.append((i + 1) * 210 + (j + t) % 210)
+    g = ctx.graph(np.array(src, np.uint32), np.array(dst, np.uint32))
+    p = g.orient_adir_omp(1).vertex_key().reshape(16, 210)
+    want = [0, 1, 2, 3, 4, 5, 6, 8, 8, 6, 5, 4, 3, 2, 1, 0]
+    assert [sorted(set(r.tolist())) for r in p] == [[w] for w in want]
+    with pytest.raises(tcamd.TCError):
+        g.orient_adir_omp(0)
+    g.close()
+
+
 @pytest.mark.parametrize("case", ["er", "rmat12", "ba", "rgg", "messy"])
 def test_kcore_numbers_exact(ctx, case):
     s, d = CASES[case]()
@@ -419,12 +466,13 @@ def test_cost_functions_match_oracle(ctx, case):
 @pytest.mark.parametrize("case", ["er", "rmat12", "messy", "hub", "k6"])
 def test_relabel_tools_match_oracle(ctx, case):
     """tools/serialize.cpp, adjust2tc.cpp, directionByReorder.cpp relabellings
-    (the oracle is pinned against those tools' own builds)."""
+    (the oracle is pinned against those tools' own builds) and Bisson's
+    A-direction relabel (pinned by hand traces)."""
     s, d = CASES[case]()
     s = (s.astype(np.int64) * 3 + 2).astype(np.uint32)   # holes in the id range
     d = (d.astype(np.int64) * 3 + 2).astype(np.uint32)
     for mode, name in ((tcamd.RELABEL_COMPACT, "compact"), (tcamd.RELABEL_FIRST_SRC, "first_src"),
-                       (tcamd.RELABEL_DEGREE, "degree")):
+                       (tcamd.RELABEL_DEGREE, "degree"), (tcamd.RELABEL_ADIR, "adir")):
         g = ctx.graph(s, d)
         perm = g.relabel(mode)
         ref = orc.relabel(s, d, name)
@@ -439,6 +487,23 @@ def test_relabel_tools_match_oracle(ctx, case):
     e = np.zeros(0, np.uint32)
     g = ctx.graph(e, e)
     assert g.relabel(tcamd.RELABEL_DEGREE).tolist() == [0]
+    g.close()
+
+
+def test_bisson_relabel_hand_traces(ctx):
+    """Bisson/A-direction.cpp on the two hand-traced record sets of the
+    oracle's known-answer test (one needs the record slot of the frontier
+    index), and on an R-MAT graph whose cascades run many levels."""
+    for s, d, want in (([4, 3, 4, 2], [4, 1, 2, 3], [4, 3, 0, 1, 2]),
+                       ([0, 0, 0, 1, 1, 2, 4, 5, 7], [1, 2, 3, 2, 3, 3, 5, 6, 7],
+                        [3, 2, 1, 0, 6, 4, 5, 7])):
+        g = ctx.graph(np.array(s, np.uint32), np.array(d, np.uint32))
+        assert g.relabel(tcamd.RELABEL_ADIR).tolist() == want
+        g.close()
+    s, d = rmat(14, 16, 21)
+    ref = orc.relabel(s, d, "adir")
+    g = ctx.graph(s, d)
+    np.testing.assert_array_equal(g.relabel(tcamd.RELABEL_ADIR), ref)
     g.close()
 
 
