@@ -34,7 +34,9 @@ __device__ inline void flush_graph(unsigned long long& acc, uint32_t g,
   acc = 0;
 }
 
-template <int CAP, uint32_t FMIN>
+// WL: tasks stream their work lists (AUTO's default schedule); otherwise the
+// push form, task u streaming N+(v) for v in N+(u) (wrp, wcol unused).
+template <int CAP, uint32_t FMIN, bool WL>
 __global__ __launch_bounds__(TB) void k_batch(const uint32_t* __restrict__ rp,
                                               const uint32_t* __restrict__ col,
                                               const uint32_t* __restrict__ wrp,
@@ -52,8 +54,14 @@ __global__ __launch_bounds__(TB) void k_batch(const uint32_t* __restrict__ rp,
     __syncthreads();
     const uint32_t t = S.t;
     if (t >= nblk) break;
-    const uint32_t u = blk[t], wb = wrp[u];
-    unsigned long long c = block_vertex<CAP, 1, true>(rp, col, u, S, wcol + wb, wrp[u + 1] - wb);
+    const uint32_t u = blk[t];
+    unsigned long long c;
+    if constexpr (WL) {
+      const uint32_t wb = wrp[u];
+      c = block_vertex<CAP, 1, true>(rp, col, u, S, wcol + wb, wrp[u + 1] - wb);
+    } else {
+      c = block_vertex<CAP, 1>(rp, col, u, S);
+    }
     flush_graph(c, gid[u], counts);
   }
   const WaveSlot W = wave_slot(S, wv);
@@ -72,30 +80,37 @@ __global__ __launch_bounds__(TB) void k_batch(const uint32_t* __restrict__ rp,
         if (cur != 0xFFFFFFFFu) flush_graph(acc, cur, counts);
         cur = g;
       }
-      const uint32_t wb = wrp[u];
-      acc += wave_vertex<FMIN, 1, true>(rp, col, u, lane, W, wcol + wb, wrp[u + 1] - wb);
+      if constexpr (WL) {
+        const uint32_t wb = wrp[u];
+        acc += wave_vertex<FMIN, 1, true>(rp, col, u, lane, W, wcol + wb, wrp[u + 1] - wb);
+      } else {
+        acc += wave_vertex<FMIN, 1>(rp, col, u, lane, W);
+      }
     }
   }
   if (cur != 0xFFFFFFFFu) flush_graph(acc, cur, counts);
 }
 
-// tasks (non-empty work list, d+ >= 2) with d+ in [lo, hi]
+// tasks with d+ in [lo, hi] and d+ >= 2 (and, with work lists wrp, a
+// non-empty work list)
 __global__ void k_bflags(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ wrp,
                          uint32_t n, uint32_t lo, uint32_t hi, uint8_t* __restrict__ f) {
   for (uint64_t u = gtid(); u < n; u += gstride()) {
     const uint32_t d = rp[u + 1] - rp[u];
-    f[u] = (wrp[u + 1] > wrp[u] && d >= 2 && d >= lo && d <= hi) ? 1 : 0;
+    f[u] = ((!wrp || wrp[u + 1] > wrp[u]) && d >= 2 && d >= lo && d <= hi) ? 1 : 0;
   }
 }
 
-// descending work (d+(x) + sum of d+(y) over W(x)) as the sort key
-__global__ void k_bwork(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ wrp,
-                        const uint32_t* __restrict__ wcol, const uint32_t* __restrict__ verts,
+// descending work as the sort key: d+(u) plus the entries the task streams
+// (d+(y) over its work list wcol[wrp[u], wrp[u+1]), or over N+(u) in the
+// push form, wrp = col and the rows of rp)
+__global__ void k_bwork(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ lrp,
+                        const uint32_t* __restrict__ lcol, const uint32_t* __restrict__ verts,
                         uint32_t nv, uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
   for (uint64_t i = gtid(); i < nv; i += gstride()) {
     const uint32_t u = verts[i];
     uint64_t w = rp[u + 1] - rp[u];
-    for (uint32_t k = wrp[u]; k < wrp[u + 1]; ++k) w += rp[wcol[k] + 1] - rp[wcol[k]];
+    for (uint32_t k = lrp[u]; k < lrp[u + 1]; ++k) w += rp[lcol[k] + 1] - rp[lcol[k]];
     key[i] = ~w;
     val[i] = u;
   }
@@ -161,15 +176,15 @@ struct tc_batch {
   uint32_t ngraphs = 0;
   std::vector<uint32_t> voff;
   DevBuf d_voff, gid, counts, cursors;
-  // schedule for one wave_max_deg
-  uint32_t sched_deg = 0;
+  // schedule for one (wave_max_deg, form)
+  uint32_t sched_deg = 0, sched_form = 0;
   DevBuf blk_lpt, blk_id, wav;
   uint64_t nblk = 0, nwav = 0;
   DevBuf blk_off, wav_off;  // per-graph ranges of blk_id / wav
   std::vector<uint32_t> h_blk_off, h_wav_off;
   // captured replay
   bool captured = false;
-  uint32_t cap_deg = 0, cap_chunk = 0, cap_grid = 0, cap_fmin = 0;
+  uint32_t cap_deg = 0, cap_chunk = 0, cap_grid = 0, cap_fmin = 0, cap_form = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -196,17 +211,27 @@ tc_batch_sched fill(const tc_batch_sched* s, tc_ctx* c) {
   return o;
 }
 
-// k_batch for the schedule's wave-variant filter threshold (1 or 16)
-template <typename... A>
-void launch_batch(const tc_batch_sched& s, dim3 grid, hipStream_t st, A... args) {
-  if (s.filter_min == 1)
-    hipLaunchKernelGGL((k_batch<2048, 1>), grid, dim3(TB), 0, st, args...);
-  else
-    hipLaunchKernelGGL((k_batch<2048, 16>), grid, dim3(TB), 0, st, args...);
+int check_sched(const tc_batch_sched* s) {
+  if (s && s->form > TC_BATCH_PUSH) return fail(TC_ERR_ARG, "unknown batched form %u", s->form);
+  return TC_OK;
 }
 
-int schedule(tc_ctx* c, tc_batch* b, uint32_t wave_max_deg) {
-  if (b->sched_deg == wave_max_deg) return TC_OK;
+// k_batch for the schedule's form and wave-variant filter threshold (1 or 16)
+template <typename... A>
+void launch_batch(const tc_batch_sched& s, dim3 grid, hipStream_t st, A... args) {
+  const bool wl = s.form == TC_BATCH_WORKLIST;
+  if (s.filter_min == 1 && wl)
+    hipLaunchKernelGGL((k_batch<2048, 1, true>), grid, dim3(TB), 0, st, args...);
+  else if (wl)
+    hipLaunchKernelGGL((k_batch<2048, 16, true>), grid, dim3(TB), 0, st, args...);
+  else if (s.filter_min == 1)
+    hipLaunchKernelGGL((k_batch<2048, 1, false>), grid, dim3(TB), 0, st, args...);
+  else
+    hipLaunchKernelGGL((k_batch<2048, 16, false>), grid, dim3(TB), 0, st, args...);
+}
+
+int schedule(tc_ctx* c, tc_batch* b, uint32_t wave_max_deg, uint32_t form) {
+  if (b->sched_deg == wave_max_deg && b->sched_form == form) return TC_OK;
   tc_graph* g = b->g;
   const uint32_t n = g->n;
   if (g->m_or == 0 || n == 0) {  // no edges: no work lists and no tasks
@@ -214,13 +239,18 @@ int schedule(tc_ctx* c, tc_batch* b, uint32_t wave_max_deg) {
     b->h_blk_off.assign((size_t)b->ngraphs + 1, 0);
     b->h_wav_off.assign((size_t)b->ngraphs + 1, 0);
     b->sched_deg = wave_max_deg;
+    b->sched_form = form;
     release_capture(b);
     return TC_OK;
   }
   const uint32_t* rp = g->rowptr.as<uint32_t>();
-  TCX_TRY(worklists(c, g));
-  const uint32_t* wrp = g->sc.wl_rp.as<uint32_t>();
-  const uint32_t* wcol = g->sc.wl_col.as<uint32_t>();
+  const uint32_t* wrp = nullptr;
+  const uint32_t* wcol = nullptr;
+  if (form == TC_BATCH_WORKLIST) {
+    TCX_TRY(worklists(c, g));
+    wrp = g->sc.wl_rp.as<uint32_t>();
+    wcol = g->sc.wl_col.as<uint32_t>();
+  }
   DevBuf flags, key, ktmp, vtmp;
   TCX_TRY(flags.reserve((size_t)n + 1));
   TCX_TRY(b->wav.reserve(4 * ((size_t)n + 1)));
@@ -240,9 +270,9 @@ int schedule(tc_ctx* c, tc_batch* b, uint32_t wave_max_deg) {
   DevBuf vals;
   TCX_TRY(vals.reserve(4 * (nb + 1)));
   if (nb) {
-    hipLaunchKernelGGL(k_bwork, dim3(grid_for(nb)), dim3(256), 0, c->stream, rp, wrp, wcol,
-                       b->blk_id.as<uint32_t>(), (uint32_t)nb, key.as<uint64_t>(),
-                       vals.as<uint32_t>());
+    hipLaunchKernelGGL(k_bwork, dim3(grid_for(nb)), dim3(256), 0, c->stream, rp, wrp ? wrp : rp,
+                       wcol ? wcol : g->col.as<uint32_t>(), b->blk_id.as<uint32_t>(), (uint32_t)nb,
+                       key.as<uint64_t>(), vals.as<uint32_t>());
     TCX_HIP(hipGetLastError());
     uint64_t* sk;
     uint32_t* sv;
@@ -269,6 +299,7 @@ int schedule(tc_ctx* c, tc_batch* b, uint32_t wave_max_deg) {
                          hipMemcpyDeviceToHost, c->stream));
   TCX_HIP(hipStreamSynchronize(c->stream));
   b->sched_deg = wave_max_deg;
+  b->sched_form = form;
   release_capture(b);
   return TC_OK;
 }
@@ -372,9 +403,10 @@ int tc_batch_count(tc_ctx* c, tc_batch* b, const tc_batch_sched* sched, uint64_t
   tc_timing tm;
   memset(&tm, 0, sizeof(tm));
   TCX_HIP(hipEventRecord(c->ev0, c->stream));
-  TCX_TRY(schedule(c, b, s.wave_max_deg));
+  TCX_TRY(check_sched(sched));
+  TCX_TRY(schedule(c, b, s.wave_max_deg, s.form));
   if (!b->captured || b->cap_chunk != s.wave_chunk || b->cap_grid != s.grid ||
-      b->cap_deg != s.wave_max_deg || b->cap_fmin != s.filter_min) {
+      b->cap_deg != s.wave_max_deg || b->cap_fmin != s.filter_min || b->cap_form != s.form) {
     release_capture(b);
     TCX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int st = enqueue_batched(c, b, s);
@@ -392,6 +424,7 @@ int tc_batch_count(tc_ctx* c, tc_batch* b, const tc_batch_sched* sched, uint64_t
     b->cap_grid = s.grid;
     b->cap_deg = s.wave_max_deg;
     b->cap_fmin = s.filter_min;
+    b->cap_form = s.form;
   }
   TCX_HIP(hipEventRecord(c->ev1, c->stream));
   TCX_HIP(hipEventSynchronize(c->ev1));
@@ -417,7 +450,8 @@ int tc_batch_count_pergraph(tc_ctx* c, tc_batch* b, const tc_batch_sched* sched,
   const tc_batch_sched s = fill(sched, c);
   tc_timing tm;
   memset(&tm, 0, sizeof(tm));
-  TCX_TRY(schedule(c, b, s.wave_max_deg));
+  TCX_TRY(check_sched(sched));
+  TCX_TRY(schedule(c, b, s.wave_max_deg, s.form));
   tc_graph* g = b->g;
   const uint32_t G = b->ngraphs;
   uint32_t* cur = b->cursors.as<uint32_t>();
@@ -467,11 +501,14 @@ int tc_batch_autotune(tc_ctx* c, tc_batch* b, tc_batch_sched* best) {
   tc_batch_sched s = fill(nullptr, c);
   float best_ms = 1e30f;
   tc_batch_sched bs = s;
-  for (uint32_t d : {16u, 32u, 64u, 128u, 256u, 512u}) {
-    s.wave_max_deg = d;
-    float ms;
-    TCX_TRY(time_of(s, &ms));
-    if (ms < best_ms) { best_ms = ms; bs = s; }
+  for (uint32_t form : {(uint32_t)TC_BATCH_WORKLIST, (uint32_t)TC_BATCH_PUSH}) {
+    s.form = form;
+    for (uint32_t d : {16u, 32u, 64u, 128u, 256u, 512u}) {
+      s.wave_max_deg = d;
+      float ms;
+      TCX_TRY(time_of(s, &ms));
+      if (ms < best_ms) { best_ms = ms; bs = s; }
+    }
   }
   s = bs;
   s.filter_min = 1;

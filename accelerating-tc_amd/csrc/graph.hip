@@ -46,10 +46,11 @@ __global__ void k_low_degree_flag(const uint32_t* deg, uint32_t n, uint32_t* fla
 
 // Orient one raw record into a packed (src<<32 | dst) key; self-loops and
 // pruned endpoints become the sentinel (n_out << 32), which sorts last.
+// tie (optional, TC_ORIENT_DEGREE): equal degrees compare tie[] before ids.
 __global__ void k_orient_keys(const uint32_t* __restrict__ s, const uint32_t* __restrict__ d,
                               uint64_t m, const uint32_t* __restrict__ deg, int mode,
                               const uint32_t* __restrict__ pruned_before, uint32_t n_out,
-                              uint64_t* __restrict__ keys) {
+                              const uint32_t* __restrict__ tie, uint64_t* __restrict__ keys) {
   const uint64_t sentinel = (uint64_t)n_out << 32;
   for (uint64_t i = gtid(); i < m; i += gstride()) {
     uint32_t a = s[i], b = d[i];
@@ -58,7 +59,8 @@ __global__ void k_orient_keys(const uint32_t* __restrict__ s, const uint32_t* __
       swap = a < b;  // src = larger id (Fox/preTC.cu:94-98)
     } else if (mode == TC_ORIENT_DEGREE) {
       uint32_t da = deg[a], db = deg[b];
-      swap = da > db || (da == db && a < b);  // Hu/preTC.cu:129
+      if (tie && da == db && tie[a] != tie[b]) swap = tie[a] > tie[b];
+      else swap = da > db || (da == db && a < b);  // Hu/preTC.cu:129
     } else {
       uint32_t da = deg[a], db = deg[b];
       swap = da > db || (da == db && a > b);  // tricount_gpu.cu:76
@@ -365,18 +367,64 @@ __global__ void k_core_expand(const uint32_t* __restrict__ frontier, uint32_t fs
 
 // keep[e] for a sorted symmetric key list: ADIR (mode 2), or KCORE (mode 3)
 // and DEGENERACY (mode 5), whose keys both point up from the lower (key, id).
+// tie (optional, A-direction): equal priorities compare tie[] before ids.
 __global__ void k_sym_keep(const uint64_t* __restrict__ keys, uint64_t m2,
-                           const int32_t* __restrict__ key, int mode, uint8_t* keep) {
+                           const int32_t* __restrict__ key, int mode,
+                           const uint32_t* __restrict__ tie, uint8_t* keep) {
   for (uint64_t e = gtid(); e < m2; e += gstride()) {
     uint64_t kk = keys[e];
     uint32_t i = (uint32_t)(kk >> 32), j = (uint32_t)kk;
     int32_t pi = key[i], pj = key[j];
     bool k;
-    if (mode == TC_ORIENT_ADIR || mode == TC_ORIENT_ADIR_OMP)
+    if (tie && pi == pj && tie[i] != tie[j])
+      k = tie[i] < tie[j];
+    else if (mode == TC_ORIENT_ADIR || mode == TC_ORIENT_ADIR_OMP)
       k = pj > pi || (pj == pi && i < j);  // A-direction-seri.cpp:259-261, A-direction.cpp:266-268
     else k = pi < pj || (pi == pj && i < j);                         // kcore_direction.cpp:137-138
     keep[e] = k ? 1 : 0;
   }
+}
+
+// Tie keys of learned orientation ties: a vertex's forced out-degree (the
+// neighbours strictly above it in the primary order: higher A-direction
+// priority, or higher degree), ascending or (desc) descending.
+__global__ void k_forced_sym(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+                             uint32_t n, const int32_t* __restrict__ key, int desc,
+                             uint32_t* __restrict__ tie) {
+  for (uint64_t v = gtid(); v < n; v += gstride()) {
+    uint32_t h = 0;
+    for (uint32_t e = rp[v]; e < rp[v + 1]; ++e) h += key[col[e]] > key[v];
+    tie[v] = desc ? ~h : h;
+  }
+}
+
+// over raw records (every record counts, as the degree does)
+__global__ void k_forced_raw(const uint32_t* __restrict__ s, const uint32_t* __restrict__ d,
+                             uint64_t m, const uint32_t* __restrict__ deg, uint32_t* h) {
+  for (uint64_t i = gtid(); i < m; i += gstride()) {
+    const uint32_t a = s[i], b = d[i];
+    if (a == b) continue;
+    if (deg[b] > deg[a]) atomicAdd(h + a, 1u);
+    else if (deg[a] > deg[b]) atomicAdd(h + b, 1u);
+  }
+}
+
+__global__ void k_tie_flip(uint32_t* __restrict__ h, uint32_t n) {
+  for (uint64_t v = gtid(); v < n; v += gstride()) h[v] = ~h[v];
+}
+
+// A-direction cost function sum_v |d+(v) - avg| (verification.cpp:43-45's
+// f_lp over the oriented CSR): per-block partial sums over a fixed grid, so
+// the host's ordered sum of the partials is deterministic.
+__global__ __launch_bounds__(256) void k_adir_cost(const uint32_t* __restrict__ rp, uint32_t n,
+                                                   double avg, double* __restrict__ partial) {
+  __shared__ double red[4];
+  double acc = 0;
+  for (uint64_t v = gtid(); v < n; v += gstride()) acc += fabs((double)(rp[v + 1] - rp[v]) - avg);
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
 // A-order helpers: oriented out-degree over raw records (A-order.cpp:88-99)
@@ -1064,7 +1112,8 @@ int tc_graph_relabel(tc_ctx* c, tc_graph* g, int mode, uint32_t* perm_out) {
   return TC_OK;
 }
 
-static int orient_peeled(tc_ctx* c, tc_graph* g, int mode, double thre_param);
+static int orient_peeled(tc_ctx* c, tc_graph* g, int mode, double thre_param, int tie);
+static int orient_impl(tc_ctx* c, tc_graph* g, int mode, uint32_t flags, int tie);
 
 int tc_graph_orient(tc_ctx* c, tc_graph* g, int mode, uint32_t flags) {
   TCX_TRY(check_records(g));
@@ -1073,6 +1122,42 @@ int tc_graph_orient(tc_ctx* c, tc_graph* g, int mode, uint32_t flags) {
     return fail(TC_ERR_ARG, "unknown orientation mode %d", mode);
   if ((flags & TC_FLAG_PRUNE_DEG1) && mode != TC_ORIENT_DEGREE)
     return fail(TC_ERR_ARG, "degree<=1 pruning applies to TC_ORIENT_DEGREE only");
+  return orient_impl(c, g, mode, flags, TC_TIE_ID);
+}
+
+int tc_graph_orient_ties(tc_ctx* c, tc_graph* g, int mode, int tie) {
+  TCX_TRY(check_records(g));
+  if (c != g->ctx) return fail(TC_ERR_ARG, "graph belongs to another context");
+  if (mode != TC_ORIENT_ADIR && mode != TC_ORIENT_DEGREE)
+    return fail(TC_ERR_ARG, "tie rules apply to TC_ORIENT_ADIR and TC_ORIENT_DEGREE");
+  if (tie < TC_TIE_ID || tie > TC_TIE_FORCED_DESC) return fail(TC_ERR_ARG, "unknown tie rule %d", tie);
+  return orient_impl(c, g, mode, 0, tie);
+}
+
+int tc_graph_adir_cost(tc_ctx* c, tc_graph* g, double* cost) {
+  if (!c || !g || !cost) return fail(TC_ERR_ARG, "NULL argument");
+  if (c != g->ctx) return fail(TC_ERR_ARG, "graph belongs to another context");
+  if (g->orient_mode < 0) return fail(TC_ERR_STATE, "graph must be oriented");
+  TCX_HIP(hipSetDevice(c->device));
+  *cost = 0;
+  if (g->n == 0) return TC_OK;
+  constexpr unsigned kGrid = 1024;
+  DevBuf part;
+  TCX_TRY(part.reserve(8 * kGrid));
+  const double avg = (double)g->m_or / (double)g->n;
+  hipLaunchKernelGGL(k_adir_cost, dim3(kGrid), dim3(256), 0, c->stream, g->rowptr.as<uint32_t>(),
+                     g->n, avg, part.as<double>());
+  TCX_HIP(hipGetLastError());
+  std::vector<double> h(kGrid);
+  TCX_HIP(hipMemcpyAsync(h.data(), part.p, 8 * kGrid, hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  double sum = 0;
+  for (double x : h) sum += x;
+  *cost = sum;
+  return TC_OK;
+}
+
+static int orient_impl(tc_ctx* c, tc_graph* g, int mode, uint32_t flags, int tie) {
   // Every pass over the raw records uses their id range; only a pruned CSR
   // has fewer rows.
   const uint32_t n = g->n_raw;
@@ -1089,6 +1174,19 @@ int tc_graph_orient(tc_ctx* c, tc_graph* g, int mode, uint32_t flags) {
     TCX_HIP(hipGetLastError());
     uint32_t n_out = n;
     const uint32_t* pruned_before = nullptr;
+    DevBuf tkey;
+    if (tie != TC_TIE_ID) {
+      TCX_TRY(tkey.reserve(4 * ((size_t)n + 1)));
+      hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n + 1)), dim3(TB), 0, c->stream,
+                         tkey.as<uint32_t>(), 0u, (uint64_t)n + 1);
+      hipLaunchKernelGGL(k_forced_raw, dim3(grid_for(m)), dim3(TB), 0, c->stream,
+                         g->src.as<uint32_t>(), g->dst.as<uint32_t>(), m, deg.as<uint32_t>(),
+                         tkey.as<uint32_t>());
+      if (tie == TC_TIE_FORCED_DESC)
+        hipLaunchKernelGGL(k_tie_flip, dim3(grid_for(n)), dim3(TB), 0, c->stream,
+                           tkey.as<uint32_t>(), n);
+      TCX_HIP(hipGetLastError());
+    }
     if (flags & TC_FLAG_PRUNE_DEG1) {
       DevBuf flag;
       TCX_TRY(flag.reserve(4 * ((size_t)n + 1)));
@@ -1108,18 +1206,19 @@ int tc_graph_orient(tc_ctx* c, tc_graph* g, int mode, uint32_t flags) {
     }
     hipLaunchKernelGGL(k_orient_keys, dim3(grid_for(m)), dim3(TB), 0, c->stream,
                        g->src.as<uint32_t>(), g->dst.as<uint32_t>(), m, deg.as<uint32_t>(), mode,
-                       pruned_before, n_out, keys.as<uint64_t>());
+                       pruned_before, n_out, tie != TC_TIE_ID ? tkey.as<uint32_t>() : nullptr,
+                       keys.as<uint64_t>());
     TCX_HIP(hipGetLastError());
     TCX_TRY(csr_from_keys(c, g, keys, tmp, m, n_out, false));
     g->orient_mode = mode;
     return TC_OK;
   }
-  return orient_peeled(c, g, mode, 0);
+  return orient_peeled(c, g, mode, 0, tie);
 }
 
 // A-direction / k-core: peel the simple symmetric graph, then keep the
 // symmetric entries that point up the peel order.
-static int orient_peeled(tc_ctx* c, tc_graph* g, int mode, double thre_param) {
+static int orient_peeled(tc_ctx* c, tc_graph* g, int mode, double thre_param, int tie) {
   const uint32_t n = g->n_raw;
   DevBuf keys, tmp, srp, scol, keep;
   uint64_t* skeys = nullptr;
@@ -1140,9 +1239,18 @@ static int orient_peeled(tc_ctx* c, tc_graph* g, int mode, double thre_param) {
                          g->vkey.as<int32_t>()));
     }
   }
+  DevBuf tkey;
+  if (tie != TC_TIE_ID && n > 0) {
+    TCX_TRY(tkey.reserve(4 * ((size_t)n + 1)));
+    hipLaunchKernelGGL(k_forced_sym, dim3(grid_for(n)), dim3(TB), 0, c->stream,
+                       srp.as<uint32_t>(), scol.as<uint32_t>(), n, g->vkey.as<int32_t>(),
+                       tie == TC_TIE_FORCED_DESC ? 1 : 0, tkey.as<uint32_t>());
+    TCX_HIP(hipGetLastError());
+  }
   TCX_TRY(keep.reserve(std::max<uint64_t>(m2, 1)));
   hipLaunchKernelGGL(k_sym_keep, dim3(grid_for(m2)), dim3(TB), 0, c->stream, skeys, m2,
-                     g->vkey.as<int32_t>(), mode, keep.as<uint8_t>());
+                     g->vkey.as<int32_t>(), mode, tkey.p ? tkey.as<uint32_t>() : nullptr,
+                     keep.as<uint8_t>());
   TCX_HIP(hipGetLastError());
   uint64_t* dst_keys = skeys == keys.as<uint64_t>() ? tmp.as<uint64_t>() : keys.as<uint64_t>();
   uint64_t mo = 0;
@@ -1160,7 +1268,7 @@ int tc_graph_orient_adir_omp(tc_ctx* c, tc_graph* g, double thre_param) {
   if (!(thre_param > 0) || !std::isfinite(thre_param))
     return fail(TC_ERR_ARG, "thre_param must be finite and positive (the threshold never grows "
                             "otherwise)");
-  return orient_peeled(c, g, TC_ORIENT_ADIR_OMP, thre_param);
+  return orient_peeled(c, g, TC_ORIENT_ADIR_OMP, thre_param, TC_TIE_ID);
 }
 
 // Stage file of an oriented CSR: "TCAMDCSR", uint32 version 1, uint32 n,

@@ -376,6 +376,11 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
   // the rows that the lanes read at once over the LDS banks
   constexpr uint32_t RS = W >= 8 ? W + 2 : W;
   __shared__ uint64_t s_M[D * RS];
+  // hashed bitmap of N+(u) (32 bits per possible member): an out-list
+  // entry that misses it is not in N+(u) and skips the binary search
+  constexpr uint32_t kFiltWords = 64 * W;
+  constexpr uint32_t kFiltShift = 32u - (uint32_t)__builtin_ctz(2048u * (uint32_t)W);
+  __shared__ uint32_t s_filt[kFiltWords];
   __shared__ uint64_t s_Cs[NW][W];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   uint64_t* s_C = s_Cs[wv];
@@ -406,11 +411,21 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
       const uint32_t base = rp[u], d = rp[u + 1] - base;
       for (uint32_t i = tid; i < d; i += 64 * NW) s_list[i] = col[base + i];
       for (uint32_t i = tid; i < d * RS; i += 64 * NW) s_M[i] = 0;
+      for (uint32_t i = tid; i < kFiltWords; i += 64 * NW) s_filt[i] = 0;
       if (tid == 0) s_next = NW;
+      __syncthreads();
+      for (uint32_t i = tid; i < d; i += 64 * NW) {
+        const uint32_t h = (s_list[i] * 0x9E3779B1u) >> kFiltShift;
+        atomicOr(&s_filt[h >> 5], 1u << (h & 31));
+      }
       __syncthreads();
       // symmetric adjacency of N+(u): bit (i, j) for every arc c_i -> c_j.
       // A wave takes four out-lists per step: when none is longer than 64,
       // each gets 16 lanes; otherwise the whole wave streams them in turn.
+      // Entries that hit the hashed bitmap of N+(u) are queued, one per
+      // lane, in registers (qi, qx: lane l holds slot l of qn); a full queue
+      // resolves 64 binary searches at once, so the searches run on full
+      // waves although only a few entries of each step pass.
       auto add_arcs = [&](uint32_t i, uint32_t x) {
         uint32_t lo = 0, len = d;
         while (len > 1) {
@@ -423,6 +438,45 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
           atomicOr((unsigned long long*)&s_M[lo * RS + (i >> 6)], 1ull << (i & 63));
         }
       };
+      uint32_t qi = 0, qx = 0, qn = 0;
+      // wave-uniform: every lane calls it; active lanes offer (i, x)
+      auto offer = [&](bool active, uint32_t i, uint32_t x) {
+        bool pass = false;
+        if (active) {
+          const uint32_t h = (x * 0x9E3779B1u) >> kFiltShift;
+          pass = (s_filt[h >> 5] >> (h & 31)) & 1u;
+        }
+        const uint64_t mask = __ballot(pass);
+        const uint32_t cnt = __popcll(mask);
+        if (cnt == 0) return;
+        // lane of the r-th set bit of mask: the largest lo with
+        // popcount(mask below lo) <= r
+        auto nth = [&](uint32_t r) -> uint32_t {
+          uint32_t lo = 0;
+          for (uint32_t step = 32; step; step >>= 1)
+            if ((uint32_t)__popcll(mask & ((1ull << (lo + step)) - 1)) <= r) lo += step;
+          return lo;
+        };
+        const uint32_t take = min(cnt, 64u - qn);
+        uint32_t src = nth((lane - qn) & 63u);
+        uint32_t vi = __shfl(i, src, 64), vx = __shfl(x, src, 64);
+        if (lane >= qn && lane < qn + take) {
+          qi = vi;
+          qx = vx;
+        }
+        qn += take;
+        if (qn == 64) {
+          add_arcs(qi, qx);
+          qn = cnt - take;
+          src = nth((take + lane) & 63u);
+          vi = __shfl(i, src, 64);
+          vx = __shfl(x, src, 64);
+          if (lane < qn) {
+            qi = vi;
+            qx = vx;
+          }
+        }
+      };
       for (uint32_t i0 = wv * 4u; i0 < d; i0 += NW * 4u) {
         const uint32_t gi = i0 + (lane >> 4);
         uint32_t eb = 0, ee = 0;
@@ -432,14 +486,24 @@ __global__ __launch_bounds__(64 * NW) void k_kcl_pivot(const uint32_t* __restric
           ee = rp[ci + 1];
         }
         if (__ballot(ee - eb > 64u) == 0) {
-          for (uint32_t e = eb + (lane & 15u); e < ee; e += 16u) add_arcs(gi, col[e]);
+          for (uint32_t k0 = 0;; k0 += 16) {
+            const uint32_t e = eb + k0 + (lane & 15u);
+            const bool act = e < ee;
+            if (__ballot(act) == 0) break;
+            offer(act, gi, act ? col[e] : 0u);
+          }
         } else {
           for (uint32_t g = 0; g < 4u && i0 + g < d; ++g) {
             const uint32_t b = __shfl(eb, 16 * g, 64), e1 = __shfl(ee, 16 * g, 64);
-            for (uint32_t e = b + lane; e < e1; e += 64) add_arcs(i0 + g, col[e]);
+            for (uint32_t e0 = b; e0 < e1; e0 += 64) {
+              const uint32_t e = e0 + lane;
+              const bool act = e < e1;
+              offer(act, i0 + g, act ? col[e] : 0u);
+            }
           }
         }
       }
+      if (lane < qn) add_arcs(qi, qx);
       __syncthreads();
 
       // Node (s_C, h, p): counted in closed form (push = false / no pivot

@@ -371,18 +371,28 @@ __global__ void k_task_flags(const uint32_t* __restrict__ rp, const uint32_t* __
 // wide one (a wave for d+ <= 8, a workgroup above); tasks above d+ 512 are
 // block tasks.  Whatever its cell runs, a task keeps the fixed rule's limits:
 // a lane holds at most kLaneMaxList streamed lists and a wave at most
-// kWlWaveWork entries of work, else the task runs one size up.
+// kWlWaveWork entries of work, else the task runs one size up.  Cell 15
+// holds the tasks above d+ 512: it has no choice, but its features and its
+// timed launch let the model predict a whole count (learned orientation
+// ties compare candidate orientations by it).
 constexpr int kCells = TC_CELLS;
+constexpr int kChoiceCells = 15;  // cells 2..14 have a narrow variant
+constexpr int kBlockCell = 15;
 constexpr int kAcc = 8;  // per-cell accumulators of k_cell_accum
 // The fixed rule (AUTO's work-list thresholds 64 / 8): narrow up to d+ 64.
 constexpr uint32_t kFixedNarrow = 0xFFCu;  // cells 2..11
-static_assert(kCells == 15, "cell layout");
+static_assert(kCells == 16, "cell layout");
 
 __device__ __host__ inline int task_cell(uint32_t d) {
   if (d < 2 || d > kWaveMax) return -1;
   if (d <= kLaneMaxDeg) return (int)d;
   const int b = 32 - __builtin_clz(d - 1);  // ceil(log2 d): 4 .. 9
   return (int)kLaneMaxDeg + 1 + (b - 4);
+}
+
+// The cell a task's features and timings go to (cells 2..15).
+__device__ __host__ inline int feature_cell(uint32_t d) {
+  return d > kWaveMax ? kBlockCell : task_cell(d);
 }
 
 // TC_VARIANT_* of a task (d >= 2, nw >= 1 lists, work = d + sum d+(y)).
@@ -420,7 +430,7 @@ __global__ void k_cell_accum(const uint32_t* __restrict__ rp, const uint32_t* __
   for (uint64_t x = gtid(); x < n; x += gstride()) {
     const uint32_t d = rp[x + 1] - rp[x];
     const uint32_t wb = wrp[x], we = wrp[x + 1];
-    const int cell = task_cell(d);
+    const int cell = feature_cell(d);
     if (we == wb || cell < 0 || !in_level((uint32_t)x, level)) continue;
     uint64_t ent = 0;
     uint32_t shortc = 0;
@@ -434,7 +444,8 @@ __global__ void k_cell_accum(const uint32_t* __restrict__ rp, const uint32_t* __
     }
     const uint32_t nw = we - wb;
     const uint64_t work = d + ent;
-    const bool ok = d <= kLaneMaxDeg ? nw <= kLaneMaxList : work <= kWlWaveWork;
+    const bool ok = d <= kLaneMaxDeg ? nw <= kLaneMaxList
+                                     : (d <= kWaveMax && work <= kWlWaveWork);
     unsigned long long* a = sa + cell * kAcc;
     atomicAdd(a + 0, 1ull);
     atomicAdd(a + 1, (unsigned long long)nw);
@@ -466,7 +477,7 @@ __global__ void k_task_var(const uint32_t* __restrict__ rp, const uint32_t* __re
     const uint32_t wb = wrp[x], we = wrp[x + 1];
     uint8_t v = 255;
     uint64_t work = 0;
-    if (we > wb && d >= 2 && (only < 0 || task_cell(d) == only) &&
+    if (we > wb && d >= 2 && (only < 0 || feature_cell(d) == only) &&
         in_level((uint32_t)x, level)) {
       work = d;
       for (uint32_t i = wb; i < we; ++i) {
@@ -730,8 +741,14 @@ int task_var(tc_ctx* c, tc_graph* g, uint32_t narrow_cells, int only, uint32_t l
 // The model's cells: predictions Y (kCells x 2, log2 us of the cell's
 // launch, wide / narrow) and the narrow-cell mask.  The fixed rule is the
 // prior: a cell leaves it only for a predicted saving above kSwitchMargin.
+// total_us (optional): the predicted launch time summed over the non-empty
+// cells under the chosen variants, cell 15 included.
 constexpr float kSwitchMargin = 0.10f;
-int learned_cells(tc_ctx* c, tc_graph* g, tc_model* md, uint32_t* mask, float* Y);
+// A tie rule other than the reference's is taken for a predicted saving
+// above this fraction of the count.
+constexpr double kTieMargin = 0.02;
+int learned_cells(tc_ctx* c, tc_graph* g, tc_model* md, uint32_t* mask, float* Y,
+                  double* total_us = nullptr);
 
 }  // namespace
 
@@ -750,7 +767,8 @@ int model_predict(tc_ctx* c, tc_model* md, const float* dX, uint32_t n, float* d
 
 namespace {
 
-int learned_cells(tc_ctx* c, tc_graph* g, tc_model* md, uint32_t* mask, float* Y) {
+int learned_cells(tc_ctx* c, tc_graph* g, tc_model* md, uint32_t* mask, float* Y,
+                  double* total_us) {
   unsigned long long acc[kCells * kAcc];
   TCX_TRY(cell_accum(c, g, md->bw, 0, acc));
   float X[kCells * F];
@@ -762,15 +780,20 @@ int learned_cells(tc_ctx* c, tc_graph* g, tc_model* md, uint32_t* mask, float* Y
   TCX_HIP(hipMemcpyAsync(Y, dY.p, 4 * kCells * O, hipMemcpyDeviceToHost, c->stream));
   TCX_HIP(hipStreamSynchronize(c->stream));
   uint32_t m = kFixedNarrow;
-  for (int cell = 2; cell < kCells; ++cell) {
+  double total = 0;
+  for (int cell = 2; cell < kChoiceCells; ++cell) {
     if (acc[cell * kAcc] == 0) continue;
     const bool prior = (kFixedNarrow >> cell) & 1u;
     const double wide = std::exp2((double)Y[cell * O + 0]);
     const double narrow = std::exp2((double)Y[cell * O + 1]);
     const double keep = prior ? narrow : wide, other = prior ? wide : narrow;
-    if (other < (1.0 - kSwitchMargin) * keep) m ^= 1u << cell;
+    const bool flip = other < (1.0 - kSwitchMargin) * keep;
+    if (flip) m ^= 1u << cell;
+    total += flip ? other : keep;
   }
+  if (acc[kBlockCell * kAcc]) total += std::exp2((double)Y[kBlockCell * O + 0]);
   *mask = m;
+  if (total_us) *total_us = total;
   return TC_OK;
 }
 
@@ -1160,7 +1183,7 @@ int tc_graph_measure_cells(tc_ctx* c, tc_graph* g, int narrow, uint32_t level, u
   unsigned long long* dacc = c->accum.as<unsigned long long>();
   DevBuf var, key, ids;
   for (int cell = 2; cell < kCells; ++cell) {
-    if (acc[cell * kAcc] == 0) continue;
+    if (acc[cell * kAcc] == 0 || (narrow && cell >= kChoiceCells)) continue;
     TCX_TRY(task_var(c, g, narrow ? (1u << cell) : 0u, cell, level, var, key, ids));
     TaskLists L;
     TCX_TRY(lists_from_var(c, g, var, key, ids, L));
@@ -1195,6 +1218,44 @@ int tc_learned_cells(tc_ctx* c, tc_graph* g, tc_model* md, uint32_t* narrow_cell
   *narrow_cells = kFixedNarrow;
   if (md && g->n && g->m_or) TCX_TRY(learned_cells(c, g, md, narrow_cells, y));
   if (Y) memcpy(Y, y, sizeof(y));
+  return TC_OK;
+}
+
+int tc_graph_orient_learned(tc_ctx* c, tc_graph* g, int mode, tc_model* md, int* chosen,
+                            double* predicted_us, double* adir_cost) {
+  if (!c || !g) return fail(TC_ERR_ARG, "NULL argument");
+  if (g->ctx != c) return fail(TC_ERR_ARG, "graph belongs to another context");
+  if (md && md->ctx != c) return fail(TC_ERR_ARG, "model belongs to another context");
+  if (mode != TC_ORIENT_ADIR && mode != TC_ORIENT_DEGREE)
+    return fail(TC_ERR_ARG, "learned ties apply to TC_ORIENT_ADIR and TC_ORIENT_DEGREE");
+  double pred[TC_TIE_CANDIDATES] = {}, cost[TC_TIE_CANDIDATES] = {};
+  for (int t = 0; t < TC_TIE_CANDIDATES; ++t) {
+    TCX_TRY(tc_graph_orient_ties(c, g, mode, t));
+    TCX_TRY(tc_graph_adir_cost(c, g, &cost[t]));
+    if (md && g->n && g->m_or) {
+      uint32_t mask = 0;
+      float Y[kCells * O];
+      TCX_TRY(worklists(c, g));
+      TCX_TRY(learned_cells(c, g, md, &mask, Y, &pred[t]));
+    }
+  }
+  // the model's prediction picks, with the reference's rule as the prior;
+  // without a model the A-direction cost function does
+  int best = 0;
+  for (int t = 1; t < TC_TIE_CANDIDATES; ++t) {
+    if (md) {
+      if (pred[t] < pred[best]) best = t;
+    } else if (cost[t] < cost[best]) {
+      best = t;
+    }
+  }
+  if (md && !(pred[best] < (1.0 - kTieMargin) * pred[0])) best = 0;
+  if (best != TC_TIE_CANDIDATES - 1) TCX_TRY(tc_graph_orient_ties(c, g, mode, best));
+  if (chosen) *chosen = best;
+  for (int t = 0; t < TC_TIE_CANDIDATES; ++t) {
+    if (predicted_us) predicted_us[t] = pred[t];
+    if (adir_cost) adir_cost[t] = cost[t];
+  }
   return TC_OK;
 }
 

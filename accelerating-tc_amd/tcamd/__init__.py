@@ -40,11 +40,14 @@ ERR_ARG, ERR_IO, ERR_FORMAT, ERR_RANGE, ERR_NOMEM, ERR_HIP, ERR_UNSUPPORTED, ERR
 ORIENT_ID, ORIENT_DEGREE, ORIENT_ADIR, ORIENT_KCORE, ORIENT_DEGREE_TRICORE = 0, 1, 2, 3, 4
 ORIENT_DEGENERACY = 5
 ORIENT_ADIR_OMP = 6  # set by Graph.orient_adir_omp
+TIE_ID, TIE_FORCED_ASC, TIE_FORCED_DESC = 0, 1, 2
+TIE_CANDIDATES = 3
 FLAG_PRUNE_DEG1 = 1
 KERNEL_HU, KERNEL_BISSON, KERNEL_FOX, KERNEL_TRICORE, KERNEL_AUTO = 0, 1, 2, 3, 4
 KERNEL_NAMES = {"hu": KERNEL_HU, "bisson": KERNEL_BISSON, "fox": KERNEL_FOX,
                 "tricore": KERNEL_TRICORE, "auto": KERNEL_AUTO}
 AUTO_WORKLIST, AUTO_PUSH = 0, 1
+BATCH_WORKLIST, BATCH_PUSH = 0, 1
 KCL_AUTO, KCL_BITMAP, KCL_MERGE, KCL_PIVOT = 0, 1, 2, 3
 RELABEL_COMPACT, RELABEL_FIRST_SRC, RELABEL_DEGREE, RELABEL_ADIR = 0, 1, 2, 3
 
@@ -136,6 +139,8 @@ _sig("tc_measure_list_bandwidth", C.c_int, [_vp, _vp, C.c_uint32, _vp])
 _sig("tc_bw_model_fit", C.c_int, [_vp, _vp, C.c_uint32, C.POINTER(BwModel)])
 _sig("tc_graph_orient", C.c_int, [_vp, _vp, C.c_int, C.c_uint32])
 _sig("tc_graph_orient_adir_omp", C.c_int, [_vp, _vp, C.c_double])
+_sig("tc_graph_orient_ties", C.c_int, [_vp, _vp, C.c_int, C.c_int])
+_sig("tc_graph_adir_cost", C.c_int, [_vp, _vp, C.POINTER(C.c_double)])
 _sig("tc_graph_relabel", C.c_int, [_vp, _vp, C.c_int, _vp])
 _sig("tc_graph_info_get", C.c_int, [_vp, C.POINTER(GraphInfo)])
 _sig("tc_graph_get_csr", C.c_int, [_vp, _vp, _vp])
@@ -184,6 +189,8 @@ _sig("tc_graph_cell_features", C.c_int, [_vp, _vp, _vp, C.c_uint32, _f32p])
 _sig("tc_graph_measure_cells", C.c_int, [_vp, _vp, C.c_int, C.c_uint32, C.c_uint32, _f32p,
                                          C.c_void_p])
 _sig("tc_learned_cells", C.c_int, [_vp, _vp, _vp, C.POINTER(C.c_uint32), C.c_void_p])
+_sig("tc_graph_orient_learned", C.c_int, [_vp, _vp, C.c_int, _vp, C.POINTER(C.c_int),
+                                          C.c_void_p, C.c_void_p])
 _sig("tc_learned_assignment", C.c_int, [_vp, _vp, _vp, _u8p])
 _sig("tc_count_learned", C.c_int, [_vp, _vp, _vp, C.POINTER(C.c_uint64), C.POINTER(Timing)])
 _sig("kcl_count", C.c_int, [_vp, _vp, C.c_uint32, C.c_int, C.POINTER(C.c_uint64),
@@ -208,11 +215,11 @@ _sig("tc_tricore_tricount_gpu", C.c_uint64, [C.c_uint64, _vp, _u64p, _u32p,
 
 class BatchSched(C.Structure):
     _fields_ = [("wave_max_deg", C.c_uint32), ("wave_chunk", C.c_uint32), ("grid", C.c_uint32),
-                ("filter_min", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+                ("filter_min", C.c_uint32), ("form", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
     def as_dict(self):
         return {"wave_max_deg": self.wave_max_deg, "wave_chunk": self.wave_chunk,
-                "grid": self.grid, "filter_min": self.filter_min}
+                "grid": self.grid, "filter_min": self.filter_min, "form": self.form}
 
 
 _sig("tc_batch_create", C.c_int, [_vp, C.c_uint32, _u64p, _vp, _vp, _u32p, C.POINTER(_vp)])
@@ -240,7 +247,8 @@ EXPORTED = [
     "tc_tricore_split", "tc_tricore_tricount_gpu", "tc_model_num_params", "tc_model_create",
     "tc_model_destroy", "tc_model_set_params", "tc_model_get_params", "tc_model_set_norm",
     "tc_model_get_norm", "tc_model_forward", "tc_model_loss_grad", "tc_model_train",
-    "tc_graph_orient_adir_omp",
+    "tc_graph_orient_adir_omp", "tc_graph_orient_ties", "tc_graph_adir_cost",
+    "tc_graph_orient_learned",
     "tc_graph_features", "tc_model_set_bw_model", "tc_graph_measure_costs",
     "tc_graph_cell_features", "tc_graph_measure_cells", "tc_learned_cells",
     "tc_learned_assignment", "tc_count_learned",
@@ -248,7 +256,8 @@ EXPORTED = [
     "tc_batch_count_pergraph", "tc_batch_stream_bytes", "tc_count_csr_noalloc", "tc_graph_save_csr", "tc_graph_load_csr",
 ]
 VARIANT_BLOCK, VARIANT_WAVE, VARIANT_LANE = 0, 1, 2
-CELLS, CELL_MAX_LEVEL = 15, 16
+CELLS, CELL_MAX_LEVEL = 16, 16
+BLOCK_CELL = 15  # tasks above d+ 512: timed and predicted wide only
 FIXED_NARROW = 0xFFC  # the fixed rule runs cells 2..11 (d+ <= 64) narrow
 N_FEATURES, N_OUTPUTS = 8, 2
 
@@ -569,6 +578,31 @@ class Graph:
                "tc_graph_orient_adir_omp")
         return self
 
+    def orient_ties(self, mode: int, tie: int):
+        """Orientation with a tie rule TIE_* (tc_graph_orient_ties)."""
+        _check(_lib.tc_graph_orient_ties(self.ctx.handle, self._h, mode, tie),
+               "tc_graph_orient_ties")
+        return self
+
+    def adir_cost(self) -> float:
+        """sum |d+(v) - m+/n| of the current orientation."""
+        out = C.c_double()
+        _check(_lib.tc_graph_adir_cost(self.ctx.handle, self._h, C.byref(out)),
+               "tc_graph_adir_cost")
+        return out.value
+
+    def orient_learned(self, mode: int, model=None):
+        """Learned orientation ties: (chosen TIE_*, predicted us per rule,
+        A-direction cost per rule); the graph keeps the chosen rule."""
+        ch = C.c_int()
+        pred = np.zeros(TIE_CANDIDATES, np.float64)
+        cost = np.zeros(TIE_CANDIDATES, np.float64)
+        _check(_lib.tc_graph_orient_learned(self.ctx.handle, self._h, mode,
+                                            model.handle if model else None, C.byref(ch),
+                                            pred.ctypes.data, cost.ctypes.data),
+               "tc_graph_orient_learned")
+        return ch.value, pred, cost
+
     def info(self) -> GraphInfo:
         i = GraphInfo()
         _check(_lib.tc_graph_info_get(self._h, C.byref(i)), "tc_graph_info_get")
@@ -837,8 +871,13 @@ class Batch:
         _check(_lib.tc_batch_info_get(self._h, C.byref(i)), "tc_batch_info_get")
         return i
 
-    def stream_bytes(self) -> float:
-        """Bytes one batched count streams (tc_batch_stream_bytes)."""
+    def stream_bytes(self, form: int = BATCH_WORKLIST) -> float:
+        """Bytes one batched count streams: the work-list bytes
+        (tc_batch_stream_bytes), or for the push form 4(n+1) + 12 m+ +
+        4 sum d- d+."""
+        if form == BATCH_PUSH:
+            i = self.info()
+            return 4.0 * (i.n + 1) + 12.0 * i.m_oriented + 4.0 * i.sum_in_out
         out = C.c_double()
         _check(_lib.tc_batch_stream_bytes(self.ctx.handle, self._h, C.byref(out)),
                "tc_batch_stream_bytes")
@@ -867,7 +906,7 @@ class Batch:
 
 
 class Model:
-    """The AUTO engine's per-task cost model (8 -> 32 -> 32 -> 2 MLP)."""
+    """The AUTO engine's cost model (8 -> 32 -> 32 -> 2 MLP)."""
     n_params = int(_lib.tc_model_num_params())
 
     def __init__(self, ctx: Context, seed: int = 0):

@@ -89,22 +89,26 @@ def feature_model(ctx, bw=None):
 
 def cell_samples(g, fm, levels=LEVELS, reps=3):
     """Per (level, non-empty cell): features X, targets T = log2 us of
-    [wide, narrow] launches, mask M, and the group id of the level."""
-    Xs, Ts, grp = [], [], []
+    [wide, narrow] launches, mask M (the block cell has no narrow variant),
+    and (level, cell)."""
+    Xs, Ts, Ms, grp = [], [], [], []
     for lv in range(levels):
         X = g.cell_features(fm, lv)
         tw, cw = g.measure_cells(False, lv, reps)
         tn, cn = g.measure_cells(True, lv, reps)
-        if not np.array_equal(cw, cn):
+        if not np.array_equal(cw[:tcamd.BLOCK_CELL], cn[:tcamd.BLOCK_CELL]):
             raise RuntimeError(f"level {lv}: the two variants count {cw} and {cn}")
         for c in range(2, tcamd.CELLS):
-            if X[c, 0] > 0 and tw[c] > 0 and tn[c] > 0:
+            block = c == tcamd.BLOCK_CELL
+            if X[c, 0] > 0 and tw[c] > 0 and (block or tn[c] > 0):
                 Xs.append(X[c])
-                Ts.append([np.log2(tw[c]), np.log2(tn[c])])
+                Ts.append([np.log2(tw[c]), 0.0 if block else np.log2(tn[c])])
+                Ms.append([1.0, 0.0 if block else 1.0])
                 grp.append((lv, c))
     X = np.array(Xs, np.float32).reshape(-1, tcamd.N_FEATURES)
     T = np.array(Ts, np.float32).reshape(-1, 2)
-    return X, T, np.ones_like(T), grp
+    M = np.array(Ms, np.float32).reshape(-1, 2)
+    return X, T, M, grp
 
 
 def collect(ctx, graphs=TRAIN_GRAPHS, levels=LEVELS, log=print, bw=None):
@@ -167,9 +171,11 @@ def evaluate(model, X, T, M, G=None):
             "mae_analytic": float(np.abs(base - T[m, j]).mean()),
             "samples": int(m.sum()),
         }
-    if G is not None and len(G):
-        cells = G[:, 2]
-        t = np.exp2(T.astype(np.float64))
+    choice = G[:, 2] < tcamd.BLOCK_CELL if G is not None and len(G) else None
+    if choice is not None and choice.any():
+        cells = G[choice, 2]
+        t = np.exp2(T[choice].astype(np.float64))
+        Y = Y[choice]
         fixed = (tcamd.FIXED_NARROW >> cells) & 1 == 1
         pick = choose(Y, cells)
         out["choice_us"] = {

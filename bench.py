@@ -233,7 +233,9 @@ def run_batched(args, w, steps, warmup, with_cpu):
     t0 = time.perf_counter()
     sched = b.autotune()
     t_tune = time.perf_counter() - t0
-    log(f"[bench] autotune {sched.as_dict()} in {t_tune:.1f}s")
+    stream = b.stream_bytes(sched.form)  # the bytes of the chosen form
+    log(f"[bench] autotune {sched.as_dict()} in {t_tune:.1f}s; the chosen form streams "
+        f"{stream/1e9:.2f} GB")
 
     def timed(fn, nsteps, nwarm):
         ref, _ = fn(sched)
@@ -275,7 +277,7 @@ def batched_desc(w):
 def main_batched(args, w):
     r = run_batched(args, w, args.steps, args.warmup, not args.no_cpu)
     info, m = r["info"], r["m"]
-    kb = r["stream"]  # k_batch streams the work lists of the union CSR
+    kb = r["stream"]  # the bytes k_batch streams under the chosen form
     achieved = kb / (r["ev_b"] * 1e-3) / 1e9
     line = {
         "metric": METRIC,
@@ -294,8 +296,9 @@ def main_batched(args, w):
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS, "traffic": load_traffic("batched", "batched"),
             "kernel": "k_batch<2048>", "bytes_per_count": kb,
-            "bytes_rule": "tc_batch_stream_bytes: the work-list bytes of the union CSR (per task "
-                          "20 + 4 d+(x), per work-list entry 12 + 4 d+(y))",
+            "bytes_rule": "work-list form: tc_batch_stream_bytes (per task 20 + 4 d+(x), per "
+                          "work-list entry 12 + 4 d+(y)); push form: 4(n+1) + 12 m+ + "
+                          "4 sum d- d+",
             "bytes_push_form": kernel_bytes(info),
             "bytes_alg_edge_model": info.bytes_alg,
         },
@@ -373,24 +376,29 @@ def main_kclique(args, w):
     per_k = {}
     for k in ks:
         t0 = time.perf_counter()
-        ref = kcount(k).count  # also the warm-up
+        r = kcount(k)  # the reference count, also the warm-up
+        ref = r.count
         log(f"[bench] k={k}: reference count {ref} in {time.perf_counter() - t0:.2f} s")
-        ms = []
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            r = kcount(k)
-            if r.count != ref:
-                raise SystemExit(f"k={k}: count changed between steps")
-            ms.append(r.timing["ms_count"])
-        wall = (time.perf_counter() - t0) / args.steps
+        if args.steps == 0:  # a count too long to repeat: the reference run is the timed one
+            wall = time.perf_counter() - t0
+            ms = [r.timing["ms_count"]]
+        else:
+            ms = []
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                r = kcount(k)
+                if r.count != ref:
+                    raise SystemExit(f"k={k}: count changed between steps")
+                ms.append(r.timing["ms_count"])
+            wall = (time.perf_counter() - t0) / args.steps
         per_k[k] = dict(count=r.count, ms_wall=wall * 1e3, ms_kernels=float(np.mean(ms)),
                         edges_per_s=m / wall)
         log(f"[bench] k={k}: {r.count} cliques, {wall*1e3:.2f} ms ({m / wall:.3e} edges/s)")
-        if kmodel is not None:
+        if kmodel is not None and args.steps > 0:
             rl = g.kclique_learned(k, kmodel)  # builds and keeps the plan
             prep = rl.timing["ms_prep"]
             lm = []
-            for _ in range(args.steps):
+            for _ in range(max(args.steps, 1)):
                 rl = g.kclique_learned(k, kmodel)
                 if rl.count != ref:
                     raise SystemExit(f"k={k}: learned plan count {rl.count} != {ref}")
@@ -439,7 +447,7 @@ def main_kclique(args, w):
                    "max_out_degree": info.max_out_degree,
                    "triangles": tri, "engine": "k-clique pivot", "root_chunks": chunks,
                    "parallelism": "single GPU"},
-        "roofline": None,
+        "roofline": kclique_roofline(args.workload, per_k),
         "cpu_baseline": cpu,
         "kclique": {str(k): v for k, v in per_k.items()},
         "end_to_end_s": {"generate": t_gen, "preprocess": t_prep},
@@ -447,6 +455,32 @@ def main_kclique(args, w):
     print(json.dumps(line), flush=True)
     g.close()
     ctx.close()
+
+
+# VALU issue peak of MI355X: 1,024 SIMDs at 2.4 GHz, a wave64 instruction
+# every 4 cycles on a 16-lane SIMD
+VALU_PEAK_WAVE_INSTS = 1024 * 2.4e9 / 4
+
+
+def kclique_roofline(workload, per_k):
+    """The pivot kernels are bound by VALU issue, not HBM: the wave
+    instructions a count issues (SQ_INSTS_VALU summed over the launch's
+    kernels, profiles/valu_<workload>.json) over its hipEvent time, against
+    the issue peak.  The largest k with a profile is reported."""
+    prof = os.path.join(ROOT, "profiles", f"valu_{workload}.json")
+    if not os.path.exists(prof):
+        return None
+    with open(prof) as f:
+        insts = json.load(f)["valu_insts"]
+    ks = [k for k in per_k if str(k) in insts]
+    if not ks:
+        return None
+    k = max(ks)
+    achieved = insts[str(k)] / (per_k[k]["ms_kernels"] * 1e-3)
+    return {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_WAVE_INSTS,
+            "unit": "wave-instructions/s", "frac": achieved / VALU_PEAK_WAVE_INSTS,
+            "traffic": None, "kernel": "k_kcl_pivot (all widths)", "k": k,
+            "valu_insts_per_count": insts[str(k)]}
 
 
 def run_triangles(args, w, kernels, with_cpu):

@@ -269,6 +269,27 @@ int tc_graph_orient(tc_ctx* ctx, tc_graph* g, int mode, uint32_t flags);
  * is the one-thread order of its loops with a frontier of exactly the
  * vertices peeled at that step.  thre_param must be positive. */
 int tc_graph_orient_adir_omp(tc_ctx* ctx, tc_graph* g, double thre_param);
+/* Orientation tie rules (no counterpart in the reference, which breaks
+ * every tie by id).  For TC_ORIENT_ADIR (ties: equal priority) and
+ * TC_ORIENT_DEGREE (ties: equal raw degree) a tie key t[v] is compared
+ * before the id:
+ *   TC_TIE_ID           no key: the reference's rule, = tc_graph_orient;
+ *   TC_TIE_FORCED_ASC   t = forced out-degree h(v), the neighbours strictly
+ *                       above v in the primary order (ADIR: over the simple
+ *                       graph; DEGREE: over the raw records), so vertices
+ *                       with few forced out-edges take the tied edges;
+ *   TC_TIE_FORCED_DESC  t = ~h(v), the reverse.
+ * Every rule is a strict total order, so the orientation is acyclic and the
+ * count is the same. */
+#define TC_TIE_ID 0
+#define TC_TIE_FORCED_ASC 1
+#define TC_TIE_FORCED_DESC 2
+#define TC_TIE_CANDIDATES 3
+int tc_graph_orient_ties(tc_ctx* ctx, tc_graph* g, int mode, int tie);
+/* The A-direction cost function of the current orientation: sum over the
+ * CSR's rows of |d+(v) - m+/n| (tools/verification.cpp:43-45's f_lp, in
+ * double; deterministic). */
+int tc_graph_adir_cost(tc_ctx* ctx, tc_graph* g, double* cost);
 int tc_graph_info_get(const tc_graph* g, tc_graph_info* info);
 /* Copies of device state (host arrays of the documented sizes). */
 int tc_graph_get_csr(const tc_graph* g, uint32_t* rowptr /*n+1*/, uint32_t* col /*m+*/);
@@ -367,6 +388,9 @@ int tc_count_csr_noalloc(tc_ctx* ctx, const uint32_t* rowptr, const uint32_t* co
  * run as workgroups.  The model predicts, from 8 features of a cell's tasks,
  * the log2 microseconds of the cell's count launch under each variant; it is
  * trained on launches the engine times on itself (tc_graph_measure_cells).
+ * Cell 15 (the tasks above d+ 512, always workgroups) is timed and predicted
+ * wide only, so the model predicts a whole count; learned orientation ties
+ * compare candidate orientations by that prediction.
  * The schedule only decides HOW each task runs; every task runs exactly
  * once, so the count is exact whatever the model predicts. */
 #define TC_MODEL_FEATURES 8
@@ -374,7 +398,8 @@ int tc_count_csr_noalloc(tc_ctx* ctx, const uint32_t* rowptr, const uint32_t* co
 #define TC_VARIANT_BLOCK 0   /* a workgroup per task                          */
 #define TC_VARIANT_WAVE 1    /* a wave per task, d+ <= 512                     */
 #define TC_VARIANT_LANE 2    /* a lane per task, d+ <= 8 (N+(x) in registers)  */
-#define TC_CELLS 15          /* cell arrays are indexed by cell, 0 and 1 unused  */
+#define TC_CELLS 16          /* cell arrays are indexed by cell, 0 and 1 unused;
+                                cell 15 holds the tasks above d+ 512 (no choice) */
 #define TC_CELL_MAX_LEVEL 16 /* sampling levels of the cell calls               */
 
 typedef struct tc_model tc_model;
@@ -440,7 +465,7 @@ int tc_graph_measure_costs(tc_ctx* ctx, tc_graph* g, int variant, float* cost_us
 int tc_graph_cell_features(tc_ctx* ctx, tc_graph* g, const tc_model* m, uint32_t level,
                            float* X);
 /* Times each non-empty cell's count launch alone, over the tasks of
- * `level`, with the cell run narrow (narrow != 0) or wide: us[cell] = the
+ * `level`, with the cell run narrow (narrow != 0; not cell 15) or wide: us[cell] = the
  * fastest of `reps` (1..64) timed launches; counts[cell] (optional) = the
  * triangles those tasks count (the same in every repetition, else
  * TC_ERR_STATE).  Entries of empty cells are 0. */
@@ -453,6 +478,17 @@ int tc_graph_measure_cells(tc_ctx* ctx, tc_graph* g, int narrow, uint32_t level,
  * when the other variant is predicted more than 10 % faster.  With m NULL,
  * the fixed rule's mask and Y = 0. */
 int tc_learned_cells(tc_ctx* ctx, tc_graph* g, tc_model* m, uint32_t* narrow_cells, float* Y);
+/* Learned orientation ties (TC_ORIENT_ADIR or TC_ORIENT_DEGREE): orients g
+ * under every tie rule TC_TIE_* in turn and scores each: predicted_us[t] =
+ * the model's predicted count time (tc_learned_cells' total over all cells)
+ * and adir_cost[t] = the A-direction cost function (tc_graph_adir_cost).
+ * With a model the lowest prediction wins, but a rule other than
+ * TC_TIE_ID only for a predicted saving above 2 %; with m NULL the lowest
+ * A-direction cost wins (the fallback).  g is left oriented under the
+ * chosen rule (*chosen).  Both score arrays (optional) have
+ * TC_TIE_CANDIDATES entries; predicted_us is 0 without a model. */
+int tc_graph_orient_learned(tc_ctx* ctx, tc_graph* g, int mode, tc_model* m, int* chosen,
+                            double* predicted_us, double* adir_cost);
 /* Variant chosen for each vertex's task (TC_VARIANT_*, 255 = no task): the
  * cell's choice within the task's limits (above).  model NULL gives the
  * fixed rule's variants (tc_count's AUTO work-list schedule). */
@@ -477,8 +513,13 @@ typedef struct tc_batch_sched {
   uint32_t grid;          /* persistent workgroups (0 = 8 per CU)                  */
   uint32_t filter_min;    /* wave tasks hash N+(u) into a bitmap filter above this
                              d+: 1 or 16 (0 = 16)                                 */
-  uint32_t reserved[4];
+  uint32_t form;          /* TC_BATCH_WORKLIST (0): tasks stream AUTO's work
+                             lists; TC_BATCH_PUSH (1): task u streams N+(v) for
+                             v in N+(u)                                           */
+  uint32_t reserved[3];
 } tc_batch_sched;
+#define TC_BATCH_WORKLIST 0
+#define TC_BATCH_PUSH 1
 
 /* Graph g holds records [edge_off[g], edge_off[g+1]) of src/dst with local
  * ids < nverts[g]; edge_off has ngraphs + 1 ascending entries, edge_off[0] = 0.
@@ -489,11 +530,12 @@ int tc_batch_create(tc_ctx* ctx, uint32_t ngraphs, const uint64_t* edge_off,
 int tc_batch_destroy(tc_batch* b);
 /* Sizes of the batch's union CSR (n, m+, algorithmic bytes of one count). */
 int tc_batch_info_get(const tc_batch* b, tc_graph_info* info);
-/* Bytes one batched count streams: the work-list bytes of the union CSR
- * (tc_auto_stream_bytes), whatever the schedule's task split. */
+/* Bytes one batched count streams in the work-list form: the work-list
+ * bytes of the union CSR (tc_auto_stream_bytes), whatever the task split.
+ * The push form streams 4(n+1) + 12 m+ + 4 sum d- d+ (tc_batch_info_get). */
 int tc_batch_stream_bytes(tc_ctx* ctx, tc_batch* b, double* bytes);
-/* Tries wave_max_deg, then filter_min, then wave_chunk candidates on the
- * batch and returns the fastest (timed with hipEvents). */
+/* Tries both forms x wave_max_deg, then filter_min, then wave_chunk
+ * candidates on the batch and returns the fastest (timed with hipEvents). */
 int tc_batch_autotune(tc_ctx* ctx, tc_batch* b, tc_batch_sched* best);
 /* Exact per-graph counts (ngraphs entries) from one replay of the captured
  * hipGraph (capture happens on the first call per schedule). */

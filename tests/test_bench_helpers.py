@@ -91,3 +91,15 @@ def test_kernel_bytes_rule(bench):
     info.n, info.m_oriented = n, m
     info.sum_in_out = int((dp * np.bincount(col, minlength=n)).sum())
     assert bench.kernel_bytes(info) == want
+
+
+def test_kclique_roofline_from_profile(bench, tmp_path, monkeypatch):
+    import json
+    (tmp_path / "profiles").mkdir()
+    (tmp_path / "profiles" / "valu_kx.json").write_text(json.dumps({"valu_insts": {"5": 6.144e10}}))
+    monkeypatch.setattr(bench, "ROOT", str(tmp_path))
+    r = bench.kclique_roofline("kx", {4: {"ms_kernels": 10.0}, 5: {"ms_kernels": 200.0}})
+    assert r["bound"] == "valu" and r["k"] == 5
+    assert r["achieved"] == pytest.approx(6.144e10 / 0.2)
+    assert r["frac"] == pytest.approx(0.5)                 # peak 6.144e11 wave instructions/s
+    assert bench.kclique_roofline("missing", {5: {"ms_kernels": 1.0}}) is None

@@ -53,7 +53,9 @@ def test_replays_interleaved_with_other_work(ctx, small_batch):
 @pytest.mark.parametrize("sched", [
     dict(wave_max_deg=1), dict(wave_max_deg=16, wave_chunk=1), dict(wave_max_deg=512, grid=1),
     dict(wave_max_deg=128, wave_chunk=64, grid=7), dict(wave_max_deg=32, wave_chunk=8),
-    dict(wave_max_deg=64, filter_min=1), dict(wave_max_deg=512, filter_min=1, grid=5)])
+    dict(wave_max_deg=64, filter_min=1), dict(wave_max_deg=512, filter_min=1, grid=5),
+    dict(form=1), dict(form=1, wave_max_deg=1), dict(form=1, wave_max_deg=512, filter_min=1),
+    dict(form=1, wave_max_deg=16, wave_chunk=1, grid=3)])
 def test_every_schedule_is_exact(ctx, small_batch, sched):
     graphs, expect = small_batch
     b = ctx.batch(graphs)
@@ -68,7 +70,7 @@ def test_autotune_picks_a_candidate_and_stays_exact(ctx, small_batch):
     s = b.autotune()
     assert s.wave_max_deg in (16, 32, 64, 128, 256, 512)
     assert s.wave_chunk in (4, 8, 16, 32, 64)
-    assert s.filter_min in (1, 16)
+    assert s.filter_min in (1, 16) and s.form in (tcamd.BATCH_WORKLIST, tcamd.BATCH_PUSH)
     assert s.grid > 0
     assert np.array_equal(b.count(s)[0], expect)
     b.close()
@@ -96,9 +98,12 @@ def test_edge_cases(ctx):
     assert expect.tolist()[:5] == [0, 1, 0, 0, 20]
     b = ctx.batch(graphs)
     for sched in (None, dict(wave_max_deg=1), dict(wave_max_deg=512, grid=3),
-                  dict(wave_max_deg=512, filter_min=1)):
+                  dict(wave_max_deg=512, filter_min=1), dict(form=1),
+                  dict(form=1, wave_max_deg=1)):
         assert np.array_equal(b.count(sched)[0], expect)
         assert np.array_equal(b.count_pergraph(sched)[0], expect)
+    with pytest.raises(tcamd.TCError):
+        b.count(dict(form=2))
     b.close()
     only_empty = ctx.batch([(e, e, 5), (e, e, 1)])
     assert only_empty.count()[0].tolist() == [0, 0]

@@ -160,6 +160,13 @@ def cell_of(d):
     return np.where((d >= 2) & (d <= 8), d, np.where((d > 8) & (d <= 512), 9 + bl - 4, -1))
 
 
+def feature_cell_of(d):
+    """The cell a task's features and timings go to: cell_of, and 15 for the
+    tasks above d+ 512."""
+    d = np.asarray(d, np.int64)
+    return np.where(d > 512, tcamd.BLOCK_CELL, cell_of(d))
+
+
 def in_level(x, level):
     if level == 0:
         return np.ones(len(x), bool)
@@ -208,8 +215,8 @@ def test_cell_features_match_restatement(ctx, make):
     ps = np.floor(tns * 1000.0 + 0.5)
     for level in (0, 1, 3):
         X = np.zeros((tcamd.CELLS, 8))
-        cell = cell_of(dp)
-        ok = np.where(dp <= 8, nw <= 16, dp + ent <= (1 << 16))
+        cell = feature_cell_of(dp)
+        ok = np.where(dp <= 8, nw <= 16, (dp <= 512) & (dp + ent <= (1 << 16)))
         for c in range(2, tcamd.CELLS):
             t = (cell == c) & (nw > 0) & in_level(np.arange(len(dp)), level)
             if not t.any():
@@ -253,14 +260,16 @@ def test_measure_cells_counts_exactly(ctx, make):
     for level in (0, 2):
         us_w, cw = g.measure_cells(False, level, 2)
         us_n, cn = g.measure_cells(True, level, 2)
-        cell = cell_of(dp)
+        cell = feature_cell_of(dp)
         lv = in_level(np.arange(len(dp)), level)
         want = np.array([per[(cell == c) & lv].sum() for c in range(tcamd.CELLS)])
         np.testing.assert_array_equal(cw, want)
-        np.testing.assert_array_equal(cn, want)
+        b = tcamd.BLOCK_CELL                         # no narrow variant: not timed
+        np.testing.assert_array_equal(cn[:b], want[:b])
+        assert cn[b] == 0 and us_n[b] == 0
         X = g.cell_features(None, level)
         nonempty = X[:, 0] > 0
-        assert np.all(us_w[nonempty] > 0) and np.all(us_n[nonempty] > 0)
+        assert np.all(us_w[nonempty] > 0) and np.all(us_n[:b][nonempty[:b]] > 0)
         assert np.all(us_w[~nonempty] == 0) and np.all(us_n[~nonempty] == 0)
     with pytest.raises(tcamd.TCError):
         g.measure_cells(True, 0, 0)
@@ -288,7 +297,7 @@ def test_learned_schedules_are_exact(ctx, make):
         # the choice follows the predictions for the cell features
         Xc = g.cell_features(m, 0)
         np.testing.assert_allclose(Y, m.forward(Xc), rtol=1e-5, atol=1e-5)
-        cells = np.arange(2, tcamd.CELLS)
+        cells = np.arange(2, tcamd.BLOCK_CELL)
         pick = learned.choose(Y[cells], cells)
         used = Xc[cells, 0] > 0
         got = (mask >> cells) & 1 == 1
@@ -345,4 +354,83 @@ def test_trained_schedule_against_fixed_rule(ctx):
     mask, _ = g.learned_cells(model)
     assert mask != tcamd.FIXED_NARROW
     assert ml <= 0.97 * mf
+    g.close()
+
+
+def ties_restated(s, d, mode, tie):
+    """Oriented edges (sorted) under a tie rule: ADIR orders by (priority,
+    t, id), DEGREE by (raw degree, t, -id); t = forced out-degree h (the
+    neighbours strictly above in the primary order), ascending or, as the
+    uint32 complement, descending."""
+    s = s.astype(np.int64)
+    d = d.astype(np.int64)
+    if mode == tcamd.ORIENT_ADIR:
+        n = orc.num_nodes(s.astype(np.uint32), d.astype(np.uint32))
+        srp, scol = orc.build_sym(s.astype(np.uint32), d.astype(np.uint32), n)
+        key = orc.adir_priority(n, srp, scol).astype(np.int64)
+        i = np.repeat(np.arange(n), np.diff(srp.astype(np.int64)))
+        j = scol.astype(np.int64)
+        h = np.bincount(i, weights=key[j] > key[i], minlength=n).astype(np.int64)
+        t = h if tie == tcamd.TIE_FORCED_ASC else (0xFFFFFFFF - h if tie else np.zeros(n, np.int64))
+        keep = (key[i] < key[j]) | ((key[i] == key[j]) & ((t[i] < t[j]) | ((t[i] == t[j]) & (i < j))))
+        es, ed = i[keep], j[keep]
+    else:
+        n = int(max(s.max(), d.max())) + 1
+        deg = np.bincount(s, minlength=n) + np.bincount(d, minlength=n)
+        loop = s == d
+        h = (np.bincount(s[~loop], weights=deg[d[~loop]] > deg[s[~loop]], minlength=n)
+             + np.bincount(d[~loop], weights=deg[s[~loop]] > deg[d[~loop]], minlength=n))
+        h = h.astype(np.int64)
+        t = h if tie == tcamd.TIE_FORCED_ASC else (0xFFFFFFFF - h if tie else np.zeros(n, np.int64))
+        a, b = s[~loop], d[~loop]
+        a_first = (deg[a] < deg[b]) | ((deg[a] == deg[b]) & ((t[a] < t[b]) | ((t[a] == t[b]) & (a > b))))
+        es, ed = np.where(a_first, a, b), np.where(a_first, b, a)
+        u = np.unique(es * (1 << 32) + ed)
+        es, ed = u >> 32, u & 0xFFFFFFFF
+    o = np.lexsort((ed, es))
+    return n, es[o], ed[o]
+
+
+def csr_edges(g):
+    rp, col = g.csr()
+    return np.repeat(np.arange(len(rp) - 1), np.diff(rp.astype(np.int64))), col.astype(np.int64)
+
+
+@pytest.mark.parametrize("make", [er, rmat, hub])
+@pytest.mark.parametrize("mode", [tcamd.ORIENT_ADIR, tcamd.ORIENT_DEGREE])
+def test_orientation_tie_rules(ctx, make, mode):
+    s, d = make()
+    expect = orc.pipeline_count(s, d, "degree")[0]
+    g = ctx.graph(s, d)
+    costs = []
+    for tie in (tcamd.TIE_ID, tcamd.TIE_FORCED_ASC, tcamd.TIE_FORCED_DESC):
+        g.orient_ties(mode, tie)
+        n, es, ed = ties_restated(s, d, mode, tie)
+        gs, gd = csr_edges(g)
+        np.testing.assert_array_equal(gs, es)
+        np.testing.assert_array_equal(gd, ed)
+        assert g.count("auto").count == expect
+        dp = np.bincount(es, minlength=g.info().n)
+        cost = np.abs(dp - len(es) / g.info().n).sum()
+        assert g.adir_cost() == pytest.approx(cost, rel=1e-12)
+        costs.append(cost)
+    # TIE_ID is tc_graph_orient itself
+    g.orient(mode)
+    np.testing.assert_array_equal(csr_edges(g)[1], ties_restated(s, d, mode, tcamd.TIE_ID)[2])
+    with pytest.raises(tcamd.TCError):
+        g.orient_ties(tcamd.ORIENT_ID, tcamd.TIE_FORCED_ASC)
+    # learned ties without a model: the lowest A-direction cost, first on ties
+    chosen, pred, cost = g.orient_learned(mode, None)
+    np.testing.assert_allclose(cost, costs, rtol=1e-12)
+    assert chosen == int(np.argmin(costs)) and not pred.any()
+    np.testing.assert_array_equal(csr_edges(g)[1], ties_restated(s, d, mode, chosen)[2])
+    # with a model: its predictions pick, the reference's rule kept unless
+    # another is predicted more than 2 % faster
+    m = tcamd.Model(ctx, 3)
+    chosen, pred, cost = g.orient_learned(mode, m)
+    best = int(np.argmin(pred))
+    want = best if pred[best] < 0.98 * pred[0] else 0
+    assert chosen == want and np.all(pred > 0)
+    np.testing.assert_array_equal(csr_edges(g)[1], ties_restated(s, d, mode, chosen)[2])
+    assert g.count_learned(m).count == expect
     g.close()
