@@ -293,23 +293,64 @@ __device__ inline uint32_t stream_list(const uint32_t* __restrict__ col, uint32_
   return c;
 }
 
+// The long lists of a wave's 64 lists (the lanes set in `mask`; lane j
+// holds col[vb, ve)) streamed back to back by the whole wave: 4 * U chunks
+// of 64 entries in flight, taken in list order across list boundaries, so
+// that lists of a few chunks each (R-MAT's work lists average about 270
+// entries) still keep 4 * U chunks of loads in flight per wave.
+template <int U>
+__device__ inline uint32_t stream_many(const uint32_t* __restrict__ col, uint64_t mask,
+                                       uint32_t vb, uint32_t ve, uint32_t lane, Probe& P) {
+  uint32_t c = 0;
+  uint32_t pos = 0, end = 0;  // wave-uniform: the current list's next chunk and its end
+  auto next_list = [&]() {
+    const uint32_t j = (uint32_t)__builtin_ctzll(mask);
+    mask &= mask - 1;
+    pos = (uint32_t)__builtin_amdgcn_readlane((int)vb, (int)j);
+    end = (uint32_t)__builtin_amdgcn_readlane((int)ve, (int)j);
+  };
+  if (mask) next_list();
+  while (pos < end) {
+    uint32_t w[4 * U];
+#pragma unroll
+    for (int s = 0; s < 4 * U; ++s) {
+      const uint32_t k = pos + lane;
+      w[s] = (pos < end && k < end) ? col[k] : kEmpty;
+      pos += 64;
+      if (pos >= end && mask) next_list();
+    }
+#pragma unroll
+    for (int h = 0; h < U; ++h) {
+      push4(P, w[4 * h], w[4 * h + 1], w[4 * h + 2], w[4 * h + 3]);
+      c += drain(P, lane, false);
+    }
+  }
+  return c;
+}
+
 // Each lane holds one list col[vb, ve) (vb == ve for none).  Lists of
-// kFlatMin entries or more are streamed one at a time by the whole wave; the
-// shorter ones are concatenated and walked 64 entries per step (two steps in
-// flight), each lane finding the list of its entry by a search over the
-// exclusive prefix of the lengths, kept with the list starts in the wave's
-// LDS slot F[128].
-template <int U = 2>
+// kFlatMin entries or more are streamed by the whole wave (with MANY, back to
+// back through stream_many; else one at a time); the shorter ones are
+// concatenated and walked 64 entries per step (two steps in flight), each
+// lane finding the list of its entry by a search over the exclusive prefix
+// of the lengths, kept with the list starts in the wave's LDS slot F[128].
+// MANY is used by the work-list kernels only: the per-vertex and plain AUTO
+// kernels keep the one-list-at-a-time stream.
+template <int U = 2, bool MANY = false>
 __device__ inline uint32_t lists64(const uint32_t* __restrict__ col, uint32_t vb, uint32_t ve,
                                    uint32_t lane, uint32_t* F, Probe& P) {
   const uint32_t len = ve - vb;
   const bool lng = len >= kFlatMin;
   uint32_t c = 0;
   uint64_t mask = __ballot(lng);
-  while (mask) {
-    const uint32_t j = (uint32_t)__builtin_ctzll(mask);
-    mask &= mask - 1;
-    c += stream_list<U>(col, __shfl(vb, j, 64), __shfl(ve, j, 64), lane, P);
+  if constexpr (MANY && U >= 2) {
+    if (mask) c += stream_many<U>(col, mask, vb, ve, lane, P);
+  } else {
+    while (mask) {
+      const uint32_t j = (uint32_t)__builtin_ctzll(mask);
+      mask &= mask - 1;
+      c += stream_list<U>(col, __shfl(vb, j, 64), __shfl(ve, j, 64), lane, P);
+    }
   }
   const uint32_t sl = lng ? 0u : len;
   const uint32_t incl = wave_incl_scan(sl, lane);
@@ -376,7 +417,7 @@ __device__ inline uint32_t wave_vertex(const uint32_t* __restrict__ rp,
       mvb = rp[v];
       mve = rp[v + 1];
     }
-    c += lists64<U>(col, mvb, mve, lane, W.F, P);
+    c += lists64<U, WL>(col, mvb, mve, lane, W.F, P);
   }
   c += drain(P, lane, true);
   if (filt)
@@ -445,7 +486,7 @@ __device__ inline uint32_t block_vertex(const uint32_t* __restrict__ rp,
       vb = rp[v];
       ve = rp[v + 1];
     }
-    c += lists64<U>(col, vb, ve, lane, S.f[wv], P);
+    c += lists64<U, WL>(col, vb, ve, lane, S.f[wv], P);
   }
   c += drain(P, lane, true);
   __syncthreads();
@@ -536,15 +577,17 @@ __device__ inline uint32_t lane_task(const uint32_t* __restrict__ rp,
 // triangle count; edges into a vertex with no out-edge are left out.
 // Phase 3 counts the lane tasks (`lan`), kLaneClaim per wave claim from
 // cursor[2].
+// Eight waves per SIMD (the persistent grid's eight workgroups per CU): the
+// register budget is held at 64 VGPRs so the occupancy survives the
+// unrolled streams.
 template <int CAP, bool STATIC_WAVES = false>
-__global__ __launch_bounds__(TB) void k_auto_wl(const uint32_t* __restrict__ rp,
-                                                const uint32_t* __restrict__ col,
-                                                const uint32_t* __restrict__ wrp,
-                                                const uint32_t* __restrict__ wcol,
-                                                const uint32_t* __restrict__ blk, uint32_t nblk,
-                                                const uint32_t* __restrict__ wav, uint32_t nwav,
-                                                const uint32_t* __restrict__ lan, uint32_t nlan,
-                                                uint32_t* cursor, unsigned long long* total) {
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_auto_wl(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+               const uint32_t* __restrict__ wrp, const uint32_t* __restrict__ wcol,
+               const uint32_t* __restrict__ blk, uint32_t nblk,
+               const uint32_t* __restrict__ wav, uint32_t nwav,
+               const uint32_t* __restrict__ lan, uint32_t nlan,
+               uint32_t* cursor, unsigned long long* total) {
   __shared__ BlockLds<CAP> S;
   __shared__ unsigned long long red[TB / 64];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
