@@ -136,8 +136,11 @@ __device__ inline bool contains(const uint32_t* a, uint32_t lo, uint32_t hi, uin
 }
 
 // Branchless membership test in an ascending LDS list a[0, n): the loop
-// count depends on n only, so the lanes of a wave never diverge.
+// count depends on n only, so the lanes of a wave never diverge.  n must be
+// wave-uniform: it is read from the first lane, so the loop's counters live
+// in scalar registers and a step costs one add, one compare and one select.
 __device__ inline uint32_t lds_has(const uint32_t* a, uint32_t n, uint32_t x) {
+  n = (uint32_t)__builtin_amdgcn_readfirstlane((int)n);
   const uint32_t* p = a;
   uint32_t len = n;
   while (len > 1) {
@@ -163,11 +166,21 @@ __device__ inline uint32_t wave_incl_scan(uint32_t x, uint32_t lane) {
   return x;
 }
 
-// Number of set bits of m below this lane.
-__device__ inline uint32_t lanes_below(uint64_t m) {
+// Number of set bits of m below this lane, plus q.
+__device__ inline uint32_t lanes_below(uint64_t m, uint32_t q = 0u) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, q));
 }
+
+// A wave-uniform value held in a scalar register.
+__device__ inline uint32_t uniform(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+// The lanes where b holds.  A predicate that is already a lane mask stays
+// one (HIP's int-valued ballot turns it into a value and back: two vector
+// instructions).
+__device__ inline uint64_t lanes(bool b) { return __builtin_amdgcn_ballot_w64(b); }
 
 // Largest j in [0, 64) with F[j] <= f, for ascending F with F[0] <= f
 // (branchless, six LDS reads).
@@ -210,16 +223,25 @@ __device__ inline uint32_t probe_search(const Probe& P, uint32_t x) {
   return (x != kEmpty && contains(P.a, 0, P.n, x)) ? 1u : 0u;
 }
 
-// Appends x (kEmpty: nothing) to the queue when it passes the filter.
+// Bitmap bit of x (any x: the word index is masked to the bitmap).  The bit
+// extract reads only the low five bits of its offset, which the mask keeps,
+// so it takes the hash before the mask.
+__device__ inline bool bm_hit(const Probe& P, uint32_t x) {
+  const uint32_t f = x ^ (x >> 15);
+  const uint32_t w = P.bm[(f & (0xFFFFFFFFu >> P.shift)) >> 5];
+  return __builtin_amdgcn_ubfe(w, f, 1u) != 0u;
+}
+
+// Appends x (kEmpty: nothing) to the queue when it passes the filter.  The
+// bitmap is read whether or not the filter is on (where it is off the probe
+// decides nothing), so the predicate is a plain lane mask.  The queue count
+// stays wave-uniform (a sum of ballot counts) and is kept in a scalar
+// register.
 __device__ inline void push(Probe& P, uint32_t x) {
-  bool pass = x != kEmpty;
-  if (P.filt && pass) {
-    const uint32_t h = fhash(x, P.shift);
-    pass = (P.bm[h >> 5] >> (h & 31)) & 1u;
-  }
-  const uint64_t m = __ballot(pass);
-  if (pass) P.Q[P.qn + lanes_below(m)] = x;
-  P.qn += (uint32_t)__popcll(m);
+  const bool pass = (x != kEmpty) & (bm_hit(P, x) | !P.filt);
+  const uint64_t m = lanes(pass);
+  if (pass) P.Q[lanes_below(m, P.qn)] = x;
+  P.qn = uniform(P.qn + (uint32_t)__popcll(m));
 }
 
 // push() of four entries, with the four bitmap probes issued before any
@@ -227,26 +249,22 @@ __device__ inline void push(Probe& P, uint32_t x) {
 // lists; short-list kernels keep the one-entry form, which needs fewer
 // registers.
 __device__ inline void push4(Probe& P, uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3) {
-  bool p0 = x0 != kEmpty, p1 = x1 != kEmpty, p2 = x2 != kEmpty, p3 = x3 != kEmpty;
-  if (P.filt) {
-    const uint32_t h0 = fhash(x0, P.shift), h1 = fhash(x1, P.shift);
-    const uint32_t h2 = fhash(x2, P.shift), h3 = fhash(x3, P.shift);
-    const uint32_t b0 = P.bm[h0 >> 5], b1 = P.bm[h1 >> 5], b2 = P.bm[h2 >> 5], b3 = P.bm[h3 >> 5];
-    p0 = p0 && ((b0 >> (h0 & 31)) & 1u);
-    p1 = p1 && ((b1 >> (h1 & 31)) & 1u);
-    p2 = p2 && ((b2 >> (h2 & 31)) & 1u);
-    p3 = p3 && ((b3 >> (h3 & 31)) & 1u);
-  }
-  const uint64_t m0 = __ballot(p0), m1 = __ballot(p1), m2 = __ballot(p2), m3 = __ballot(p3);
+  // the four probes are read whatever the entries and the filter (no short
+  // circuit), so the reads overlap and the predicates combine as lane masks
+  const bool nf = !P.filt;
+  const bool h0 = bm_hit(P, x0), h1 = bm_hit(P, x1), h2 = bm_hit(P, x2), h3 = bm_hit(P, x3);
+  const bool p0 = (x0 != kEmpty) & (h0 | nf), p1 = (x1 != kEmpty) & (h1 | nf);
+  const bool p2 = (x2 != kEmpty) & (h2 | nf), p3 = (x3 != kEmpty) & (h3 | nf);
+  const uint64_t m0 = lanes(p0), m1 = lanes(p1), m2 = lanes(p2), m3 = lanes(p3);
   uint32_t q = P.qn;
-  if (p0) P.Q[q + lanes_below(m0)] = x0;
+  if (p0) P.Q[lanes_below(m0, q)] = x0;
   q += (uint32_t)__popcll(m0);
-  if (p1) P.Q[q + lanes_below(m1)] = x1;
+  if (p1) P.Q[lanes_below(m1, q)] = x1;
   q += (uint32_t)__popcll(m1);
-  if (p2) P.Q[q + lanes_below(m2)] = x2;
+  if (p2) P.Q[lanes_below(m2, q)] = x2;
   q += (uint32_t)__popcll(m2);
-  if (p3) P.Q[q + lanes_below(m3)] = x3;
-  P.qn = q + (uint32_t)__popcll(m3);
+  if (p3) P.Q[lanes_below(m3, q)] = x3;
+  P.qn = uniform(q + (uint32_t)__popcll(m3));
 }
 
 // Resolves queued survivors 64 at a time; `all` also resolves a partial tail.
@@ -342,7 +360,7 @@ __device__ inline uint32_t lists64(const uint32_t* __restrict__ col, uint32_t vb
   const uint32_t len = ve - vb;
   const bool lng = len >= kFlatMin;
   uint32_t c = 0;
-  uint64_t mask = __ballot(lng);
+  uint64_t mask = lanes(lng);
   if constexpr (MANY && U >= 2) {
     if (mask) c += stream_many<U>(col, mask, vb, ve, lane, P);
   } else {
