@@ -1,6 +1,7 @@
 // List-read bandwidth of this GPU, for A-order's bandwidth model
 // (vertex_ordering/A-order/A-order.cpp:20-36 ships numbers measured on the
-// GPU the reference targets).  One wave reads one list of `len` uint32
+// GPU the reference targets), and the vector-ALU issue rate that the
+// issue-bound kernels are measured against.  One wave reads one list of `len` uint32
 // entries starting at a random 4-byte offset of a 1 GiB array (as the
 // counting kernels read N+(v) rows of a CSR far larger than the caches); the
 // whole GPU is kept busy; the rate is list entries read per second.
@@ -45,6 +46,32 @@ __global__ __launch_bounds__(256) void k_list_read(const uint32_t* __restrict__ 
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
   if (lane == 0) out[wave] = acc;
+}
+
+// Issue-rate probe of the vector ALU: every lane runs `iters` rounds of
+// eight independent chains of 32-bit integer operations (shift, xor, add,
+// and/or: the mix of the counting kernels), so nothing waits on memory or on
+// a result; the xor of the chains goes to out so that none is dropped.  The
+// grid fills every SIMD with eight waves.  The instructions issued are read
+// from the SQ_INSTS_VALU counter of a profiled run.
+__global__ __launch_bounds__(256) void k_valu_probe(uint32_t iters, uint32_t seed,
+                                                    uint32_t* __restrict__ out) {
+  uint32_t x[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) x[i] = seed + threadIdx.x * 8u + (uint32_t)i;
+  for (uint32_t r = 0; r < iters; ++r) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      x[i] ^= x[i] >> 13;
+      x[i] += 0x9E3779B9u;
+      x[i] ^= x[i] << 7;
+      x[i] = (x[i] & 0x0F0F0F0Fu) | (x[i] >> 4);
+    }
+  }
+  uint32_t acc = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) acc ^= x[i];
+  out[(uint64_t)blockIdx.x * blockDim.x + threadIdx.x] = acc;
 }
 
 unsigned grid_for(uint64_t n) {
@@ -109,6 +136,26 @@ int tc_measure_list_bandwidth(tc_ctx* c, const uint32_t* lens, uint32_t count, f
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return st;
+}
+
+int tc_measure_valu_rate(tc_ctx* c, uint32_t iters, float* ms) {
+  if (!c || !ms) return fail(TC_ERR_ARG, "NULL argument");
+  if (iters == 0) return fail(TC_ERR_RANGE, "iters must be positive");
+  TCX_HIP(hipSetDevice(c->device));
+  const uint32_t G = (uint32_t)c->num_cus * 8u;  // 8 waves per SIMD
+  DevBuf out;
+  TCX_TRY(out.reserve(4ull * G * 256));
+  hipLaunchKernelGGL(k_valu_probe, dim3(G), dim3(256), 0, c->stream, iters, 1u,
+                     out.as<uint32_t>());  // warm-up
+  TCX_HIP(hipGetLastError());
+  TCX_HIP(hipEventRecord(c->ev0, c->stream));
+  hipLaunchKernelGGL(k_valu_probe, dim3(G), dim3(256), 0, c->stream, iters, 2u,
+                     out.as<uint32_t>());
+  TCX_HIP(hipGetLastError());
+  TCX_HIP(hipEventRecord(c->ev1, c->stream));
+  TCX_HIP(hipEventSynchronize(c->ev1));
+  TCX_HIP(hipEventElapsedTime(ms, c->ev0, c->ev1));
+  return TC_OK;
 }
 
 int tc_bw_model_fit(const uint32_t* lens, const float* gbps, uint32_t count, tc_bw_model* out) {

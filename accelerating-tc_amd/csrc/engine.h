@@ -223,14 +223,17 @@ __device__ inline uint32_t probe_search(const Probe& P, uint32_t x) {
   return (x != kEmpty && contains(P.a, 0, P.n, x)) ? 1u : 0u;
 }
 
-// Bitmap bit of x (any x: the word index is masked to the bitmap).  The bit
-// extract reads only the low five bits of its offset, which the mask keeps,
-// so it takes the hash before the mask.
-__device__ inline bool bm_hit(const Probe& P, uint32_t x) {
+// Bitmap bit of x, 0 or 1 (any x: the word index is masked to the bitmap).
+// The bit extract reads only the low five bits of its offset, which the
+// mask keeps, so it takes the hash before the mask.
+__device__ inline uint32_t bm_bit(const Probe& P, uint32_t x) {
   const uint32_t f = x ^ (x >> 15);
-  const uint32_t w = P.bm[(f & (0xFFFFFFFFu >> P.shift)) >> 5];
-  return __builtin_amdgcn_ubfe(w, f, 1u) != 0u;
+  // word (fhash(x) >> 5): bits 5 .. 31 - shift of f
+  const uint32_t w = P.bm[__builtin_amdgcn_ubfe(f, 5u, 27u - P.shift)];
+  return __builtin_amdgcn_ubfe(w, f, 1u);
 }
+
+__device__ inline bool bm_hit(const Probe& P, uint32_t x) { return bm_bit(P, x) != 0u; }
 
 // Appends x (kEmpty: nothing) to the queue when it passes the filter.  The
 // bitmap is read whether or not the filter is on (where it is off the probe
@@ -247,23 +250,25 @@ __device__ inline void push(Probe& P, uint32_t x) {
 // push() of four entries, with the four bitmap probes issued before any
 // queue write so that their LDS reads overlap.  Faster for long streamed
 // lists; short-list kernels keep the one-entry form, which needs fewer
-// registers.
+// registers.  Entries past a list's end (kEmpty) are not told apart: one
+// whose bitmap bit is set (or any, with the filter off) is queued, and the
+// search rejects it, as no vertex id is kEmpty.  Each lane mask is the
+// compare of the extracted bit.
 __device__ inline void push4(Probe& P, uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3) {
-  // the four probes are read whatever the entries and the filter (no short
-  // circuit), so the reads overlap and the predicates combine as lane masks
   const bool nf = !P.filt;
-  const bool h0 = bm_hit(P, x0), h1 = bm_hit(P, x1), h2 = bm_hit(P, x2), h3 = bm_hit(P, x3);
-  const bool p0 = (x0 != kEmpty) & (h0 | nf), p1 = (x1 != kEmpty) & (h1 | nf);
-  const bool p2 = (x2 != kEmpty) & (h2 | nf), p3 = (x3 != kEmpty) & (h3 | nf);
-  const uint64_t m0 = lanes(p0), m1 = lanes(p1), m2 = lanes(p2), m3 = lanes(p3);
+  const uint64_t all = nf ? ~0ull : 0ull;
+  const uint32_t b0 = bm_bit(P, x0), b1 = bm_bit(P, x1), b2 = bm_bit(P, x2), b3 = bm_bit(P, x3);
+  const bool c0 = b0 != 0u, c1 = b1 != 0u, c2 = b2 != 0u, c3 = b3 != 0u;
+  const uint64_t m0 = lanes(c0) | all, m1 = lanes(c1) | all;
+  const uint64_t m2 = lanes(c2) | all, m3 = lanes(c3) | all;
   uint32_t q = P.qn;
-  if (p0) P.Q[lanes_below(m0, q)] = x0;
+  if (c0 || nf) P.Q[lanes_below(m0, q)] = x0;
   q += (uint32_t)__popcll(m0);
-  if (p1) P.Q[lanes_below(m1, q)] = x1;
+  if (c1 || nf) P.Q[lanes_below(m1, q)] = x1;
   q += (uint32_t)__popcll(m1);
-  if (p2) P.Q[lanes_below(m2, q)] = x2;
+  if (c2 || nf) P.Q[lanes_below(m2, q)] = x2;
   q += (uint32_t)__popcll(m2);
-  if (p3) P.Q[lanes_below(m3, q)] = x3;
+  if (c3 || nf) P.Q[lanes_below(m3, q)] = x3;
   P.qn = uniform(q + (uint32_t)__popcll(m3));
 }
 
@@ -417,7 +422,7 @@ __device__ inline uint32_t wave_vertex(const uint32_t* __restrict__ rp,
                                        uint32_t lane, const WaveSlot& W,
                                        const uint32_t* __restrict__ wl = nullptr,
                                        uint32_t wn = 0) {
-  const uint32_t ub = rp[u], du = rp[u + 1] - ub;
+  const uint32_t ub = rp[u], du = uniform(rp[u + 1] - ub);
   const bool filt = du > FMIN;
   for (uint32_t i = lane; i < du; i += 64) {
     const uint32_t x = col[ub + i];
@@ -484,7 +489,7 @@ __device__ inline uint32_t block_vertex(const uint32_t* __restrict__ rp,
                                         uint32_t wn = 0) {
   constexpr uint32_t shift = BlockLds<CAP>::kBlockShift;
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const uint32_t ub = rp[u], du = rp[u + 1] - ub;
+  const uint32_t ub = rp[u], du = uniform(rp[u + 1] - ub);
   const bool in_lds = du <= (uint32_t)CAP;
   for (uint32_t i = tid; i < du; i += TB) {
     const uint32_t x = col[ub + i];

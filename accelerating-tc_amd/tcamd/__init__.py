@@ -136,6 +136,7 @@ class BwModel(C.Structure):
 _sig("tc_graph_relabel_aorder_model", C.c_int, [_vp, _vp, C.POINTER(BwModel), _vp])
 _sig("tc_bw_model_reference", C.c_int, [C.POINTER(BwModel)])
 _sig("tc_measure_list_bandwidth", C.c_int, [_vp, _vp, C.c_uint32, _vp])
+_sig("tc_measure_valu_rate", C.c_int, [_vp, C.c_uint32, C.POINTER(C.c_float)])
 _sig("tc_bw_model_fit", C.c_int, [_vp, _vp, C.c_uint32, C.POINTER(BwModel)])
 _sig("tc_graph_orient", C.c_int, [_vp, _vp, C.c_int, C.c_uint32])
 _sig("tc_graph_orient_adir_omp", C.c_int, [_vp, _vp, C.c_double])
@@ -236,7 +237,7 @@ EXPORTED = [
     "tc_gen_er", "tc_gen_rmat", "tc_gen_ba", "tc_gen_rgg", "tc_graph_create", "tc_graph_destroy",
     "tc_graph_simplify", "tc_graph_relabel_aorder", "tc_graph_relabel", "tc_graph_orient", "tc_graph_info_get",
     "tc_graph_relabel_aorder_model", "tc_bw_model_reference", "tc_measure_list_bandwidth",
-    "tc_bw_model_fit",
+    "tc_measure_valu_rate", "tc_bw_model_fit",
     "tc_graph_get_csr", "tc_graph_get_edges", "tc_graph_get_vertex_key", "tc_graph_device_csr",
     "tc_hu_workload", "tc_hu_block_starts", "tc_fox_workload", "tc_fox_bin_sizes", "tc_count",
     "tc_count_per_vertex", "tc_auto_stream_bytes", "tc_cost_functions", "tc_approx_ratio", "kcl_count", "kcl_gpu_solver",
@@ -396,6 +397,14 @@ def measure_list_bandwidth(ctx: "Context", lens) -> np.ndarray:
     _check(_lib.tc_measure_list_bandwidth(ctx.handle, lens.ctypes.data, len(lens),
                                           out.ctypes.data), "tc_measure_list_bandwidth")
     return out[:len(lens)]
+
+
+def measure_valu_rate(ctx: "Context", iters: int) -> float:
+    """hipEvent milliseconds of one issue-rate probe launch of `iters` rounds
+    (its wave instructions come from a profiled run's SQ_INSTS_VALU)."""
+    ms = C.c_float()
+    _check(_lib.tc_measure_valu_rate(ctx.handle, iters, C.byref(ms)), "tc_measure_valu_rate")
+    return ms.value
 
 
 def fit_bw_model(lens, gbps) -> BwModel:

@@ -457,9 +457,19 @@ def main_kclique(args, w):
     ctx.close()
 
 
-# VALU issue peak of MI355X: 1,024 SIMDs at 2.4 GHz, a wave64 instruction
-# every 4 cycles on a 16-lane SIMD
-VALU_PEAK_WAVE_INSTS = 1024 * 2.4e9 / 4
+# Nominal VALU issue peak of MI355X: 1,024 SIMDs at 2.4 GHz, a wave64
+# instruction every 4 cycles on a 16-lane SIMD.  The measured peak (the
+# issue-rate probe, profiles/valu_calibration.json) replaces it when present.
+VALU_NOMINAL_WAVE_INSTS = 1024 * 2.4e9 / 4
+
+
+def valu_peak():
+    """(wave instructions per second, source) of the VALU issue roofline."""
+    prof = os.path.join(ROOT, "profiles", "valu_calibration.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            return json.load(f)["wave_insts_per_s"], "measured (valu_calibration.json)"
+    return VALU_NOMINAL_WAVE_INSTS, "nominal (4 cycles per wave64 instruction)"
 
 
 def kclique_roofline(workload, per_k):
@@ -476,9 +486,10 @@ def kclique_roofline(workload, per_k):
     if not ks:
         return None
     k = max(ks)
+    peak, source = valu_peak()
     achieved = insts[str(k)] / (per_k[k]["ms_kernels"] * 1e-3)
-    return {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_WAVE_INSTS,
-            "unit": "wave-instructions/s", "frac": achieved / VALU_PEAK_WAVE_INSTS,
+    return {"bound": "valu", "achieved": achieved, "peak": peak,
+            "unit": "wave-instructions/s", "frac": achieved / peak, "peak_source": source,
             "traffic": None, "kernel": "k_kcl_pivot (all widths)", "k": k,
             "valu_insts_per_count": insts[str(k)]}
 
@@ -633,7 +644,19 @@ def main():
                     help="k-clique workloads: orientation (default: degeneracy)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
+    ap.add_argument("--valu-calibration", action="store_true",
+                    help="only run the vector-ALU issue-rate probe (profile it for SQ_INSTS_VALU)")
     args = ap.parse_args()
+    if args.valu_calibration:
+        import tcamd
+        ctx = tcamd.Context(0)
+        iters = 100000
+        ms = [tcamd.measure_valu_rate(ctx, iters) for _ in range(3)]
+        import torch
+        waves = torch.cuda.get_device_properties(0).multi_processor_count * 8 * 4
+        print(json.dumps({"probe_iters": iters, "waves": waves, "ms": ms,
+                          "valu_insts_per_round": 48}))
+        return
     if args.copy_calibration:
         import torch
         x = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")

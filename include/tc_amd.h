@@ -218,6 +218,13 @@ int tc_bw_model_reference(tc_bw_model* out);
  * per list, the whole GPU busy (the access the counting kernels make to
  * N+(v)).  No counterpart in the reference, which ships measured numbers. */
 int tc_measure_list_bandwidth(tc_ctx* ctx, const uint32_t* lens, uint32_t count, float* gbps);
+/* Vector-ALU issue-rate probe (no counterpart in the reference; it prices
+ * the issue-bound kernels): one timed launch of `iters` rounds of eight
+ * independent integer-operation chains per lane, eight waves on every SIMD;
+ * *ms receives its hipEvent time.  The wave instructions it issued come from
+ * a profiled run (SQ_INSTS_VALU); their count over *ms is the GPU's
+ * attainable issue rate for such code. */
+int tc_measure_valu_rate(tc_ctx* ctx, uint32_t iters, float* ms);
 /* A model fitted to such measurements: table[len] = the measured value for
  * 2 <= len <= 9 (all of them must be present), (a, b) the least-squares fit
  * of log gbps on log len over the lengths >= 10 (at least two), lambda and

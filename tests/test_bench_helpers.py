@@ -102,4 +102,11 @@ def test_kclique_roofline_from_profile(bench, tmp_path, monkeypatch):
     assert r["bound"] == "valu" and r["k"] == 5
     assert r["achieved"] == pytest.approx(6.144e10 / 0.2)
     assert r["frac"] == pytest.approx(0.5)                 # peak 6.144e11 wave instructions/s
+    assert r["peak_source"].startswith("nominal")
     assert bench.kclique_roofline("missing", {5: {"ms_kernels": 1.0}}) is None
+    # a measured issue rate replaces the nominal peak
+    (tmp_path / "profiles" / "valu_calibration.json").write_text(
+        json.dumps({"wave_insts_per_s": 1.2288e12}))
+    r = bench.kclique_roofline("kx", {5: {"ms_kernels": 200.0}})
+    assert r["peak"] == 1.2288e12 and r["frac"] == pytest.approx(0.25)
+    assert r["peak_source"].startswith("measured")
