@@ -549,6 +549,16 @@ def test_list_bandwidth_measurement(ctx):
         tcamd.measure_list_bandwidth(ctx, [0])
 
 
+def test_valu_rate_probe(ctx):
+    # the probe's work is a fixed number of rounds per lane: its time grows
+    # with the rounds (eight times the rounds, four to twelve times the time)
+    t1 = min(tcamd.measure_valu_rate(ctx, 2000) for _ in range(2))
+    t8 = min(tcamd.measure_valu_rate(ctx, 16000) for _ in range(2))
+    assert t1 > 0 and 4.0 < t8 / t1 < 12.0
+    with pytest.raises(tcamd.TCError):
+        tcamd.measure_valu_rate(ctx, 0)
+
+
 @pytest.mark.parametrize("case", list(CASES))
 def test_auto_strided_and_claimed_wave_tasks(ctx, case):
     # AUTO strides over the wave tasks when they are few (at most 16 per
