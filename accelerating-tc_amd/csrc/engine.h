@@ -139,10 +139,27 @@ __device__ inline bool contains(const uint32_t* a, uint32_t lo, uint32_t hi, uin
 // count depends on n only, so the lanes of a wave never diverge.  n must be
 // wave-uniform: it is read from the first lane, so the loop's counters live
 // in scalar registers and a step costs one add, one compare and one select.
+// UNROLL (the work-list kernels): four halvings per trip while at least four
+// remain (len > 15), a quarter of the loop's scalar bookkeeping.  The
+// per-vertex and push-form kernels keep the one-halving loop: with the
+// unrolled search the compiler moved their block variant out of line, and
+// the per-vertex kernel in that form once stopped returning on R-MAT
+// graphs (the cause was not found; the inlined form has always finished).
+template <bool UNROLL = false>
 __device__ inline uint32_t lds_has(const uint32_t* a, uint32_t n, uint32_t x) {
   n = (uint32_t)__builtin_amdgcn_readfirstlane((int)n);
   const uint32_t* p = a;
   uint32_t len = n;
+  if constexpr (UNROLL) {
+    while (len > 15) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t half = len >> 1;
+        p = p[half] <= x ? p + half : p;
+        len -= half;
+      }
+    }
+  }
   while (len > 1) {
     const uint32_t half = len >> 1;
     p = p[half] <= x ? p + half : p;
@@ -218,8 +235,9 @@ struct Probe {
   bool global;
 };
 
+template <bool UNROLL = false>
 __device__ inline uint32_t probe_search(const Probe& P, uint32_t x) {
-  if (!P.global) return lds_has(P.a, P.n, x);
+  if (!P.global) return lds_has<UNROLL>(P.a, P.n, x);
   return (x != kEmpty && contains(P.a, 0, P.n, x)) ? 1u : 0u;
 }
 
@@ -253,9 +271,12 @@ __device__ inline void push(Probe& P, uint32_t x) {
 // registers.  Entries past a list's end (kEmpty) are not told apart: one
 // whose bitmap bit is set (or any, with the filter off) is queued, and the
 // search rejects it, as no vertex id is kEmpty.  Each lane mask is the
-// compare of the extracted bit.
+// compare of the extracted bit.  FILT: the caller guarantees P.filt (the
+// work-list kernels build the bitmap for every task), which drops the
+// filter-off masks.
+template <bool FILT = false>
 __device__ inline void push4(Probe& P, uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3) {
-  const bool nf = !P.filt;
+  const bool nf = !FILT && !P.filt;
   const uint64_t all = nf ? ~0ull : 0ull;
   const uint32_t b0 = bm_bit(P, x0), b1 = bm_bit(P, x1), b2 = bm_bit(P, x2), b3 = bm_bit(P, x3);
   const bool c0 = b0 != 0u, c1 = b1 != 0u, c2 = b2 != 0u, c3 = b3 != 0u;
@@ -273,16 +294,18 @@ __device__ inline void push4(Probe& P, uint32_t x0, uint32_t x1, uint32_t x2, ui
 }
 
 // Resolves queued survivors 64 at a time; `all` also resolves a partial tail.
+// WL: the work-list kernels' search (see lds_has).
+template <bool WL = false>
 __device__ inline uint32_t drain(Probe& P, uint32_t lane, bool all) {
   uint32_t c = 0;
   if (P.qn >= 64 || (all && P.qn)) {
     wave_sync();
     while (P.qn >= 64) {
       P.qn -= 64;
-      c += probe_search(P, P.Q[P.qn + lane]);
+      c += probe_search<WL>(P, P.Q[P.qn + lane]);
     }
     if (all && P.qn) {
-      c += probe_search(P, lane < P.qn ? P.Q[lane] : kEmpty);
+      c += probe_search<WL>(P, lane < P.qn ? P.Q[lane] : kEmpty);
       P.qn = 0;
     }
     wave_sync();
@@ -293,7 +316,7 @@ __device__ inline uint32_t drain(Probe& P, uint32_t lane, bool all) {
 // Streams col[vb, ve) with the whole wave, 4 * U 64-entry chunks in flight
 // (the queue is drained after every four).  U = 2 suits long lists (R-MAT
 // hubs); U = 1 wastes fewer lanes on short ones (many small graphs).
-template <int U = 2>
+template <int U = 2, bool WL = false>
 __device__ inline uint32_t stream_list(const uint32_t* __restrict__ col, uint32_t vb, uint32_t ve,
                                        uint32_t lane, Probe& P) {
   uint32_t c = 0;
@@ -310,7 +333,7 @@ __device__ inline uint32_t stream_list(const uint32_t* __restrict__ col, uint32_
 #pragma unroll
         for (int j = 0; j < 4; ++j) push(P, w[4 * h + j]);
       }
-      c += drain(P, lane, false);
+      c += drain<WL>(P, lane, false);
     }
   }
   return c;
@@ -337,15 +360,20 @@ __device__ inline uint32_t stream_many(const uint32_t* __restrict__ col, uint64_
     uint32_t w[4 * U];
 #pragma unroll
     for (int s = 0; s < 4 * U; ++s) {
-      const uint32_t k = pos + lane;
-      w[s] = (pos < end && k < end) ? col[k] : kEmpty;
+      const uint32_t k = pos + lane;  // no wrap: CSRs hold fewer than 2^32 - 512 entries
+      w[s] = k < end ? col[k] : kEmpty;
       pos += 64;
-      if (pos >= end && mask) next_list();
+      if (pos >= end) {
+        if (mask)
+          next_list();
+        else
+          pos = end = 0;  // all lists done: the remaining chunks load nothing
+      }
     }
 #pragma unroll
     for (int h = 0; h < U; ++h) {
-      push4(P, w[4 * h], w[4 * h + 1], w[4 * h + 2], w[4 * h + 3]);
-      c += drain(P, lane, false);
+      push4<true>(P, w[4 * h], w[4 * h + 1], w[4 * h + 2], w[4 * h + 3]);
+      c += drain<true>(P, lane, false);
     }
   }
   return c;
@@ -357,22 +385,23 @@ __device__ inline uint32_t stream_many(const uint32_t* __restrict__ col, uint64_
 // concatenated and walked 64 entries per step (two steps in flight), each
 // lane finding the list of its entry by a search over the exclusive prefix
 // of the lengths, kept with the list starts in the wave's LDS slot F[128].
-// MANY is used by the work-list kernels only: the per-vertex and plain AUTO
-// kernels keep the one-list-at-a-time stream.
-template <int U = 2, bool MANY = false>
+// WL (the work-list kernels) streams through stream_many and searches with
+// the unrolled loop; the per-vertex and push-form kernels keep the
+// one-list-at-a-time stream and the one-halving search.
+template <int U = 2, bool WL = false>
 __device__ inline uint32_t lists64(const uint32_t* __restrict__ col, uint32_t vb, uint32_t ve,
                                    uint32_t lane, uint32_t* F, Probe& P) {
   const uint32_t len = ve - vb;
   const bool lng = len >= kFlatMin;
   uint32_t c = 0;
   uint64_t mask = lanes(lng);
-  if constexpr (MANY && U >= 2) {
+  if constexpr (WL && U >= 2) {
     if (mask) c += stream_many<U>(col, mask, vb, ve, lane, P);
   } else {
     while (mask) {
       const uint32_t j = (uint32_t)__builtin_ctzll(mask);
       mask &= mask - 1;
-      c += stream_list<U>(col, __shfl(vb, j, 64), __shfl(ve, j, 64), lane, P);
+      c += stream_list<U, WL>(col, __shfl(vb, j, 64), __shfl(ve, j, 64), lane, P);
     }
   }
   const uint32_t sl = lng ? 0u : len;
@@ -395,7 +424,7 @@ __device__ inline uint32_t lists64(const uint32_t* __restrict__ col, uint32_t vb
     }
     push(P, xa);
     push(P, xb);
-    c += drain(P, lane, false);
+    c += drain<WL>(P, lane, false);
   }
   wave_sync();  // F is rewritten by the next call
   return c;
@@ -423,7 +452,8 @@ __device__ inline uint32_t wave_vertex(const uint32_t* __restrict__ rp,
                                        const uint32_t* __restrict__ wl = nullptr,
                                        uint32_t wn = 0) {
   const uint32_t ub = rp[u], du = uniform(rp[u + 1] - ub);
-  const bool filt = du > FMIN;
+  // the work-list stream (stream_many) relies on the filter being on
+  const bool filt = (WL && U >= 2) || du > FMIN;
   for (uint32_t i = lane; i < du; i += 64) {
     const uint32_t x = col[ub + i];
     W.L[i] = x;
@@ -442,7 +472,7 @@ __device__ inline uint32_t wave_vertex(const uint32_t* __restrict__ rp,
     }
     c += lists64<U, WL>(col, mvb, mve, lane, W.F, P);
   }
-  c += drain(P, lane, true);
+  c += drain<WL>(P, lane, true);
   if (filt)
     for (uint32_t i = lane; i < du; i += 64) W.bm[fhash(W.L[i], W.shift) >> 5] = 0;
   wave_sync();
@@ -511,7 +541,7 @@ __device__ inline uint32_t block_vertex(const uint32_t* __restrict__ rp,
     }
     c += lists64<U, WL>(col, vb, ve, lane, S.f[wv], P);
   }
-  c += drain(P, lane, true);
+  c += drain<WL>(P, lane, true);
   __syncthreads();
   for (uint32_t i = tid; i < du; i += TB) S.bm[fhash(a[i], shift) >> 5] = 0;
   __syncthreads();

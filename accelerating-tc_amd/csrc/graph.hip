@@ -559,7 +559,7 @@ static int csr_from_keys(tc_ctx* c, tc_graph* g, DevBuf& keys, DevBuf& tmp, uint
       if ((uint32_t)(last >> 32) == n_out) --nuniq;  // sentinel
     }
   }
-  if (nuniq >= 0xFFFFFFFFull)
+  if (nuniq > kMaxCsrEntries)
     return fail(TC_ERR_RANGE, "%llu oriented edges exceed the 32-bit CSR offsets",
                 (unsigned long long)nuniq);
   g->n = n_out;
@@ -1331,7 +1331,7 @@ int tc_graph_load_csr(tc_ctx* c, const char* path, tc_graph** out) {
   if (fread(&h, sizeof(h), 1, f) != 1 || memcmp(h.magic, kCsrMagic, 8) != 0 || h.version != 1 ||
       h.orient_mode < TC_ORIENT_ID || h.orient_mode > TC_ORIENT_ADIR_OMP ||
       h.has_key != (has_vertex_key(h.orient_mode) ? 1u : 0u) || h.n > 0x7FFFFFFEu ||
-      h.m >= 0xFFFFFFFFull) {
+      h.m > kMaxCsrEntries) {
     fclose(f);
     return fail(TC_ERR_FORMAT, "%s is not an oriented-CSR stage file", path);
   }

@@ -14,6 +14,10 @@
 
 namespace tcx {
 
+// Most oriented edges a CSR may hold: 32-bit offsets, with the last list's
+// 64-entry chunks (up to 512 entries past a list's first) still below 2^32.
+constexpr uint64_t kMaxCsrEntries = 0xFFFFFFFFull - 512;
+
 // ---- error plumbing ------------------------------------------------------
 void set_error(const char* fmt, ...);
 int fail(int status, const char* fmt, ...);

@@ -362,7 +362,8 @@ int tc_count_per_vertex(tc_ctx* ctx, tc_graph* g, uint64_t* tri);
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): the exact triangle count of an oriented CSR the caller
  * already holds in device memory (rowptr[n+1], col[rowptr[n]], every row
- * ascending and without repeats, as tc_graph_device_csr gives), written to
+ * ascending and without repeats, rowptr[n] below 2^32 - 512, as
+ * tc_graph_device_csr gives), written to
  * the caller's device uint64 *count.  The work is enqueued on ctx's stream
  * and not waited for.  The kernels are tc_count's AUTO push form without
  * its task lists.  Their scratch memory (two claim cursors and one partial
