@@ -653,7 +653,10 @@ void k_auto_wl(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col
     if (t >= nblk) break;
     const uint32_t x = blk[t];
     const uint32_t wb = wrp[x];
-    cnt += block_vertex<CAP, 2, true>(rp, col, x, S, wcol + wb, wrp[x + 1] - wb);
+    // inlined: the workgroup's LDS stays a known address (no generic
+    // pointer, no call) and the kernel fits 64 VGPRs without spilling
+    [[clang::always_inline]] cnt += block_vertex<CAP, 2, true>(rp, col, x, S, wcol + wb,
+                                                              wrp[x + 1] - wb);
   }
   const WaveSlot W = wave_slot(S, wv);
   if constexpr (STATIC_WAVES) {
