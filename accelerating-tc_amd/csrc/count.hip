@@ -169,6 +169,21 @@ __global__ void k_low32(const uint64_t* __restrict__ keys, uint64_t m, uint32_t*
   for (uint64_t i = gtid(); i < m; i += gstride()) out[i] = (uint32_t)keys[i];
 }
 
+// Row x of the lane tasks' layout: N+(x) in 8 words, kEmpty after its
+// end; rows of longer lists are left all kEmpty (no lane task reads them:
+// a lane task and every list it streams hold at most eight entries).
+__global__ void k_ell8(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+                       uint32_t n, uint4* __restrict__ ell) {
+  for (uint64_t x = gtid(); x < n; x += gstride()) {
+    const uint32_t b = rp[x], d = rp[x + 1] - b;
+    uint32_t r[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) r[k] = (d <= 8 && k < d) ? col[b + k] : dev::kEmpty;
+    ell[2 * x] = make_uint4(r[0], r[1], r[2], r[3]);
+    ell[2 * x + 1] = make_uint4(r[4], r[5], r[6], r[7]);
+  }
+}
+
 __global__ void k_eq_flags(const uint8_t* __restrict__ in, uint64_t n, uint8_t v,
                            uint8_t* __restrict__ out) {
   for (uint64_t i = gtid(); i < n; i += gstride()) out[i] = in[i] == v ? 1 : 0;
@@ -824,6 +839,12 @@ static int wl_schedule(tc_ctx* c, tc_graph* g, uint32_t wmax, uint32_t hmin, uin
   TCX_HIP(hipGetLastError());
   TCX_TRY(sc.wl_lane.reserve(4 * ((size_t)n + 1)));
   TCX_TRY(select_indices(c, flags.as<uint8_t>(), n, sc.wl_lane.as<uint32_t>(), &sc.nw_lane));
+  // the lane tasks' 32-byte rows (n of them; a learned schedule may run any
+  // task of d+ <= 8 as a lane)
+  TCX_TRY(sc.wl_ell.reserve(32 * ((size_t)n + 1)));
+  hipLaunchKernelGGL(k_ell8, dim3(grid_for(n)), dim3(TB), 0, c->stream, rp,
+                     g->col.as<uint32_t>(), n, sc.wl_ell.as<uint4>());
+  TCX_HIP(hipGetLastError());
   uint64_t* sk;
   uint32_t* sv;
   TCX_TRY(sort_pairs_u64_u32(c, key.as<uint64_t>(), ktmp.as<uint64_t>(), ids.as<uint32_t>(),
@@ -1076,16 +1097,17 @@ int tc_count(tc_ctx* c, tc_graph* g, const tc_sched* sched_in, uint64_t* count,
         // cursors: [0] block, [1] wave, [2] lane tasks; [4..6] the 8192-entry launch
         const uint32_t* wrp = sc.wl_rp.as<uint32_t>();
         const uint32_t* wcol = sc.wl_col.as<uint32_t>();
+        const uint4* ell = sc.wl_ell.as<uint4>();
         if (sc.nw_large || sc.nw_small || sc.nw_lane) {
           TCX_TRY(lm.mark());
           if (sc.nw_small <= (uint64_t)kStaticWaves * G * (TB / 64))
             hipLaunchKernelGGL((k_auto_wl<2048, true>), dim3(G), dim3(TB), 0, c->stream, rp, col,
-                               wrp, wcol, sc.wl_large.as<uint32_t>(), (uint32_t)sc.nw_large,
+                               wrp, wcol, ell, sc.wl_large.as<uint32_t>(), (uint32_t)sc.nw_large,
                                sc.wl_small.as<uint32_t>(), (uint32_t)sc.nw_small,
                                sc.wl_lane.as<uint32_t>(), (uint32_t)sc.nw_lane, cursor, acc);
           else
             hipLaunchKernelGGL((k_auto_wl<2048>), dim3(G), dim3(TB), 0, c->stream, rp, col, wrp,
-                               wcol, sc.wl_large.as<uint32_t>(), (uint32_t)sc.nw_large,
+                               wcol, ell, sc.wl_large.as<uint32_t>(), (uint32_t)sc.nw_large,
                                sc.wl_small.as<uint32_t>(), (uint32_t)sc.nw_small,
                                sc.wl_lane.as<uint32_t>(), (uint32_t)sc.nw_lane, cursor, acc);
           tm.launches++;
@@ -1094,7 +1116,7 @@ int tc_count(tc_ctx* c, tc_graph* g, const tc_sched* sched_in, uint64_t* count,
           TCX_TRY(lm.mark());
           hipLaunchKernelGGL((k_auto_wl<8192>),
                              dim3(std::min<uint64_t>((uint64_t)c->num_cus * 4, sc.nw_huge)),
-                             dim3(TB), 0, c->stream, rp, col, wrp, wcol, sc.wl_huge.as<uint32_t>(),
+                             dim3(TB), 0, c->stream, rp, col, wrp, wcol, ell, sc.wl_huge.as<uint32_t>(),
                              (uint32_t)sc.nw_huge, (const uint32_t*)nullptr, 0u,
                              (const uint32_t*)nullptr, 0u, cursor + 4, acc);
           tm.launches++;

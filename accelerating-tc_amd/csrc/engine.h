@@ -594,29 +594,34 @@ __global__ __launch_bounds__(TB) void k_auto(const uint32_t* __restrict__ rp,
 }
 
 // Lane task: one lane counts task x of the work-list schedule when
-// d+(x) <= kLaneMaxDeg.  N+(x) is held in registers; every list N+(y), y in
-// W(x), has at most d+(x) entries (the work-list rule streams the shorter
-// list), and each of its entries is compared with all of N+(x).  Unused
-// register slots hold kEmpty and unused list slots kEmpty - 1, which match
-// nothing (vertex ids are below 2^31 - 1).
-__device__ inline uint32_t lane_task(const uint32_t* __restrict__ rp,
-                                     const uint32_t* __restrict__ col,
+// d+(x) <= kLaneMaxDeg.  Out-lists come from `ell`, the out-lists of up to
+// eight entries as fixed 32-byte rows padded with kEmpty: a row is two
+// 16-byte loads from one cache line and needs no row-offset read (with
+// lists of a few entries at random rows, the offsets' line and a list that
+// straddles two lines made up most of the traffic).  N+(x) is held in
+// registers; every list N+(y), y in W(x), has at most d+(x) entries (the
+// work-list rule streams the shorter list), and each of its entries is
+// compared with all of N+(x).  The padding of N+(x) is turned into
+// kEmpty - 1 so that it matches none of a streamed row's (vertex ids are
+// below 2^31 - 1).
+static_assert(kLaneMaxDeg == 8, "lane rows are 32 bytes");
+__device__ inline uint32_t lane_task(const uint4* __restrict__ ell,
                                      const uint32_t* __restrict__ wrp,
                                      const uint32_t* __restrict__ wcol, uint32_t x) {
-  const uint32_t b = rp[x], dx = rp[x + 1] - b;
-  uint32_t s[kLaneMaxDeg];
+  const uint4 a0 = ell[2 * (uint64_t)x], a1 = ell[2 * (uint64_t)x + 1];
+  uint32_t s[kLaneMaxDeg] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
 #pragma unroll
-  for (uint32_t k = 0; k < kLaneMaxDeg; ++k) s[k] = k < dx ? col[b + k] : kEmpty;
+  for (uint32_t k = 0; k < kLaneMaxDeg; ++k) s[k] = s[k] == kEmpty ? kEmpty - 1 : s[k];
   const uint32_t wb = wrp[x], we = wrp[x + 1];
   uint32_t c = 0;
   for (uint32_t i = wb; i < we; ++i) {
     const uint32_t y = wcol[i];
-    const uint32_t yb = rp[y], dy = rp[y + 1] - yb;
+    const uint4 b0 = ell[2 * (uint64_t)y], b1 = ell[2 * (uint64_t)y + 1];
+    const uint32_t e[kLaneMaxDeg] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
 #pragma unroll
     for (uint32_t k = 0; k < kLaneMaxDeg; ++k) {
-      const uint32_t e = k < dy ? col[yb + k] : kEmpty - 1;
 #pragma unroll
-      for (uint32_t j = 0; j < kLaneMaxDeg; ++j) c += e == s[j] ? 1u : 0u;
+      for (uint32_t j = 0; j < kLaneMaxDeg; ++j) c += e[k] == s[j] ? 1u : 0u;
     }
   }
   return c;
@@ -629,7 +634,8 @@ __device__ inline uint32_t lane_task(const uint32_t* __restrict__ rp,
 // d+(v) <= d+(u), else v), so the sum over tasks of |N+(x) & N+(y)| is the
 // triangle count; edges into a vertex with no out-edge are left out.
 // Phase 3 counts the lane tasks (`lan`), kLaneClaim per wave claim from
-// cursor[2].
+// cursor[2], from the 32-byte rows `ell` (see lane_task; unused when nlan is
+// zero).
 // Eight waves per SIMD (the persistent grid's eight workgroups per CU): the
 // register budget is held at 64 VGPRs so the occupancy survives the
 // unrolled streams.
@@ -637,6 +643,7 @@ template <int CAP, bool STATIC_WAVES = false>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void k_auto_wl(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                const uint32_t* __restrict__ wrp, const uint32_t* __restrict__ wcol,
+               const uint4* __restrict__ ell,
                const uint32_t* __restrict__ blk, uint32_t nblk,
                const uint32_t* __restrict__ wav, uint32_t nwav,
                const uint32_t* __restrict__ lan, uint32_t nlan,
@@ -686,7 +693,7 @@ void k_auto_wl(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col
     i0 = __shfl(i0, 0, 64);
     if (i0 >= nlan) break;
     for (uint32_t i = i0 + lane; i < min(nlan, i0 + kLaneClaim); i += 64)
-      cnt += lane_task(rp, col, wrp, wcol, lan[i]);
+      cnt += lane_task(ell, wrp, wcol, lan[i]);
   }
   flush_block(cnt, total, red);
 }
