@@ -34,13 +34,14 @@ __device__ inline void flush_graph(unsigned long long& acc, uint32_t g,
   acc = 0;
 }
 
-// WL: tasks stream their work lists (AUTO's default schedule); otherwise the
-// push form, task u streaming N+(v) for v in N+(u) (wrp, wcol unused).
+// WL: tasks stream their work lists (AUTO's default schedule; wse holds each
+// entry's list bounds); otherwise the push form, task u streaming N+(v) for
+// v in N+(u) (wrp, wse unused).
 template <int CAP, uint32_t FMIN, bool WL>
 __global__ __launch_bounds__(TB) void k_batch(const uint32_t* __restrict__ rp,
                                               const uint32_t* __restrict__ col,
                                               const uint32_t* __restrict__ wrp,
-                                              const uint32_t* __restrict__ wcol,
+                                              const uint2* __restrict__ wse,
                                               const uint32_t* __restrict__ gid,
                                               const uint32_t* __restrict__ blk, uint32_t nblk,
                                               const uint32_t* __restrict__ wav, uint32_t nwav,
@@ -58,7 +59,7 @@ __global__ __launch_bounds__(TB) void k_batch(const uint32_t* __restrict__ rp,
     unsigned long long c;
     if constexpr (WL) {
       const uint32_t wb = wrp[u];
-      c = block_vertex<CAP, 1, true>(rp, col, u, S, wcol + wb, wrp[u + 1] - wb);
+      c = block_vertex<CAP, 1, true>(rp, col, u, S, wse + wb, wrp[u + 1] - wb);
     } else {
       c = block_vertex<CAP, 1>(rp, col, u, S);
     }
@@ -82,7 +83,7 @@ __global__ __launch_bounds__(TB) void k_batch(const uint32_t* __restrict__ rp,
       }
       if constexpr (WL) {
         const uint32_t wb = wrp[u];
-        acc += wave_vertex<FMIN, 1, true>(rp, col, u, lane, W, wcol + wb, wrp[u + 1] - wb);
+        acc += wave_vertex<FMIN, 1, true>(rp, col, u, lane, W, wse + wb, wrp[u + 1] - wb);
       } else {
         acc += wave_vertex<FMIN, 1>(rp, col, u, lane, W);
       }
@@ -309,7 +310,7 @@ int enqueue_batched(tc_ctx* c, tc_batch* b, const tc_batch_sched& s) {
   hipLaunchKernelGGL(k_batch_reset, dim3(grid_for(2ull + b->ngraphs)), dim3(256), 0, c->stream,
                      b->cursors.as<uint32_t>(), 2u, b->counts.as<unsigned long long>(), b->ngraphs);
   launch_batch(s, dim3(s.grid), c->stream, g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(),
-               g->sc.wl_rp.as<uint32_t>(), g->sc.wl_col.as<uint32_t>(),
+               g->sc.wl_rp.as<uint32_t>(), g->sc.wl_se.as<uint2>(),
                b->gid.as<uint32_t>(), b->blk_lpt.as<uint32_t>(), (uint32_t)b->nblk,
                b->wav.as<uint32_t>(), (uint32_t)b->nwav, s.wave_chunk, b->cursors.as<uint32_t>(),
                b->counts.as<unsigned long long>());
@@ -466,7 +467,7 @@ int tc_batch_count_pergraph(tc_ctx* c, tc_batch* b, const tc_batch_sched* sched,
     const uint64_t need = (b1 - b0) + ((w1 - w0) + 3) / 4;
     const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(s.grid, need));
     launch_batch(s, dim3(grid), c->stream, g->rowptr.as<uint32_t>(), g->col.as<uint32_t>(),
-                 g->sc.wl_rp.as<uint32_t>(), g->sc.wl_col.as<uint32_t>(),
+                 g->sc.wl_rp.as<uint32_t>(), g->sc.wl_se.as<uint2>(),
                  b->gid.as<uint32_t>(), b->blk_id.as<uint32_t>() + b0, b1 - b0,
                  b->wav.as<uint32_t>() + w0, w1 - w0, s.wave_chunk, cur + 2 * q,
                  b->counts.as<unsigned long long>());

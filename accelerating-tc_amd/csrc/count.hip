@@ -107,7 +107,9 @@ __global__ void k_wl_keys(const uint32_t* __restrict__ rp, const uint32_t* __res
 // W(x) (as ~work, for a descending sort), its class (0 none: W(x) empty or
 // d+(x) < 2; 1 wave task; 2 block task; 3 block task of the 8192-entry
 // kernel; 4 lane task) and the bytes it streams (task id, row bounds of x in
-// both CSRs, N+(x), and per y: its id, its row bounds and N+(y)).
+// both CSRs, N+(x), and per y: its row bounds, stored with the entry, and
+// N+(y)).  Lane tasks read fixed 32-byte rows instead; the figure is the
+// same for every split of the tasks.
 __global__ void k_wl_work(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ wrp,
                           const uint32_t* __restrict__ wcol, uint32_t n, uint32_t wmax,
                           uint32_t hmin, uint32_t lmax, uint64_t wave_work,
@@ -129,7 +131,7 @@ __global__ void k_wl_work(const uint32_t* __restrict__ rp, const uint32_t* __res
       else if (dx > hmin) k = 3;
       else if (dx > wmax || work > wave_work) k = 2;
       else k = 1;
-      b += 20ull + 4ull * dx + 12ull * (we - wb) + 4ull * streamed;
+      b += 20ull + 4ull * dx + 8ull * (we - wb) + 4ull * streamed;
     }
     key[x] = ~work;
     cls[x] = k;
@@ -181,6 +183,15 @@ __global__ void k_ell8(const uint32_t* __restrict__ rp, const uint32_t* __restri
     for (uint32_t k = 0; k < 8; ++k) r[k] = (d <= 8 && k < d) ? col[b + k] : dev::kEmpty;
     ell[2 * x] = make_uint4(r[0], r[1], r[2], r[3]);
     ell[2 * x + 1] = make_uint4(r[4], r[5], r[6], r[7]);
+  }
+}
+
+// se[i] = the bounds of N+(y) for the work-list entry y = wl[i].
+__global__ void k_wl_bounds(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ wl,
+                            uint64_t m, uint2* __restrict__ se) {
+  for (uint64_t i = gtid(); i < m; i += gstride()) {
+    const uint32_t y = wl[i];
+    se[i] = make_uint2(rp[y], rp[y + 1]);
   }
 }
 
@@ -809,9 +820,13 @@ static int wl_schedule(tc_ctx* c, tc_graph* g, uint32_t wmax, uint32_t hmin, uin
     TCX_HIP(hipStreamSynchronize(c->stream));
     sc.wl_entries = kept;
     TCX_TRY(sc.wl_col.reserve(4 * std::max<uint64_t>(kept, 1)));
-    if (kept)
+    TCX_TRY(sc.wl_se.reserve(8 * std::max<uint64_t>(kept, 1)));
+    if (kept) {
       hipLaunchKernelGGL(k_low32, dim3(grid_for(kept)), dim3(TB), 0, c->stream, sorted,
                          (uint64_t)kept, sc.wl_col.as<uint32_t>());
+      hipLaunchKernelGGL(k_wl_bounds, dim3(grid_for(kept)), dim3(TB), 0, c->stream, rp,
+                         sc.wl_col.as<uint32_t>(), (uint64_t)kept, sc.wl_se.as<uint2>());
+    }
     TCX_HIP(hipGetLastError());
   }
   DevBuf key, ktmp, ids, itmp, cls, flags, dflag;
@@ -1097,17 +1112,18 @@ int tc_count(tc_ctx* c, tc_graph* g, const tc_sched* sched_in, uint64_t* count,
         // cursors: [0] block, [1] wave, [2] lane tasks; [4..6] the 8192-entry launch
         const uint32_t* wrp = sc.wl_rp.as<uint32_t>();
         const uint32_t* wcol = sc.wl_col.as<uint32_t>();
+        const uint2* wse = sc.wl_se.as<uint2>();
         const uint4* ell = sc.wl_ell.as<uint4>();
         if (sc.nw_large || sc.nw_small || sc.nw_lane) {
           TCX_TRY(lm.mark());
           if (sc.nw_small <= (uint64_t)kStaticWaves * G * (TB / 64))
             hipLaunchKernelGGL((k_auto_wl<2048, true>), dim3(G), dim3(TB), 0, c->stream, rp, col,
-                               wrp, wcol, ell, sc.wl_large.as<uint32_t>(), (uint32_t)sc.nw_large,
+                               wrp, wcol, wse, ell, sc.wl_large.as<uint32_t>(), (uint32_t)sc.nw_large,
                                sc.wl_small.as<uint32_t>(), (uint32_t)sc.nw_small,
                                sc.wl_lane.as<uint32_t>(), (uint32_t)sc.nw_lane, cursor, acc);
           else
             hipLaunchKernelGGL((k_auto_wl<2048>), dim3(G), dim3(TB), 0, c->stream, rp, col, wrp,
-                               wcol, ell, sc.wl_large.as<uint32_t>(), (uint32_t)sc.nw_large,
+                               wcol, wse, ell, sc.wl_large.as<uint32_t>(), (uint32_t)sc.nw_large,
                                sc.wl_small.as<uint32_t>(), (uint32_t)sc.nw_small,
                                sc.wl_lane.as<uint32_t>(), (uint32_t)sc.nw_lane, cursor, acc);
           tm.launches++;
@@ -1116,7 +1132,7 @@ int tc_count(tc_ctx* c, tc_graph* g, const tc_sched* sched_in, uint64_t* count,
           TCX_TRY(lm.mark());
           hipLaunchKernelGGL((k_auto_wl<8192>),
                              dim3(std::min<uint64_t>((uint64_t)c->num_cus * 4, sc.nw_huge)),
-                             dim3(TB), 0, c->stream, rp, col, wrp, wcol, ell, sc.wl_huge.as<uint32_t>(),
+                             dim3(TB), 0, c->stream, rp, col, wrp, wcol, wse, ell, sc.wl_huge.as<uint32_t>(),
                              (uint32_t)sc.nw_huge, (const uint32_t*)nullptr, 0u,
                              (const uint32_t*)nullptr, 0u, cursor + 4, acc);
           tm.launches++;

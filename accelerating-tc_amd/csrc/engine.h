@@ -430,6 +430,22 @@ __device__ inline uint32_t lists64(const uint32_t* __restrict__ col, uint32_t vb
   return c;
 }
 
+// Bounds col[vb, ve) of list j of a work list: `wl` holds vertex ids (the
+// bounds read from rp: a random row-offset line per list) or, as uint2, the
+// bounds themselves (read in sequence with the list).
+__device__ inline void wl_bounds(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ wl,
+                                 uint32_t j, uint32_t& vb, uint32_t& ve) {
+  const uint32_t v = wl[j];
+  vb = rp[v];
+  ve = rp[v + 1];
+}
+__device__ inline void wl_bounds(const uint32_t* __restrict__, const uint2* __restrict__ wl,
+                                 uint32_t j, uint32_t& vb, uint32_t& ve) {
+  const uint2 b = wl[j];
+  vb = b.x;
+  ve = b.y;
+}
+
 // LDS of one wave for the wave variant.
 struct WaveSlot {
   uint32_t* L;   // N+(u), kWaveMax entries
@@ -444,13 +460,13 @@ struct WaveSlot {
 // built for d+(u) > FMIN and its words are cleared again at the end (below
 // that, every entry goes straight to the search).
 // With WL, the lists streamed against N+(u) are N+(y) for y in the work list
-// wl[0, wn) (global memory) instead of N+(v) for v in N+(u).
-template <uint32_t FMIN = kFilterMin, int U = 2, bool WL = false>
+// wl[0, wn) (global memory; ids or bounds, see wl_bounds) instead of N+(v)
+// for v in N+(u).
+template <uint32_t FMIN = kFilterMin, int U = 2, bool WL = false, typename WLE = uint32_t>
 __device__ inline uint32_t wave_vertex(const uint32_t* __restrict__ rp,
                                        const uint32_t* __restrict__ col, uint32_t u,
                                        uint32_t lane, const WaveSlot& W,
-                                       const uint32_t* __restrict__ wl = nullptr,
-                                       uint32_t wn = 0) {
+                                       const WLE* __restrict__ wl = nullptr, uint32_t wn = 0) {
   const uint32_t ub = rp[u], du = uniform(rp[u + 1] - ub);
   // the work-list stream (stream_many) relies on the filter being on
   const bool filt = (WL && U >= 2) || du > FMIN;
@@ -466,9 +482,13 @@ __device__ inline uint32_t wave_vertex(const uint32_t* __restrict__ rp,
   for (uint32_t j0 = 0; j0 < nl; j0 += 64) {
     uint32_t mvb = 0, mve = 0;
     if (j0 + lane < nl) {
-      const uint32_t v = WL ? wl[j0 + lane] : W.L[j0 + lane];
-      mvb = rp[v];
-      mve = rp[v + 1];
+      if constexpr (WL) {
+        wl_bounds(rp, wl, j0 + lane, mvb, mve);
+      } else {
+        const uint32_t v = W.L[j0 + lane];
+        mvb = rp[v];
+        mve = rp[v + 1];
+      }
     }
     c += lists64<U, WL>(col, mvb, mve, lane, W.F, P);
   }
@@ -511,11 +531,12 @@ __device__ inline void clear_bitmap(BlockLds<CAP>& S) {
 // hashed into the bitmap; wave w takes the out-neighbours w, w+4, w+8, ...
 // 64 at a time.  Every thread must call it; it ends with the bitmap clear
 // and a barrier.  With WL, the lists streamed are N+(y) for y in the work
-// list wl[0, wn) (global memory), wave w taking entries w, w+4, w+8, ...
-template <int CAP, int U = 2, bool WL = false>
+// list wl[0, wn) (global memory; ids or bounds, see wl_bounds), wave w
+// taking entries w, w+4, w+8, ...
+template <int CAP, int U = 2, bool WL = false, typename WLE = uint32_t>
 __device__ inline uint32_t block_vertex(const uint32_t* __restrict__ rp,
                                         const uint32_t* __restrict__ col, uint32_t u,
-                                        BlockLds<CAP>& S, const uint32_t* __restrict__ wl = nullptr,
+                                        BlockLds<CAP>& S, const WLE* __restrict__ wl = nullptr,
                                         uint32_t wn = 0) {
   constexpr uint32_t shift = BlockLds<CAP>::kBlockShift;
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -535,9 +556,13 @@ __device__ inline uint32_t block_vertex(const uint32_t* __restrict__ rp,
   for (uint32_t g0 = 0; g0 < mine; g0 += 64) {
     uint32_t vb = 0, ve = 0;
     if (g0 + lane < mine) {
-      const uint32_t v = WL ? wl[4 * (g0 + lane) + wv] : a[4 * (g0 + lane) + wv];
-      vb = rp[v];
-      ve = rp[v + 1];
+      if constexpr (WL) {
+        wl_bounds(rp, wl, 4 * (g0 + lane) + wv, vb, ve);
+      } else {
+        const uint32_t v = a[4 * (g0 + lane) + wv];
+        vb = rp[v];
+        ve = rp[v + 1];
+      }
     }
     c += lists64<U, WL>(col, vb, ve, lane, S.f[wv], P);
   }
@@ -633,6 +658,9 @@ __device__ inline uint32_t lane_task(const uint4* __restrict__ ell,
 // list, at the endpoint whose task streams the shorter out-list (u when
 // d+(v) <= d+(u), else v), so the sum over tasks of |N+(x) & N+(y)| is the
 // triangle count; edges into a vertex with no out-edge are left out.
+// Block and wave tasks read each entry's list bounds from wse (wse[i] =
+// (rp[y], rp[y+1]) for y = wcol[i]), in sequence, instead of a random
+// row-offset line per list.
 // Phase 3 counts the lane tasks (`lan`), kLaneClaim per wave claim from
 // cursor[2], from the 32-byte rows `ell` (see lane_task; unused when nlan is
 // zero).
@@ -643,7 +671,7 @@ template <int CAP, bool STATIC_WAVES = false>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void k_auto_wl(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                const uint32_t* __restrict__ wrp, const uint32_t* __restrict__ wcol,
-               const uint4* __restrict__ ell,
+               const uint2* __restrict__ wse, const uint4* __restrict__ ell,
                const uint32_t* __restrict__ blk, uint32_t nblk,
                const uint32_t* __restrict__ wav, uint32_t nwav,
                const uint32_t* __restrict__ lan, uint32_t nlan,
@@ -662,7 +690,7 @@ void k_auto_wl(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col
     const uint32_t wb = wrp[x];
     // inlined: the workgroup's LDS stays a known address (no generic
     // pointer, no call) and the kernel fits 64 VGPRs without spilling
-    [[clang::always_inline]] cnt += block_vertex<CAP, 2, true>(rp, col, x, S, wcol + wb,
+    [[clang::always_inline]] cnt += block_vertex<CAP, 2, true>(rp, col, x, S, wse + wb,
                                                               wrp[x + 1] - wb);
   }
   const WaveSlot W = wave_slot(S, wv);
@@ -671,7 +699,7 @@ void k_auto_wl(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col
     for (uint32_t i = blockIdx.x * (TB / 64) + wv; i < nwav; i += nw) {
       const uint32_t x = wav[i];
       const uint32_t wb = wrp[x];
-      cnt += wave_vertex<kFilterMin, 2, true>(rp, col, x, lane, W, wcol + wb, wrp[x + 1] - wb);
+      cnt += wave_vertex<kFilterMin, 2, true>(rp, col, x, lane, W, wse + wb, wrp[x + 1] - wb);
     }
   } else {
     for (bool first = true;; first = false) {
@@ -683,7 +711,7 @@ void k_auto_wl(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col
       for (uint32_t i = i0; i < i1; ++i) {
         const uint32_t x = wav[i];
         const uint32_t wb = wrp[x];
-        cnt += wave_vertex<kFilterMin, 2, true>(rp, col, x, lane, W, wcol + wb, wrp[x + 1] - wb);
+        cnt += wave_vertex<kFilterMin, 2, true>(rp, col, x, lane, W, wse + wb, wrp[x + 1] - wb);
       }
     }
   }

@@ -90,6 +90,7 @@ struct SchedCache {
   bool wl_ok = false;
   uint32_t wl_wmax = 0, wl_hmin = 0, wl_lmax = 0;
   DevBuf wl_rp, wl_col;            // uint32[n+1], uint32[entries]
+  DevBuf wl_se;                    // uint2[entries]: (rp[y], rp[y+1]) of wl_col's y
   DevBuf wl_small, wl_large, wl_huge, wl_lane;
   DevBuf wl_ell;                   // uint32[8 n]: out-lists of <= 8 entries, kEmpty-padded rows
   uint64_t nw_small = 0, nw_large = 0, nw_huge = 0, nw_lane = 0;

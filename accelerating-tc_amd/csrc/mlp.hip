@@ -271,7 +271,7 @@ template <int CAP>
 __global__ __launch_bounds__(TB) void k_time_wl_block(const uint32_t* __restrict__ rp,
                                                       const uint32_t* __restrict__ col,
                                                       const uint32_t* __restrict__ wrp,
-                                                      const uint32_t* __restrict__ wcol,
+                                                      const uint2* __restrict__ wse,
                                                       const uint32_t* __restrict__ verts,
                                                       uint32_t n, uint32_t* cursor,
                                                       float* __restrict__ cost,
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(TB) void k_time_wl_block(const uint32_t* __restrict
     const uint32_t t = S.t;
     if (t >= n) break;
     const uint32_t x = verts[t], wb = wrp[x];
-    cnt += block_vertex<CAP, 2, true>(rp, col, x, S, wcol + wb, wrp[x + 1] - wb);  // ends with a barrier
+    cnt += block_vertex<CAP, 2, true>(rp, col, x, S, wse + wb, wrp[x + 1] - wb);  // ends with a barrier
     if (tid == 0) cost[x] = 4.f * (float)(wall_clock64() - t0);
   }
   flush_count(cnt, total);
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(TB) void k_time_wl_block(const uint32_t* __restrict
 __global__ __launch_bounds__(TB) void k_time_wl_wave(const uint32_t* __restrict__ rp,
                                                      const uint32_t* __restrict__ col,
                                                      const uint32_t* __restrict__ wrp,
-                                                     const uint32_t* __restrict__ wcol,
+                                                     const uint2* __restrict__ wse,
                                                      const uint32_t* __restrict__ verts, uint32_t n,
                                                      uint32_t* cursor, float* __restrict__ cost,
                                                      unsigned long long* total) {
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(TB) void k_time_wl_wave(const uint32_t* __restrict_
     const uint32_t i1 = min(n, i0 + kWaveChunk);
     for (uint32_t i = i0; i < i1; ++i) {
       const uint32_t x = verts[i], wb = wrp[x];
-      cnt += wave_vertex<kFilterMin, 2, true>(rp, col, x, lane, W, wcol + wb, wrp[x + 1] - wb);
+      cnt += wave_vertex<kFilterMin, 2, true>(rp, col, x, lane, W, wse + wb, wrp[x + 1] - wb);
       const uint64_t t1 = wall_clock64();
       if (lane == 0) cost[x] = (float)(t1 - t0);
       t0 = t1;
@@ -693,21 +693,22 @@ int launch_lists(tc_ctx* c, tc_graph* g, const uint32_t* blk, uint64_t nb, const
   const uint32_t* col = g->col.as<uint32_t>();
   const uint32_t* wrp = g->sc.wl_rp.as<uint32_t>();
   const uint32_t* wcol = g->sc.wl_col.as<uint32_t>();
+  const uint2* wse = g->sc.wl_se.as<uint2>();
   const uint4* ell = g->sc.wl_ell.as<uint4>();
   if (nb || nw || nl) {
     const uint32_t G = (uint32_t)c->num_cus * 8u;
     if (nw <= (uint64_t)kStaticWaves * G * (TB / 64))
       hipLaunchKernelGGL((k_auto_wl<2048, true>), dim3(G), dim3(TB), 0, c->stream, rp, col, wrp,
-                         wcol, ell, blk, (uint32_t)nb, wav, (uint32_t)nw, lan, (uint32_t)nl, cursor,
+                         wcol, wse, ell, blk, (uint32_t)nb, wav, (uint32_t)nw, lan, (uint32_t)nl, cursor,
                          acc);
     else
       hipLaunchKernelGGL((k_auto_wl<2048>), dim3(G), dim3(TB), 0, c->stream, rp, col, wrp, wcol,
-                         ell, blk, (uint32_t)nb, wav, (uint32_t)nw, lan, (uint32_t)nl, cursor, acc);
+                         wse, ell, blk, (uint32_t)nb, wav, (uint32_t)nw, lan, (uint32_t)nl, cursor, acc);
     if (launches) ++*launches;
   }
   if (nh) {
     hipLaunchKernelGGL((k_auto_wl<8192>), dim3(std::min<uint64_t>((uint64_t)c->num_cus * 4, nh)),
-                       dim3(TB), 0, c->stream, rp, col, wrp, wcol, ell, huge, (uint32_t)nh,
+                       dim3(TB), 0, c->stream, rp, col, wrp, wcol, wse, ell, huge, (uint32_t)nh,
                        (const uint32_t*)nullptr, 0u, (const uint32_t*)nullptr, 0u, cursor + 4,
                        acc);
     if (launches) ++*launches;
@@ -1092,6 +1093,7 @@ int tc_graph_measure_costs(tc_ctx* c, tc_graph* g, int variant, float* cost_us, 
   const uint32_t* col = g->col.as<uint32_t>();
   const uint32_t* wrp = g->sc.wl_rp.as<uint32_t>();
   const uint32_t* wcol = g->sc.wl_col.as<uint32_t>();
+  const uint2* wse = g->sc.wl_se.as<uint2>();
   uint64_t nv = 0;
   const uint32_t G = (uint32_t)c->num_cus * 8u;
   auto select = [&](uint32_t lo, uint32_t hi) -> int {
@@ -1103,18 +1105,18 @@ int tc_graph_measure_costs(tc_ctx* c, tc_graph* g, int variant, float* cost_us, 
   if (variant == TC_VARIANT_BLOCK) {
     TCX_TRY(select(2u, 2048u));
     if (nv)
-      hipLaunchKernelGGL(k_time_wl_block<2048>, dim3(G), dim3(TB), 0, c->stream, rp, col, wrp, wcol,
+      hipLaunchKernelGGL(k_time_wl_block<2048>, dim3(G), dim3(TB), 0, c->stream, rp, col, wrp, wse,
                          verts.as<uint32_t>(), (uint32_t)nv, cursor, cost.as<float>(), acc);
     TCX_TRY(select(2049u, 0xFFFFFFFFu));
     if (nv)
       hipLaunchKernelGGL(k_time_wl_block<8192>,
                          dim3(std::min<uint64_t>(nv, (uint64_t)c->num_cus * 4)), dim3(TB), 0,
-                         c->stream, rp, col, wrp, wcol, verts.as<uint32_t>(), (uint32_t)nv,
+                         c->stream, rp, col, wrp, wse, verts.as<uint32_t>(), (uint32_t)nv,
                          cursor + 1, cost.as<float>(), acc);
   } else if (variant == TC_VARIANT_WAVE) {
     TCX_TRY(select(2u, kWaveMax));
     if (nv)
-      hipLaunchKernelGGL(k_time_wl_wave, dim3(G), dim3(TB), 0, c->stream, rp, col, wrp, wcol,
+      hipLaunchKernelGGL(k_time_wl_wave, dim3(G), dim3(TB), 0, c->stream, rp, col, wrp, wse,
                          verts.as<uint32_t>(), (uint32_t)nv, cursor, cost.as<float>(), acc);
   } else {
     TCX_TRY(select(2u, kLaneMaxDeg));

@@ -297,7 +297,7 @@ def main_batched(args, w):
             "frac": achieved / HBM_PEAK_GBPS, "traffic": load_traffic("batched", "batched"),
             "kernel": "k_batch<2048>", "bytes_per_count": kb,
             "bytes_rule": "work-list form: tc_batch_stream_bytes (per task 20 + 4 d+(x), per "
-                          "work-list entry 12 + 4 d+(y)); push form: 4(n+1) + 12 m+ + "
+                          "work-list entry 8 + 4 d+(y)); push form: 4(n+1) + 12 m+ + "
                           "4 sum d- d+",
             "bytes_push_form": kernel_bytes(info),
             "bytes_alg_edge_model": info.bytes_alg,
@@ -590,8 +590,8 @@ def triangle_roofline(info, hr, traffic, stream):
         "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic, "kernel": "k_auto_wl<2048>",
         "bytes_per_count": stream,
         "bytes_rule": "tc_auto_stream_bytes: per task x, 20 + 4 d+(x) (task id, row bounds in "
-                      "both CSRs, N+(x)); per work-list entry y, 12 + 4 d+(y) (id, row bounds, "
-                      "N+(y))",
+                      "both CSRs, N+(x)); per work-list entry y, 8 + 4 d+(y) (its row bounds, "
+                      "stored with the entry, and N+(y))",
         "bytes_push_form": kernel_bytes(info),
         "bytes_alg_edge_model": info.bytes_alg,
     }

@@ -348,8 +348,8 @@ int tc_count(tc_ctx* ctx, tc_graph* g, const tc_sched* sched, uint64_t* count,
              tc_timing* timing);
 /* Bytes one AUTO count streams under sched (NULL: defaults): for
  * TC_AUTO_WORKLIST the sum over tasks x of 20 + 4 d+(x) (task id, row bounds
- * in both CSRs, N+(x)) and over the work list of x of 12 + 4 d+(y) (id, row
- * bounds, N+(y)); for TC_AUTO_PUSH 4(n+1) + 12 m+ + 4 sum_v d-(v) d+(v).
+ * in both CSRs, N+(x)) and over the work list of x of 8 + 4 d+(y) (the row
+ * bounds of y, stored with the entry, and N+(y)); for TC_AUTO_PUSH 4(n+1) + 12 m+ + 4 sum_v d-(v) d+(v).
  * Builds the schedule when needed.  No counterpart in the reference. */
 int tc_auto_stream_bytes(tc_ctx* ctx, tc_graph* g, const tc_sched* sched, double* bytes);
 /* Per-vertex forward counts tri[u] (n entries): the triangles whose first

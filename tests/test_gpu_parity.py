@@ -578,7 +578,8 @@ def worklist_bytes(rp, col):
     """Bytes of the work-list schedule, recounted from a CSR: each edge u->v
     goes to the task of u when d+(v) <= d+(u), else of v; it is dropped when
     v has no out-edge or the owner u has d+(u) < 2.  A task x (d+(x) >= 2,
-    non-empty list) reads 20 + 4 d+(x) bytes, each list entry y 12 + 4 d+(y)."""
+    non-empty list) reads 20 + 4 d+(x) bytes, each list entry y 8 + 4 d+(y)
+    (the row bounds of y, stored with the entry, and N+(y))."""
     n = len(rp) - 1
     dp = np.diff(rp.astype(np.int64))
     src = np.repeat(np.arange(n), dp)
@@ -590,7 +591,7 @@ def worklist_bytes(rp, col):
     nw = np.bincount(owner, minlength=n)
     tasks = (nw > 0) & (dp >= 2)
     assert nw[~tasks].sum() == 0  # every kept entry belongs to a runnable task
-    return int((20 + 4 * dp[tasks]).sum() + (12 + 4 * dp[streamed]).sum())
+    return int((20 + 4 * dp[tasks]).sum() + (8 + 4 * dp[streamed]).sum())
 
 
 @pytest.mark.parametrize("case", list(CASES))
