@@ -499,31 +499,34 @@ int tc_batch_autotune(tc_ctx* c, tc_batch* b, tc_batch_sched* best) {
     *ms = acc / 2;
     return TC_OK;
   };
-  tc_batch_sched s = fill(nullptr, c);
+  // per form: the wave/block degree split, then the filter threshold and the
+  // wave-task chunk, each stage keeping the fastest so far; the faster form
+  // wins
   float best_ms = 1e30f;
-  tc_batch_sched bs = s;
+  tc_batch_sched bs = fill(nullptr, c);
   for (uint32_t form : {(uint32_t)TC_BATCH_WORKLIST, (uint32_t)TC_BATCH_PUSH}) {
+    tc_batch_sched s = fill(nullptr, c), fb = s;
+    float fms = 1e30f;
+    auto trial = [&](const tc_batch_sched& t) -> int {
+      float ms;
+      TCX_TRY(time_of(t, &ms));
+      if (ms < fms) { fms = ms; fb = t; }
+      return TC_OK;
+    };
     s.form = form;
     for (uint32_t d : {16u, 32u, 64u, 128u, 256u, 512u}) {
       s.wave_max_deg = d;
-      float ms;
-      TCX_TRY(time_of(s, &ms));
-      if (ms < best_ms) { best_ms = ms; bs = s; }
+      TCX_TRY(trial(s));
     }
-  }
-  s = bs;
-  s.filter_min = 1;
-  {
-    float ms;
-    TCX_TRY(time_of(s, &ms));
-    if (ms < best_ms) { best_ms = ms; bs = s; }
-  }
-  s = bs;
-  for (uint32_t ch : {4u, 8u, 32u, 64u}) {
-    s.wave_chunk = ch;
-    float ms;
-    TCX_TRY(time_of(s, &ms));
-    if (ms < best_ms) { best_ms = ms; bs = s; }
+    s = fb;
+    s.filter_min = 1;
+    TCX_TRY(trial(s));
+    s = fb;
+    for (uint32_t ch : {4u, 8u, 32u, 64u}) {
+      s.wave_chunk = ch;
+      TCX_TRY(trial(s));
+    }
+    if (fms < best_ms) { best_ms = fms; bs = fb; }
   }
   *best = bs;
   return TC_OK;

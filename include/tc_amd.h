@@ -542,8 +542,10 @@ int tc_batch_info_get(const tc_batch* b, tc_graph_info* info);
  * bytes of the union CSR (tc_auto_stream_bytes), whatever the task split.
  * The push form streams 4(n+1) + 12 m+ + 4 sum d- d+ (tc_batch_info_get). */
 int tc_batch_stream_bytes(tc_ctx* ctx, tc_batch* b, double* bytes);
-/* Tries both forms x wave_max_deg, then filter_min, then wave_chunk
- * candidates on the batch and returns the fastest (timed with hipEvents). */
+/* For each form, tries the wave_max_deg candidates, then filter_min, then
+ * the wave_chunk candidates (each stage from the fastest so far) on the
+ * batch, and returns the fastest schedule of either form (timed with
+ * hipEvents). */
 int tc_batch_autotune(tc_ctx* ctx, tc_batch* b, tc_batch_sched* best);
 /* Exact per-graph counts (ngraphs entries) from one replay of the captured
  * hipGraph (capture happens on the first call per schedule). */
