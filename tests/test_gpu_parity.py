@@ -312,6 +312,27 @@ def test_schedule_variants(ctx, case, sched):
         g.close()
 
 
+@pytest.mark.parametrize("width", [3, 8, 9])
+def test_lane_rows_full_and_partial(ctx, width):
+    # circulant graph C_n(1..width) under the ID orientation: almost every
+    # vertex has d+ = width and a work list of width entries, so with width
+    # 8 the lane tasks read full 32-byte rows (no padding), with 3 mostly
+    # padding, and with 9 no task runs as a lane
+    n = 3000
+    i = np.arange(n, dtype=np.uint32)
+    s = np.concatenate([i] * width)
+    d = np.concatenate([(i + k) % n for k in range(1, width + 1)]).astype(np.uint32)
+    rp_o, col_o = oracle_csr(s, d, "id")
+    expect = orc.count(rp_o, col_o)
+    print(f"width={width} expect={expect}", flush=True)
+    assert np.median(np.diff(rp_o)) == width  # the wrap-around vertices hold up to 2 width
+    g = ctx.graph(s, d).orient(tcamd.ORIENT_ID)
+    for sched in (dict(), dict(auto_thresholds=[8, 8, 0, 8], grid_size=1),
+                  dict(auto_thresholds=[2, 2, 0, 4]), dict(auto_thresholds=[64, 2048, 0, 1])):
+        assert g.count("auto", **sched).count == expect, sched
+    g.close()
+
+
 def test_auto_hub_beyond_lds(ctx):
     """A row longer than the 8192-entry LDS slot is searched in global memory."""
     n = 12000
