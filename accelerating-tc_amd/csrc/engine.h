@@ -652,9 +652,347 @@ __device__ inline uint32_t lane_task(const uint4* __restrict__ ell,
   return c;
 }
 
+// ---- Rank-indexed membership (the work-list kernels) -----------------------
+// N+(x) is hashed (fhash) into a bitmap whose words sit in `bw` as pairs
+// (bits, prefix): .y holds the number of set bits in the words before
+// (bits 0..30) and, in bit 31, a flag for a word in which two members share
+// a bit.  The member whose hash is the r-th set bit is stored at H[r], and
+// H[total] = kEmpty - 1 (no id, and not the padding kEmpty).  An entry y is
+// a member iff H[popcount(bits below y's bit) + prefix] == y: one 8-byte and
+// one 4-byte LDS read and a handful of vector instructions, with no queue
+// and no search.  A colliding member may have lost its H slot, so a miss in
+// a flagged word is queued and resolved by a binary search of N+(x) in
+// global memory (sorted), 64 at a time.  The flag bit drops out of the H
+// address (r << 2).
+
+// LDS of the work-list kernel: one table region, seen as the workgroup's
+// (block tasks: CAP members, CAP * 16 bits) or as four wave slots (wave
+// tasks: CAP / 4 members, CAP * 4 bits each).
+template <int CAP>
+struct RankLds {
+  static constexpr uint32_t kWords = CAP / 2;                  // block bitmap words
+  static constexpr uint32_t kWaveWords = CAP / 8;              // wave slot bitmap words
+  static constexpr uint32_t kWaveH = CAP / 4 + 2;              // >= CAP / 4 + 1, even
+  static constexpr uint32_t kSlot = 2 * kWaveWords + kWaveH;   // words per wave slot
+  static constexpr uint32_t kRegion = 4 * kSlot;               // words
+  static_assert(kRegion >= 2 * kWords + CAP + 1, "the block view needs CAP + 1 H entries");
+  static_assert(CAP / 4 >= (int)kWaveMax, "wave slots must hold kWaveMax members");
+  static constexpr uint32_t kQ = 96;                           // secondary queue per wave
+  alignas(16) uint32_t region[kRegion];
+  uint32_t f[TB / 64][128];
+  uint32_t q[TB / 64][kQ];
+  uint32_t wsum[TB / 64];
+  uint32_t t;
+};
+
+struct RankSlot {
+  uint2* bw;
+  uint32_t* H;
+  uint32_t nwm;   // bitmap words - 1
+  uint32_t* F;
+  uint32_t* Q;
+};
+
+template <int CAP>
+__device__ inline RankSlot rank_block(RankLds<CAP>& S, uint32_t wv) {
+  return RankSlot{reinterpret_cast<uint2*>(S.region), S.region + 2 * RankLds<CAP>::kWords,
+                  RankLds<CAP>::kWords - 1, S.f[wv], S.q[wv]};
+}
+
+template <int CAP>
+__device__ inline RankSlot rank_wave(RankLds<CAP>& S, uint32_t wv) {
+  uint32_t* b = S.region + wv * RankLds<CAP>::kSlot;
+  return RankSlot{reinterpret_cast<uint2*>(b), b + 2 * RankLds<CAP>::kWaveWords,
+                  RankLds<CAP>::kWaveWords - 1, S.f[wv], S.q[wv]};
+}
+
+__device__ inline void rank_insert(uint2* bw, uint32_t nwm, uint32_t x) {
+  const uint32_t f = x ^ (x >> 15);
+  const uint32_t wi = (f >> 5) & nwm, bit = 1u << (f & 31);
+  const uint32_t old = atomicOr(&bw[wi].x, bit);
+  if (old & bit) atomicOr(&bw[wi].y, 0x80000000u);
+}
+
+__device__ inline void rank_place(const uint2* bw, uint32_t* H, uint32_t nwm, uint32_t x) {
+  const uint32_t f = x ^ (x >> 15);
+  const uint2 e = bw[(f >> 5) & nwm];
+  H[__builtin_popcount(__builtin_amdgcn_ubfe(e.x, 0u, f)) + (e.y & 0x7FFFFFFFu)] = x;
+}
+
+__device__ inline void rank_clear(uint2* bw, uint32_t nwm, uint32_t x) {
+  const uint32_t f = x ^ (x >> 15);
+  bw[(f >> 5) & nwm] = make_uint2(0u, 0u);
+}
+
+// Prefix of the set bits over the NW words of bw (the whole workgroup; K
+// consecutive words per thread); returns the total.  Ends with a barrier.
+template <uint32_t NW>
+__device__ inline uint32_t rank_scan_block(uint2* bw, uint32_t* wsum) {
+  constexpr uint32_t K = NW / TB;
+  static_assert(K * TB == NW, "words per thread");
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  uint32_t c[K], s = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < K; ++k) {
+    c[k] = (uint32_t)__builtin_popcount(bw[tid * K + k].x);
+    s += c[k];
+  }
+  const uint32_t incl = wave_incl_scan(s, lane);
+  if (lane == 63) wsum[wv] = incl;
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < TB / 64; ++w) {
+    const uint32_t t = wsum[w];
+    before += w < wv ? t : 0u;
+    total += t;
+  }
+  uint32_t p = before + incl - s;
+#pragma unroll
+  for (uint32_t k = 0; k < K; ++k) {
+    bw[tid * K + k].y = (bw[tid * K + k].y & 0x80000000u) | p;
+    p += c[k];
+  }
+  __syncthreads();
+  return total;
+}
+
+// Per-wave probe state (all fields wave-uniform).
+struct RProbe {
+  const uint2* bw;
+  const uint32_t* H;
+  uint32_t amask;            // byte offsets of bw words: (nwm << 3)
+  uint32_t* Q;               // secondary queue (RankLds::kQ entries)
+  uint32_t qn;
+  const uint32_t* sa;        // N+(x) in global memory, ascending
+  uint32_t sn;
+  uint32_t hits;
+};
+
+// Resolves up to 64 queued entries by binary search in global memory.
+__device__ inline void rank_resolve(RProbe& P, uint32_t lane) {
+  wave_sync();
+  const uint32_t take = P.qn < 64u ? P.qn : 64u;
+  const uint32_t y = lane < take ? P.Q[P.qn - take + lane] : kEmpty;
+  const bool hit = y != kEmpty && contains(P.sa, 0, P.sn, y);
+  P.hits = uniform(P.hits + (uint32_t)__popcll(lanes(hit)));
+  P.qn = uniform(P.qn - take);
+  wave_sync();
+}
+
+// N entries (kEmpty: none) probed together: the table reads of all N are
+// issued before their compares.
+template <int N>
+__device__ inline void rank_probe(RProbe& P, const uint32_t (&x)[N], uint32_t lane) {
+  uint32_t f[N], h[N];
+  uint2 e[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    f[i] = x[i] ^ (x[i] >> 15);
+    e[i] = *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(P.bw) +
+                                           ((f[i] >> 2) & P.amask));
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const uint32_t r = (uint32_t)__builtin_popcount(__builtin_amdgcn_ubfe(e[i].x, 0u, f[i])) + e[i].y;
+    h[i] = *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(P.H) + (r << 2));
+  }
+  uint32_t hs = 0;  // scalar: the hits of the N entries
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const uint64_t hm = lanes(h[i] == x[i]);
+    // misses on a set bit of a flagged word (an unset bit holds no member)
+    const uint64_t sm = lanes((int)e[i].y < 0) & lanes(__builtin_amdgcn_ubfe(e[i].x, f[i], 1u) != 0u) & ~hm;
+    hs += (uint32_t)__popcll(hm);
+    if (sm) {
+      if ((sm >> lane) & 1ull) P.Q[lanes_below(sm, P.qn)] = x[i];
+      P.qn = uniform(P.qn + (uint32_t)__popcll(sm));
+      if (P.qn >= 32) rank_resolve(P, lane);
+    }
+  }
+  P.hits = uniform(P.hits + hs);
+}
+
+__device__ inline void rank_finish(RProbe& P, uint32_t lane) {
+  while (P.qn) rank_resolve(P, lane);
+}
+
+// stream_many with rank probes.
+__device__ inline void rank_stream_many(const uint32_t* __restrict__ col, uint64_t mask,
+                                        uint32_t vb, uint32_t ve, uint32_t lane, RProbe& P) {
+  uint32_t pos = 0, end = 0;
+  auto next_list = [&]() {
+    const uint32_t j = (uint32_t)__builtin_ctzll(mask);
+    mask &= mask - 1;
+    pos = (uint32_t)__builtin_amdgcn_readlane((int)vb, (int)j);
+    end = (uint32_t)__builtin_amdgcn_readlane((int)ve, (int)j);
+  };
+  if (mask) next_list();
+  while (pos < end) {
+    uint32_t w[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const uint32_t k = pos + lane;  // no wrap: CSRs hold fewer than 2^32 - 512 entries
+      w[s] = k < end ? col[k] : kEmpty;
+      pos += 64;
+      if (pos >= end) {
+        if (mask)
+          next_list();
+        else
+          pos = end = 0;
+      }
+    }
+    const uint32_t a[4] = {w[0], w[1], w[2], w[3]};
+    rank_probe<4>(P, a, lane);
+    const uint32_t b[4] = {w[4], w[5], w[6], w[7]};
+    rank_probe<4>(P, b, lane);
+  }
+}
+
+// lists64 with rank probes (long lists back to back, short ones
+// concatenated across lanes through F).
+__device__ inline void rank_lists64(const uint32_t* __restrict__ col, uint32_t vb, uint32_t ve,
+                                    uint32_t lane, uint32_t* F, RProbe& P) {
+  const uint32_t len = ve - vb;
+  const bool lng = len >= kFlatMin;
+  const uint64_t mask = lanes(lng);
+  if (mask) rank_stream_many(col, mask, vb, ve, lane, P);
+  const uint32_t sl = lng ? 0u : len;
+  const uint32_t incl = wave_incl_scan(sl, lane);
+  const uint32_t total = __shfl(incl, 63, 64);
+  if (total == 0) return;
+  F[lane] = incl - sl;
+  F[64 + lane] = vb;
+  wave_sync();
+  for (uint32_t f0 = 0; f0 < total; f0 += 128) {
+    const uint32_t fa = f0 + lane, fb = fa + 64;
+    uint32_t xa = kEmpty, xb = kEmpty;
+    if (fa < total) {
+      const uint32_t j = seg64(F, fa);
+      xa = col[F[64 + j] + (fa - F[j])];
+    }
+    if (fb < total) {
+      const uint32_t j = seg64(F, fb);
+      xb = col[F[64 + j] + (fb - F[j])];
+    }
+    const uint32_t a[2] = {xa, xb};
+    rank_probe<2>(P, a, lane);
+  }
+  wave_sync();
+}
+
+// Block task x (d+(x) <= CAP): the workgroup builds the table of N+(x) and
+// wave w streams the lists of work-list entries w, w+4, ...  Every thread
+// must call it; it ends with the table cleared and a barrier.  Returns the
+// count in lane 0 of each wave (0 elsewhere).
+template <int CAP>
+__device__ inline uint32_t block_rank(const uint32_t* __restrict__ rp,
+                                      const uint32_t* __restrict__ col, uint32_t x,
+                                      RankLds<CAP>& S, const uint2* __restrict__ wl,
+                                      uint32_t wn) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const RankSlot T = rank_block(S, wv);
+  const uint32_t ub = rp[x], du = uniform(rp[x + 1] - ub);
+  for (uint32_t i = tid; i < du; i += TB) rank_insert(T.bw, T.nwm, col[ub + i]);
+  __syncthreads();
+  const uint32_t total = rank_scan_block<RankLds<CAP>::kWords>(T.bw, S.wsum);
+  if (tid == 0) T.H[total] = kEmpty - 1;
+  for (uint32_t i = tid; i < du; i += TB) rank_place(T.bw, T.H, T.nwm, col[ub + i]);
+  __syncthreads();
+  RProbe P{T.bw, T.H, T.nwm << 3, T.Q, 0u, col + ub, du, 0u};
+  const uint32_t mine = wn > wv ? (wn - wv + 3) / 4 : 0;
+  for (uint32_t g0 = 0; g0 < mine; g0 += 64) {
+    uint32_t vb = 0, ve = 0;
+    if (g0 + lane < mine) {
+      const uint2 b = wl[4 * (g0 + lane) + wv];
+      vb = b.x;
+      ve = b.y;
+    }
+    rank_lists64(col, vb, ve, lane, T.F, P);
+  }
+  rank_finish(P, lane);
+  __syncthreads();
+  for (uint32_t i = tid; i < total; i += TB) rank_clear(T.bw, T.nwm, T.H[i]);
+  __syncthreads();
+  return lane == 0 ? P.hits : 0u;
+}
+
+// Block task whose N+(x) exceeds the table (d+(x) > CAP): every streamed
+// entry is searched in N+(x) in global memory.  Per-lane counts.
+__device__ inline uint32_t block_global(const uint32_t* __restrict__ rp,
+                                        const uint32_t* __restrict__ col, uint32_t x,
+                                        const uint2* __restrict__ wl, uint32_t wn) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const uint32_t ub = rp[x], ue = rp[x + 1];
+  uint32_t c = 0;
+  for (uint32_t j = wv; j < wn; j += TB / 64) {
+    const uint2 b = wl[j];
+    for (uint32_t k = b.x + lane; k < b.y; k += 64) c += contains(col, ub, ue, col[k]) ? 1u : 0u;
+  }
+  return c;
+}
+
+// Prefix of the set bits over the first nw words of a wave slot (nw a
+// multiple of 64, wave-uniform); returns the total (wave-uniform).
+__device__ inline uint32_t rank_scan_wave(uint2* bw, uint32_t nw, uint32_t lane) {
+  const uint32_t K = nw >> 6;
+  uint32_t s = 0;
+  for (uint32_t k = 0; k < K; ++k) s += (uint32_t)__builtin_popcount(bw[lane * K + k].x);
+  const uint32_t incl = wave_incl_scan(s, lane);
+  const uint32_t total = uniform(__shfl(incl, 63, 64));
+  uint32_t p = incl - s;
+  for (uint32_t k = 0; k < K; ++k) {
+    const uint2 e = bw[lane * K + k];
+    bw[lane * K + k].y = (e.y & 0x80000000u) | p;
+    p += (uint32_t)__builtin_popcount(e.x);
+  }
+  return total;
+}
+
+// Wave task x (d+(x) <= kWaveMax) in the wave's slot.  The table takes
+// about 16 bits per member (64 words and up, the slot's at most), so the
+// prefix pass of a short task is one word per lane.  Returns the count in
+// lane 0 (0 elsewhere).
+template <int CAP>
+__device__ inline uint32_t wave_rank(const uint32_t* __restrict__ rp,
+                                     const uint32_t* __restrict__ col, uint32_t x, uint32_t lane,
+                                     const RankSlot& T, const uint2* __restrict__ wl,
+                                     uint32_t wn) {
+  static_assert(RankLds<CAP>::kWaveWords >= 64, "at least one word per lane");
+  const uint32_t ub = rp[x], du = uniform(rp[x + 1] - ub);
+  uint32_t nw = 64;
+  while (nw < RankLds<CAP>::kWaveWords && 2 * nw < du) nw <<= 1;
+  const uint32_t nwm = nw - 1;
+  for (uint32_t i = lane; i < du; i += 64) rank_insert(T.bw, nwm, col[ub + i]);
+  wave_sync();
+  const uint32_t total = rank_scan_wave(T.bw, nw, lane);
+  if (lane == 0) T.H[total] = kEmpty - 1;
+  wave_sync();
+  for (uint32_t i = lane; i < du; i += 64) rank_place(T.bw, T.H, nwm, col[ub + i]);
+  wave_sync();
+  RProbe P{T.bw, T.H, nwm << 3, T.Q, 0u, col + ub, du, 0u};
+  for (uint32_t j0 = 0; j0 < wn; j0 += 64) {
+    uint32_t vb = 0, ve = 0;
+    if (j0 + lane < wn) {
+      const uint2 b = wl[j0 + lane];
+      vb = b.x;
+      ve = b.y;
+    }
+    rank_lists64(col, vb, ve, lane, T.F, P);
+  }
+  rank_finish(P, lane);
+  wave_sync();
+  for (uint32_t i = lane; i < total; i += 64) rank_clear(T.bw, nwm, T.H[i]);
+  wave_sync();
+  return lane == 0 ? P.hits : 0u;
+}
+
 // The AUTO count kernel over work lists (the default schedule): task x
-// stages N+(x) as k_auto does and streams N+(y) for y in W(x) =
-// wcol[wrp[x], wrp[x+1]).  Each oriented edge u->v sits in exactly one work
+// builds the rank-indexed table of N+(x) in LDS (block tasks: the
+// workgroup's, d+(x) <= CAP; wave tasks: the wave's slot) and streams N+(y)
+// for y in W(x) = wcol[wrp[x], wrp[x+1]), each entry decided by one table
+// lookup.  A block task whose N+(x) exceeds CAP searches N+(x) in global
+// memory instead.  Each oriented edge u->v sits in exactly one work
 // list, at the endpoint whose task streams the shorter out-list (u when
 // d+(v) <= d+(u), else v), so the sum over tasks of |N+(x) & N+(y)| is the
 // triangle count; edges into a vertex with no out-edge are left out.
@@ -665,8 +1003,8 @@ __device__ inline uint32_t lane_task(const uint4* __restrict__ ell,
 // cursor[2], from the 32-byte rows `ell` (see lane_task; unused when nlan is
 // zero).
 // Eight waves per SIMD (the persistent grid's eight workgroups per CU): the
-// register budget is held at 64 VGPRs so the occupancy survives the
-// unrolled streams.
+// register budget is held at 64 VGPRs and the LDS (RankLds<2048>) at 20 KB
+// per workgroup so the occupancy survives the unrolled streams.
 template <int CAP, bool STATIC_WAVES = false>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void k_auto_wl(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
@@ -676,10 +1014,11 @@ void k_auto_wl(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col
                const uint32_t* __restrict__ wav, uint32_t nwav,
                const uint32_t* __restrict__ lan, uint32_t nlan,
                uint32_t* cursor, unsigned long long* total) {
-  __shared__ BlockLds<CAP> S;
+  __shared__ RankLds<CAP> S;
   __shared__ unsigned long long red[TB / 64];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  clear_bitmap(S);
+  for (uint32_t i = tid; i < RankLds<CAP>::kRegion; i += TB) S.region[i] = 0;
+  __syncthreads();
   unsigned long long cnt = 0;
   for (bool first = true;; first = false) {
     if (tid == 0) S.t = claim(cursor, nblk, 1u, first);
@@ -687,19 +1026,24 @@ void k_auto_wl(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col
     const uint32_t t = S.t;
     if (t >= nblk) break;
     const uint32_t x = blk[t];
-    const uint32_t wb = wrp[x];
-    // inlined: the workgroup's LDS stays a known address (no generic
-    // pointer, no call) and the kernel fits 64 VGPRs without spilling
-    [[clang::always_inline]] cnt += block_vertex<CAP, 2, true>(rp, col, x, S, wse + wb,
-                                                              wrp[x + 1] - wb);
+    const uint32_t wb = wrp[x], wn = wrp[x + 1] - wb;
+    if (uniform(rp[x + 1] - rp[x]) <= (uint32_t)CAP) {
+      [[clang::always_inline]] cnt += block_rank<CAP>(rp, col, x, S, wse + wb, wn);
+    } else {
+      cnt += block_global(rp, col, x, wse + wb, wn);
+      __syncthreads();  // every wave has read S.t before it is claimed again
+    }
   }
-  const WaveSlot W = wave_slot(S, wv);
+  // the wave slots overlap the block view's table: start them from zero
+  for (uint32_t i = tid; i < RankLds<CAP>::kRegion; i += TB) S.region[i] = 0;
+  __syncthreads();
+  const RankSlot W = rank_wave(S, wv);
   if constexpr (STATIC_WAVES) {
     const uint32_t nw = gridDim.x * (TB / 64);
     for (uint32_t i = blockIdx.x * (TB / 64) + wv; i < nwav; i += nw) {
       const uint32_t x = wav[i];
       const uint32_t wb = wrp[x];
-      cnt += wave_vertex<kFilterMin, 2, true>(rp, col, x, lane, W, wse + wb, wrp[x + 1] - wb);
+      cnt += wave_rank<CAP>(rp, col, x, lane, W, wse + wb, wrp[x + 1] - wb);
     }
   } else {
     for (bool first = true;; first = false) {
@@ -711,7 +1055,7 @@ void k_auto_wl(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col
       for (uint32_t i = i0; i < i1; ++i) {
         const uint32_t x = wav[i];
         const uint32_t wb = wrp[x];
-        cnt += wave_vertex<kFilterMin, 2, true>(rp, col, x, lane, W, wse + wb, wrp[x + 1] - wb);
+        cnt += wave_rank<CAP>(rp, col, x, lane, W, wse + wb, wrp[x + 1] - wb);
       }
     }
   }

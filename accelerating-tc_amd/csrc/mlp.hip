@@ -276,9 +276,10 @@ __global__ __launch_bounds__(TB) void k_time_wl_block(const uint32_t* __restrict
                                                       uint32_t n, uint32_t* cursor,
                                                       float* __restrict__ cost,
                                                       unsigned long long* total) {
-  __shared__ BlockLds<CAP> S;
+  __shared__ RankLds<CAP> S;
   const uint32_t tid = threadIdx.x;
-  clear_bitmap(S);
+  for (uint32_t i = tid; i < RankLds<CAP>::kRegion; i += TB) S.region[i] = 0;
+  __syncthreads();
   unsigned long long cnt = 0;
   for (;;) {
     // the task's cost includes fetching it (cursor atomic + barrier)
@@ -290,8 +291,13 @@ __global__ __launch_bounds__(TB) void k_time_wl_block(const uint32_t* __restrict
     __syncthreads();
     const uint32_t t = S.t;
     if (t >= n) break;
-    const uint32_t x = verts[t], wb = wrp[x];
-    cnt += block_vertex<CAP, 2, true>(rp, col, x, S, wse + wb, wrp[x + 1] - wb);  // ends with a barrier
+    const uint32_t x = verts[t], wb = wrp[x], wn = wrp[x + 1] - wb;
+    if (uniform(rp[x + 1] - rp[x]) <= (uint32_t)CAP) {
+      cnt += block_rank<CAP>(rp, col, x, S, wse + wb, wn);  // ends with a barrier
+    } else {
+      cnt += block_global(rp, col, x, wse + wb, wn);
+      __syncthreads();
+    }
     if (tid == 0) cost[x] = 4.f * (float)(wall_clock64() - t0);
   }
   flush_count(cnt, total);
@@ -304,10 +310,11 @@ __global__ __launch_bounds__(TB) void k_time_wl_wave(const uint32_t* __restrict_
                                                      const uint32_t* __restrict__ verts, uint32_t n,
                                                      uint32_t* cursor, float* __restrict__ cost,
                                                      unsigned long long* total) {
-  __shared__ BlockLds<2048> S;
+  __shared__ RankLds<2048> S;
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  clear_bitmap(S);
-  const WaveSlot W = wave_slot(S, wv);
+  for (uint32_t i = threadIdx.x; i < RankLds<2048>::kRegion; i += TB) S.region[i] = 0;
+  __syncthreads();
+  const RankSlot W = rank_wave(S, wv);
   unsigned long long cnt = 0;
   // fetched kWaveChunk at a time, as the count kernel does; each task's cost
   // carries its share of the fetch
@@ -320,7 +327,7 @@ __global__ __launch_bounds__(TB) void k_time_wl_wave(const uint32_t* __restrict_
     const uint32_t i1 = min(n, i0 + kWaveChunk);
     for (uint32_t i = i0; i < i1; ++i) {
       const uint32_t x = verts[i], wb = wrp[x];
-      cnt += wave_vertex<kFilterMin, 2, true>(rp, col, x, lane, W, wse + wb, wrp[x + 1] - wb);
+      cnt += wave_rank<2048>(rp, col, x, lane, W, wse + wb, wrp[x + 1] - wb);
       const uint64_t t1 = wall_clock64();
       if (lane == 0) cost[x] = (float)(t1 - t0);
       t0 = t1;
