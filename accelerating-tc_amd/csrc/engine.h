@@ -963,12 +963,18 @@ __device__ inline uint32_t wave_rank(const uint32_t* __restrict__ rp,
   uint32_t nw = 64;
   while (nw < RankLds<CAP>::kWaveWords && 2 * nw < du) nw <<= 1;
   const uint32_t nwm = nw - 1;
-  for (uint32_t i = lane; i < du; i += 64) rank_insert(T.bw, nwm, col[ub + i]);
+  // the first 128 members are kept in F (free until the stream) for the
+  // place pass: a short task reads N+(x) from memory once
+  for (uint32_t i = lane; i < du; i += 64) {
+    const uint32_t v = col[ub + i];
+    if (i < 128) T.F[i] = v;
+    rank_insert(T.bw, nwm, v);
+  }
   wave_sync();
   const uint32_t total = rank_scan_wave(T.bw, nw, lane);
   if (lane == 0) T.H[total] = kEmpty - 1;
   wave_sync();
-  for (uint32_t i = lane; i < du; i += 64) rank_place(T.bw, T.H, nwm, col[ub + i]);
+  for (uint32_t i = lane; i < du; i += 64) rank_place(T.bw, T.H, nwm, i < 128 ? T.F[i] : col[ub + i]);
   wave_sync();
   RProbe P{T.bw, T.H, nwm << 3, T.Q, 0u, col + ub, du, 0u};
   for (uint32_t j0 = 0; j0 < wn; j0 += 64) {
