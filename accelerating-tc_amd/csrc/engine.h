@@ -183,6 +183,22 @@ __device__ inline uint32_t wave_incl_scan(uint32_t x, uint32_t lane) {
   return x;
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave in six DPP adds (no LDS
+// round trips): shifts by 1 and 2 within each 16-lane row give every lane
+// the sum of its last four; shifts by 4 and 8 (on the banks above the
+// first one and two) complete each row; the last lane of rows 0 and 2 is
+// then added to rows 1 and 3, and that of row 1 to rows 2 and 3.  Lanes
+// whose source is disabled or outside the row add 0.
+__device__ inline uint32_t wave_incl_scan_dpp(uint32_t x) {
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xe, false);  // row_shr:4
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xc, false);  // row_shr:8
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return x;
+}
+
 // Number of set bits of m below this lane, plus q.
 __device__ inline uint32_t lanes_below(uint64_t m, uint32_t q = 0u) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
@@ -737,7 +753,7 @@ __device__ inline uint32_t rank_scan_block(uint2* bw, uint32_t* wsum) {
     c[k] = (uint32_t)__builtin_popcount(bw[tid * K + k].x);
     s += c[k];
   }
-  const uint32_t incl = wave_incl_scan(s, lane);
+  const uint32_t incl = wave_incl_scan_dpp(s);
   if (lane == 63) wsum[wv] = incl;
   __syncthreads();
   uint32_t before = 0, total = 0;
@@ -858,7 +874,7 @@ __device__ inline void rank_lists64(const uint32_t* __restrict__ col, uint32_t v
   const uint64_t mask = lanes(lng);
   if (mask) rank_stream_many(col, mask, vb, ve, lane, P);
   const uint32_t sl = lng ? 0u : len;
-  const uint32_t incl = wave_incl_scan(sl, lane);
+  const uint32_t incl = wave_incl_scan_dpp(sl);
   const uint32_t total = __shfl(incl, 63, 64);
   if (total == 0) return;
   F[lane] = incl - sl;
@@ -938,7 +954,7 @@ __device__ inline uint32_t rank_scan_wave(uint2* bw, uint32_t nw, uint32_t lane)
   const uint32_t K = nw >> 6;
   uint32_t s = 0;
   for (uint32_t k = 0; k < K; ++k) s += (uint32_t)__builtin_popcount(bw[lane * K + k].x);
-  const uint32_t incl = wave_incl_scan(s, lane);
+  const uint32_t incl = wave_incl_scan_dpp(s);
   const uint32_t total = uniform(__shfl(incl, 63, 64));
   uint32_t p = incl - s;
   for (uint32_t k = 0; k < K; ++k) {
