@@ -34,10 +34,10 @@ __device__ inline void flush_graph(unsigned long long& acc, uint32_t g,
   acc = 0;
 }
 
-// WL: tasks stream their work lists (AUTO's default schedule; wse holds each
-// entry's list bounds); otherwise the push form, task u streaming N+(v) for
-// v in N+(u) (wrp, wse unused).
-template <int CAP, uint32_t FMIN, bool WL>
+// The push form: task u streams N+(v) for v in N+(u), each entry tested
+// against the hashed bitmap of N+(u) and the survivors searched (wrp and
+// wse unused; they keep the work-list kernel's signature).
+template <int CAP, uint32_t FMIN>
 __global__ __launch_bounds__(TB) void k_batch(const uint32_t* __restrict__ rp,
                                               const uint32_t* __restrict__ col,
                                               const uint32_t* __restrict__ wrp,
@@ -56,13 +56,7 @@ __global__ __launch_bounds__(TB) void k_batch(const uint32_t* __restrict__ rp,
     const uint32_t t = S.t;
     if (t >= nblk) break;
     const uint32_t u = blk[t];
-    unsigned long long c;
-    if constexpr (WL) {
-      const uint32_t wb = wrp[u];
-      c = block_vertex<CAP, 1, true>(rp, col, u, S, wse + wb, wrp[u + 1] - wb);
-    } else {
-      c = block_vertex<CAP, 1>(rp, col, u, S);
-    }
+    unsigned long long c = block_vertex<CAP, 1>(rp, col, u, S);
     flush_graph(c, gid[u], counts);
   }
   const WaveSlot W = wave_slot(S, wv);
@@ -81,12 +75,66 @@ __global__ __launch_bounds__(TB) void k_batch(const uint32_t* __restrict__ rp,
         if (cur != 0xFFFFFFFFu) flush_graph(acc, cur, counts);
         cur = g;
       }
-      if constexpr (WL) {
-        const uint32_t wb = wrp[u];
-        acc += wave_vertex<FMIN, 1, true>(rp, col, u, lane, W, wse + wb, wrp[u + 1] - wb);
-      } else {
-        acc += wave_vertex<FMIN, 1>(rp, col, u, lane, W);
+      acc += wave_vertex<FMIN, 1>(rp, col, u, lane, W);
+    }
+  }
+  if (cur != 0xFFFFFFFFu) flush_graph(acc, cur, counts);
+}
+
+// The work-list form (AUTO's default schedule; wse holds each entry's list
+// bounds): each task builds the rank-indexed table of N+(u) and decides each
+// streamed entry by one lookup, as k_auto_wl does; a block task whose N+(u)
+// exceeds CAP searches it in global memory.
+template <int CAP>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch_wl(const uint32_t* __restrict__ rp,
+                                                 const uint32_t* __restrict__ col,
+                                                 const uint32_t* __restrict__ wrp,
+                                                 const uint2* __restrict__ wse,
+                                                 const uint32_t* __restrict__ gid,
+                                                 const uint32_t* __restrict__ blk, uint32_t nblk,
+                                                 const uint32_t* __restrict__ wav, uint32_t nwav,
+                                                 uint32_t wave_chunk, uint32_t* cursor,
+                                                 unsigned long long* counts) {
+  __shared__ RankLds<CAP> S;
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (uint32_t i = tid; i < RankLds<CAP>::kRegion; i += TB) S.region[i] = 0;
+  __syncthreads();
+  for (bool first = true;; first = false) {
+    if (tid == 0) S.t = claim(cursor, nblk, 1u, first);
+    __syncthreads();
+    const uint32_t t = S.t;
+    if (t >= nblk) break;
+    const uint32_t u = blk[t], wb = wrp[u], wn = wrp[u + 1] - wb;
+    unsigned long long c;
+    if (uniform(rp[u + 1] - rp[u]) <= (uint32_t)CAP) {
+      c = block_rank<CAP>(rp, col, u, S, wse + wb, wn);  // ends with a barrier
+    } else {
+      c = block_global(rp, col, u, wse + wb, wn);
+      __syncthreads();  // every wave has read S.t before it is claimed again
+    }
+    flush_graph(c, gid[u], counts);
+  }
+  // the wave slots overlap the block view's table: start them from zero
+  for (uint32_t i = tid; i < RankLds<CAP>::kRegion; i += TB) S.region[i] = 0;
+  __syncthreads();
+  const RankSlot W = rank_wave(S, wv);
+  unsigned long long acc = 0;
+  uint32_t cur = 0xFFFFFFFFu;
+  for (bool first = true;; first = false) {
+    uint32_t i0 = 0;
+    if (lane == 0) i0 = claim(cursor + 1, nwav, wave_chunk, first);
+    i0 = __shfl(i0, 0, 64);
+    if (i0 >= nwav) break;
+    const uint32_t i1 = min(nwav, i0 + wave_chunk);
+    for (uint32_t i = i0; i < i1; ++i) {
+      const uint32_t u = wav[i];
+      const uint32_t g = gid[u];
+      if (g != cur) {
+        if (cur != 0xFFFFFFFFu) flush_graph(acc, cur, counts);
+        cur = g;
       }
+      const uint32_t wb = wrp[u];
+      acc += wave_rank<CAP>(rp, col, u, lane, W, wse + wb, wrp[u + 1] - wb);
     }
   }
   if (cur != 0xFFFFFFFFu) flush_graph(acc, cur, counts);
@@ -217,18 +265,16 @@ int check_sched(const tc_batch_sched* s) {
   return TC_OK;
 }
 
-// k_batch for the schedule's form and wave-variant filter threshold (1 or 16)
+// The batched kernel for the schedule's form (the push form also for its
+// wave-variant filter threshold, 1 or 16; the work-list form has none)
 template <typename... A>
 void launch_batch(const tc_batch_sched& s, dim3 grid, hipStream_t st, A... args) {
-  const bool wl = s.form == TC_BATCH_WORKLIST;
-  if (s.filter_min == 1 && wl)
-    hipLaunchKernelGGL((k_batch<2048, 1, true>), grid, dim3(TB), 0, st, args...);
-  else if (wl)
-    hipLaunchKernelGGL((k_batch<2048, 16, true>), grid, dim3(TB), 0, st, args...);
+  if (s.form == TC_BATCH_WORKLIST)
+    hipLaunchKernelGGL((k_batch_wl<2048>), grid, dim3(TB), 0, st, args...);
   else if (s.filter_min == 1)
-    hipLaunchKernelGGL((k_batch<2048, 1, false>), grid, dim3(TB), 0, st, args...);
+    hipLaunchKernelGGL((k_batch<2048, 1>), grid, dim3(TB), 0, st, args...);
   else
-    hipLaunchKernelGGL((k_batch<2048, 16, false>), grid, dim3(TB), 0, st, args...);
+    hipLaunchKernelGGL((k_batch<2048, 16>), grid, dim3(TB), 0, st, args...);
 }
 
 int schedule(tc_ctx* c, tc_batch* b, uint32_t wave_max_deg, uint32_t form) {

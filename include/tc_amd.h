@@ -519,8 +519,9 @@ typedef struct tc_batch_sched {
   uint32_t wave_max_deg;  /* vertices with 2 <= d+ <= this run as wave tasks (64) */
   uint32_t wave_chunk;    /* wave tasks fetched per cursor bump (16)              */
   uint32_t grid;          /* persistent workgroups (0 = 8 per CU)                  */
-  uint32_t filter_min;    /* wave tasks hash N+(u) into a bitmap filter above this
-                             d+: 1 or 16 (0 = 16)                                 */
+  uint32_t filter_min;    /* push form: wave tasks hash N+(u) into a bitmap filter
+                             above this d+: 1 or 16 (0 = 16); the work-list form
+                             decides every entry by a rank-indexed table of N+(u) */
   uint32_t form;          /* TC_BATCH_WORKLIST (0): tasks stream AUTO's work
                              lists; TC_BATCH_PUSH (1): task u streams N+(v) for
                              v in N+(u)                                           */
