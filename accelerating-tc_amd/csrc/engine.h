@@ -13,9 +13,9 @@
 //     branchless binary search of N+(u).
 // Step 2 replaces a log2(d+(u))-deep chain of dependent LDS reads per entry
 // by one read; step 3 keeps the searches at full lane occupancy.  The
-// per-vertex, push-form and batched kernels work this way.  The work-list
-// count kernel (k_auto_wl) replaces steps 2-4 by one lookup per entry in a
-// rank-indexed table of N+(u) (see RankLds below).
+// per-vertex kernel and the push forms work this way.  The work-list
+// kernels (k_auto_wl, the batched work-list form) replace steps 2-4 by one
+// lookup per entry in a rank-indexed table of N+(u) (see RankLds below).
 #pragma once
 
 #include <hip/hip_runtime.h>
